@@ -1,0 +1,122 @@
+/*
+ * pra.h -- C ABI of the MI355X backend for parallel-RA's semi-naive hot path.
+ *
+ * The reference exposes no FFI: its "operator API" is the C++ class surface
+ * that generated Datalog programs call (relation, RAM, parallel_join,
+ * parallel_copy, parallel_acopy, parallel_copy_filter, LIE).  Each entry point
+ * below replaces one piece of that surface; the C++ facade under
+ * include/parallel_RA_inc.h forwards to it so such programs compile unchanged.
+ *
+ * Conventions: every function returns 0 on success or a negative PRA_E_* code;
+ * pra_last_error() describes the most recent failure of a context.  Handles are
+ * owned by the caller and freed with the matching *_destroy.  One context is
+ * one LIE instance on one device and is not thread safe.  Rows are arrays of
+ * (arity + 1) uint64 words, the last word being the intern id column.
+ */
+#ifndef PRA_H
+#define PRA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pra_ctx pra_ctx;
+typedef struct pra_rel pra_rel;
+typedef struct pra_scc pra_scc;
+
+/* relation versions: src/relation/balanced_hash_relation.h:13 (DELTA=0, FULL) */
+#define PRA_DELTA 0
+#define PRA_FULL 1
+#define PRA_NEWT 2
+
+/* copy_filter predicates over input columns 0 and 1 */
+#define PRA_FILTER_EQ 0
+#define PRA_FILTER_NEQ 1
+#define PRA_FILTER_LT 2
+#define PRA_FILTER_LE 3
+#define PRA_FILTER_GT 4
+#define PRA_FILTER_GE 5
+#define PRA_FILTER_TRUE 6
+#define PRA_FILTER_FALSE 7
+
+#define PRA_OK 0
+#define PRA_E_ARG -1        /* invalid argument or shape */
+#define PRA_E_HIP -2        /* HIP runtime failure */
+#define PRA_E_IO -3         /* file missing or malformed */
+#define PRA_E_UNSUPPORTED -4
+#define PRA_E_COLLISION -5  /* 64-bit key hash collision detected */
+#define PRA_E_STATE -6      /* call out of order */
+
+/* Context = one LIE (src/lie/lie.h:13-153) on one device with `nbuckets`
+ * virtual buckets, i.e. the layout of the reference run on `nbuckets` MPI
+ * ranks (mpi_comm::create, src/comm/comm.cpp:22-28).  hbm_budget 0 = no cap. */
+int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out);
+int pra_ctx_destroy(pra_ctx* ctx);
+const char* pra_last_error(pra_ctx* ctx);
+/* LIE::set_batch_size (src/lie/lie.h:116) */
+int pra_ctx_set_batch_size(pra_ctx* ctx, uint32_t batch_size);
+/* the 20,000,000-output threshold of RAM::local_compute (src/RAM/RA_tasks.cpp:312) */
+int pra_ctx_set_join_threshold(pra_ctx* ctx, int64_t threshold);
+/* total (delta row, inner row) join pairs so far, counted before dedup */
+int pra_ctx_joined_tuples(pra_ctx* ctx, uint64_t* out);
+/* fixed-point iterations executed so far over all SCCs */
+int pra_ctx_iterations(pra_ctx* ctx, uint64_t* out);
+/* per-kernel timing with HIP events on the context stream (0 = off) */
+int pra_ctx_set_profiling(pra_ctx* ctx, int on);
+int pra_ctx_kernel_stat(pra_ctx* ctx, int idx, const char** name, uint64_t* launches, double* ms, double* bytes);
+int pra_ctx_synchronize(pra_ctx* ctx);
+
+/* relation(jcc, is_canonical, arity, tag, name, file, version)
+ * (src/relation/balanced_hash_relation.h:90-102) */
+int pra_rel_create(pra_ctx* ctx, uint32_t arity, uint32_t jcc, uint32_t tag, int canonical, pra_rel** out);
+int pra_rel_destroy(pra_rel* rel);
+/* populate_full / populate_delta from host rows
+ * (src/relation/balanced_hash_relation.cpp:773-815) */
+int pra_rel_load_rows(pra_rel* rel, const uint64_t* rows, uint64_t nrows, int version);
+/* tsv_to_bin format: <path> raw LE rows, <path>.size "rows\ncols"
+ * (src/IO/parallel_io.cpp:115-235, utility/tsv_to_bin.cpp) */
+int pra_rel_load_bin(pra_rel* rel, const char* path, int version);
+/* element count of a version (get_full_element_count and friends) */
+int pra_rel_count(pra_rel* rel, int version, uint64_t* out);
+/* rows of a version, bucket-major and lexicographic inside each bucket (the
+ * trie walk order of src/relation/shmap_relation_exp.cpp:77-112) */
+int pra_rel_download(pra_rel* rel, int version, uint64_t* rows, uint64_t cap, uint64_t* nrows);
+/* device pointer to the sorted rows of a version (valid until the next call
+ * that changes the relation) */
+int pra_rel_device_rows(pra_rel* rel, int version, const uint64_t** rows, uint64_t* nrows);
+
+/* RAM(bool run_to_fixed_point, int id) (src/RAM/RA_tasks.h:52) */
+int pra_scc_create(pra_ctx* ctx, int fixpoint, int id, pra_scc** out);
+int pra_scc_destroy(pra_scc* scc);
+/* RAM::add_relation(relation*&, bool move_full_to_delta) (src/RAM/RA_tasks.cpp:34-39) */
+int pra_scc_add_relation(pra_scc* scc, pra_rel* rel, int move_full_to_delta);
+/* parallel_join(out, in0, ver0, in1, ver1, projection) (src/RA/parallel_join.h:55-58) */
+int pra_scc_add_join(pra_scc* scc, pra_rel* out, pra_rel* in0, int ver0, pra_rel* in1, int ver1, const int* proj,
+                     int nproj);
+/* parallel_copy(dest, src, ver, reorder) (src/RA/parallel_copy.h:28-32) */
+int pra_scc_add_copy(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, const int* reorder, int n);
+/* parallel_acopy(dest, src, ver, reorder) (src/RA/parallel_acopy.h:45-49) */
+int pra_scc_add_acopy(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, const int* reorder, int n);
+/* parallel_copy_filter(dest, src, ver, reorder, predicate) with the predicate
+ * given as a PRA_FILTER_* code (src/RA/parallel_copy_filter.h:29-33) */
+int pra_scc_add_copy_filter(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, const int* reorder, int n, int op);
+/* LIE::add_scc_dependance(before, after) (src/lie/lie.cpp:79-102) */
+int pra_scc_add_dependency(pra_scc* before, pra_scc* after);
+/* Run one SCC the way LIE::execute does (src/lie/lie.cpp:286-441); delta_hist
+ * receives the global delta count after every batch (RAM::check_for_fixed_point,
+ * src/RAM/RA_tasks.cpp:1072-1085). */
+int pra_scc_run(pra_scc* scc, uint64_t* delta_hist, uint64_t cap, uint64_t* nbatches);
+/* number of batches run by the SCC and their delta counts */
+int pra_scc_history(pra_scc* scc, uint64_t* out, uint64_t cap, uint64_t* n);
+
+/* LIE::execute (src/lie/lie.cpp:197-448): run every SCC created on the
+ * context in dependency order, first runnable in creation order. */
+int pra_execute(pra_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,98 @@
+// Device-side building blocks shared by the semi-naive RA kernels (gfx950).
+//
+// Tuples are rows of w = arity+1 little-endian u64 words, the last word being
+// the intern id.  Every sorted row array is ordered by (bucket, col0, col1, ..),
+// bucket = tuple_hash(row[0:jcc]) % nbuckets, which is the order in which the
+// reference walks its per-rank nested B-tree tries
+// (src/relation/shmap_relation_exp.cpp:77-112) concatenated over ranks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+#define PRA_BLOCK 256
+#define PRA_WAVES (PRA_BLOCK / 64)
+#define PRA_ITEMS 8
+#define PRA_TILE (PRA_BLOCK * PRA_ITEMS)
+#define PRA_RADIX_BITS 8
+#define PRA_RADIX (1 << PRA_RADIX_BITS)
+#define PRA_MAXW 32
+
+// flags word bits reported back to the host
+#define PRA_FLAG_COLLISION 1u
+#define PRA_FLAG_TABLE_FULL 2u
+#define PRA_FLAG_INDEX_COLLISION 4u
+
+// FNV-1a-64 over the 8 little-endian bytes of each word, exactly the
+// reference's tuple_hash (src/hash/hash.h:11-30).
+__host__ __device__ inline u64 pra_fnv(const u64* t, int n)
+{
+    u64 h = 14695981039346656037ULL;
+    for (int i = 0; i < n; i++) {
+        u64 w = t[i];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            h ^= (w >> (8 * b)) & 255ULL;
+            h *= 1099511628211ULL;
+        }
+    }
+    return h;
+}
+
+__host__ __device__ inline u64 pra_mix64(u64 x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+// 64-bit table hash of a key of n words; never 0 (0 marks an empty slot).
+__host__ __device__ inline u64 pra_keyhash(const u64* k, int n)
+{
+    u64 h = 0x9E3779B97F4A7C15ULL ^ (u64)n;
+    for (int i = 0; i < n; i++) h = pra_mix64(h ^ (k[i] + 0x632BE59BD9B4E019ULL * (u64)(i + 1)));
+    return h ? h : 1;
+}
+
+// One radix digit of a row: either bits of a column or bits of the bucket.
+struct DigitSpec {
+    int col;       // column index, or -1 for the bucket digit
+    int shift;     // bit offset of the digit
+    int jcc;       // bucket: number of leading columns hashed
+    int pad;
+    u64 nb;        // bucket count
+};
+
+__device__ inline u32 pra_digit(const u64* row, const DigitSpec& ds)
+{
+    u64 v;
+    if (ds.col >= 0)
+        v = row[ds.col];
+    else
+        v = pra_fnv(row, ds.jcc) % ds.nb;
+    return (u32)((v >> ds.shift) & (PRA_RADIX - 1));
+}
+
+__device__ inline u64 pra_lanemask_lt()
+{
+    u32 lane = __lane_id();
+    return lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+}
+
+// Mask of the active lanes holding the same `bits`-bit value as this lane.
+template <int BITS>
+__device__ inline u64 pra_match(u32 v, u64 active)
+{
+    u64 peers = active;
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        u64 m = __ballot((v >> b) & 1u);
+        peers &= ((v >> b) & 1u) ? m : ~m;
+    }
+    return peers;
+}

@@ -1,0 +1,1233 @@
+// Host engine of the MI355X semi-naive RA backend: device-resident relation
+// versions, the per-iteration pipeline (join / copy emit -> arrival-ordered
+// candidates -> dedup + intern -> stable bucket partition -> LSD sort) and the
+// LIE-style SCC scheduler, exported through the C ABI of include/pra.h.
+//
+// Layout in HBM: every version (full, delta, newt) of a relation is one array
+// of rows of w = arity+1 u64 words.  delta and newt are kept sorted by
+// (bucket, lexicographic row); full is an append log that is re-sorted only
+// when an ordered walk of it is needed (copy from FULL, join probe side).
+// Membership of full u delta is an open-addressing table keyed by the dedup
+// columns; join probe sides get a (key -> first row, row count) index.
+#include "../../include/pra.h"
+#include "pra_kernels.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+
+enum { RA_COPY = 0, RA_COPY_FILTER = 1, RA_ACOPY = 3, RA_JOIN = 4 };
+
+struct PraError : std::runtime_error {
+    int code;
+    PraError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            throw PraError(PRA_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+static void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw PraError(PRA_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // grow to at least `bytes`; `keep` preserves the first `used` bytes
+    void reserve(size_t bytes, hipStream_t st, size_t used = 0)
+    {
+        if (bytes <= cap) return;
+        size_t nc = std::max(bytes, cap + cap / 2);
+        nc = std::max<size_t>(nc, 256);
+        void* np = nullptr;
+        HIPCHK(hipMalloc(&np, nc));
+        if (p && used) HIPCHK(hipMemcpyAsync(np, p, used, hipMemcpyDeviceToDevice, st));
+        if (p) {
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipFree(p));
+        }
+        p = np;
+        cap = nc;
+    }
+    template <typename T> T* as() { return static_cast<T*>(p); }
+    void swap(DBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
+};
+
+struct Rows {
+    DBuf buf;
+    u64 n = 0;
+    bool sorted = true;
+    u64* d() { return buf.as<u64>(); }
+};
+
+static u64 pow2_at_least(u64 x)
+{
+    u64 c = 1;
+    while (c < x) c <<= 1;
+    return c;
+}
+
+} // namespace
+
+struct pra_scc;
+
+struct pra_rel {
+    pra_ctx* ctx = nullptr;
+    int arity = 0, jcc = 0, canonical = 1, w = 0;
+    u32 tag = 0;
+    Rows ver[3];  // PRA_DELTA, PRA_FULL, PRA_NEWT
+    // membership table of full u delta keyed on mt_kw leading columns
+    DBuf mt;
+    u64 mt_cap = 0, mt_entries = 0;
+    int mt_kw = -1;
+    bool mt_valid = false;
+    // join index (key -> first row, count) per version
+    DBuf idx[2];
+    u64 idx_cap[2] = {0, 0};
+    bool idx_valid[2] = {false, false};
+    // bucket bounds per version
+    DBuf bnd[2];
+    bool bnd_valid[2] = {false, false};
+    void touched(int v)
+    {
+        if (v == PRA_DELTA || v == PRA_FULL) {
+            idx_valid[v] = false;
+            bnd_valid[v] = false;
+        }
+    }
+};
+
+namespace {
+
+struct RADesc {
+    int type = 0;
+    pra_rel *out = nullptr, *in0 = nullptr, *in1 = nullptr;
+    int v0 = 0, v1 = 0, op = PRA_F_TRUE;
+    std::vector<int> map;
+    // per-iteration state
+    DBuf cnt, off, istart, posadj;
+    u64 n_outer = 0;
+    pra_rel* outer = nullptr;
+    int outer_v = 0;
+    pra_rel* inner = nullptr;
+    int inner_v = 0;
+    int left = 1;
+};
+
+struct KStat {
+    const char* name;
+    uint64_t launches = 0;
+    double ms = 0, bytes = 0;
+};
+
+enum {
+    K_JOIN_COUNT, K_JOIN_EXPAND, K_COPY_EXPAND, K_DEDUP_CLAIM, K_DEDUP_DECIDE, K_DEDUP_VERIFY, K_DEDUP_SCATTER,
+    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_COUNT
+};
+static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand", "dedup_claim", "dedup_decide",
+                                      "dedup_verify_hist", "dedup_scatter", "radix_hist", "radix_scatter",
+                                      "table_build", "index_build", "scan"};
+
+} // namespace
+
+struct pra_scc {
+    pra_ctx* ctx = nullptr;
+    int fixpoint = 1, id = 0;
+    std::vector<pra_rel*> rels;
+    std::vector<int> status;
+    std::vector<RADesc> ras;
+    bool done = false;
+    std::vector<u64> hist;
+};
+
+struct pra_ctx {
+    int device = 0, nb = 1;
+    hipStream_t st = nullptr;
+    u64 budget = 0;
+    u32 batch_size = 10;  // LIE::batch_size default (src/lie/lie.h:17)
+    long long threshold = 20000000;
+    std::vector<pra_rel*> rels;
+    std::vector<pra_scc*> sccs;
+    std::vector<std::pair<pra_scc*, pra_scc*>> deps;
+    std::map<u32, std::unique_ptr<DBuf>> seq;  // per intern tag: nb counters
+    // scratch
+    DBuf partials, hist, offs, sort_tmp, flags_buf, stage, stage2, orand, tmp64a, tmp64b, tmp64c;
+    DBuf cand, dest, slot_of, keep;
+    u64* host_stage = nullptr;  // pinned
+    size_t host_stage_cap = 0;
+    u64 joined = 0, iterations = 0;
+    std::string err;
+    // profiling
+    bool prof = false;
+    KStat kst[K_COUNT];
+    struct Pending { int k; hipEvent_t a, b; double bytes; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> evpool;
+
+    pra_ctx()
+    {
+        for (int i = 0; i < K_COUNT; i++) kst[i].name = kNames[i];
+    }
+    ~pra_ctx()
+    {
+        resolve_prof();
+        for (auto e : evpool) (void)hipEventDestroy(e);
+        if (host_stage) (void)hipHostFree(host_stage);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    hipEvent_t ev()
+    {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    int prof_begin(int k, hipEvent_t* a)
+    {
+        if (!prof) return -1;
+        *a = ev();
+        HIPCHK(hipEventRecord(*a, st));
+        return k;
+    }
+    void prof_end(int k, hipEvent_t a, double bytes) noexcept
+    {
+        if (k < 0) return;
+        hipEvent_t b = nullptr;
+        if (!evpool.empty()) {
+            b = evpool.back();
+            evpool.pop_back();
+        } else if (hipEventCreate(&b) != hipSuccess) {
+            return;
+        }
+        if (hipEventRecord(b, st) != hipSuccess) return;
+        pending.push_back({k, a, b, bytes});
+    }
+    void resolve_prof()
+    {
+        if (pending.empty()) return;
+        if (hipStreamSynchronize(st) != hipSuccess) return;
+        for (auto& p : pending) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                kst[p.k].ms += ms;
+                kst[p.k].launches++;
+                kst[p.k].bytes += p.bytes;
+            }
+            evpool.push_back(p.a);
+            evpool.push_back(p.b);
+        }
+        pending.clear();
+    }
+
+    u64* host_buf(size_t words)
+    {
+        if (words > host_stage_cap) {
+            if (host_stage) HIPCHK(hipHostFree(host_stage));
+            host_stage = nullptr;
+            size_t c = std::max<size_t>(words, 1024);
+            HIPCHK(hipHostMalloc((void**)&host_stage, c * sizeof(u64)));
+            host_stage_cap = c;
+        }
+        return host_stage;
+    }
+    u32* dflags() { return flags_buf.as<u32>(); }
+};
+
+namespace {
+
+struct Prof {
+    pra_ctx* c;
+    int k;
+    hipEvent_t a;
+    double bytes;
+    Prof(pra_ctx* c_, int k_, double b) : c(c_), k(-1), a(nullptr), bytes(b) { k = c->prof_begin(k_, &a); }
+    ~Prof() { c->prof_end(k, a, bytes); }
+};
+
+// ----------------------------------------------------------------- scans ---
+
+static void scan_u64(pra_ctx* c, const u64* in, u64* out, u64 n, u64* total)
+{
+    Prof p(c, K_SCAN, 16.0 * n);
+    u64 np = std::max<u64>(1, pra_scan_partials_count(n));
+    c->partials.reserve(np * 8, c->st);
+    pra_scan_u64(c->st, in, out, n, c->partials.as<u64>(), total);
+    check_launch("scan_u64");
+}
+
+static void scan_u32(pra_ctx* c, const u32* in, u32* out, u64 n, u32* total)
+{
+    Prof p(c, K_SCAN, 8.0 * n);
+    u64 np = std::max<u64>(1, pra_scan_partials_count(n));
+    c->partials.reserve(np * 8, c->st);
+    pra_scan_u32(c->st, in, out, n, c->partials.as<u32>(), total);
+    check_launch("scan_u32");
+}
+
+static void clear_flags(pra_ctx* c)
+{
+    c->flags_buf.reserve(64, c->st);
+    pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
+}
+
+// read back `words` u64 from device memory (one synchronization)
+static u64* readback(pra_ctx* c, const void* src, size_t words)
+{
+    u64* h = c->host_buf(words + 1);
+    if (words) HIPCHK(hipMemcpyAsync(h, src, words * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return h;
+}
+
+static u32 read_flags(pra_ctx* c)
+{
+    u64* h = readback(c, c->flags_buf.p, 1);
+    return (u32)h[0];
+}
+
+static void check_flags(u32 f)
+{
+    if (f & (PRA_FLAG_COLLISION | PRA_FLAG_INDEX_COLLISION))
+        throw PraError(PRA_E_COLLISION, "64-bit key hash collision between distinct keys");
+    if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
+}
+
+// ------------------------------------------------------------------ sort ---
+
+// LSD radix sort of rows by (bucket, col0 .. col[keycols-1]); digits that are
+// equal for every row are skipped.
+static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols)
+{
+    u64 n = r.n;
+    if (n <= 1) {
+        r.sorted = true;
+        return;
+    }
+    int kc = std::min(keycols, 16);
+    c->orand.reserve(32 * 8, c->st);
+    u64* om = c->orand.as<u64>();
+    pra_launch_fill_u64(c->st, om, 16, 0);
+    pra_launch_fill_u64(c->st, om + 16, 16, ~0ULL);
+    pra_launch_col_orand(c->st, r.d(), n, w, kc, om);
+    check_launch("col_orand");
+    u64* h = readback(c, om, 32);
+    std::vector<DigitSpec> passes;
+    for (int col = keycols - 1; col >= 0; col--) {
+        u64 vary = col < 16 ? (h[col] ^ h[16 + col]) : ~0ULL;
+        for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
+            if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{col, sh, 0, 0, 0});
+    }
+    if (c->nb > 1)
+        for (int sh = 0; ((u64)(c->nb - 1) >> sh) != 0; sh += PRA_RADIX_BITS)
+            passes.push_back(DigitSpec{-1, sh, jcc, 0, (u64)c->nb});
+    if (passes.empty()) {
+        r.sorted = true;
+        return;
+    }
+    u32 nblocks = pra_tiles(n);
+    c->hist.reserve((size_t)PRA_RADIX * nblocks * 4, c->st);
+    c->offs.reserve((size_t)PRA_RADIX * nblocks * 4 + 8, c->st);
+    c->sort_tmp.reserve(n * w * 8, c->st);
+    for (auto& ds : passes) {
+        {
+            Prof p(c, K_RADIX_HIST, 8.0 * n * (ds.col >= 0 ? 1 : jcc));
+            pra_launch_radix_hist(c->st, r.d(), n, w, ds, c->hist.as<u32>(), nblocks);
+            check_launch("radix_hist");
+        }
+        scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)PRA_RADIX * nblocks, nullptr);
+        {
+            Prof p(c, K_RADIX_SCATTER, 16.0 * n * w);
+            pra_launch_radix_scatter(c->st, r.d(), c->sort_tmp.as<u64>(), n, w, ds, c->offs.as<u32>(), nblocks);
+            check_launch("radix_scatter");
+        }
+        r.buf.swap(c->sort_tmp);
+    }
+    r.sorted = true;
+}
+
+// remove rows equal (in all w columns) to their predecessor; rows sorted
+static void unique_rows(pra_ctx* c, Rows& r, int w)
+{
+    u64 n = r.n;
+    if (n <= 1) return;
+    c->tmp64a.reserve(n * 8, c->st);
+    c->tmp64b.reserve(n * 8 + 8, c->st);
+    c->sort_tmp.reserve(n * w * 8, c->st);
+    pra_launch_group_flags(c->st, r.d(), n, w, w, c->tmp64a.as<u64>());
+    scan_u64(c, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), n, c->tmp64b.as<u64>() + n);
+    pra_launch_compact_rows(c->st, r.d(), n, w, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), c->sort_tmp.as<u64>());
+    check_launch("compact_rows");
+    u64* h = readback(c, c->tmp64b.as<u64>() + n, 1);
+    r.buf.swap(c->sort_tmp);
+    r.n = h[0];
+}
+
+static void ensure_sorted(pra_ctx* c, pra_rel* R, int v)
+{
+    Rows& r = R->ver[v];
+    if (r.sorted) return;
+    sort_rows(c, r, R->w, R->jcc, R->w);
+    R->touched(v);
+}
+
+static const u64* bucket_bounds(pra_ctx* c, pra_rel* R, int v)
+{
+    Rows& r = R->ver[v];
+    R->bnd[v].reserve((size_t)(c->nb + 1) * 8, c->st);
+    if (!R->bnd_valid[v]) {
+        if (c->nb == 1) {
+            pra_launch_fill_u64(c->st, R->bnd[v].as<u64>(), 1, 0);
+            pra_launch_fill_u64(c->st, R->bnd[v].as<u64>() + 1, 1, r.n);
+            check_launch("bucket_bounds");
+        } else {
+            pra_launch_bucket_bounds(c->st, r.d(), r.n, R->w, R->jcc, (u64)c->nb, R->bnd[v].as<u64>());
+            check_launch("bucket_bounds");
+        }
+        R->bnd_valid[v] = true;
+    }
+    return R->bnd[v].as<u64>();
+}
+
+// (key -> first row, count) index over the sorted rows of a version
+static void ensure_index(pra_ctx* c, pra_rel* R, int v)
+{
+    if (R->idx_valid[v]) return;
+    ensure_sorted(c, R, v);
+    Rows& r = R->ver[v];
+    u64 n = r.n;
+    if (n == 0) {
+        R->idx_cap[v] = 0;
+        R->idx_valid[v] = true;
+        return;
+    }
+    Prof p(c, K_INDEX_BUILD, 8.0 * n * R->jcc);
+    c->tmp64a.reserve(n * 8, c->st);
+    c->tmp64b.reserve((n + 1) * 8, c->st);
+    c->tmp64c.reserve((n + 1) * 8, c->st);
+    pra_launch_group_flags(c->st, r.d(), n, R->w, R->jcc, c->tmp64a.as<u64>());
+    scan_u64(c, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), n, c->tmp64b.as<u64>() + n);
+    pra_launch_group_compact(c->st, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), n, c->tmp64c.as<u64>());
+    check_launch("group_compact");
+    u64 ng = readback(c, c->tmp64b.as<u64>() + n, 1)[0];
+    u64 cap = pow2_at_least(std::max<u64>(64, 2 * ng));
+    R->idx[v].reserve(cap * 3 * 8, c->st);
+    pra_launch_table_clear(c->st, R->idx[v].as<u64>(), cap, 3);
+    clear_flags(c);
+    pra_launch_index_insert(c->st, r.d(), R->w, R->jcc, c->tmp64c.as<u64>(), ng, R->idx[v].as<u64>(), cap,
+                            c->dflags());
+    check_launch("index_insert");
+    check_flags(read_flags(c));
+    R->idx_cap[v] = cap;
+    R->idx_valid[v] = true;
+}
+
+// membership table of full u delta keyed on kw leading columns, with room for
+// `incoming` more keys at load factor <= 1/2
+static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
+{
+    int stride = 2 + kw;
+    if (!R->mt_valid || R->mt_kw != kw) {
+        Prof p(c, K_TABLE_BUILD, 8.0 * (R->ver[PRA_FULL].n + R->ver[PRA_DELTA].n) * kw);
+        u64 entries = R->ver[PRA_FULL].n + R->ver[PRA_DELTA].n;
+        u64 cap = pow2_at_least(std::max<u64>(1024, 2 * (entries + incoming)));
+        R->mt.reserve(cap * stride * 8, c->st);
+        pra_launch_table_clear(c->st, R->mt.as<u64>(), cap, stride);
+        clear_flags(c);
+        for (int v : {PRA_FULL, PRA_DELTA}) {
+            Rows& r = R->ver[v];
+            pra_launch_table_insert_rows(c->st, r.d(), r.n, R->w, kw, R->mt.as<u64>(), cap, c->dflags());
+            check_launch("table_insert_rows");
+        }
+        for (int v : {PRA_FULL, PRA_DELTA}) {
+            Rows& r = R->ver[v];
+            pra_launch_table_verify_rows(c->st, r.d(), r.n, R->w, kw, R->mt.as<u64>(), cap, c->dflags());
+            check_launch("table_verify_rows");
+        }
+        check_flags(read_flags(c));
+        R->mt_cap = cap;
+        R->mt_entries = entries;
+        R->mt_kw = kw;
+        R->mt_valid = true;
+        return;
+    }
+    if (2 * (R->mt_entries + incoming) <= R->mt_cap) return;
+    Prof p(c, K_TABLE_BUILD, 8.0 * R->mt_cap * stride);
+    u64 cap = pow2_at_least(2 * (R->mt_entries + incoming));
+    DBuf nt;
+    nt.reserve(cap * stride * 8, c->st);
+    pra_launch_table_clear(c->st, nt.as<u64>(), cap, stride);
+    clear_flags(c);
+    pra_launch_table_rehash(c->st, R->mt.as<u64>(), R->mt_cap, nt.as<u64>(), cap, kw, c->dflags());
+    check_launch("table_rehash");
+    check_flags(read_flags(c));
+    R->mt.swap(nt);
+    R->mt_cap = cap;
+}
+
+static DBuf& seq_counters(pra_ctx* c, u32 tag)
+{
+    auto it = c->seq.find(tag);
+    if (it != c->seq.end()) return *it->second;
+    std::unique_ptr<DBuf> b(new DBuf());
+    b->reserve((size_t)c->nb * 8, c->st);
+    pra_launch_fill_u64(c->st, b->as<u64>(), (u64)c->nb, 0);
+    DBuf& ref = *b;
+    c->seq[tag] = std::move(b);
+    return ref;
+}
+
+// append src rows (already on device) to a version
+static void append_rows(pra_ctx* c, pra_rel* R, int v, const u64* src, u64 n)
+{
+    if (!n) return;
+    Rows& r = R->ver[v];
+    size_t used = (size_t)r.n * R->w * 8;
+    r.buf.reserve(used + (size_t)n * R->w * 8, c->st, used);
+    HIPCHK(hipMemcpyAsync(r.d() + r.n * R->w, src, (size_t)n * R->w * 8, hipMemcpyDeviceToDevice, c->st));
+    r.n += n;
+    r.sorted = false;
+    R->touched(v);
+}
+
+// ------------------------------------------------------------ iteration ---
+
+static bool is_intern_type(int type) { return type != RA_ACOPY; }
+
+// One semi-naive iteration of an SCC (src/RAM/RA_tasks.cpp:1121-1269).
+static void run_iteration(pra_scc* S)
+{
+    pra_ctx* c = S->ctx;
+    const int nb = c->nb;
+    const int nra = (int)S->ras.size();
+
+    // 0. inputs ready: sorted versions, join indexes
+    for (auto& ra : S->ras) {
+        if (ra.type == RA_JOIN) {
+            if (ra.v0 == PRA_DELTA) {
+                ra.outer = ra.in0; ra.outer_v = PRA_DELTA; ra.inner = ra.in1; ra.inner_v = ra.v1; ra.left = 1;
+            } else {
+                ra.outer = ra.in1; ra.outer_v = ra.v1; ra.inner = ra.in0; ra.inner_v = PRA_FULL; ra.left = 0;
+            }
+            ensure_sorted(c, ra.outer, ra.outer_v);
+            ensure_index(c, ra.inner, ra.inner_v);
+        } else {
+            ra.outer = ra.in0; ra.outer_v = ra.v0; ra.inner = nullptr; ra.left = 1;
+            ensure_sorted(c, ra.outer, ra.outer_v);
+        }
+    }
+
+    // 1. per-RA output counts and per-source offsets
+    c->stage.reserve((size_t)std::max(1, nra) * (nb + 1) * 8, c->st);
+    for (int r = 0; r < nra; r++) {
+        RADesc& ra = S->ras[r];
+        Rows& O = ra.outer->ver[ra.outer_v];
+        u64 n = O.n;
+        ra.n_outer = n;
+        ra.cnt.reserve(std::max<u64>(1, n) * 8, c->st);
+        ra.off.reserve((n + 1) * 8, c->st);
+        ra.istart.reserve(std::max<u64>(1, n) * 8, c->st);
+        if (ra.type == RA_JOIN) {
+            Rows& I = ra.inner->ver[ra.inner_v];
+            if (I.n == 0 || ra.inner->idx_cap[ra.inner_v] == 0) {
+                pra_launch_fill_u64(c->st, ra.cnt.as<u64>(), n, 0);
+            } else {
+                Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + 16.0 * n);
+                pra_launch_join_count(c->st, O.d(), n, ra.outer->w, ra.in0->jcc, ra.inner->idx[ra.inner_v].as<u64>(),
+                                      ra.inner->idx_cap[ra.inner_v], I.d(), ra.inner->w, ra.cnt.as<u64>(),
+                                      ra.istart.as<u64>());
+                check_launch("join_count");
+            }
+        } else {
+            int op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
+            pra_launch_copy_count(c->st, O.d(), n, ra.outer->w, op, ra.cnt.as<u64>());
+            check_launch("copy_count");
+        }
+        scan_u64(c, ra.cnt.as<u64>(), ra.off.as<u64>(), n, ra.off.as<u64>() + n);
+        const u64* bs = bucket_bounds(c, ra.outer, ra.outer_v);
+        pra_launch_gather_srcoff(c->st, ra.off.as<u64>(), bs, (u64)nb, c->stage.as<u64>() + (size_t)r * (nb + 1));
+        check_launch("gather_srcoff");
+    }
+    std::vector<u64> srcoff((size_t)nra * (nb + 1));
+    {
+        u64* h = readback(c, c->stage.p, srcoff.size());
+        std::copy(h, h + srcoff.size(), srcoff.begin());
+    }
+
+    // 2. arrival order per output relation: source-major, RA-minor
+    //    (src/comm/all_to_allv_comm.cpp:64-142, src/RAM/RA_tasks.cpp:831-835)
+    std::vector<pra_rel*> outs;
+    for (auto& ra : S->ras)
+        if (std::find(outs.begin(), outs.end(), ra.out) == outs.end()) outs.push_back(ra.out);
+    std::vector<u64> totals(outs.size(), 0);
+    std::vector<u64> posadj((size_t)nra * nb);
+    for (size_t oi = 0; oi < outs.size(); oi++) {
+        u64 tot = 0;
+        for (int s = 0; s < nb; s++)
+            for (int r = 0; r < nra; r++) {
+                if (S->ras[r].out != outs[oi]) continue;
+                const u64* so = &srcoff[(size_t)r * (nb + 1)];
+                posadj[(size_t)r * nb + s] = tot - so[s];
+                u64 cnt = so[s + 1] - so[s];
+                if (S->ras[r].type == RA_JOIN && (long long)cnt > c->threshold)
+                    throw PraError(PRA_E_UNSUPPORTED,
+                                   "a source bucket produced more join outputs than the local-join threshold; "
+                                   "threshold rounds are not implemented on the device");
+                tot += cnt;
+            }
+        totals[oi] = tot;
+    }
+    for (int r = 0; r < nra; r++)
+        if (S->ras[r].type == RA_JOIN) c->joined += srcoff[(size_t)r * (nb + 1) + nb];
+
+    // 3+4. emit candidates and deduplicate, one output relation at a time
+    std::vector<int> out_intern(outs.size(), 1);
+    for (size_t oi = 0; oi < outs.size(); oi++) {
+        int type0 = -1;
+        for (auto& ra : S->ras)
+            if (ra.out == outs[oi]) {
+                if (type0 < 0) type0 = is_intern_type(ra.type);
+                else if (type0 != (int)is_intern_type(ra.type))
+                    throw PraError(PRA_E_UNSUPPORTED, "relation is the target of both acopy and join/copy rules");
+            }
+        out_intern[oi] = type0;
+        pra_rel* R = outs[oi];
+        if (totals[oi]) ensure_member_table(c, R, type0 ? R->arity : R->w, totals[oi]);
+    }
+    c->stage2.reserve(std::max<size_t>(1, outs.size()) * 8, c->st);
+    clear_flags(c);
+    for (size_t oi = 0; oi < outs.size(); oi++) {
+        pra_rel* R = outs[oi];
+        u64 J = totals[oi];
+        const bool intern = out_intern[oi] == 1;
+        const int cw = intern ? R->arity : R->arity + 1;
+        const int kw = cw;
+        Rows& NT = R->ver[PRA_NEWT];
+        NT.n = 0;
+        if (J == 0) {
+            pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi, 1, 0);
+            continue;
+        }
+        c->cand.reserve(J * cw * 8, c->st);
+        c->dest.reserve(J * 4, c->st);
+        c->slot_of.reserve(J * 8, c->st);
+        c->keep.reserve(J * 4, c->st);
+        for (int r = 0; r < nra; r++) {
+            RADesc& ra = S->ras[r];
+            if (ra.out != R) continue;
+            ra.posadj.reserve((size_t)nb * 8, c->st);
+            HIPCHK(hipMemcpyAsync(ra.posadj.p, &posadj[(size_t)r * nb], (size_t)nb * 8, hipMemcpyHostToDevice, c->st));
+            Rows& O = ra.outer->ver[ra.outer_v];
+            u64 tot_r = srcoff[(size_t)r * (nb + 1) + nb];
+            if (ra.type == RA_JOIN) {
+                JoinExpandArgs a;
+                memset(&a, 0, sizeof(a));
+                a.outer = O.d();
+                a.inner = ra.inner->ver[ra.inner_v].d();
+                a.off = ra.off.as<u64>();
+                a.istart = ra.istart.as<u64>();
+                a.posadj = ra.posadj.as<u64>();
+                a.cand = c->cand.as<u64>();
+                a.dest = c->dest.as<u32>();
+                a.n_outer = ra.n_outer;
+                a.total = tot_r;
+                a.nb = (u64)nb;
+                a.wo = ra.outer->w;
+                a.wi = ra.inner->w;
+                a.jcc = ra.in0->jcc;
+                a.left = ra.left;
+                a.cw = cw;
+                a.out_jcc = R->jcc;
+                for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+                Prof p(c, K_JOIN_EXPAND, (double)tot_r * (8.0 * ra.inner->w + 8.0 * cw));
+                pra_launch_join_expand(c->st, a);
+                check_launch("join_expand");
+            } else {
+                CopyExpandArgs a;
+                memset(&a, 0, sizeof(a));
+                a.rows = O.d();
+                a.off = ra.off.as<u64>();
+                a.posadj = ra.posadj.as<u64>();
+                a.cand = c->cand.as<u64>();
+                a.dest = c->dest.as<u32>();
+                a.n = ra.n_outer;
+                a.nb = (u64)nb;
+                a.w = ra.outer->w;
+                a.in_jcc = ra.outer->jcc;
+                a.cw = cw;
+                a.out_jcc = R->jcc;
+                for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+                Prof p(c, K_COPY_EXPAND, (double)ra.n_outer * 8.0 * ra.outer->w + (double)tot_r * 8.0 * cw);
+                pra_launch_copy_expand(c->st, a);
+                check_launch("copy_expand");
+            }
+        }
+        u64* tab = R->mt.as<u64>();
+        {
+            Prof p(c, K_DEDUP_CLAIM, (double)J * 8.0 * (kw + 1));
+            pra_launch_dedup_claim(c->st, c->cand.as<u64>(), J, cw, kw, tab, R->mt_cap, c->slot_of.as<u64>(),
+                                   c->dflags());
+            check_launch("dedup_claim");
+        }
+        {
+            Prof p(c, K_DEDUP_DECIDE, (double)J * 12.0);
+            pra_launch_dedup_decide(c->st, c->cand.as<u64>(), J, cw, kw, tab, c->slot_of.as<u64>(),
+                                    c->keep.as<u32>());
+            check_launch("dedup_decide");
+        }
+        u32 nblocks = pra_tiles(J);
+        c->hist.reserve((size_t)nb * nblocks * 4, c->st);
+        c->offs.reserve((size_t)nb * nblocks * 4 + 8, c->st);
+        {
+            Prof p(c, K_DEDUP_VERIFY, (double)J * (8.0 * kw + 16.0));
+            pra_launch_dedup_verify_hist(c->st, c->cand.as<u64>(), J, cw, kw, tab, c->slot_of.as<u64>(),
+                                         c->keep.as<u32>(), c->dest.as<u32>(), (u32)nb, c->hist.as<u32>(), nblocks,
+                                         c->dflags());
+            check_launch("dedup_verify_hist");
+        }
+        u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
+        scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
+        NT.buf.reserve(J * R->w * 8, c->st);
+        DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
+        {
+            NewtScatterArgs a;
+            memset(&a, 0, sizeof(a));
+            a.cand = c->cand.as<u64>();
+            a.keep = c->keep.as<u32>();
+            a.dest = c->dest.as<u32>();
+            a.offs = c->offs.as<u32>();
+            a.seqbase = sq ? sq->as<u64>() : nullptr;
+            a.newt = NT.d();
+            a.n = J;
+            a.nblocks = nblocks;
+            a.nb = (u32)nb;
+            a.cw = cw;
+            a.wn = R->w;
+            a.intern = intern ? 1 : 0;
+            a.tag = R->tag;
+            Prof p(c, K_DEDUP_SCATTER, (double)J * (8.0 * cw + 8.0));
+            pra_launch_dedup_scatter(c->st, a);
+            check_launch("dedup_scatter");
+        }
+        pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
+                              c->stage2.as<u64>() + oi, intern ? 1 : 0);
+        check_launch("seq_update");
+    }
+
+    // 5. new-tuple counts + error flags: the per-iteration host readback
+    {
+        size_t no = outs.size();
+        u64* h = readback(c, c->stage2.p, no);
+        std::vector<u64> cnts(h, h + no);
+        check_flags(read_flags(c));
+        for (size_t oi = 0; oi < no; oi++) {
+            pra_rel* R = outs[oi];
+            R->ver[PRA_NEWT].n = cnts[oi];
+            R->ver[PRA_NEWT].sorted = false;
+            R->mt_entries += cnts[oi];
+        }
+    }
+
+    // 6. sort newt into trie order, then delta -> full, newt -> delta
+    //    (src/RAM/RA_tasks.cpp:1044-1058)
+    for (size_t oi = 0; oi < outs.size(); oi++) {
+        pra_rel* R = outs[oi];
+        sort_rows(c, R->ver[PRA_NEWT], R->w, R->jcc, out_intern[oi] ? R->arity : R->w);
+    }
+    for (pra_rel* R : S->rels) {
+        Rows& D = R->ver[PRA_DELTA];
+        append_rows(c, R, PRA_FULL, D.d(), D.n);
+        D.buf.swap(R->ver[PRA_NEWT].buf);
+        D.n = R->ver[PRA_NEWT].n;
+        D.sorted = true;
+        R->ver[PRA_NEWT].n = 0;
+        R->touched(PRA_DELTA);
+    }
+    c->iterations++;
+}
+
+static u64 delta_sum(pra_scc* S)
+{
+    u64 d = 0;
+    for (pra_rel* R : S->rels) d += R->ver[PRA_DELTA].n;
+    return d;
+}
+
+// full -> delta at SCC start (src/relation/balanced_hash_relation.cpp:1152-1179)
+static void full_to_delta(pra_ctx* c, pra_rel* R)
+{
+    Rows& F = R->ver[PRA_FULL];
+    Rows& D = R->ver[PRA_DELTA];
+    if (F.n == 0) return;
+    if (D.n == 0) {
+        D.buf.swap(F.buf);
+        D.n = F.n;
+        D.sorted = F.sorted;
+        F.n = 0;
+        F.sorted = true;
+        R->touched(PRA_FULL);
+        R->touched(PRA_DELTA);
+        ensure_sorted(c, R, PRA_DELTA);
+        return;
+    }
+    append_rows(c, R, PRA_DELTA, F.d(), F.n);
+    F.n = 0;
+    F.sorted = true;
+    R->touched(PRA_FULL);
+    sort_rows(c, D, R->w, R->jcc, R->w);
+    unique_rows(c, D, R->w);
+    R->touched(PRA_DELTA);
+}
+
+// delta -> full (src/relation/balanced_hash_relation.cpp:1120-1148)
+static void delta_to_full(pra_ctx* c, pra_rel* R)
+{
+    Rows& D = R->ver[PRA_DELTA];
+    if (D.n == 0) return;
+    append_rows(c, R, PRA_FULL, D.d(), D.n);
+    D.n = 0;
+    R->touched(PRA_DELTA);
+}
+
+static void scc_run(pra_scc* S)
+{
+    pra_ctx* c = S->ctx;
+    for (size_t i = 0; i < S->rels.size(); i++)
+        if (S->status[i]) full_to_delta(c, S->rels[i]);
+    if (!S->fixpoint) {
+        run_iteration(S);
+        S->hist.push_back(delta_sum(S));
+    } else {
+        u64 d;
+        do {
+            for (u32 b = 0; b < c->batch_size; b++) run_iteration(S);
+            d = delta_sum(S);
+            S->hist.push_back(d);
+        } while (d != 0);
+    }
+    for (pra_rel* R : S->rels) delta_to_full(c, R);
+    S->done = true;
+}
+
+static pra_scc* one_runnable(pra_ctx* c)
+{
+    for (pra_scc* s : c->sccs) {
+        if (s->done) continue;
+        bool blocked = false;
+        for (auto& d : c->deps)
+            if (d.second == s && !d.first->done) blocked = true;
+        if (!blocked) return s;
+    }
+    return nullptr;
+}
+
+static int fail(pra_ctx* c, int code, const std::string& m)
+{
+    if (c) c->err = m;
+    return code;
+}
+
+#define API_BEGIN(ctxexpr)                                                                         \
+    pra_ctx* ctx_ = (ctxexpr);                                                                     \
+    try {
+#define API_END                                                                                    \
+    }                                                                                              \
+    catch (const PraError& e) { return fail(ctx_, e.code, e.what()); }                           \
+    catch (const std::bad_alloc&) { return fail(ctx_, PRA_E_HIP, "host out of memory"); }        \
+    catch (const std::exception& e) { return fail(ctx_, PRA_E_HIP, e.what()); }                  \
+    return PRA_OK;
+
+static bool valid_ver(int v) { return v == PRA_DELTA || v == PRA_FULL || v == PRA_NEWT; }
+
+} // namespace
+
+// ----------------------------------------------------------------- C ABI ---
+
+extern "C" {
+
+int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
+{
+    if (!out || nbuckets < 1 || nbuckets > 256) return PRA_E_ARG;
+    *out = nullptr;
+    std::unique_ptr<pra_ctx> c(new pra_ctx());
+    try {
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) return PRA_E_ARG;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        c->device = device;
+        c->nb = nbuckets;
+        c->budget = hbm_budget;
+        c->flags_buf.reserve(64, c->st);
+        pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
+        check_launch("init");
+        HIPCHK(hipStreamSynchronize(c->st));
+    } catch (const PraError& e) {
+        return e.code;
+    }
+    *out = c.release();
+    return PRA_OK;
+}
+
+int pra_ctx_destroy(pra_ctx* ctx)
+{
+    if (!ctx) return PRA_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    for (auto* s : ctx->sccs) delete s;
+    for (auto* r : ctx->rels) delete r;
+    ctx->sccs.clear();
+    ctx->rels.clear();
+    delete ctx;
+    return PRA_OK;
+}
+
+const char* pra_last_error(pra_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pra_ctx_set_batch_size(pra_ctx* ctx, uint32_t bs)
+{
+    if (!ctx || bs == 0) return PRA_E_ARG;
+    ctx->batch_size = bs;
+    return PRA_OK;
+}
+
+int pra_ctx_set_join_threshold(pra_ctx* ctx, int64_t t)
+{
+    if (!ctx || t < 0) return PRA_E_ARG;
+    ctx->threshold = t;
+    return PRA_OK;
+}
+
+int pra_ctx_joined_tuples(pra_ctx* ctx, uint64_t* out)
+{
+    if (!ctx || !out) return PRA_E_ARG;
+    *out = ctx->joined;
+    return PRA_OK;
+}
+
+int pra_ctx_iterations(pra_ctx* ctx, uint64_t* out)
+{
+    if (!ctx || !out) return PRA_E_ARG;
+    *out = ctx->iterations;
+    return PRA_OK;
+}
+
+int pra_ctx_set_profiling(pra_ctx* ctx, int on)
+{
+    if (!ctx) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    ctx->resolve_prof();
+    ctx->prof = on != 0;
+    if (on)
+        for (auto& k : ctx->kst) { k.launches = 0; k.ms = 0; k.bytes = 0; }
+    API_END
+}
+
+int pra_ctx_kernel_stat(pra_ctx* ctx, int idx, const char** name, uint64_t* launches, double* ms, double* bytes)
+{
+    if (!ctx || idx < 0 || idx >= K_COUNT) return PRA_E_ARG;
+    ctx->resolve_prof();
+    if (name) *name = ctx->kst[idx].name;
+    if (launches) *launches = ctx->kst[idx].launches;
+    if (ms) *ms = ctx->kst[idx].ms;
+    if (bytes) *bytes = ctx->kst[idx].bytes;
+    return PRA_OK;
+}
+
+int pra_ctx_synchronize(pra_ctx* ctx)
+{
+    if (!ctx) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    API_END
+}
+
+int pra_rel_create(pra_ctx* ctx, uint32_t arity, uint32_t jcc, uint32_t tag, int canonical, pra_rel** out)
+{
+    if (!ctx || !out) return PRA_E_ARG;
+    if (arity < 1 || arity + 1 > PRA_MAXW || jcc > arity || tag >= (1u << 18))
+        return fail(ctx, PRA_E_ARG, "relation shape out of range");
+    pra_rel* r = new pra_rel();
+    r->ctx = ctx;
+    r->arity = (int)arity;
+    r->jcc = (int)jcc;
+    r->tag = tag;
+    r->canonical = canonical;
+    r->w = (int)arity + 1;
+    ctx->rels.push_back(r);
+    *out = r;
+    return PRA_OK;
+}
+
+int pra_rel_destroy(pra_rel* rel)
+{
+    if (!rel) return PRA_E_ARG;
+    pra_ctx* c = rel->ctx;
+    (void)hipStreamSynchronize(c->st);
+    c->rels.erase(std::remove(c->rels.begin(), c->rels.end(), rel), c->rels.end());
+    delete rel;
+    return PRA_OK;
+}
+
+int pra_rel_load_rows(pra_rel* rel, const uint64_t* rows, uint64_t nrows, int version)
+{
+    if (!rel || (nrows && !rows) || (version != PRA_FULL && version != PRA_DELTA)) return PRA_E_ARG;
+    API_BEGIN(rel->ctx)
+    pra_ctx* c = rel->ctx;
+    if (nrows) {
+        DBuf tmp;
+        tmp.reserve(nrows * rel->w * 8, c->st);
+        HIPCHK(hipMemcpyAsync(tmp.p, rows, nrows * rel->w * 8, hipMemcpyHostToDevice, c->st));
+        append_rows(c, rel, version, tmp.as<u64>(), nrows);
+        Rows& r = rel->ver[version];
+        sort_rows(c, r, rel->w, rel->jcc, rel->w);
+        unique_rows(c, r, rel->w);
+        HIPCHK(hipStreamSynchronize(c->st));
+        rel->touched(version);
+        rel->mt_valid = false;
+    }
+    API_END
+}
+
+int pra_rel_load_bin(pra_rel* rel, const char* path, int version)
+{
+    if (!rel || !path) return PRA_E_ARG;
+    pra_ctx* c = rel->ctx;
+    std::string sp = std::string(path) + ".size";
+    FILE* f = fopen(sp.c_str(), "r");
+    if (!f) return fail(c, PRA_E_IO, "cannot open " + sp);
+    long long rows = -1, cols = -1;
+    int got = fscanf(f, "%lld\n%lld", &rows, &cols);
+    fclose(f);
+    if (got != 2 || rows < 0) return fail(c, PRA_E_IO, "malformed " + sp);
+    if (cols != rel->w) return fail(c, PRA_E_IO, "column count in " + sp + " does not match arity+1");
+    std::vector<u64> buf((size_t)rows * cols);
+    if (rows) {
+        FILE* d = fopen(path, "rb");
+        if (!d) return fail(c, PRA_E_IO, std::string("cannot open ") + path);
+        size_t n = fread(buf.data(), 8, buf.size(), d);
+        fclose(d);
+        if (n != buf.size()) return fail(c, PRA_E_IO, std::string("short read of ") + path);
+    }
+    return pra_rel_load_rows(rel, buf.data(), (uint64_t)rows, version);
+}
+
+int pra_rel_count(pra_rel* rel, int version, uint64_t* out)
+{
+    if (!rel || !out || !valid_ver(version)) return PRA_E_ARG;
+    *out = rel->ver[version].n;
+    return PRA_OK;
+}
+
+int pra_rel_device_rows(pra_rel* rel, int version, const uint64_t** rows, uint64_t* nrows)
+{
+    if (!rel || !rows || !nrows || !valid_ver(version)) return PRA_E_ARG;
+    API_BEGIN(rel->ctx)
+    pra_ctx* c = rel->ctx;
+    if (version != PRA_NEWT) ensure_sorted(c, rel, version);
+    HIPCHK(hipStreamSynchronize(c->st));
+    *rows = rel->ver[version].d();
+    *nrows = rel->ver[version].n;
+    API_END
+}
+
+int pra_rel_download(pra_rel* rel, int version, uint64_t* rows, uint64_t cap, uint64_t* nrows)
+{
+    if (!rel || !nrows || !valid_ver(version)) return PRA_E_ARG;
+    API_BEGIN(rel->ctx)
+    pra_ctx* c = rel->ctx;
+    if (version != PRA_NEWT) ensure_sorted(c, rel, version);
+    Rows& r = rel->ver[version];
+    *nrows = r.n;
+    u64 m = std::min<u64>(cap, r.n);
+    if (m && rows) HIPCHK(hipMemcpyAsync(rows, r.d(), m * rel->w * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    API_END
+}
+
+int pra_scc_create(pra_ctx* ctx, int fixpoint, int id, pra_scc** out)
+{
+    if (!ctx || !out) return PRA_E_ARG;
+    pra_scc* s = new pra_scc();
+    s->ctx = ctx;
+    s->fixpoint = fixpoint != 0;
+    s->id = id;
+    ctx->sccs.push_back(s);
+    *out = s;
+    return PRA_OK;
+}
+
+int pra_scc_destroy(pra_scc* scc)
+{
+    if (!scc) return PRA_E_ARG;
+    pra_ctx* c = scc->ctx;
+    c->sccs.erase(std::remove(c->sccs.begin(), c->sccs.end(), scc), c->sccs.end());
+    c->deps.erase(std::remove_if(c->deps.begin(), c->deps.end(),
+                                 [&](const std::pair<pra_scc*, pra_scc*>& d) {
+                                     return d.first == scc || d.second == scc;
+                                 }),
+                  c->deps.end());
+    delete scc;
+    return PRA_OK;
+}
+
+int pra_scc_add_relation(pra_scc* scc, pra_rel* rel, int move_full_to_delta)
+{
+    if (!scc || !rel || rel->ctx != scc->ctx) return PRA_E_ARG;
+    if (scc->rels.size() >= 64) return fail(scc->ctx, PRA_E_ARG, "at most 64 relations per SCC");
+    scc->rels.push_back(rel);
+    scc->status.push_back(move_full_to_delta != 0);
+    return PRA_OK;
+}
+
+static bool same_ctx(pra_scc* s, pra_rel* a, pra_rel* b = nullptr, pra_rel* d = nullptr)
+{
+    return a && a->ctx == s->ctx && (!b || b->ctx == s->ctx) && (!d || d->ctx == s->ctx);
+}
+
+int pra_scc_add_join(pra_scc* scc, pra_rel* out, pra_rel* in0, int ver0, pra_rel* in1, int ver1, const int* proj,
+                     int nproj)
+{
+    if (!scc || !same_ctx(scc, out, in0, in1) || (nproj && !proj)) return PRA_E_ARG;
+    pra_ctx* c = scc->ctx;
+    if ((ver0 != PRA_DELTA && ver0 != PRA_FULL) || (ver1 != PRA_DELTA && ver1 != PRA_FULL))
+        return fail(c, PRA_E_ARG, "join versions must be DELTA or FULL");
+    if (nproj != out->arity) return fail(c, PRA_E_ARG, "projection width must equal the output arity");
+    if (in0->jcc != in1->jcc) return fail(c, PRA_E_ARG, "join inputs must share the join column count");
+    int span = in0->w + in1->w - in0->jcc;
+    for (int i = 0; i < nproj; i++)
+        if (proj[i] < 0 || proj[i] >= span) return fail(c, PRA_E_ARG, "projection index out of range");
+    RADesc ra;
+    ra.type = RA_JOIN;
+    ra.out = out;
+    ra.in0 = in0;
+    ra.v0 = ver0;
+    ra.in1 = in1;
+    ra.v1 = ver1;
+    ra.map.assign(proj, proj + nproj);
+    scc->ras.push_back(std::move(ra));
+    return PRA_OK;
+}
+
+static int add_copylike(pra_scc* scc, int type, pra_rel* out, pra_rel* in, int ver, const int* map, int n, int op)
+{
+    if (!scc || !same_ctx(scc, out, in) || (n && !map)) return PRA_E_ARG;
+    pra_ctx* c = scc->ctx;
+    if (ver != PRA_DELTA && ver != PRA_FULL) return fail(c, PRA_E_ARG, "input version must be DELTA or FULL");
+    int want = type == RA_ACOPY ? out->w : out->arity;
+    if (n != want) return fail(c, PRA_E_ARG, "reorder width does not match the output relation");
+    for (int i = 0; i < n; i++)
+        if (map[i] < 0 || map[i] >= in->w) return fail(c, PRA_E_ARG, "reorder index out of range");
+    if (type == RA_COPY_FILTER && (op < PRA_FILTER_EQ || op > PRA_FILTER_FALSE))
+        return fail(c, PRA_E_ARG, "unknown filter predicate");
+    if (type == RA_COPY_FILTER && in->w < 2) return fail(c, PRA_E_ARG, "filter reads two input columns");
+    RADesc ra;
+    ra.type = type;
+    ra.out = out;
+    ra.in0 = in;
+    ra.v0 = ver;
+    ra.op = op;
+    ra.map.assign(map, map + n);
+    scc->ras.push_back(std::move(ra));
+    return PRA_OK;
+}
+
+int pra_scc_add_copy(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, const int* reorder, int n)
+{
+    return add_copylike(scc, RA_COPY, out, in, ver, reorder, n, PRA_F_TRUE);
+}
+
+int pra_scc_add_acopy(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, const int* reorder, int n)
+{
+    return add_copylike(scc, RA_ACOPY, out, in, ver, reorder, n, PRA_F_TRUE);
+}
+
+int pra_scc_add_copy_filter(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, const int* reorder, int n, int op)
+{
+    return add_copylike(scc, RA_COPY_FILTER, out, in, ver, reorder, n, op);
+}
+
+int pra_scc_add_dependency(pra_scc* before, pra_scc* after)
+{
+    if (!before || !after || before->ctx != after->ctx) return PRA_E_ARG;
+    before->ctx->deps.push_back({before, after});
+    return PRA_OK;
+}
+
+int pra_scc_run(pra_scc* scc, uint64_t* delta_hist, uint64_t cap, uint64_t* nbatches)
+{
+    if (!scc) return PRA_E_ARG;
+    API_BEGIN(scc->ctx)
+    HIPCHK(hipSetDevice(scc->ctx->device));
+    size_t before = scc->hist.size();
+    scc_run(scc);
+    size_t nb = scc->hist.size() - before;
+    if (nbatches) *nbatches = nb;
+    for (size_t i = 0; delta_hist && i < nb && i < cap; i++) delta_hist[i] = scc->hist[before + i];
+    API_END
+}
+
+int pra_scc_history(pra_scc* scc, uint64_t* out, uint64_t cap, uint64_t* n)
+{
+    if (!scc || !n) return PRA_E_ARG;
+    *n = scc->hist.size();
+    for (size_t i = 0; out && i < scc->hist.size() && i < cap; i++) out[i] = scc->hist[i];
+    return PRA_OK;
+}
+
+int pra_execute(pra_ctx* ctx)
+{
+    if (!ctx) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    HIPCHK(hipSetDevice(ctx->device));
+    pra_scc* s;
+    while ((s = one_runnable(ctx)) != nullptr) scc_run(s);
+    API_END
+}
+
+} // extern "C"

@@ -1,0 +1,790 @@
+// HIP kernels of the semi-naive RA hot path for gfx950 (MI355X).
+//
+// Work is integer hashing, scanning, partitioning and sorting over rows of
+// u64 words; it is HBM-bandwidth bound, so kernels stream rows in tiles of
+// PRA_TILE rows per 256-thread workgroup (4 wave64s) and keep per-tile
+// histograms in LDS.  Launch wrappers at the bottom are called by the host
+// engine (pra_engine.cpp) on the context's stream.
+#include "pra_device.hpp"
+#include "pra_kernels.hpp"
+
+// ------------------------------------------------------------------ utils --
+
+__global__ void k_fill_u64(u64* p, u64 n, u64 v)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// Table slots of `stride` words: [hash, minpos, key...]; empty = hash 0,
+// minpos = all ones so that atomicMin by the first claimant wins.
+__global__ void k_table_clear(u64* t, u64 cap, int stride)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 step = (u64)gridDim.x * blockDim.x;
+    for (; i < cap; i += step) {
+        u64* s = t + i * stride;
+        s[0] = 0;
+        s[1] = ~0ULL;
+        for (int c = 2; c < stride; c++) s[c] = 0;
+    }
+}
+
+template <typename T>
+__device__ inline T block_exclusive_scan(T v, T* lds, T* total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (int w = 0; w < PRA_WAVES; w++) {
+            T t = lds[w];
+            lds[w] = acc;
+            acc += t;
+        }
+        lds[PRA_WAVES] = acc;
+    }
+    __syncthreads();
+    T res = x - v + lds[wave];
+    *total = lds[PRA_WAVES];
+    __syncthreads();
+    return res;
+}
+
+// ---- device-wide exclusive scan: reduce tiles, scan tile sums, apply ------
+
+template <typename T>
+__global__ void k_scan_reduce(const T* in, u64 n, T* partials)
+{
+    __shared__ T lds[PRA_WAVES + 1];
+    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)threadIdx.x * PRA_ITEMS;
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++)
+        if (base + k < n) s += in[base + k];
+    T tot;
+    block_exclusive_scan<T>(s, lds, &tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ void k_scan_partials(T* partials, u64 np, T* total)
+{
+    __shared__ T lds[PRA_WAVES + 1];
+    T carry = 0;
+    for (u64 off = 0; off < np; off += PRA_BLOCK) {
+        u64 i = off + threadIdx.x;
+        T v = i < np ? partials[i] : 0;
+        T tot;
+        T ex = block_exclusive_scan<T>(v, lds, &tot);
+        if (i < np) partials[i] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <typename T>
+__global__ void k_scan_apply(const T* in, T* out, u64 n, const T* partials)
+{
+    __shared__ T lds[PRA_WAVES + 1];
+    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)threadIdx.x * PRA_ITEMS;
+    T v[PRA_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    T tot;
+    T ex = block_exclusive_scan<T>(s, lds, &tot) + partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+}
+
+// --------------------------------------------------------- radix sort ------
+
+__global__ void k_radix_hist(const u64* rows, u64 n, int w, DigitSpec ds, u32* hist, u32 nblocks)
+{
+    __shared__ u32 h[PRA_RADIX];
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
+    __syncthreads();
+    u64 base = (u64)blockIdx.x * PRA_TILE;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 idx = base + (u64)k * PRA_BLOCK + threadIdx.x;
+        if (idx < n) atomicAdd(&h[pra_digit(rows + idx * w, ds)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
+}
+
+// Stable scatter of one LSD pass.  Wave v of a workgroup owns the contiguous
+// rows [tile + v*TILE/WAVES, ...) processed 64 at a time; ranks inside a wave
+// come from ballot-based digit matching, ranks across waves from an LDS prefix.
+__global__ void k_radix_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* offs, u32 nblocks)
+{
+    __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
+    for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u64 lt = pra_lanemask_lt();
+    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
+    u32 dg[PRA_ITEMS], lc[PRA_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 idx = base + (u64)k * 64 + lane;
+        bool valid = idx < n;
+        u32 d = valid ? pra_digit(in + idx * w, ds) : 0u;
+        u64 act = __ballot(valid);
+        u64 peers = pra_match<PRA_RADIX_BITS>(d, act);
+        dg[k] = d;
+        lc[k] = 0;
+        if (valid) {
+            u32 before = wh[wave][d];
+            lc[k] = before + (u32)__popcll(peers & lt);
+            if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
+        }
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < PRA_RADIX; d += PRA_BLOCK) {
+        u32 acc = offs[(u64)d * nblocks + blockIdx.x];
+        for (int v = 0; v < PRA_WAVES; v++) {
+            u32 t = wh[v][d];
+            wh[v][d] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 idx = base + (u64)k * 64 + lane;
+        if (idx < n) {
+            u64 pos = (u64)wh[wave][dg[k]] + lc[k];
+            const u64* src = in + idx * w;
+            u64* dst = out + pos * w;
+            for (int c = 0; c < w; c++) dst[c] = src[c];
+        }
+    }
+}
+
+// bitwise OR and AND of each of the first kc columns (kc <= 16), used to skip
+// radix passes whose digit is the same for every row
+__global__ void k_col_orand(const u64* rows, u64 n, int w, int kc, u64* out)
+{
+    u64 o[16], a[16];
+    for (int c = 0; c < 16; c++) { o[c] = 0; a[c] = ~0ULL; }
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 step = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += step) {
+        const u64* r = rows + i * w;
+        for (int c = 0; c < kc; c++) { o[c] |= r[c]; a[c] &= r[c]; }
+    }
+    for (int c = 0; c < kc; c++) {
+        u64 x = o[c], y = a[c];
+        for (int d = 32; d >= 1; d >>= 1) {
+            x |= __shfl_xor(x, d, 64);
+            y &= __shfl_xor(y, d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicOr((unsigned long long*)&out[c], (unsigned long long)x);
+            atomicAnd((unsigned long long*)&out[16 + c], (unsigned long long)y);
+        }
+    }
+}
+
+// bstart[s] = first row whose bucket is >= s, for s in [0, nb]
+__global__ void k_bucket_bounds(const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart)
+{
+    u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nb) return;
+    u64 lo = 0, hi = n;
+    while (lo < hi) {
+        u64 mid = (lo + hi) >> 1;
+        u64 b = pra_fnv(rows + mid * w, jcc) % nb;
+        if (b < s) lo = mid + 1; else hi = mid;
+    }
+    bstart[s] = lo;
+}
+
+// keep rows whose flag is set, in order
+__global__ void k_compact_rows(const u64* in, u64 n, int w, const u64* flags, const u64* pos, u64* out)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    const u64* src = in + i * w;
+    u64* dst = out + pos[i] * w;
+    for (int c = 0; c < w; c++) dst[c] = src[c];
+}
+
+// ------------------------------------------------------- join index --------
+
+__global__ void k_group_flags(const u64* rows, u64 n, int w, int jcc, u64* flags)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 f = 1;
+    if (i > 0) {
+        f = 0;
+        const u64* a = rows + i * w;
+        const u64* b = a - w;
+        for (int c = 0; c < jcc; c++)
+            if (a[c] != b[c]) { f = 1; break; }
+    }
+    flags[i] = f;
+}
+
+__global__ void k_group_compact(const u64* flags, const u64* gpos, u64 n, u64* gstart)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        gstart[n == 0 ? 0 : gpos[n - 1] + flags[n - 1]] = n;
+        return;
+    }
+    if (flags[i]) gstart[gpos[i]] = i;
+}
+
+// index slots: [hash, start row, row count]
+__global__ void k_index_insert(const u64* rows, int w, int jcc, const u64* gstart, u64 ng, u64* table, u64 cap,
+                               u32* flags)
+{
+    u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    u64 i0 = gstart[g], cnt = gstart[g + 1] - i0;
+    u64 h = pra_keyhash(rows + i0 * w, jcc);
+    u64 m = cap - 1, s = h & m;
+    for (u64 p = 0; p < cap; p++) {
+        u64* slot = table + s * 3;
+        u64 prev = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
+        if (prev == 0) {
+            slot[1] = i0;
+            slot[2] = cnt;
+            return;
+        }
+        if (prev == h) {
+            atomicOr(flags, PRA_FLAG_INDEX_COLLISION);
+            return;
+        }
+        s = (s + 1) & m;
+    }
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+// --------------------------------------------------------------- join ------
+
+// Count the inner rows matching each outer row's first jcc columns.
+__global__ void k_join_count(const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
+                             const u64* inner, int wi, u64* cnt, u64* istart)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* key = outer + i * wo;
+    u64 h = pra_keyhash(key, jcc);
+    u64 m = cap - 1, s = h & m, c = 0, st = 0;
+    for (u64 p = 0; p < cap; p++) {
+        const u64* slot = table + s * 3;
+        u64 sh = slot[0];
+        if (sh == 0) break;
+        if (sh == h) {
+            const u64* f = inner + slot[1] * wi;
+            bool eq = true;
+            for (int x = 0; x < jcc; x++)
+                if (f[x] != key[x]) { eq = false; break; }
+            if (eq) { c = slot[2]; st = slot[1]; }
+            break;
+        }
+        s = (s + 1) & m;
+    }
+    cnt[i] = c;
+    istart[i] = st;
+}
+
+// One thread per join pair: locate the outer row by binary search over the
+// exclusive offsets, build the projection of (outer ++ inner[jcc:]) (LEFT) or
+// (inner ++ outer[jcc:]) (RIGHT), and write it at its arrival position.
+__global__ void k_join_expand(JoinExpandArgs a)
+{
+    u64 o = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= a.total) return;
+    u64 lo = 0, hi = a.n_outer;   // find largest i with off[i] <= o
+    while (lo < hi) {
+        u64 mid = (lo + hi + 1) >> 1;
+        if (a.off[mid] <= o) lo = mid; else hi = mid - 1;
+    }
+    u64 i = lo;
+    const u64* d = a.outer + i * a.wo;
+    const u64* f = a.inner + (a.istart[i] + (o - a.off[i])) * a.wi;
+    u64 s = a.nb == 1 ? 0 : pra_fnv(d, a.jcc) % a.nb;
+    u64 pos = a.posadj[s] + o;
+    u64* dst = a.cand + pos * a.cw;
+    u64 tmp[PRA_MAXW];
+    for (int c = 0; c < a.cw; c++) {
+        int m = a.map[c];
+        u64 v;
+        if (a.left)
+            v = m < a.wo ? d[m] : f[a.jcc + (m - a.wo)];
+        else
+            v = m < a.wi ? f[m] : d[a.jcc + (m - a.wi)];
+        tmp[c] = v;
+        dst[c] = v;
+    }
+    a.dest[pos] = a.nb == 1 ? 0u : (u32)(pra_fnv(tmp, a.out_jcc) % a.nb);
+}
+
+__device__ inline bool pra_filter(int op, u64 x, u64 y)
+{
+    switch (op) {
+    case PRA_F_EQ: return x == y;
+    case PRA_F_NEQ: return x != y;
+    case PRA_F_LT: return x < y;
+    case PRA_F_LE: return x <= y;
+    case PRA_F_GT: return x > y;
+    case PRA_F_GE: return x >= y;
+    case PRA_F_TRUE: return true;
+    default: return false;
+    }
+}
+
+__global__ void k_copy_count(const u64* rows, u64 n, int w, int op, u64* cnt)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* r = rows + i * w;
+    cnt[i] = pra_filter(op, r[0], r[1]) ? 1 : 0;
+}
+
+__global__ void k_copy_expand(CopyExpandArgs a)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    if (a.off[i + 1] == a.off[i]) return;
+    const u64* r = a.rows + i * a.w;
+    u64 s = a.nb == 1 ? 0 : pra_fnv(r, a.in_jcc) % a.nb;
+    u64 pos = a.posadj[s] + a.off[i];
+    u64* dst = a.cand + pos * a.cw;
+    u64 tmp[PRA_MAXW];
+    for (int c = 0; c < a.cw; c++) {
+        tmp[c] = r[a.map[c]];
+        dst[c] = tmp[c];
+    }
+    a.dest[pos] = a.nb == 1 ? 0u : (u32)(pra_fnv(tmp, a.out_jcc) % a.nb);
+}
+
+// srcoff[s] = off[bstart[s]] for s in [0, nb]
+__global__ void k_gather_srcoff(const u64* off, const u64* bstart, u64 nb, u64* srcoff)
+{
+    u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= nb) srcoff[s] = off[bstart[s]];
+}
+
+// ---------------------------------------------------- dedup / newt ---------
+
+// Claim (or find) the slot of each candidate's key and record the earliest
+// arrival position (p+1; 0 marks a key already in full or delta).
+__global__ void k_dedup_claim(const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of,
+                              u32* flags)
+{
+    u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const u64* key = cand + p * cw;
+    const int stride = 2 + kw;
+    u64 h = pra_keyhash(key, kw);
+    u64 m = cap - 1, s = h & m;
+    for (u64 q = 0; q < cap; q++) {
+        u64* slot = table + s * stride;
+        u64 cur = slot[0];
+        if (cur == 0)
+            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
+        if (cur == 0 || cur == h) {
+            atomicMin((unsigned long long*)(slot + 1), (unsigned long long)(p + 1));
+            slot_of[p] = s;
+            return;
+        }
+        s = (s + 1) & m;
+    }
+    slot_of[p] = ~0ULL;
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+__global__ void k_dedup_decide(const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
+                               u32* keep)
+{
+    u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    u64 s = slot_of[p];
+    u32 k = 0;
+    if (s != ~0ULL) {
+        u64* slot = table + s * (2 + kw);
+        if (slot[1] == p + 1) {
+            const u64* key = cand + p * cw;
+            for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
+            k = 1;
+        }
+    }
+    keep[p] = k;
+}
+
+// Verify keys against their slots (a 64-bit hash collision between distinct
+// keys is reported, never silently merged), turn winners into "present"
+// entries, and count kept candidates per destination bucket per tile.
+__global__ void k_dedup_verify_hist(const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
+                                    const u32* keep, const u32* dest, u32 nb, u32* hist, u32 nblocks, u32* flags)
+{
+    __shared__ u32 h[PRA_RADIX];
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
+    __syncthreads();
+    u64 base = (u64)blockIdx.x * PRA_TILE;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 p = base + (u64)k * PRA_BLOCK + threadIdx.x;
+        if (p < n) {
+            u64 s = slot_of[p];
+            if (s != ~0ULL) {
+                u64* slot = table + s * (2 + kw);
+                const u64* key = cand + p * cw;
+                bool eq = true;
+                for (int c = 0; c < kw; c++)
+                    if (slot[2 + c] != key[c]) { eq = false; break; }
+                if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
+                if (keep[p]) {
+                    slot[1] = 0;
+                    atomicAdd(&h[dest[p]], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nb; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
+}
+
+// Stable partition of the kept candidates by destination bucket; JOIN-type
+// rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
+// (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
+__global__ void k_dedup_scatter(NewtScatterArgs a)
+{
+    __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
+    for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u64 lt = pra_lanemask_lt();
+    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
+    u32 dg[PRA_ITEMS], lc[PRA_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 p = base + (u64)k * 64 + lane;
+        bool valid = p < a.n && a.keep[p];
+        u32 d = valid ? a.dest[p] : 0u;
+        u64 act = __ballot(valid);
+        u64 peers = pra_match<PRA_RADIX_BITS>(d, act);
+        dg[k] = d;
+        lc[k] = 0;
+        if (valid) {
+            u32 before = wh[wave][d];
+            lc[k] = before + (u32)__popcll(peers & lt);
+            if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
+        }
+    }
+    __syncthreads();
+    for (u32 d = threadIdx.x; d < a.nb; d += PRA_BLOCK) {
+        u32 acc = a.offs[(u64)d * a.nblocks + blockIdx.x];
+        for (int v = 0; v < PRA_WAVES; v++) {
+            u32 t = wh[v][d];
+            wh[v][d] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 p = base + (u64)k * 64 + lane;
+        if (p < a.n && a.keep[p]) {
+            u32 d = dg[k];
+            u64 pos = (u64)wh[wave][d] + lc[k];
+            const u64* src = a.cand + p * a.cw;
+            u64* dst = a.newt + pos * a.wn;
+            for (int c = 0; c < a.cw; c++) dst[c] = src[c];
+            if (a.intern) {
+                u64 rank = pos - a.offs[(u64)d * a.nblocks];
+                u64 key = ((u64)a.tag << 46) | ((u64)d << 28);
+                dst[a.cw] = key | (a.seqbase[d] + rank);
+            }
+        }
+    }
+}
+
+// seq counters advance by the number of new tuples per bucket; the kept
+// total is reported to the host
+__global__ void k_seq_update(u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total, u64* count_out,
+                             int intern)
+{
+    u32 d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d == 0) *count_out = *total;
+    if (d >= nb || !intern) return;
+    u32 start = offs[(u64)d * nblocks];
+    u32 end = d + 1 < nb ? offs[(u64)(d + 1) * nblocks] : *total;
+    seqbase[d] += (u64)(end - start);
+}
+
+// ----------------------------------------- membership table maintenance -----
+
+// Insert keys of already-present rows (minpos 0).  Duplicate keys (rows that
+// differ only after the key) land on the same slot and are skipped.
+__global__ void k_table_insert_rows(const u64* rows, u64 n, int w, int kw, u64* table, u64 cap, u32* flags)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* key = rows + i * w;
+    const int stride = 2 + kw;
+    u64 h = pra_keyhash(key, kw);
+    u64 m = cap - 1, s = h & m;
+    for (u64 q = 0; q < cap; q++) {
+        u64* slot = table + s * stride;
+        u64 cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
+        if (cur == 0) {
+            slot[1] = 0;
+            for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
+            return;
+        }
+        if (cur == h) return;
+        s = (s + 1) & m;
+    }
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+__global__ void k_table_verify_rows(const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap, u32* flags)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* key = rows + i * w;
+    const int stride = 2 + kw;
+    u64 h = pra_keyhash(key, kw);
+    u64 m = cap - 1, s = h & m;
+    for (u64 q = 0; q < cap; q++) {
+        const u64* slot = table + s * stride;
+        if (slot[0] == h) {
+            for (int c = 0; c < kw; c++)
+                if (slot[2 + c] != key[c]) { atomicOr(flags, PRA_FLAG_COLLISION); break; }
+            return;
+        }
+        if (slot[0] == 0) { atomicOr(flags, PRA_FLAG_COLLISION); return; }
+        s = (s + 1) & m;
+    }
+}
+
+__global__ void k_table_rehash(const u64* old, u64 oldcap, u64* nt, u64 cap, int kw, u32* flags)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= oldcap) return;
+    const int stride = 2 + kw;
+    const u64* os = old + i * stride;
+    u64 h = os[0];
+    if (h == 0) return;
+    u64 m = cap - 1, s = h & m;
+    for (u64 q = 0; q < cap; q++) {
+        u64* slot = nt + s * stride;
+        u64 cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
+        if (cur == 0) {
+            slot[1] = 0;
+            for (int c = 0; c < kw; c++) slot[2 + c] = os[2 + c];
+            return;
+        }
+        s = (s + 1) & m;
+    }
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+// ------------------------------------------------------- launch wrappers ---
+
+static inline unsigned grid_for(u64 n, unsigned bs = PRA_BLOCK)
+{
+    u64 g = (n + bs - 1) / bs;
+    return (unsigned)(g ? g : 1);
+}
+
+void pra_launch_fill_u64(hipStream_t st, u64* p, u64 n, u64 v)
+{
+    if (!n) return;
+    u64 g = (n + PRA_BLOCK - 1) / PRA_BLOCK;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, p, n, v);
+}
+
+void pra_launch_table_clear(hipStream_t st, u64* t, u64 cap, int stride)
+{
+    u64 g = (cap + PRA_BLOCK - 1) / PRA_BLOCK;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_table_clear, dim3((unsigned)(g ? g : 1)), dim3(PRA_BLOCK), 0, st, t, cap, stride);
+}
+
+u64 pra_scan_partials_count(u64 n) { return (n + PRA_TILE - 1) / PRA_TILE; }
+
+template <typename T>
+static void scan_impl(hipStream_t st, const T* in, T* out, u64 n, T* partials, T* total)
+{
+    u64 nb = pra_scan_partials_count(n);
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(PRA_BLOCK), 0, st, in, n, partials);
+    hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(PRA_BLOCK), 0, st, partials, nb, total);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(PRA_BLOCK), 0, st, in, out, n, partials);
+}
+
+void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* partials, u64* total)
+{
+    scan_impl<u64>(st, in, out, n, partials, total);
+}
+
+void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* partials, u32* total)
+{
+    scan_impl<u32>(st, in, out, n, partials, total);
+}
+
+u32 pra_tiles(u64 n) { return (u32)((n + PRA_TILE - 1) / PRA_TILE); }
+
+void pra_launch_radix_hist(hipStream_t st, const u64* rows, u64 n, int w, DigitSpec ds, u32* hist, u32 nblocks)
+{
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, rows, n, w, ds, hist, nblocks);
+}
+
+void pra_launch_radix_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* offs,
+                              u32 nblocks)
+{
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, offs, nblocks);
+}
+
+void pra_launch_col_orand(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out)
+{
+    u64 g = (n + PRA_BLOCK - 1) / PRA_BLOCK;
+    if (g > 1024) g = 1024;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_col_orand, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, rows, n, w, kc, out);
+}
+
+void pra_launch_bucket_bounds(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart)
+{
+    hipLaunchKernelGGL(k_bucket_bounds, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, rows, n, w, jcc, nb, bstart);
+}
+
+void pra_launch_group_flags(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64* flags)
+{
+    hipLaunchKernelGGL(k_group_flags, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, jcc, flags);
+}
+
+void pra_launch_group_compact(hipStream_t st, const u64* flags, const u64* gpos, u64 n, u64* gstart)
+{
+    hipLaunchKernelGGL(k_group_compact, dim3(grid_for(n + 1)), dim3(PRA_BLOCK), 0, st, flags, gpos, n, gstart);
+}
+
+void pra_launch_compact_rows(hipStream_t st, const u64* in, u64 n, int w, const u64* flags, const u64* pos, u64* out)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, in, n, w, flags, pos, out);
+}
+
+void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, const u64* gstart, u64 ng, u64* table,
+                             u64 cap, u32* flags)
+{
+    if (!ng) return;
+    hipLaunchKernelGGL(k_index_insert, dim3(grid_for(ng)), dim3(PRA_BLOCK), 0, st, rows, w, jcc, gstart, ng, table,
+                       cap, flags);
+}
+
+void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
+                           const u64* inner, int wi, u64* cnt, u64* istart)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_join_count, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, outer, n, wo, jcc, table, cap, inner,
+                       wi, cnt, istart);
+}
+
+void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a)
+{
+    if (!a.total) return;
+    hipLaunchKernelGGL(k_join_expand, dim3(grid_for(a.total)), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy_count, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, op, cnt);
+}
+
+void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)
+{
+    if (!a.n) return;
+    hipLaunchKernelGGL(k_copy_expand, dim3(grid_for(a.n)), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart, u64 nb, u64* srcoff)
+{
+    hipLaunchKernelGGL(k_gather_srcoff, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, off, bstart, nb, srcoff);
+}
+
+void pra_launch_dedup_claim(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of,
+                            u32* flags)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_dedup_claim, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, cap, slot_of,
+                       flags);
+}
+
+void pra_launch_dedup_decide(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
+                             u32* keep)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_dedup_decide, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
+                       keep);
+}
+
+void pra_launch_dedup_verify_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table,
+                                  const u64* slot_of, const u32* keep, const u32* dest, u32 nb, u32* hist,
+                                  u32 nblocks, u32* flags)
+{
+    hipLaunchKernelGGL(k_dedup_verify_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
+                       keep, dest, nb, hist, nblocks, flags);
+}
+
+void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
+{
+    hipLaunchKernelGGL(k_dedup_scatter, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
+                           u64* count_out, int intern)
+{
+    hipLaunchKernelGGL(k_seq_update, dim3(grid_for(nb)), dim3(PRA_BLOCK), 0, st, seqbase, offs, nblocks, nb, total,
+                       count_out, intern);
+}
+
+void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
+                                  u32* flags)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_table_insert_rows, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, kw, table, cap,
+                       flags);
+}
+
+void pra_launch_table_verify_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap,
+                                  u32* flags)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_table_verify_rows, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, kw, table, cap,
+                       flags);
+}
+
+void pra_launch_table_rehash(hipStream_t st, const u64* old, u64 oldcap, u64* nt, u64 cap, int kw, u32* flags)
+{
+    if (!oldcap) return;
+    hipLaunchKernelGGL(k_table_rehash, dim3(grid_for(oldcap)), dim3(PRA_BLOCK), 0, st, old, oldcap, nt, cap, kw,
+                       flags);
+}

@@ -1,0 +1,93 @@
+// Launch interface of the HIP kernels (pra_kernels.hip) used by the engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pra_device.hpp"
+
+// copy_filter predicates over columns 0 and 1 of the input row
+#define PRA_F_EQ 0
+#define PRA_F_NEQ 1
+#define PRA_F_LT 2
+#define PRA_F_LE 3
+#define PRA_F_GT 4
+#define PRA_F_GE 5
+#define PRA_F_TRUE 6
+#define PRA_F_FALSE 7
+
+struct JoinExpandArgs {
+    const u64* outer;
+    const u64* inner;
+    const u64* off;      // n_outer + 1 exclusive offsets
+    const u64* istart;   // first matching inner row per outer row
+    const u64* posadj;   // per source bucket: arrival base minus source offset
+    u64* cand;
+    u32* dest;
+    u64 n_outer;
+    u64 total;
+    u64 nb;
+    int wo, wi, jcc, left, cw, out_jcc;
+    int map[PRA_MAXW];
+};
+
+struct CopyExpandArgs {
+    const u64* rows;
+    const u64* off;
+    const u64* posadj;
+    u64* cand;
+    u32* dest;
+    u64 n;
+    u64 nb;
+    int w, in_jcc, cw, out_jcc;
+    int map[PRA_MAXW];
+};
+
+struct NewtScatterArgs {
+    const u64* cand;
+    const u32* keep;
+    const u32* dest;
+    const u32* offs;
+    const u64* seqbase;
+    u64* newt;
+    u64 n;
+    u32 nblocks, nb;
+    int cw, wn, intern;
+    u32 tag;
+};
+
+void pra_launch_fill_u64(hipStream_t st, u64* p, u64 n, u64 v);
+void pra_launch_table_clear(hipStream_t st, u64* t, u64 cap, int stride);
+u64 pra_scan_partials_count(u64 n);
+void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* partials, u64* total);
+void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* partials, u32* total);
+u32 pra_tiles(u64 n);
+void pra_launch_radix_hist(hipStream_t st, const u64* rows, u64 n, int w, DigitSpec ds, u32* hist, u32 nblocks);
+void pra_launch_radix_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* offs,
+                              u32 nblocks);
+void pra_launch_col_orand(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
+void pra_launch_bucket_bounds(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart);
+void pra_launch_group_flags(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64* flags);
+void pra_launch_group_compact(hipStream_t st, const u64* flags, const u64* gpos, u64 n, u64* gstart);
+void pra_launch_compact_rows(hipStream_t st, const u64* in, u64 n, int w, const u64* flags, const u64* pos, u64* out);
+void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, const u64* gstart, u64 ng, u64* table,
+                             u64 cap, u32* flags);
+void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
+                           const u64* inner, int wi, u64* cnt, u64* istart);
+void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a);
+void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
+void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
+void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart, u64 nb, u64* srcoff);
+void pra_launch_dedup_claim(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of,
+                            u32* flags);
+void pra_launch_dedup_decide(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
+                             u32* keep);
+void pra_launch_dedup_verify_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table,
+                                  const u64* slot_of, const u32* keep, const u32* dest, u32 nb, u32* hist,
+                                  u32 nblocks, u32* flags);
+void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
+void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
+                           u64* count_out, int intern);
+void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
+                                  u32* flags);
+void pra_launch_table_verify_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap,
+                                  u32* flags);
+void pra_launch_table_rehash(hipStream_t st, const u64* old, u64 oldcap, u64* nt, u64 cap, int kw, u32* flags);

@@ -1,0 +1,247 @@
+"""Python binding of libpra_amd.so, the MI355X backend of parallel-RA's
+semi-naive relational-algebra hot path.
+
+The calls mirror the reference's C++ program surface (relation, RAM,
+parallel_join / parallel_copy / parallel_acopy / parallel_copy_filter,
+LIE::add_scc_dependance, LIE::execute) through the C ABI declared in
+include/pra.h.  Every call goes to the HIP library; there is no CPU path, and
+loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpra_amd.so")
+
+DELTA, FULL, NEWT = 0, 1, 2
+F_EQ, F_NEQ, F_LT, F_LE, F_GT, F_GE, F_TRUE, F_FALSE = range(8)
+
+# every symbol include/pra.h declares
+EXPORTS = (
+    "pra_ctx_create", "pra_ctx_destroy", "pra_last_error", "pra_ctx_set_batch_size",
+    "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
+    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize",
+    "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
+    "pra_rel_download", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
+    "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
+    "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_history",
+    "pra_execute",
+)
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_intp = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class PraError(RuntimeError):
+    pass
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    """Load libpra_amd.so (build it first with __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PraError("libpra_amd.so is not built: run __graft_entry__.build() (make -C parallel-ra_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "pra_ctx_create": [ctypes.c_int, ctypes.c_int, _u64, ctypes.POINTER(_vp)],
+            "pra_ctx_destroy": [_vp],
+            "pra_ctx_set_batch_size": [_vp, ctypes.c_uint32],
+            "pra_ctx_set_join_threshold": [_vp, ctypes.c_int64],
+            "pra_ctx_joined_tuples": [_vp, _u64p],
+            "pra_ctx_iterations": [_vp, _u64p],
+            "pra_ctx_set_profiling": [_vp, ctypes.c_int],
+            "pra_ctx_kernel_stat": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _u64p,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+            "pra_ctx_synchronize": [_vp],
+            "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                               ctypes.POINTER(_vp)],
+            "pra_rel_destroy": [_vp],
+            "pra_rel_load_rows": [_vp, _u64p, _u64, ctypes.c_int],
+            "pra_rel_load_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
+            "pra_rel_count": [_vp, ctypes.c_int, _u64p],
+            "pra_rel_download": [_vp, ctypes.c_int, _u64p, _u64, _u64p],
+            "pra_rel_device_rows": [_vp, ctypes.c_int, ctypes.POINTER(_u64p), _u64p],
+            "pra_scc_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+            "pra_scc_destroy": [_vp],
+            "pra_scc_add_relation": [_vp, _vp, ctypes.c_int],
+            "pra_scc_add_join": [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _intp, ctypes.c_int],
+            "pra_scc_add_copy": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int],
+            "pra_scc_add_acopy": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int],
+            "pra_scc_add_copy_filter": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int, ctypes.c_int],
+            "pra_scc_add_dependency": [_vp, _vp],
+            "pra_scc_run": [_vp, _u64p, _u64, _u64p],
+            "pra_scc_history": [_vp, _u64p, _u64, _u64p],
+            "pra_execute": [_vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.pra_last_error.argtypes = [_vp]
+        L.pra_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ints(xs):
+    return (ctypes.c_int * max(1, len(xs)))(*[int(x) for x in xs])
+
+
+class Context:
+    """One LIE on one GPU with `nbuckets` virtual buckets (= reference ranks)."""
+
+    def __init__(self, nbuckets=1, device=0, hbm_budget=0, batch_size=1):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.pra_ctx_create(int(device), int(nbuckets), int(hbm_budget), ctypes.byref(h))
+        if rc != 0:
+            raise PraError(f"pra_ctx_create failed ({rc})")
+        self.c = h
+        self.nb = int(nbuckets)
+        self.arity = {}
+        self.rels = []
+        self.sccs = []
+        self.scc_rels = {}
+        self.set_batch_size(batch_size)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.pra_last_error(self.c)
+            raise PraError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "c", None):
+            self.L.pra_ctx_destroy(self.c)
+            self.c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_batch_size(self, bs):
+        self._chk(self.L.pra_ctx_set_batch_size(self.c, int(bs)), "set_batch_size")
+
+    def set_join_threshold(self, t):
+        self._chk(self.L.pra_ctx_set_join_threshold(self.c, int(t)), "set_join_threshold")
+
+    def set_profiling(self, on):
+        self._chk(self.L.pra_ctx_set_profiling(self.c, int(bool(on))), "set_profiling")
+
+    def kernel_stats(self):
+        out = {}
+        i = 0
+        while True:
+            name = ctypes.c_char_p()
+            n = _u64()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            if self.L.pra_ctx_kernel_stat(self.c, i, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms),
+                                          ctypes.byref(by)) != 0:
+                break
+            out[name.value.decode()] = {"launches": int(n.value), "ms": ms.value, "bytes": by.value}
+            i += 1
+        return out
+
+    def synchronize(self):
+        self._chk(self.L.pra_ctx_synchronize(self.c), "synchronize")
+
+    def joined(self):
+        v = _u64()
+        self._chk(self.L.pra_ctx_joined_tuples(self.c, ctypes.byref(v)), "joined")
+        return int(v.value)
+
+    def iterations_run(self):
+        v = _u64()
+        self._chk(self.L.pra_ctx_iterations(self.c, ctypes.byref(v)), "iterations")
+        return int(v.value)
+
+    # relations
+    def rel_create(self, arity, jcc, tag, canonical=True):
+        h = _vp()
+        self._chk(self.L.pra_rel_create(self.c, arity, jcc, tag, int(bool(canonical)), ctypes.byref(h)),
+                  "rel_create")
+        self.rels.append(h)
+        r = len(self.rels) - 1
+        self.arity[r] = arity
+        return r
+
+    def load_rows(self, rel, rows, version=FULL):
+        a = np.ascontiguousarray(np.asarray(rows, dtype=np.uint64).reshape(-1, self.arity[rel] + 1))
+        self._chk(self.L.pra_rel_load_rows(self.rels[rel], a.ctypes.data_as(_u64p), a.shape[0], version),
+                  "load_rows")
+
+    def load_bin(self, rel, path, version=FULL):
+        self._chk(self.L.pra_rel_load_bin(self.rels[rel], str(path).encode(), version), "load_bin")
+
+    def count(self, rel, version=FULL):
+        v = _u64()
+        self._chk(self.L.pra_rel_count(self.rels[rel], version, ctypes.byref(v)), "count")
+        return int(v.value)
+
+    def rows(self, rel, version=FULL):
+        n = self.count(rel, version)
+        w = self.arity[rel] + 1
+        out = np.zeros((max(n, 1), w), dtype=np.uint64)
+        got = _u64()
+        self._chk(self.L.pra_rel_download(self.rels[rel], version, out.ctypes.data_as(_u64p), n,
+                                          ctypes.byref(got)), "download")
+        assert got.value == n
+        return out[:n]
+
+    # SCCs
+    def scc_create(self, fixpoint, sid):
+        h = _vp()
+        self._chk(self.L.pra_scc_create(self.c, int(bool(fixpoint)), int(sid), ctypes.byref(h)), "scc_create")
+        self.sccs.append(h)
+        s = len(self.sccs) - 1
+        self.scc_rels[s] = []
+        return s
+
+    def scc_add_relation(self, scc, rel, move_full_to_delta):
+        self._chk(self.L.pra_scc_add_relation(self.sccs[scc], self.rels[rel], int(bool(move_full_to_delta))),
+                  "scc_add_relation")
+        self.scc_rels[scc].append(rel)
+
+    def add_join(self, scc, out, in0, v0, in1, v1, proj):
+        self._chk(self.L.pra_scc_add_join(self.sccs[scc], self.rels[out], self.rels[in0], v0, self.rels[in1], v1,
+                                          _ints(proj), len(proj)), "add_join")
+
+    def add_copy(self, scc, out, src, v, reorder):
+        self._chk(self.L.pra_scc_add_copy(self.sccs[scc], self.rels[out], self.rels[src], v, _ints(reorder),
+                                          len(reorder)), "add_copy")
+
+    def add_acopy(self, scc, out, src, v, reorder):
+        self._chk(self.L.pra_scc_add_acopy(self.sccs[scc], self.rels[out], self.rels[src], v, _ints(reorder),
+                                           len(reorder)), "add_acopy")
+
+    def add_copy_filter(self, scc, out, src, v, reorder, op):
+        self._chk(self.L.pra_scc_add_copy_filter(self.sccs[scc], self.rels[out], self.rels[src], v,
+                                                 _ints(reorder), len(reorder), int(op)), "add_copy_filter")
+
+    def add_dependency(self, before, after):
+        self._chk(self.L.pra_scc_add_dependency(self.sccs[before], self.sccs[after]), "add_dependency")
+
+    def execute(self, batch_size=None):
+        if batch_size is not None:
+            self.set_batch_size(batch_size)
+        self._chk(self.L.pra_execute(self.c), "execute")
+
+    def history(self, scc):
+        n = _u64()
+        self._chk(self.L.pra_scc_history(self.sccs[scc], None, 0, ctypes.byref(n)), "history")
+        buf = (_u64 * max(1, n.value))()
+        self._chk(self.L.pra_scc_history(self.sccs[scc], buf, n.value, ctypes.byref(n)), "history")
+        return [int(buf[i]) for i in range(n.value)]

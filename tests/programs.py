@@ -1,0 +1,282 @@
+"""Programs, inputs and closed forms shared by the tests and the benchmark.
+
+Programs are written against a backend object with the program-building
+calls of the oracle (oracle/oracle.py) and of the HIP binding
+(parallel-ra_amd/pra_amd.py), both mirroring the reference's C++ surface.
+"""
+import os
+import random
+from collections import deque
+
+import numpy as np
+
+DELTA, FULL = 0, 1
+F_NEQ = 1
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------ tsv_to_bin ----
+
+def read_bin(path):
+    """tsv_to_bin format: <path> = rows of LE u64, <path>.size = "rows\\ncols"
+    (utility/tsv_to_bin.cpp, src/IO/parallel_io.cpp:115-235)."""
+    with open(path + ".size") as f:
+        rows, cols = (int(x) for x in f.read().split())
+    if rows == 0:
+        return np.zeros((0, cols), dtype=np.uint64)
+    a = np.fromfile(path, dtype="<u8")
+    return a.reshape(rows, cols).astype(np.uint64)
+
+
+def write_bin(path, rows):
+    rows = np.ascontiguousarray(np.asarray(rows, dtype="<u8"))
+    rows.tofile(path)
+    with open(path + ".size", "w") as f:
+        f.write(f"{rows.shape[0]}\n{rows.shape[1]}")
+
+
+def with_rowid(pairs):
+    """(src, dst) pairs -> tsv_to_bin rows (src, dst, rowid)."""
+    p = np.asarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    return np.column_stack([p, np.arange(p.shape[0], dtype=np.uint64)])
+
+
+def golden(name):
+    return read_bin(os.path.join(GOLDEN, name))
+
+
+# -------------------------------------------------------------- programs ----
+
+def tc_program(b, edges, seed_rows=None):
+    """tests/transitive_closure.cpp:56-77 on backend `b`: returns (T, G, scc_fix, scc_copy)."""
+    T = b.rel_create(2, 1, 257)
+    G = b.rel_create(2, 1, 256)
+    if seed_rows is not None and len(seed_rows):
+        b.load_rows(T, seed_rows, FULL)
+    b.load_rows(G, edges, FULL)
+    s13237 = b.scc_create(True, 13237)
+    b.scc_add_relation(s13237, G, False)
+    b.scc_add_relation(s13237, T, True)
+    b.add_join(s13237, T, T, DELTA, G, FULL, [3, 1])
+    s13239 = b.scc_create(False, 13239)
+    b.scc_add_relation(s13239, G, False)
+    b.scc_add_relation(s13239, T, True)
+    b.add_copy(s13239, T, G, FULL, [1, 0])
+    b.add_dependency(s13239, s13237)
+    return T, G, s13237, s13239
+
+
+def sg_program(b, edges):
+    """Same generation over `edges` (p, x, id):
+         sg(x,y) :- edge(p,x), edge(p,y), x != y.
+         sg(x,y) :- edge(a,x), sg(a,b), edge(b,y).
+    The recursive rule is a join chain re-keyed by an acopy:
+         A(x,b)    <- sg(a,b) JOIN edge(a,x)          on a
+         B(b,x,id) <- acopy of A                       (re-key on b)
+         sg(x,y)   <- B(b,x) JOIN edge(b,y)            on b
+    Returns dict of relation handles and SCC handles."""
+    E = b.rel_create(2, 1, 300)
+    I = b.rel_create(2, 1, 301)     # edge(p,x) JOIN edge(p,y) -> (x, y)
+    SG = b.rel_create(2, 1, 302)
+    A = b.rel_create(2, 1, 303)
+    B = b.rel_create(2, 1, 304)
+    b.load_rows(E, edges, FULL)
+    s_base = b.scc_create(False, 1)
+    b.scc_add_relation(s_base, E, False)
+    b.scc_add_relation(s_base, I, False)
+    b.add_join(s_base, I, E, FULL, E, FULL, [1, 3])
+    s_filter = b.scc_create(False, 2)
+    b.scc_add_relation(s_filter, I, True)
+    b.scc_add_relation(s_filter, SG, False)
+    b.add_copy_filter(s_filter, SG, I, DELTA, [0, 1], F_NEQ)
+    s_rec = b.scc_create(True, 3)
+    b.scc_add_relation(s_rec, E, False)
+    b.scc_add_relation(s_rec, SG, True)
+    b.scc_add_relation(s_rec, A, False)
+    b.scc_add_relation(s_rec, B, False)
+    b.add_join(s_rec, A, SG, DELTA, E, FULL, [3, 1])
+    b.add_acopy(s_rec, B, A, DELTA, [1, 0, 2])
+    b.add_join(s_rec, SG, B, DELTA, E, FULL, [1, 3])
+    b.add_dependency(s_base, s_filter)
+    b.add_dependency(s_filter, s_rec)
+    return {"E": E, "I": I, "SG": SG, "A": A, "B": B, "base": s_base, "filter": s_filter, "rec": s_rec}
+
+
+# ------------------------------------------------------------ generators ----
+
+def config1_graph(seed, n=40, m=60):
+    """40 vertices, 60 distinct edges uniform without self-loops, random.Random(seed)."""
+    rng = random.Random(seed)
+    edges = []
+    seen = set()
+    while len(edges) < m:
+        u, v = rng.randrange(n), rng.randrange(n)
+        if u != v and (u, v) not in seen:
+            seen.add((u, v))
+            edges.append((u, v))
+    return with_rowid(edges)
+
+
+def distinct_u64(n, seed):
+    rng = np.random.default_rng(seed)
+    ids = np.unique(rng.integers(0, 2**64 - 1, size=n + 64, dtype=np.uint64, endpoint=True))
+    while ids.size < n:
+        ids = np.unique(np.concatenate([ids, rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True)]))
+    ids = ids[rng.permutation(ids.size)[:n]]
+    return ids
+
+
+def grid_graph(n, seed=None):
+    """n x n grid, edges right and down; vertex ids permuted into distinct
+    uint64 values from `seed` (identity ids when seed is None)."""
+    idx = np.arange(n * n, dtype=np.int64).reshape(n, n)
+    right = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], axis=1)
+    down = np.stack([idx[:-1, :].ravel(), idx[1:, :].ravel()], axis=1)
+    e = np.concatenate([right, down], axis=0)
+    ids = np.arange(n * n, dtype=np.uint64) if seed is None else distinct_u64(n * n, seed)
+    return with_rowid(ids[e])
+
+
+def layered_dag(seed, nvert=10000, nlayers=12, nedges=14000, lam=1.4):
+    """Random DAG of `nlayers` layers; out-degree Poisson(lam) towards the next
+    layer, topped up or trimmed to exactly `nedges` distinct edges; uint64 ids."""
+    rng = np.random.default_rng(seed)
+    sizes = [nvert // nlayers + (1 if i < nvert % nlayers else 0) for i in range(nlayers)]
+    starts = np.cumsum([0] + sizes)
+    edges = []
+    seen = set()
+    for L in range(nlayers - 1):
+        for v in range(starts[L], starts[L + 1]):
+            k = min(int(rng.poisson(lam)), sizes[L + 1])
+            for t in rng.choice(sizes[L + 1], size=k, replace=False):
+                e = (v, int(starts[L + 1] + t))
+                if e not in seen:
+                    seen.add(e)
+                    edges.append(e)
+    while len(edges) < nedges:
+        L = int(rng.integers(0, nlayers - 1))
+        v = int(starts[L] + rng.integers(0, sizes[L]))
+        t = int(starts[L + 1] + rng.integers(0, sizes[L + 1]))
+        if (v, t) not in seen:
+            seen.add((v, t))
+            edges.append((v, t))
+    edges = edges[:nedges]
+    ids = distinct_u64(nvert, seed + 7919)
+    e = np.asarray(edges, dtype=np.int64)
+    return with_rowid(ids[e])
+
+
+def random_graph(n, m, seed):
+    """Uniform random directed graph with n vertices and m distinct edges, no
+    self-loops, uint64 vertex ids (the batched/served TC instances)."""
+    rng = np.random.default_rng(seed)
+    seen = set()
+    out = []
+    while len(out) < m:
+        u = rng.integers(0, n, size=2 * (m - len(out)) + 16)
+        v = rng.integers(0, n, size=u.size)
+        for a, c in zip(u.tolist(), v.tolist()):
+            if a != c and (a, c) not in seen:
+                seen.add((a, c))
+                out.append((a, c))
+                if len(out) == m:
+                    break
+    ids = distinct_u64(n, seed + 104729)
+    e = np.asarray(out, dtype=np.int64)
+    return with_rowid(ids[e])
+
+
+# ------------------------------------------------------------ closed forms ---
+
+def grid_tc_size(n):
+    return (n * (n + 1) // 2) ** 2 - n * n
+
+
+def grid_N(n, d):
+    """number of (source, target) pairs at grid distance d"""
+    return sum((n - a) * (n - (d - a)) for a in range(max(0, d - n + 1), min(d, n - 1) + 1))
+
+
+def grid_deltas(n):
+    """fixed-point delta sequence of right-linear TC on the n x n grid:
+    distances 2 .. 2n-2, then the empty iteration"""
+    return [grid_N(n, d) for d in range(2, 2 * n - 1)] + [0]
+
+
+def bfs_distance_histogram(pairs):
+    """For right-linear TC seeded with the edges: the k-th fixed-point delta is
+    the number of (x, z) with shortest-path length k+1 (x == z counts through
+    its shortest cycle).  Returns (final |TC|, delta list ending in 0)."""
+    adj = {}
+    for u, v in pairs:
+        adj.setdefault(u, set()).add(v)
+    hist = {}
+    total = 0
+    for x in adj:
+        dist = {}
+        q = deque()
+        for v in adj[x]:
+            if v not in dist:
+                dist[v] = 1
+                q.append(v)
+        while q:
+            u = q.popleft()
+            for v in adj.get(u, ()):
+                if v not in dist:
+                    dist[v] = dist[u] + 1
+                    q.append(v)
+        total += len(dist)
+        for d in dist.values():
+            hist[d] = hist.get(d, 0) + 1
+    maxd = max(hist) if hist else 1
+    return total, [hist.get(d, 0) for d in range(2, maxd + 1)] + [0]
+
+
+def tc_pairs_bruteforce(pairs):
+    """set of (x, z) with a path x -> z"""
+    adj = {}
+    for u, v in pairs:
+        adj.setdefault(u, set()).add(v)
+    out = set()
+    for x in adj:
+        seen = set()
+        stack = list(adj[x])
+        while stack:
+            u = stack.pop()
+            if u in seen:
+                continue
+            seen.add(u)
+            stack.extend(adj.get(u, ()))
+        out.update((x, z) for z in seen)
+    return out
+
+
+def tc_join_pairs(pairs):
+    """joined tuples of right-linear TC: sum over paths x -> y of outdeg(y)"""
+    outdeg = {}
+    for u, _ in pairs:
+        outdeg[u] = outdeg.get(u, 0) + 1
+    return sum(outdeg.get(z, 0) for (_, z) in tc_pairs_bruteforce(pairs))
+
+
+def sg_bruteforce(pairs):
+    """same-generation closure by naive iteration over python sets"""
+    par = {}
+    for p, x in pairs:
+        par.setdefault(p, set()).add(x)
+    sg = set()
+    for kids in par.values():
+        for x in kids:
+            for y in kids:
+                if x != y:
+                    sg.add((x, y))
+    while True:
+        new = set()
+        for (a, bb) in sg:
+            for x in par.get(a, ()):
+                for y in par.get(bb, ()):
+                    if (x, y) not in sg:
+                        new.add((x, y))
+        if not new:
+            return sg
+        sg |= new

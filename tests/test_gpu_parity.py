@@ -1,0 +1,149 @@
+"""Parity of the HIP backend (through the C ABI) with the CPU oracle.
+
+Integer set arithmetic: every comparison is bit-exact, including the intern-id
+column and the per-iteration delta counts.  Full-size inputs are checked
+through size-independent properties (closed-form sizes and delta sequences,
+sortedness, uniqueness, id-field structure).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
+                      grid_tc_size, layered_dag, random_graph, sg_bruteforce, sg_program, tc_program, with_rowid)
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_ctx(nb, batch_size=1):
+    import pra_amd
+    return pra_amd.Context(nbuckets=nb, batch_size=batch_size)
+
+
+def both(nb, build, batch_size=1):
+    o = O.Oracle(nb)
+    g = gpu_ctx(nb, batch_size)
+    ho = build(o)
+    hg = build(g)
+    o.execute(batch_size)
+    g.execute()
+    return o, ho, g, hg
+
+
+def assert_same_rel(o, ro, g, rg, what=""):
+    a = o.rows(ro)
+    b = g.rows(rg)
+    assert a.shape == b.shape, f"{what}: {a.shape} vs {b.shape}"
+    assert np.array_equal(a, b), f"{what}: rows differ"
+
+
+def check_tc(nb, edges, batch_size=1):
+    print(f"[tc] nb={nb} edges={len(edges)} batch={batch_size}", flush=True)
+    o, (To, Go, fo, co), g, (Tg, Gg, fg, cg) = both(nb, lambda b: tc_program(b, edges), batch_size)
+    assert_same_rel(o, To, g, Tg, "T")
+    assert_same_rel(o, Go, g, Gg, "G")
+    assert o.history(fo) == g.history(fg)
+    assert o.history(co) == g.history(cg)
+    assert o.joined() == g.joined()
+    return o, g, To, Tg, fg
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 5, 8])
+def test_tc_golden_graphs(nb):
+    o, g, To, Tg, fg = check_tc(nb, golden("g13236_edge_2_1_2"))
+    assert g.count(Tg) == 573
+    assert g.history(fg) == [99, 132, 126, 82, 35, 19, 11, 7, 2, 0]
+    o, g, To, Tg, fg = check_tc(nb, golden("g5955_edge_2_1_2"))
+    assert g.count(Tg) == 10
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tc_config1(seed):
+    edges = config1_graph(seed)
+    pairs = [tuple(r) for r in edges[:, :2].tolist()]
+    total, deltas = bfs_distance_histogram(pairs)
+    for nb in [1, 3, 5]:
+        o, g, To, Tg, fg = check_tc(nb, edges)
+        assert g.count(Tg) == total
+        assert g.history(fg) == deltas
+
+
+def test_tc_batch_size():
+    check_tc(3, config1_graph(11), batch_size=3)
+
+
+def test_tc_empty_and_tiny():
+    check_tc(3, np.zeros((0, 3), dtype=np.uint64))
+    check_tc(2, with_rowid([(5, 5)]))              # self loop
+    check_tc(4, with_rowid([(1, 2), (2, 1)]))      # 2-cycle
+
+
+@pytest.mark.parametrize("nb", [1, 8])
+def test_tc_grid_random_ids(nb):
+    n = 30
+    o, g, To, Tg, fg = check_tc(nb, grid_graph(n, seed=2024))
+    assert g.count(Tg) == grid_tc_size(n)
+    assert g.history(fg) == grid_deltas(n)
+
+
+def test_tc_random_graph_medium():
+    check_tc(8, random_graph(1500, 1950, seed=5))
+
+
+def test_same_generation_parity():
+    edges = layered_dag(4, nvert=600, nlayers=6, nedges=840)
+    pairs = [tuple(r) for r in edges[:, :2].tolist()]
+    want = sg_bruteforce(pairs)
+    for nb in [1, 3]:
+        print(f"[sg] nb={nb}", flush=True)
+        o, ho, g, hg = both(nb, lambda b: sg_program(b, edges))
+        for k in ["E", "I", "SG", "A", "B"]:
+            assert_same_rel(o, ho[k], g, hg[k], k)
+        for k in ["base", "filter", "rec"]:
+            assert o.history(ho[k]) == g.history(hg[k])
+        assert set(map(tuple, g.rows(hg["SG"])[:, :2].tolist())) == want
+
+
+def test_acopy_and_filter_parity():
+    rows = np.array([[1, 2, 100], [1, 2, 101], [3, 4, 102], [7, 7, 103]], dtype=np.uint64)
+
+    def build(b):
+        A = b.rel_create(2, 1, 10)
+        B = b.rel_create(2, 1, 11)
+        C = b.rel_create(2, 1, 12)
+        b.load_rows(A, rows, DELTA)
+        s = b.scc_create(False, 1)
+        b.scc_add_relation(s, A, False)
+        b.scc_add_relation(s, B, False)
+        b.scc_add_relation(s, C, False)
+        b.add_acopy(s, B, A, DELTA, [1, 0, 2])
+        b.add_copy_filter(s, C, A, DELTA, [1, 0], O.F_NEQ)
+        return A, B, C
+
+    for nb in [1, 2]:
+        o, ho, g, hg = both(nb, build)
+        for i in range(3):
+            assert_same_rel(o, ho[i], g, hg[i], f"rel{i}")
+
+
+def test_config2_grid150_properties():
+    # the full-size config: 150x150 grid, uint64 ids, 8 virtual buckets
+    n, nb = 150, 8
+    print("[config2] grid150 nb=8", flush=True)
+    g = gpu_ctx(nb)
+    T, G, sfix, scopy = tc_program(g, grid_graph(n, seed=150))
+    g.execute()
+    assert g.count(T) == grid_tc_size(n) == 128233125
+    assert g.history(sfix) == grid_deltas(n)
+    assert g.joined() == 253069050
+    rows = g.rows(T)
+    # bucket-major, lexicographic within bucket, no duplicate keys
+    b = np.array([O.tuple_hash([int(x)]) % nb for x in rows[::100000, 0]])
+    assert np.all(np.diff(b) >= 0)
+    ids = rows[:, 2]
+    assert np.all((ids >> np.uint64(46)) == np.uint64(257))
+    seq = ids & np.uint64((1 << 28) - 1)
+    bucket = (ids >> np.uint64(28)) & np.uint64((1 << 18) - 1)
+    for bb in range(nb):
+        s = np.sort(seq[bucket == bb])
+        assert np.array_equal(s, np.arange(s.size, dtype=np.uint64))
