@@ -1,0 +1,553 @@
+/*
+ * parallel_RA_inc.h -- drop-in C++ facade of parallel-RA's program surface over
+ * the MI355X backend (pra.h).
+ *
+ * Generated Datalog programs (e.g. the reference's tests/transitive_closure.cpp)
+ * include "../src/parallel_RA_inc.h" and use relation, RAM, parallel_join,
+ * parallel_copy, parallel_acopy, parallel_copy_filter, LIE and mpi_comm.  This
+ * header declares the same names with the same constructor signatures and
+ * ownership rules, so such a program compiles unchanged and runs on one GPU.
+ *
+ * One process drives one GPU.  The rank layout of an `mpirun -n k` run of the
+ * reference (bucket = tuple_hash(key) % k) is reproduced with k virtual buckets,
+ * taken from the PRA_BUCKETS environment variable (default 1), which makes the
+ * intern-id column identical to the k-rank reference run.  PRA_DEVICE selects
+ * the GPU (default 0).
+ *
+ * Interface correspondence (reference file:line):
+ *   relation            src/relation/balanced_hash_relation.h:17-233
+ *   parallel_RA         src/RA/parallel_RA.h:11-72
+ *   parallel_join       src/RA/parallel_join.h:10-58
+ *   parallel_copy       src/RA/parallel_copy.h:28-32
+ *   parallel_acopy      src/RA/parallel_acopy.h:45-49
+ *   parallel_copy_filter src/RA/parallel_copy_filter.h:29-33
+ *   parallel_copy_generate src/RA/parallel_copy_generate.h:29-33 (rejected at execute)
+ *   RAM                 src/RAM/RA_tasks.h:13-165
+ *   LIE                 src/lie/lie.h:13-153
+ *   mpi_comm            src/comm/comm.h:8-76
+ */
+#pragma once
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <iostream>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "pra.h"
+#include "pra_mpi_single.h"
+
+typedef int8_t s8;
+typedef int16_t s16;
+typedef int32_t s32;
+typedef int64_t s64;
+typedef uint8_t u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef char c8;
+typedef wchar_t c16;
+
+/* src/relation/balanced_hash_relation.h:12-15 */
+enum { LEFT = 0, RIGHT };
+enum { DELTA = 0, FULL, FULL_AND_DELTA };
+enum { COPY = 0, COPY_FILTER, COPY_GENERATE, ACOPY, JOIN };
+enum { STATIC = 0, DYNAMIC };
+
+/* FNV-1a-64 over LE bytes (src/hash/hash.h:11-30) */
+inline u64 tuple_hash(const u64* start_ptr, u64 prefix_len)
+{
+    u64 h = 14695981039346656037ULL;
+    for (u64 i = 0; i < prefix_len; ++i)
+        for (int b = 0; b < 8; b++) {
+            h ^= (start_ptr[i] >> (8 * b)) & 255ULL;
+            h *= 1099511628211ULL;
+        }
+    return h;
+}
+
+namespace pra_facade {
+
+inline void die(pra_ctx* ctx, int rc, const char* what)
+{
+    fprintf(stderr, "parallel-RA (MI355X backend): %s failed (%d): %s\n", what, rc,
+            ctx ? pra_last_error(ctx) : "");
+    exit(1);
+}
+
+inline void chk(pra_ctx* ctx, int rc, const char* what)
+{
+    if (rc != PRA_OK) die(ctx, rc, what);
+}
+
+inline int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+/* Map a copy_filter predicate over columns 0 and 1 to a backend predicate code
+ * by evaluating it on probe rows; predicates that are not one of the
+ * comparisons below are rejected (a host function cannot run on the GPU). */
+inline int classify_filter(bool (*f)(const u64* const))
+{
+    static const u64 probes[][2] = {{5, 5}, {4, 5}, {5, 4}, {0, 0}, {1, 2}, {100, 3}, {0x8000000000000001ULL, 7},
+                                    {7, 0xFFFFFFFFFFFFFFFFULL}, {0xFFFFFFFFFFFFFFFFULL, 0xFFFFFFFFFFFFFFFFULL}};
+    const int codes[] = {PRA_FILTER_EQ, PRA_FILTER_NEQ, PRA_FILTER_LT, PRA_FILTER_LE,
+                         PRA_FILTER_GT, PRA_FILTER_GE, PRA_FILTER_TRUE, PRA_FILTER_FALSE};
+    for (int code : codes) {
+        bool ok = true;
+        for (const auto& p : probes) {
+            u64 row[8] = {p[0], p[1], 0, 0, 0, 0, 0, 0};
+            bool want;
+            switch (code) {
+            case PRA_FILTER_EQ: want = p[0] == p[1]; break;
+            case PRA_FILTER_NEQ: want = p[0] != p[1]; break;
+            case PRA_FILTER_LT: want = p[0] < p[1]; break;
+            case PRA_FILTER_LE: want = p[0] <= p[1]; break;
+            case PRA_FILTER_GT: want = p[0] > p[1]; break;
+            case PRA_FILTER_GE: want = p[0] >= p[1]; break;
+            case PRA_FILTER_TRUE: want = true; break;
+            default: want = false; break;
+            }
+            if (f(row) != want) { ok = false; break; }
+        }
+        if (ok) return code;
+    }
+    return -1;
+}
+
+} // namespace pra_facade
+
+/* ------------------------------------------------------------- mpi_comm --- */
+
+class mpi_comm
+{
+    int rank = 0, nprocs = 1, local_rank = 0, local_nprocs = 1;
+    MPI_Comm world_comm = MPI_COMM_WORLD, local_comm = MPI_COMM_WORLD;
+
+public:
+    mpi_comm() {}
+    MPI_Comm get_comm() { return world_comm; }
+    int get_nprocs() { return nprocs; }
+    int get_rank() { return rank; }
+    int get_local_nprocs() { return local_nprocs; }
+    int get_local_rank() { return local_rank; }
+    MPI_Comm get_local_comm() { return local_comm; }
+    void set_local_comm(MPI_Comm c) { local_comm = c; }
+    void create(int, char**) {}
+    void destroy() {}
+};
+
+/* ------------------------------------------------------------- relation --- */
+
+class relation
+{
+    u32 join_column_count;
+    bool is_canonical;
+    u32 arity;
+    u32 intern_tag;
+    std::string debug_id;
+    int initailization_type = -1;
+    std::string filename;
+    pra_rel* handle = nullptr;
+
+public:
+    relation(u32 jcc, bool is_c, u32 ar, u32 tg, std::string fname, int version)
+        : join_column_count(jcc), is_canonical(is_c), arity(ar), intern_tag(tg), initailization_type(version),
+          filename(fname)
+    {
+    }
+    relation(u32 jcc, bool is_c, u32 ar, u32 tg, std::string did, std::string fname, int version)
+        : join_column_count(jcc), is_canonical(is_c), arity(ar), intern_tag(tg), debug_id(did),
+          initailization_type(version), filename(fname)
+    {
+    }
+    ~relation()
+    {
+        if (handle) pra_rel_destroy(handle);
+    }
+
+    u32 get_arity() { return arity; }
+    int get_join_column_count() { return (int)join_column_count; }
+    bool get_is_canonical() { return is_canonical; }
+    u32 get_intern_tag() { return intern_tag; }
+    std::string get_debug_id() { return debug_id; }
+    std::string get_filename() { return filename; }
+    void set_filename(std::string f) { filename = f; }
+    void set_initailization_type(int x) { initailization_type = x; }
+    void set_restart_flag(bool) {}
+    void set_offset_io(bool) {}
+    void set_share_io(bool) {}
+    void set_separate_io(bool) {}
+
+    /* counts of this relation's versions */
+    int get_full_element_count()
+    {
+        uint64_t n = 0;
+        if (handle) pra_rel_count(handle, PRA_FULL, &n);
+        return (int)n;
+    }
+    int get_delta_element_count()
+    {
+        uint64_t n = 0;
+        if (handle) pra_rel_count(handle, PRA_DELTA, &n);
+        return (int)n;
+    }
+    u32 get_global_full_element_count() { return (u32)get_full_element_count(); }
+    u32 get_global_delta_element_count() { return (u32)get_delta_element_count(); }
+
+    /* rows of a version in trie order (bucket-major, lexicographic) */
+    std::vector<u64> rows(int version = FULL)
+    {
+        std::vector<u64> out;
+        if (!handle) return out;
+        uint64_t n = 0;
+        pra_rel_count(handle, version, &n);
+        out.resize((size_t)n * (arity + 1));
+        uint64_t got = 0;
+        pra_rel_download(handle, version, out.data(), n, &got);
+        return out;
+    }
+
+    /* backend side: created by LIE::execute */
+    pra_rel* backend() { return handle; }
+    void initialize_relation_on(pra_ctx* ctx)
+    {
+        if (handle) return;
+        pra_facade::chk(ctx, pra_rel_create(ctx, arity, join_column_count, intern_tag, is_canonical, &handle),
+                        "relation create");
+        if (initailization_type == DELTA || initailization_type == FULL) {
+            int rc = pra_rel_load_bin(handle, filename.c_str(), initailization_type == DELTA ? PRA_DELTA : PRA_FULL);
+            if (rc != PRA_OK) {
+                std::cout << filename << " Wrong IO: " << pra_last_error(ctx) << std::endl;
+                MPI_Abort(MPI_COMM_WORLD, -1);
+            }
+        }
+    }
+};
+
+/* ----------------------------------------------------------- RA operators - */
+
+class parallel_RA
+{
+protected:
+    u32 RA_type = 0;
+    mpi_comm mcomm;
+
+public:
+    parallel_RA() {}
+    virtual ~parallel_RA() {}
+    u32 get_RA_type() { return RA_type; }
+    void set_comm(mpi_comm& m) { mcomm = m; }
+    virtual void add_to(pra_scc* scc, pra_ctx* ctx) = 0;
+};
+
+class parallel_join : public parallel_RA
+{
+    relation* join_input0_table;
+    int join_input0_graph_type;
+    relation* join_input1_table;
+    int join_input1_graph_type;
+    relation* join_output_table;
+    std::vector<int> projection_reorder_index_array;
+
+public:
+    parallel_join(relation* output, relation* G, int G_type, relation* T, int T_type, std::vector<int> proj)
+        : join_input0_table(G), join_input0_graph_type(G_type), join_input1_table(T), join_input1_graph_type(T_type),
+          join_output_table(output), projection_reorder_index_array(proj)
+    {
+        RA_type = JOIN;
+    }
+    relation* get_join_input0() { return join_input0_table; }
+    int get_join_input0_graph_type() { return join_input0_graph_type; }
+    relation* get_join_input1() { return join_input1_table; }
+    int get_join_input1_graph_type() { return join_input1_graph_type; }
+    relation* get_join_output() { return join_output_table; }
+    void add_to(pra_scc* scc, pra_ctx* ctx) override
+    {
+        pra_facade::chk(ctx,
+                        pra_scc_add_join(scc, join_output_table->backend(), join_input0_table->backend(),
+                                         join_input0_graph_type, join_input1_table->backend(), join_input1_graph_type,
+                                         projection_reorder_index_array.data(),
+                                         (int)projection_reorder_index_array.size()),
+                        "parallel_join");
+    }
+};
+
+class parallel_copy : public parallel_RA
+{
+    relation* copy_input0_table;
+    int copy_input0_graph_type;
+    relation* copy_output_table;
+    std::vector<int> copy_reorder_index_array;
+
+public:
+    parallel_copy(relation* dest, relation* src, int src_version, std::vector<int> reorder)
+        : copy_input0_table(src), copy_input0_graph_type(src_version), copy_output_table(dest),
+          copy_reorder_index_array(reorder)
+    {
+        RA_type = COPY;
+    }
+    relation* get_copy_input() { return copy_input0_table; }
+    relation* get_copy_output() { return copy_output_table; }
+    void add_to(pra_scc* scc, pra_ctx* ctx) override
+    {
+        pra_facade::chk(ctx,
+                        pra_scc_add_copy(scc, copy_output_table->backend(), copy_input0_table->backend(),
+                                         copy_input0_graph_type, copy_reorder_index_array.data(),
+                                         (int)copy_reorder_index_array.size()),
+                        "parallel_copy");
+    }
+};
+
+class parallel_acopy : public parallel_RA
+{
+    relation* acopy_input0_table;
+    int acopy_input0_graph_type;
+    relation* acopy_output_table;
+    std::vector<int> acopy_reorder_index_array;
+
+public:
+    parallel_acopy(relation* dest, relation* src, int src_version, std::vector<int> reorder)
+        : acopy_input0_table(src), acopy_input0_graph_type(src_version), acopy_output_table(dest),
+          acopy_reorder_index_array(reorder)
+    {
+        RA_type = ACOPY;
+    }
+    relation* get_acopy_input() { return acopy_input0_table; }
+    relation* get_acopy_output() { return acopy_output_table; }
+    void add_to(pra_scc* scc, pra_ctx* ctx) override
+    {
+        pra_facade::chk(ctx,
+                        pra_scc_add_acopy(scc, acopy_output_table->backend(), acopy_input0_table->backend(),
+                                          acopy_input0_graph_type, acopy_reorder_index_array.data(),
+                                          (int)acopy_reorder_index_array.size()),
+                        "parallel_acopy");
+    }
+};
+
+class parallel_copy_filter : public parallel_RA
+{
+    relation* copy_filter_input0_table;
+    int copy_filter_input0_graph_type;
+    relation* copy_filter_output_table;
+    std::vector<int> copy_filter_reorder_index_array;
+    bool (*lambda)(const u64* const);
+
+public:
+    parallel_copy_filter(relation* dest, relation* src, int src_version, std::vector<int> reorder,
+                         bool (*func)(const u64* const))
+        : copy_filter_input0_table(src), copy_filter_input0_graph_type(src_version), copy_filter_output_table(dest),
+          copy_filter_reorder_index_array(reorder), lambda(func)
+    {
+        RA_type = COPY_FILTER;
+    }
+    relation* get_copy_filter_input() { return copy_filter_input0_table; }
+    relation* get_copy_filter_output() { return copy_filter_output_table; }
+    void add_to(pra_scc* scc, pra_ctx* ctx) override
+    {
+        int op = pra_facade::classify_filter(lambda);
+        if (op < 0) {
+            fprintf(stderr, "parallel_copy_filter: predicate is not a comparison of columns 0 and 1\n");
+            exit(1);
+        }
+        pra_facade::chk(ctx,
+                        pra_scc_add_copy_filter(scc, copy_filter_output_table->backend(),
+                                                copy_filter_input0_table->backend(), copy_filter_input0_graph_type,
+                                                copy_filter_reorder_index_array.data(),
+                                                (int)copy_filter_reorder_index_array.size(), op),
+                        "parallel_copy_filter");
+    }
+};
+
+class parallel_copy_generate : public parallel_RA
+{
+public:
+    parallel_copy_generate(relation*, relation*, int, int (*)(const u64* const, u64* const))
+    {
+        RA_type = COPY_GENERATE;
+    }
+    void add_to(pra_scc*, pra_ctx*) override
+    {
+        /* the reference's RAM cannot route it either (src/RAM/RA_tasks.cpp:844-845) */
+        fprintf(stderr, "parallel_copy_generate is not supported by this backend\n");
+        exit(1);
+    }
+};
+
+/* ------------------------------------------------------------------ RAM --- */
+
+class RAM
+{
+    int ram_id;
+    bool fixpoint;
+    u32 ram_relation_count = 0;
+    relation* ram_relations[64];
+    bool ram_relation_status[64];
+    std::vector<parallel_RA*> RA_list;
+    bool logging = false;
+    mpi_comm mcomm;
+
+public:
+    RAM(bool ic, int r_id) : ram_id(r_id), fixpoint(ic) {}
+    ~RAM()
+    {
+        for (auto* r : RA_list) delete r;
+    }
+    void add_relation(relation*& G, bool i_status)
+    {
+        ram_relations[ram_relation_count] = G;
+        ram_relation_status[ram_relation_count] = i_status;
+        ram_relation_count++;
+    }
+    void add_rule(parallel_RA* pj) { RA_list.push_back(pj); }
+    void set_comm(mpi_comm& m) { mcomm = m; }
+    void enable_logging() { logging = true; }
+    int get_id() { return ram_id; }
+    int get_iteration_count() { return fixpoint ? -1 : 1; }
+    u32 get_ram_relation_count() { return ram_relation_count; }
+    relation** get_RAM_relations() { return ram_relations; }
+    bool* get_RAM_relations_status() { return ram_relation_status; }
+    u32 get_bucket_count() { return 1; }
+
+    pra_scc* build_on(pra_ctx* ctx)
+    {
+        pra_scc* s = nullptr;
+        pra_facade::chk(ctx, pra_scc_create(ctx, fixpoint ? 1 : 0, ram_id, &s), "RAM");
+        for (u32 i = 0; i < ram_relation_count; i++)
+            pra_facade::chk(ctx, pra_scc_add_relation(s, ram_relations[i]->backend(), ram_relation_status[i]),
+                            "RAM::add_relation");
+        for (auto* r : RA_list) r->add_to(s, ctx);
+        return s;
+    }
+};
+
+/* ------------------------------------------------------------------ LIE --- */
+
+class LIE
+{
+    u32 batch_size = 10;
+    double batch_time = 0;
+    mpi_comm mcomm;
+    std::string app_name;
+    u32 lie_relation_count = 0;
+    relation* lie_relations[64];
+    u32 lie_sccs_count = 0;
+    RAM* lie_sccs[64];
+    std::map<RAM*, std::set<RAM*>> taskgraph;
+    std::vector<int> executed_scc_id;
+    int loop_counter = 0;
+    std::string output_dir;
+    int cp_iteration = 0;
+    pra_ctx* ctx = nullptr;
+
+    /* first SCC in add order that no remaining SCC points to (src/lie/lie.cpp:27-60) */
+    RAM* one_runnable_tasks()
+    {
+        for (u32 i = 0; i < lie_sccs_count; i++) {
+            if (!lie_sccs[i]) continue;
+            bool blocked = false;
+            for (auto& e : taskgraph)
+                if (e.second.count(lie_sccs[i])) blocked = true;
+            if (!blocked) return lie_sccs[i];
+        }
+        return nullptr;
+    }
+    void update_task_graph(RAM* t)
+    {
+        for (u32 i = 0; i < lie_sccs_count; i++)
+            if (lie_sccs[i] == t) {
+                taskgraph.erase(t);
+                delete t;
+                lie_sccs[i] = nullptr;
+            }
+    }
+
+public:
+    LIE() {}
+    ~LIE()
+    {
+        for (u32 i = 0; i < lie_relation_count; i++) delete lie_relations[i];
+        for (u32 i = 0; i < lie_sccs_count; i++) delete lie_sccs[i];
+        if (ctx) pra_ctx_destroy(ctx);
+    }
+
+    void add_relation(relation* rel) { lie_relations[lie_relation_count++] = rel; }
+    void add_scc(RAM* ra) { lie_sccs[lie_sccs_count++] = ra; }
+    void add_scc_dependance(RAM* src_task, RAM* destination_task) { taskgraph[src_task].insert(destination_task); }
+    void set_comm(mpi_comm comm) { mcomm = comm; }
+    void set_name(std::string name) { app_name = name; }
+    void set_batch_size(u32 bs) { batch_size = bs; }
+    u32 get_batch_size() { return batch_size; }
+    void set_batch_time(double bt) { batch_time = bt; }
+    double get_batch_time() { return batch_time; }
+
+    /* checkpoint / restart controls: accepted, checkpoints are not written */
+    void set_cp_iteration(int it) { cp_iteration = it; }
+    void set_output_dir(std::string o) { output_dir = o; }
+    void set_executed_scc_id(std::vector<int> id) { executed_scc_id = id; }
+    std::vector<int> get_executed_scc_id() { return executed_scc_id; }
+    void set_loop_counter(int c) { loop_counter = c; }
+    int get_loop_counter() { return loop_counter; }
+    void enable_separate_io() {}
+    void enable_all_to_all_dump() {}
+    void enable_offset_io() {}
+    void enable_share_io() {}
+    void set_restart_dir_name(std::string) {}
+    void set_restart_flag(bool) {}
+    void enable_IO() {}
+    void write_checkpoint_dump(int, std::vector<int>, int = 0) {}
+
+    pra_ctx* backend() { return ctx; }
+
+    /* LIE::execute (src/lie/lie.cpp:197-448) */
+    bool execute()
+    {
+        int nb = pra_facade::env_int("PRA_BUCKETS", 1);
+        int dev = pra_facade::env_int("PRA_DEVICE", 0);
+        int rc = pra_ctx_create(dev, nb, 0, &ctx);
+        if (rc != PRA_OK) pra_facade::die(nullptr, rc, "GPU context");
+        pra_facade::chk(ctx, pra_ctx_set_batch_size(ctx, batch_size), "set_batch_size");
+        for (u32 i = 0; i < lie_relation_count; i++) lie_relations[i]->initialize_relation_on(ctx);
+        std::cout << "----------------- Initialization Complete ---------------------" << std::endl << std::endl;
+
+        RAM* task;
+        while ((task = one_runnable_tasks()) != nullptr) {
+            std::cout << "-------------------Executing SCC " << task->get_id() << "------------------" << std::endl;
+            pra_scc* s = task->build_on(ctx);
+            double t0 = MPI_Wtime();
+            uint64_t nbatches = 0;
+            pra_facade::chk(ctx, pra_scc_run(s, nullptr, 0, &nbatches), "SCC execute");
+            double t1 = MPI_Wtime();
+            if (task->get_iteration_count() == 1)
+                print_all_relation_size();
+            else
+                std::cout << "SCC " << task->get_id() << ": " << nbatches << " batches to the fixed point in "
+                          << (t1 - t0) << " s" << std::endl;
+            executed_scc_id.push_back(task->get_id());
+            loop_counter += (int)nbatches;
+            pra_scc_destroy(s);
+            update_task_graph(task);
+        }
+        return true;
+    }
+
+    /* LIE::print_all_relation_size (src/lie/lie.cpp:128-150) */
+    void print_all_relation_size()
+    {
+        u64 total = 0;
+        for (u32 i = 0; i < lie_relation_count; i++) {
+            relation* r = lie_relations[i];
+            u64 n = (u64)(u32)r->get_full_element_count();
+            std::cout << r->get_debug_id() << ": {" << r->get_arity() << "}. (" << n << " total facts)" << std::endl;
+            total += n;
+        }
+        std::cout << "Total facts across all relations " << total << std::endl << std::endl;
+    }
+};

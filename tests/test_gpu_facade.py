@@ -1,0 +1,58 @@
+"""Programs written against the reference's C++ surface run on the GPU through
+the facade: the reference's own tests/transitive_closure.cpp (compiled
+unchanged by build()) and the same-generation example program."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from programs import GOLDEN, layered_dag, sg_program, write_bin
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "parallel-ra_amd", "build")
+
+
+def sizes(stdout):
+    return {m.group(1): int(m.group(3)) for m in re.finditer(r"^(\S+): \{(\d+)\}\. \((\d+) total facts\)", stdout, re.M)}
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BUILD, "TC_reference")), reason="reference TC not built")
+@pytest.mark.parametrize("nb", [1, 3])
+def test_reference_tc_program_on_gpu(tmp_path, nb):
+    # 2-argument form: T is seeded from ../data/g5955/path_2_1_2 relative to cwd
+    d = tmp_path / "data" / "g5955"
+    d.mkdir(parents=True)
+    shutil.copyfile(os.path.join(GOLDEN, "empty_path_2_1_2"), d / "path_2_1_2")
+    shutil.copyfile(os.path.join(GOLDEN, "empty_path_2_1_2.size"), d / "path_2_1_2.size")
+    cwd = tmp_path / "tests"
+    cwd.mkdir()
+    env = dict(os.environ, PRA_BUCKETS=str(nb))
+    print(f"[facade] TC_reference nb={nb}", flush=True)
+    r = subprocess.run([os.path.join(BUILD, "TC_reference"), os.path.join(GOLDEN, "g13236_edge_2_1_2")],
+                       cwd=cwd, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    s = sizes(r.stdout)
+    assert s["T"] == 573 and s["G"] == 60, r.stdout
+    assert "Total facts across all relations 633" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BUILD, "same_generation")), reason="example not built")
+def test_same_generation_program_on_gpu(tmp_path):
+    edges = layered_dag(9, nvert=2000, nlayers=8, nedges=2800)
+    path = str(tmp_path / "edge_2_1_2")
+    write_bin(path, edges)
+    o = O.Oracle(2)
+    h = sg_program(o, edges)
+    o.execute(1)
+    print("[facade] same_generation nb=2", flush=True)
+    r = subprocess.run([os.path.join(BUILD, "same_generation"), path], env=dict(os.environ, PRA_BUCKETS="2"),
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    s = sizes(r.stdout)
+    assert s["sg_2_1_2"] == o.count(h["SG"])
+    assert s["edge_2_1_2"] == 2800
