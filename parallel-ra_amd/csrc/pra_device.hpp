@@ -24,6 +24,14 @@ typedef uint32_t u32;
 #define PRA_FLAG_COLLISION 1u
 #define PRA_FLAG_TABLE_FULL 2u
 #define PRA_FLAG_INDEX_COLLISION 4u
+#define PRA_FLAG_SORT_STALL 8u
+
+#define PRA_MAXPASS 40
+#define PRA_LOOKBACK_SPIN_LIMIT (1u << 22)
+
+// decoupled look-back status words: [63:48] epoch, [47:46] flag, [45:0] count
+#define PRA_ST_AGG 1ULL
+#define PRA_ST_INC 2ULL
 
 // FNV-1a-64 over the 8 little-endian bytes of each word, exactly the
 // reference's tuple_hash (src/hash/hash.h:11-30).
@@ -66,6 +74,12 @@ struct DigitSpec {
     int jcc;       // bucket: number of leading columns hashed
     int pad;
     u64 nb;        // bucket count
+};
+
+struct PassList {
+    int np;
+    int pad;
+    DigitSpec p[PRA_MAXPASS];
 };
 
 __device__ inline u32 pra_digit(const u64* row, const DigitSpec& ds)

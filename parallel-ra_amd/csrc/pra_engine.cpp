@@ -193,8 +193,14 @@ struct pra_ctx {
     // scratch
     DBuf partials, hist, offs, sort_tmp, flags_buf, stage, stage2, orand, tmp64a, tmp64b, tmp64c;
     DBuf cand, dest, slot_of, keep;
+    DBuf ghist, gofs, lb_status, lb_ctr;
+    size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
+    u32 sort_epoch = 0;
+    u64 dedup_epoch = 0;
     u64* host_stage = nullptr;  // pinned
     size_t host_stage_cap = 0;
+    u64* host_pos = nullptr;    // pinned
+    size_t host_pos_cap = 0;
     u64 joined = 0, iterations = 0;
     std::string err;
     // profiling
@@ -213,6 +219,7 @@ struct pra_ctx {
         resolve_prof();
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (host_stage) (void)hipHostFree(host_stage);
+        if (host_pos) (void)hipHostFree(host_pos);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -275,6 +282,20 @@ struct pra_ctx {
         }
         return host_stage;
     }
+    u64* host_posadj(size_t words)
+    {
+        if (words > host_pos_cap) {
+            if (host_pos) {
+                HIPCHK(hipStreamSynchronize(st));
+                HIPCHK(hipHostFree(host_pos));
+            }
+            host_pos = nullptr;
+            size_t c = std::max<size_t>(words, 256);
+            HIPCHK(hipHostMalloc((void**)&host_pos, c * sizeof(u64)));
+            host_pos_cap = c;
+        }
+        return host_pos;
+    }
     u32* dflags() { return flags_buf.as<u32>(); }
 };
 
@@ -309,42 +330,111 @@ static void scan_u32(pra_ctx* c, const u32* in, u32* out, u64 n, u32* total)
     check_launch("scan_u32");
 }
 
-static void clear_flags(pra_ctx* c)
-{
-    c->flags_buf.reserve(64, c->st);
-    pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
-}
-
-// read back `words` u64 from device memory (one synchronization)
-static u64* readback(pra_ctx* c, const void* src, size_t words)
-{
-    u64* h = c->host_buf(words + 1);
-    if (words) HIPCHK(hipMemcpyAsync(h, src, words * 8, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    return h;
-}
-
-static u32 read_flags(pra_ctx* c)
-{
-    u64* h = readback(c, c->flags_buf.p, 1);
-    return (u32)h[0];
-}
-
 static void check_flags(u32 f)
 {
     if (f & (PRA_FLAG_COLLISION | PRA_FLAG_INDEX_COLLISION))
         throw PraError(PRA_E_COLLISION, "64-bit key hash collision between distinct keys");
     if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
+    if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort look-back did not complete");
 }
+
+// Read back `words` u64 from device memory in one synchronization, together
+// with the device error flags.  Flags are sticky: kernels only ever set them
+// on a detected error, and every synchronization point checks them.
+static u64* readback(pra_ctx* c, const void* src, size_t words)
+{
+    u64* h = c->host_buf(words + 1);
+    if (words) HIPCHK(hipMemcpyAsync(h, src, words * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(h + words, c->flags_buf.p, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    check_flags((u32)h[words]);
+    return h;
+}
+
+static void sync_check(pra_ctx* c) { readback(c, nullptr, 0); }
 
 // ------------------------------------------------------------------ sort ---
 
-// LSD radix sort of rows by (bucket, col0 .. col[keycols-1]); digits that are
-// equal for every row are skipped.
-static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols)
+// One-sweep LSD radix sort of rows by (bucket, col0 .. col[keycols-1]).
+// `orm` holds the OR (words 0..15) and AND (16..31) of the first 16 key
+// columns: digits equal in every row are skipped.  When the bucket digit is
+// sorted and nb <= 256, the bucket bounds are written to `bnd` (nb+1 words)
+// and true is returned.
+static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, const u64* orm, u64* bnd)
 {
     u64 n = r.n;
     if (n <= 1) {
+        r.sorted = true;
+        return false;
+    }
+    std::vector<DigitSpec> passes;
+    for (int col = keycols - 1; col >= 0; col--) {
+        u64 vary = (orm && col < 16) ? (orm[col] ^ orm[16 + col]) : ~0ULL;
+        for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
+            if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{col, sh, 0, 0, 0});
+    }
+    bool bucket_last = false;
+    if (c->nb > 1) {
+        for (int sh = 0; ((u64)(c->nb - 1) >> sh) != 0; sh += PRA_RADIX_BITS)
+            passes.push_back(DigitSpec{-1, sh, jcc, 0, (u64)c->nb});
+        bucket_last = c->nb <= PRA_RADIX;
+    }
+    if (passes.empty()) {
+        r.sorted = true;
+        return false;
+    }
+    u32 nt = pra_tiles(n);
+    size_t need = (size_t)nt * PRA_RADIX;
+    if (need > c->lb_zeroed) {
+        c->lb_status.reserve(need * 8, c->st);
+        size_t words = c->lb_status.cap / 8;
+        pra_launch_fill_u64(c->st, c->lb_status.as<u64>(), words, 0);
+        c->lb_zeroed = words;
+        c->sort_epoch = 0;
+    }
+    c->sort_tmp.reserve(n * w * 8, c->st);
+    c->lb_ctr.reserve(PRA_MAXPASS * 4 + 8, c->st);
+    c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
+    c->gofs.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
+    bool wrote_bnd = false;
+    for (size_t first = 0; first < passes.size(); first += PRA_MAXPASS) {
+        size_t np = std::min<size_t>(PRA_MAXPASS, passes.size() - first);
+        PassList pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.np = (int)np;
+        for (size_t i = 0; i < np; i++) pl.p[i] = passes[first + i];
+        bool last_chunk = first + np == passes.size();
+        pra_launch_fill_u64(c->st, c->ghist.as<u64>(), np * PRA_RADIX / 2, 0);
+        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (np + 1) / 2, 0);
+        {
+            Prof p(c, K_RADIX_HIST, 8.0 * n * keycols);
+            pra_launch_sort_hist(c->st, r.d(), n, w, pl, c->ghist.as<u32>());
+            check_launch("sort_hist");
+        }
+        u64* bout = (bnd && bucket_last && last_chunk) ? bnd : nullptr;
+        pra_launch_sort_scan(c->st, c->ghist.as<u32>(), (int)np, c->gofs.as<u32>(), bout, (u32)c->nb, n);
+        check_launch("sort_scan");
+        wrote_bnd = wrote_bnd || bout != nullptr;
+        for (size_t i = 0; i < np; i++) {
+            if (++c->sort_epoch >= 0xFFFF) {
+                pra_launch_fill_u64(c->st, c->lb_status.as<u64>(), c->lb_zeroed, 0);
+                c->sort_epoch = 1;
+            }
+            Prof p(c, K_RADIX_SCATTER, 16.0 * n * w);
+            pra_launch_sort_scatter(c->st, r.d(), c->sort_tmp.as<u64>(), n, w, pl.p[i],
+                                    c->gofs.as<u32>() + i * PRA_RADIX, c->lb_status.as<u64>(),
+                                    c->lb_ctr.as<u32>() + i, c->sort_epoch, c->dflags());
+            check_launch("sort_scatter");
+            r.buf.swap(c->sort_tmp);
+        }
+    }
+    r.sorted = true;
+    return wrote_bnd;
+}
+
+static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols)
+{
+    if (r.n <= 1) {
         r.sorted = true;
         return;
     }
@@ -353,41 +443,12 @@ static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols)
     u64* om = c->orand.as<u64>();
     pra_launch_fill_u64(c->st, om, 16, 0);
     pra_launch_fill_u64(c->st, om + 16, 16, ~0ULL);
-    pra_launch_col_orand(c->st, r.d(), n, w, kc, om);
+    pra_launch_col_orand2(c->st, r.d(), r.n, w, kc, om);
     check_launch("col_orand");
     u64* h = readback(c, om, 32);
-    std::vector<DigitSpec> passes;
-    for (int col = keycols - 1; col >= 0; col--) {
-        u64 vary = col < 16 ? (h[col] ^ h[16 + col]) : ~0ULL;
-        for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
-            if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{col, sh, 0, 0, 0});
-    }
-    if (c->nb > 1)
-        for (int sh = 0; ((u64)(c->nb - 1) >> sh) != 0; sh += PRA_RADIX_BITS)
-            passes.push_back(DigitSpec{-1, sh, jcc, 0, (u64)c->nb});
-    if (passes.empty()) {
-        r.sorted = true;
-        return;
-    }
-    u32 nblocks = pra_tiles(n);
-    c->hist.reserve((size_t)PRA_RADIX * nblocks * 4, c->st);
-    c->offs.reserve((size_t)PRA_RADIX * nblocks * 4 + 8, c->st);
-    c->sort_tmp.reserve(n * w * 8, c->st);
-    for (auto& ds : passes) {
-        {
-            Prof p(c, K_RADIX_HIST, 8.0 * n * (ds.col >= 0 ? 1 : jcc));
-            pra_launch_radix_hist(c->st, r.d(), n, w, ds, c->hist.as<u32>(), nblocks);
-            check_launch("radix_hist");
-        }
-        scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)PRA_RADIX * nblocks, nullptr);
-        {
-            Prof p(c, K_RADIX_SCATTER, 16.0 * n * w);
-            pra_launch_radix_scatter(c->st, r.d(), c->sort_tmp.as<u64>(), n, w, ds, c->offs.as<u32>(), nblocks);
-            check_launch("radix_scatter");
-        }
-        r.buf.swap(c->sort_tmp);
-    }
-    r.sorted = true;
+    u64 m[32];
+    std::copy(h, h + 32, m);
+    sort_rows_masked(c, r, w, jcc, keycols, m, nullptr);
 }
 
 // remove rows equal (in all w columns) to their predecessor; rows sorted
@@ -457,11 +518,10 @@ static void ensure_index(pra_ctx* c, pra_rel* R, int v)
     u64 cap = pow2_at_least(std::max<u64>(64, 2 * ng));
     R->idx[v].reserve(cap * 3 * 8, c->st);
     pra_launch_table_clear(c->st, R->idx[v].as<u64>(), cap, 3);
-    clear_flags(c);
     pra_launch_index_insert(c->st, r.d(), R->w, R->jcc, c->tmp64c.as<u64>(), ng, R->idx[v].as<u64>(), cap,
                             c->dflags());
     check_launch("index_insert");
-    check_flags(read_flags(c));
+    sync_check(c);
     R->idx_cap[v] = cap;
     R->idx_valid[v] = true;
 }
@@ -477,7 +537,6 @@ static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
         u64 cap = pow2_at_least(std::max<u64>(1024, 2 * (entries + incoming)));
         R->mt.reserve(cap * stride * 8, c->st);
         pra_launch_table_clear(c->st, R->mt.as<u64>(), cap, stride);
-        clear_flags(c);
         for (int v : {PRA_FULL, PRA_DELTA}) {
             Rows& r = R->ver[v];
             pra_launch_table_insert_rows(c->st, r.d(), r.n, R->w, kw, R->mt.as<u64>(), cap, c->dflags());
@@ -488,7 +547,7 @@ static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
             pra_launch_table_verify_rows(c->st, r.d(), r.n, R->w, kw, R->mt.as<u64>(), cap, c->dflags());
             check_launch("table_verify_rows");
         }
-        check_flags(read_flags(c));
+        sync_check(c);
         R->mt_cap = cap;
         R->mt_entries = entries;
         R->mt_kw = kw;
@@ -497,14 +556,15 @@ static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
     }
     if (2 * (R->mt_entries + incoming) <= R->mt_cap) return;
     Prof p(c, K_TABLE_BUILD, 8.0 * R->mt_cap * stride);
+    // grow 4x while the table is small, 2x beyond 2^28 slots
     u64 cap = pow2_at_least(2 * (R->mt_entries + incoming));
+    if (cap < (1ULL << 28)) cap *= 2;
     DBuf nt;
     nt.reserve(cap * stride * 8, c->st);
     pra_launch_table_clear(c->st, nt.as<u64>(), cap, stride);
-    clear_flags(c);
     pra_launch_table_rehash(c->st, R->mt.as<u64>(), R->mt_cap, nt.as<u64>(), cap, kw, c->dflags());
     check_launch("table_rehash");
-    check_flags(read_flags(c));
+    sync_check(c);
     R->mt.swap(nt);
     R->mt_cap = cap;
 }
@@ -638,8 +698,15 @@ static void run_iteration(pra_scc* S)
         pra_rel* R = outs[oi];
         if (totals[oi]) ensure_member_table(c, R, type0 ? R->arity : R->w, totals[oi]);
     }
-    c->stage2.reserve(std::max<size_t>(1, outs.size()) * 8, c->st);
-    clear_flags(c);
+    // per output relation: [0] new-tuple count, [8..23] OR, [24..39] AND of the key columns
+    const size_t SW = 40;
+    c->stage2.reserve(std::max<size_t>(1, outs.size()) * SW * 8, c->st);
+    for (size_t oi = 0; oi < outs.size(); oi++) {
+        pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW, 24, 0);
+        pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW + 24, 16, ~0ULL);
+    }
+    u64* hpos = c->host_posadj((size_t)nra * nb);
+    std::copy(posadj.begin(), posadj.end(), hpos);
     for (size_t oi = 0; oi < outs.size(); oi++) {
         pra_rel* R = outs[oi];
         u64 J = totals[oi];
@@ -648,10 +715,7 @@ static void run_iteration(pra_scc* S)
         const int kw = cw;
         Rows& NT = R->ver[PRA_NEWT];
         NT.n = 0;
-        if (J == 0) {
-            pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi, 1, 0);
-            continue;
-        }
+        if (J == 0) continue;
         c->cand.reserve(J * cw * 8, c->st);
         c->dest.reserve(J * 4, c->st);
         c->slot_of.reserve(J * 8, c->st);
@@ -660,7 +724,7 @@ static void run_iteration(pra_scc* S)
             RADesc& ra = S->ras[r];
             if (ra.out != R) continue;
             ra.posadj.reserve((size_t)nb * 8, c->st);
-            HIPCHK(hipMemcpyAsync(ra.posadj.p, &posadj[(size_t)r * nb], (size_t)nb * 8, hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(ra.posadj.p, hpos + (size_t)r * nb, (size_t)nb * 8, hipMemcpyHostToDevice, c->st));
             Rows& O = ra.outer->ver[ra.outer_v];
             u64 tot_r = srcoff[(size_t)r * (nb + 1) + nb];
             if (ra.type == RA_JOIN) {
@@ -707,27 +771,22 @@ static void run_iteration(pra_scc* S)
             }
         }
         u64* tab = R->mt.as<u64>();
+        const u64 epoch = ++c->dedup_epoch;
         {
             Prof p(c, K_DEDUP_CLAIM, (double)J * 8.0 * (kw + 1));
-            pra_launch_dedup_claim(c->st, c->cand.as<u64>(), J, cw, kw, tab, R->mt_cap, c->slot_of.as<u64>(),
-                                   c->dflags());
+            pra_launch_dedup_claim2(c->st, c->cand.as<u64>(), J, cw, kw, tab, R->mt_cap, c->slot_of.as<u64>(), epoch,
+                                    c->dflags());
             check_launch("dedup_claim");
-        }
-        {
-            Prof p(c, K_DEDUP_DECIDE, (double)J * 12.0);
-            pra_launch_dedup_decide(c->st, c->cand.as<u64>(), J, cw, kw, tab, c->slot_of.as<u64>(),
-                                    c->keep.as<u32>());
-            check_launch("dedup_decide");
         }
         u32 nblocks = pra_tiles(J);
         c->hist.reserve((size_t)nb * nblocks * 4, c->st);
         c->offs.reserve((size_t)nb * nblocks * 4 + 8, c->st);
         {
             Prof p(c, K_DEDUP_VERIFY, (double)J * (8.0 * kw + 16.0));
-            pra_launch_dedup_verify_hist(c->st, c->cand.as<u64>(), J, cw, kw, tab, c->slot_of.as<u64>(),
+            pra_launch_dedup_decide_hist(c->st, c->cand.as<u64>(), J, cw, kw, tab, c->slot_of.as<u64>(), epoch,
                                          c->keep.as<u32>(), c->dest.as<u32>(), (u32)nb, c->hist.as<u32>(), nblocks,
                                          c->dflags());
-            check_launch("dedup_verify_hist");
+            check_launch("dedup_decide_hist");
         }
         u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
         scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
@@ -736,6 +795,7 @@ static void run_iteration(pra_scc* S)
         {
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
+            a.orand = c->stage2.as<u64>() + oi * SW + 8;
             a.cand = c->cand.as<u64>();
             a.keep = c->keep.as<u32>();
             a.dest = c->dest.as<u32>();
@@ -754,29 +814,40 @@ static void run_iteration(pra_scc* S)
             check_launch("dedup_scatter");
         }
         pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
-                              c->stage2.as<u64>() + oi, intern ? 1 : 0);
+                              c->stage2.as<u64>() + oi * SW, intern ? 1 : 0);
         check_launch("seq_update");
     }
 
-    // 5. new-tuple counts + error flags: the per-iteration host readback
+    // 5. new-tuple counts, key-column masks and error flags: the per-iteration
+    //    host readback
+    size_t no = outs.size();
+    std::vector<u64> st2(no * SW);
     {
-        size_t no = outs.size();
-        u64* h = readback(c, c->stage2.p, no);
-        std::vector<u64> cnts(h, h + no);
-        check_flags(read_flags(c));
+        u64* h = readback(c, c->stage2.p, no * SW);
+        std::copy(h, h + no * SW, st2.begin());
         for (size_t oi = 0; oi < no; oi++) {
             pra_rel* R = outs[oi];
-            R->ver[PRA_NEWT].n = cnts[oi];
+            u64 cnt = totals[oi] ? st2[oi * SW] : 0;
+            R->ver[PRA_NEWT].n = cnt;
             R->ver[PRA_NEWT].sorted = false;
-            R->mt_entries += cnts[oi];
+            R->mt_entries += cnt;
         }
     }
 
     // 6. sort newt into trie order, then delta -> full, newt -> delta
     //    (src/RAM/RA_tasks.cpp:1044-1058)
-    for (size_t oi = 0; oi < outs.size(); oi++) {
+    std::vector<char> bnd_ok(no, 0);
+    for (size_t oi = 0; oi < no; oi++) {
         pra_rel* R = outs[oi];
-        sort_rows(c, R->ver[PRA_NEWT], R->w, R->jcc, out_intern[oi] ? R->arity : R->w);
+        Rows& NT = R->ver[PRA_NEWT];
+        R->bnd[PRA_DELTA].reserve((size_t)(nb + 1) * 8, c->st);
+        u64* bnd = R->bnd[PRA_DELTA].as<u64>();
+        bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, &st2[oi * SW + 8], bnd);
+        if (!bnd_ok[oi] && nb == 1) {
+            pra_launch_fill_u64(c->st, bnd, 1, 0);
+            pra_launch_fill_u64(c->st, bnd + 1, 1, NT.n);
+            bnd_ok[oi] = 1;
+        }
     }
     for (pra_rel* R : S->rels) {
         Rows& D = R->ver[PRA_DELTA];
@@ -786,6 +857,8 @@ static void run_iteration(pra_scc* S)
         D.sorted = true;
         R->ver[PRA_NEWT].n = 0;
         R->touched(PRA_DELTA);
+        auto it = std::find(outs.begin(), outs.end(), R);
+        if (it != outs.end() && bnd_ok[it - outs.begin()]) R->bnd_valid[PRA_DELTA] = true;
     }
     c->iterations++;
 }
@@ -850,6 +923,7 @@ static void scc_run(pra_scc* S)
         } while (d != 0);
     }
     for (pra_rel* R : S->rels) delta_to_full(c, R);
+    sync_check(c);
     S->done = true;
 }
 

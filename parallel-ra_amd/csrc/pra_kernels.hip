@@ -178,6 +178,155 @@ __global__ void k_radix_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec
     }
 }
 
+
+// ---------------------------------------------------- one-sweep LSD sort ---
+//
+// k_sort_hist counts the digits of every pass in one read of the rows;
+// k_sort_scan turns them into per-pass digit offsets; one k_sort_scatter per
+// pass then finds its tile's offset inside each digit by decoupled look-back
+// over the preceding tiles' published counts (tiles are numbered in dispatch
+// order, so a tile only ever waits on tiles that are already running).
+
+__global__ void k_sort_hist(const u64* rows, u64 n, int w, PassList pl, u32* ghist)
+{
+    extern __shared__ u32 sh[];
+    const int nbins = pl.np * PRA_RADIX;
+    for (int i = threadIdx.x; i < nbins; i += PRA_BLOCK) sh[i] = 0;
+    __syncthreads();
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 step = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += step) {
+        const u64* r = rows + i * w;
+        for (int p = 0; p < pl.np; p++) atomicAdd(&sh[p * PRA_RADIX + pra_digit(r, pl.p[p])], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nbins; j += PRA_BLOCK)
+        if (sh[j]) atomicAdd(&ghist[j], sh[j]);
+}
+
+// exclusive scan of each pass's digit counts; when `bnd` is given the last
+// pass is the bucket digit and its offsets are the bucket bounds
+__global__ void k_sort_scan(const u32* ghist, int np, u32* gofs, u64* bnd, u32 nb, u64 n)
+{
+    __shared__ u32 lds[PRA_WAVES + 1];
+    for (int p = 0; p < np; p++) {
+        u32 v = ghist[p * PRA_RADIX + threadIdx.x];
+        u32 tot;
+        u32 ex = block_exclusive_scan<u32>(v, lds, &tot);
+        gofs[p * PRA_RADIX + threadIdx.x] = ex;
+        if (bnd && p == np - 1) {
+            if (threadIdx.x < nb) bnd[threadIdx.x] = ex;
+            if (threadIdx.x == 0) bnd[nb] = n;
+        }
+    }
+}
+
+static_assert(PRA_BLOCK == PRA_RADIX, "look-back assigns one digit per thread");
+
+__device__ inline u64 st_pack(u32 epoch, u64 flag, u64 cnt) { return ((u64)epoch << 48) | (flag << 46) | cnt; }
+
+__global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs, u64* status,
+                               u32* tile_ctr, u32 epoch, u32* flags)
+{
+    __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
+    __shared__ u32 s_base[PRA_RADIX];
+    __shared__ u32 s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
+    __syncthreads();
+    const u32 tile = s_tile;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u64 lt = pra_lanemask_lt();
+    u64 base = (u64)tile * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
+    u32 dg[PRA_ITEMS], lc[PRA_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 idx = base + (u64)k * 64 + lane;
+        bool valid = idx < n;
+        u32 d = valid ? pra_digit(in + idx * w, ds) : 0u;
+        u64 act = __ballot(valid);
+        u64 peers = pra_match<PRA_RADIX_BITS>(d, act);
+        dg[k] = d;
+        lc[k] = 0;
+        if (valid) {
+            u32 before = wh[wave][d];
+            lc[k] = before + (u32)__popcll(peers & lt);
+            if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
+        }
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;   // PRA_BLOCK == PRA_RADIX: one digit per thread
+        u32 acc = 0;
+        for (int v = 0; v < PRA_WAVES; v++) {
+            u32 t = wh[v][d];
+            wh[v][d] = acc;
+            acc += t;
+        }
+        u64* my = status + (u64)tile * PRA_RADIX + d;
+        u64 excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(my, st_pack(epoch, PRA_ST_INC, acc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, st_pack(epoch, PRA_ST_AGG, acc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            u32 t = tile - 1;
+            u32 spins = 0;
+            for (;;) {
+                u64 v = __hip_atomic_load(status + (u64)t * PRA_RADIX + d, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                u64 fl = (v >> 46) & 3ULL;
+                if ((u32)(v >> 48) != epoch || fl == 0) {
+                    if (++spins > PRA_LOOKBACK_SPIN_LIMIT) {
+                        atomicOr(flags, PRA_FLAG_SORT_STALL);
+                        break;
+                    }
+                    continue;
+                }
+                excl += v & ((1ULL << 46) - 1);
+                if (fl == PRA_ST_INC || t == 0) break;
+                t--;
+            }
+            __hip_atomic_store(my, st_pack(epoch, PRA_ST_INC, excl + acc), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_base[d] = gofs[d] + (u32)excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 idx = base + (u64)k * 64 + lane;
+        if (idx < n) {
+            u64 pos = (u64)s_base[dg[k]] + wh[wave][dg[k]] + lc[k];
+            const u64* src = in + idx * w;
+            u64* dst = out + pos * w;
+            for (int c = 0; c < w; c++) dst[c] = src[c];
+        }
+    }
+}
+
+// bitwise OR and AND of column blockIdx.y over all rows, used to skip radix
+// passes whose digit is the same for every row
+__global__ void k_col_orand2(const u64* rows, u64 n, int w, u64* out /* [0..15] or, [16..31] and */)
+{
+    const int c = blockIdx.y;
+    u64 o = 0, a = ~0ULL;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 step = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += step) {
+        u64 v = rows[i * w + c];
+        o |= v;
+        a &= v;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        o |= __shfl_xor(o, d, 64);
+        a &= __shfl_xor(a, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr((unsigned long long*)&out[c], (unsigned long long)o);
+        atomicAnd((unsigned long long*)&out[16 + c], (unsigned long long)a);
+    }
+}
+
 // bitwise OR and AND of each of the first kc columns (kc <= 16), used to skip
 // radix passes whose digit is the same for every row
 __global__ void k_col_orand(const u64* rows, u64 n, int w, int kc, u64* out)
@@ -468,6 +617,79 @@ __global__ void k_dedup_verify_hist(const u64* cand, u64 n, int cw, int kw, u64*
     for (u32 i = threadIdx.x; i < nb; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
 }
 
+
+// Claim (or find) the slot of each candidate's key.  The first claimant
+// stores the key; every candidate then lowers the slot's arrival mark to
+// (epoch << 40 | p+1), so the earliest arrival of this iteration wins, while a
+// mark from an earlier iteration (key already in full or delta) always wins.
+__global__ void k_dedup_claim2(const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of, u64 epoch,
+                               u32* flags)
+{
+    u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const u64* key = cand + p * cw;
+    const int stride = 2 + kw;
+    const u64 mark = (epoch << 40) | (p + 1);
+    u64 h = pra_keyhash(key, kw);
+    u64 m = cap - 1, s = h & m;
+    for (u64 q = 0; q < cap; q++) {
+        u64* slot = table + s * stride;
+        u64 cur = slot[0];
+        bool mine = false;
+        if (cur == 0) {
+            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
+            if (cur == 0) {
+                mine = true;
+                for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
+            }
+        }
+        if (mine || cur == h) {
+            u64 old = slot[1];
+            if (mine || old >= (epoch << 40)) atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
+            slot_of[p] = s;
+            return;
+        }
+        s = (s + 1) & m;
+    }
+    slot_of[p] = ~0ULL;
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+// Keep the winners, verify every key against its slot (a 64-bit hash
+// collision between distinct keys is reported, never merged) and count kept
+// candidates per destination bucket per tile.
+__global__ void k_dedup_decide_hist(const u64* cand, u64 n, int cw, int kw, const u64* table, const u64* slot_of,
+                                    u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist, u32 nblocks, u32* flags)
+{
+    __shared__ u32 h[PRA_RADIX];
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
+    __syncthreads();
+    u64 base = (u64)blockIdx.x * PRA_TILE;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        u64 p = base + (u64)k * PRA_BLOCK + threadIdx.x;
+        if (p < n) {
+            u64 s = slot_of[p];
+            u32 kp = 0;
+            if (s != ~0ULL) {
+                const u64* slot = table + s * (2 + kw);
+                const u64* key = cand + p * cw;
+                bool eq = true;
+                for (int c = 0; c < kw; c++)
+                    if (slot[2 + c] != key[c]) { eq = false; break; }
+                if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
+                if (slot[1] == ((epoch << 40) | (p + 1))) {
+                    kp = 1;
+                    atomicAdd(&h[dest[p]], 1u);
+                }
+            }
+            keep[p] = kp;
+        }
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nb; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
+}
+
 // Stable partition of the kept candidates by destination bucket; JOIN-type
 // rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
@@ -505,6 +727,10 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     __syncthreads();
+    const int kc = a.cw < 16 ? a.cw : 16;
+    u64 o[16], an[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) { o[c] = 0; an[c] = ~0ULL; }
 #pragma unroll
     for (int k = 0; k < PRA_ITEMS; k++) {
         u64 p = base + (u64)k * 64 + lane;
@@ -514,11 +740,25 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
             const u64* src = a.cand + p * a.cw;
             u64* dst = a.newt + pos * a.wn;
             for (int c = 0; c < a.cw; c++) dst[c] = src[c];
+#pragma unroll
+            for (int c = 0; c < 16; c++)
+                if (c < kc) { o[c] |= src[c]; an[c] &= src[c]; }
             if (a.intern) {
                 u64 rank = pos - a.offs[(u64)d * a.nblocks];
                 u64 key = ((u64)a.tag << 46) | ((u64)d << 28);
                 dst[a.cw] = key | (a.seqbase[d] + rank);
             }
+        }
+    }
+    for (int c = 0; c < kc; c++) {
+        u64 x = o[c], y = an[c];
+        for (int dd = 32; dd >= 1; dd >>= 1) {
+            x |= __shfl_xor(x, dd, 64);
+            y &= __shfl_xor(y, dd, 64);
+        }
+        if (lane == 0) {
+            atomicOr((unsigned long long*)&a.orand[c], (unsigned long long)x);
+            atomicAnd((unsigned long long*)&a.orand[16 + c], (unsigned long long)y);
         }
     }
 }
@@ -787,4 +1027,50 @@ void pra_launch_table_rehash(hipStream_t st, const u64* old, u64 oldcap, u64* nt
     if (!oldcap) return;
     hipLaunchKernelGGL(k_table_rehash, dim3(grid_for(oldcap)), dim3(PRA_BLOCK), 0, st, old, oldcap, nt, cap, kw,
                        flags);
+}
+
+void pra_launch_sort_hist(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u32* ghist)
+{
+    u64 g = (n + PRA_BLOCK * 4 - 1) / (PRA_BLOCK * 4);
+    if (g > 2048) g = 2048;
+    if (g == 0) g = 1;
+    size_t lds = (size_t)pl.np * PRA_RADIX * sizeof(u32);
+    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)g), dim3(PRA_BLOCK), lds, st, rows, n, w, pl, ghist);
+}
+
+void pra_launch_sort_scan(hipStream_t st, const u32* ghist, int np, u32* gofs, u64* bnd, u32 nb, u64 n)
+{
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(PRA_BLOCK), 0, st, ghist, np, gofs, bnd, nb, n);
+}
+
+void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs,
+                             u64* status, u32* tile_ctr, u32 epoch, u32* flags)
+{
+    u32 nt = pra_tiles(n);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, gofs, status, tile_ctr,
+                       epoch, flags);
+}
+
+void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out)
+{
+    u64 g = (n + PRA_BLOCK - 1) / PRA_BLOCK;
+    if (g > 512) g = 512;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_col_orand2, dim3((unsigned)g, (unsigned)kc), dim3(PRA_BLOCK), 0, st, rows, n, w, out);
+}
+
+void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap,
+                             u64* slot_of, u64 epoch, u32* flags)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_dedup_claim2, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, cap, slot_of,
+                       epoch, flags);
+}
+
+void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, const u64* table,
+                                  const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
+                                  u32 nblocks, u32* flags)
+{
+    hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
+                       epoch, keep, dest, nb, hist, nblocks, flags);
 }

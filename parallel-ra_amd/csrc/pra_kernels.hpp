@@ -42,6 +42,7 @@ struct CopyExpandArgs {
 };
 
 struct NewtScatterArgs {
+    u64* orand;        // [0..15] OR, [16..31] AND of the written key columns
     const u64* cand;
     const u32* keep;
     const u32* dest;
@@ -91,3 +92,13 @@ void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w,
 void pra_launch_table_verify_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap,
                                   u32* flags);
 void pra_launch_table_rehash(hipStream_t st, const u64* old, u64 oldcap, u64* nt, u64 cap, int kw, u32* flags);
+void pra_launch_sort_hist(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u32* ghist);
+void pra_launch_sort_scan(hipStream_t st, const u32* ghist, int np, u32* gofs, u64* bnd, u32 nb, u64 n);
+void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs,
+                             u64* status, u32* tile_ctr, u32 epoch, u32* flags);
+void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
+void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap,
+                             u64* slot_of, u64 epoch, u32* flags);
+void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, const u64* table,
+                                  const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
+                                  u32 nblocks, u32* flags);
