@@ -63,8 +63,9 @@ int pra_ctx_set_join_threshold(pra_ctx* ctx, int64_t threshold);
 int pra_ctx_joined_tuples(pra_ctx* ctx, uint64_t* out);
 /* fixed-point iterations executed so far over all SCCs */
 int pra_ctx_iterations(pra_ctx* ctx, uint64_t* out);
-/* per-kernel timing with HIP events on the context stream (0 = off) */
-int pra_ctx_set_profiling(pra_ctx* ctx, int on);
+/* per-kernel timing with HIP events on the context stream: bit k of `mask`
+ * times kernel class k (see pra_ctx_kernel_stat), 0 = off, -1 = all */
+int pra_ctx_set_profiling(pra_ctx* ctx, int mask);
 int pra_ctx_kernel_stat(pra_ctx* ctx, int idx, const char** name, uint64_t* launches, double* ms, double* bytes);
 int pra_ctx_synchronize(pra_ctx* ctx);
 

@@ -125,6 +125,8 @@ struct pra_rel {
     DBuf idx[2];
     u64 idx_cap[2] = {0, 0};
     bool idx_valid[2] = {false, false};
+    // packed sort keys of newt rows
+    DBuf nkeys;
     // bucket bounds per version
     DBuf bnd[2];
     bool bnd_valid[2] = {false, false};
@@ -162,11 +164,11 @@ struct KStat {
 
 enum {
     K_JOIN_COUNT, K_JOIN_EXPAND, K_COPY_EXPAND, K_DEDUP_CLAIM, K_DEDUP_DECIDE, K_DEDUP_VERIFY, K_DEDUP_SCATTER,
-    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_COUNT
+    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_COUNT
 };
 static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand", "dedup_claim", "dedup_decide",
                                       "dedup_verify_hist", "dedup_scatter", "radix_hist", "radix_scatter",
-                                      "table_build", "index_build", "scan"};
+                                      "table_build", "index_build", "scan", "gather_rows"};
 
 } // namespace
 
@@ -194,6 +196,10 @@ struct pra_ctx {
     DBuf partials, hist, offs, sort_tmp, flags_buf, stage, stage2, orand, tmp64a, tmp64b, tmp64c;
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
+    // order-preserving value dictionary for packed sort keys
+    DBuf dict, ktmp, idxa, idxb;
+    u64 dict_cap = 0, dict_n = 0;
+    int dict_bits = 0, dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
     u32 sort_epoch = 0;
     u64 dedup_epoch = 0;
@@ -204,7 +210,7 @@ struct pra_ctx {
     u64 joined = 0, iterations = 0;
     std::string err;
     // profiling
-    bool prof = false;
+    u32 prof_mask = 0;           // bit k: time kernel class k
     KStat kst[K_COUNT];
     struct Pending { int k; hipEvent_t a, b; double bytes; };
     std::vector<Pending> pending;
@@ -236,7 +242,7 @@ struct pra_ctx {
     }
     int prof_begin(int k, hipEvent_t* a)
     {
-        if (!prof) return -1;
+        if (!(prof_mask & (1u << k))) return -1;
         *a = ev();
         HIPCHK(hipEventRecord(*a, st));
         return k;
@@ -355,12 +361,38 @@ static void sync_check(pra_ctx* c) { readback(c, nullptr, 0); }
 
 // ------------------------------------------------------------------ sort ---
 
+// look-back status words for `n` rows, zero-initialised once per allocation
+static void prepare_lookback(pra_ctx* c, u64 n)
+{
+    size_t need = (size_t)pra_tiles(n) * PRA_RADIX;
+    if (need > c->lb_zeroed) {
+        c->lb_status.reserve(need * 8, c->st);
+        size_t words = c->lb_status.cap / 8;
+        pra_launch_fill_u64(c->st, c->lb_status.as<u64>(), words, 0);
+        c->lb_zeroed = words;
+        c->sort_epoch = 0;
+    }
+    c->lb_ctr.reserve(PRA_MAXPASS * 4 + 8, c->st);
+    c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
+    c->gofs.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
+}
+
+static u32 next_sort_epoch(pra_ctx* c)
+{
+    if (++c->sort_epoch >= 0xFFFF) {
+        pra_launch_fill_u64(c->st, c->lb_status.as<u64>(), c->lb_zeroed, 0);
+        c->sort_epoch = 1;
+    }
+    return c->sort_epoch;
+}
+
 // One-sweep LSD radix sort of rows by (bucket, col0 .. col[keycols-1]).
 // `orm` holds the OR (words 0..15) and AND (16..31) of the first 16 key
 // columns: digits equal in every row are skipped.  When the bucket digit is
 // sorted and nb <= 256, the bucket bounds are written to `bnd` (nb+1 words)
 // and true is returned.
-static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, const u64* orm, u64* bnd)
+static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, const u64* orm, u64* bnd,
+                             bool by_bucket = true)
 {
     u64 n = r.n;
     if (n <= 1) {
@@ -374,7 +406,7 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
             if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{col, sh, 0, 0, 0});
     }
     bool bucket_last = false;
-    if (c->nb > 1) {
+    if (c->nb > 1 && by_bucket) {
         for (int sh = 0; ((u64)(c->nb - 1) >> sh) != 0; sh += PRA_RADIX_BITS)
             passes.push_back(DigitSpec{-1, sh, jcc, 0, (u64)c->nb});
         bucket_last = c->nb <= PRA_RADIX;
@@ -383,19 +415,8 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
         r.sorted = true;
         return false;
     }
-    u32 nt = pra_tiles(n);
-    size_t need = (size_t)nt * PRA_RADIX;
-    if (need > c->lb_zeroed) {
-        c->lb_status.reserve(need * 8, c->st);
-        size_t words = c->lb_status.cap / 8;
-        pra_launch_fill_u64(c->st, c->lb_status.as<u64>(), words, 0);
-        c->lb_zeroed = words;
-        c->sort_epoch = 0;
-    }
+    prepare_lookback(c, n);
     c->sort_tmp.reserve(n * w * 8, c->st);
-    c->lb_ctr.reserve(PRA_MAXPASS * 4 + 8, c->st);
-    c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
-    c->gofs.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
     bool wrote_bnd = false;
     for (size_t first = 0; first < passes.size(); first += PRA_MAXPASS) {
         size_t np = std::min<size_t>(PRA_MAXPASS, passes.size() - first);
@@ -416,14 +437,11 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
         check_launch("sort_scan");
         wrote_bnd = wrote_bnd || bout != nullptr;
         for (size_t i = 0; i < np; i++) {
-            if (++c->sort_epoch >= 0xFFFF) {
-                pra_launch_fill_u64(c->st, c->lb_status.as<u64>(), c->lb_zeroed, 0);
-                c->sort_epoch = 1;
-            }
+            u32 ep = next_sort_epoch(c);
             Prof p(c, K_RADIX_SCATTER, 16.0 * n * w);
             pra_launch_sort_scatter(c->st, r.d(), c->sort_tmp.as<u64>(), n, w, pl.p[i],
                                     c->gofs.as<u32>() + i * PRA_RADIX, c->lb_status.as<u64>(),
-                                    c->lb_ctr.as<u32>() + i, c->sort_epoch, c->dflags());
+                                    c->lb_ctr.as<u32>() + i, ep, c->dflags(), nullptr, nullptr);
             check_launch("sort_scatter");
             r.buf.swap(c->sort_tmp);
         }
@@ -432,7 +450,7 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
     return wrote_bnd;
 }
 
-static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols)
+static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols, bool by_bucket = true)
 {
     if (r.n <= 1) {
         r.sorted = true;
@@ -448,7 +466,71 @@ static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols)
     u64* h = readback(c, om, 32);
     u64 m[32];
     std::copy(h, h + 32, m);
-    sort_rows_masked(c, r, w, jcc, keycols, m, nullptr);
+    sort_rows_masked(c, r, w, jcc, keycols, m, nullptr, by_bucket);
+}
+
+// Sort rows by their packed keys (bucket, rank0, .., rank[a-1]): LSD passes
+// over the varying key bytes move (key, row index) pairs, then one gather
+// moves the rows.  Writes the bucket bounds to `bnd` (nb + 1 words).
+static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, u64 kor, u64 kand, int bucket_shift, u64* bnd)
+{
+    u64 n = r.n;
+    u64 vary = kor ^ kand;
+    std::vector<DigitSpec> passes;
+    for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
+        if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{0, sh, 0, 0, 0});
+    if (n > 1 && !passes.empty()) {
+        prepare_lookback(c, n);
+        c->ktmp.reserve(n * 8, c->st);
+        c->idxa.reserve(n * 4, c->st);
+        c->idxb.reserve(n * 4, c->st);
+        PassList pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.np = (int)passes.size();
+        for (size_t i = 0; i < passes.size(); i++) pl.p[i] = passes[i];
+        pra_launch_fill_u64(c->st, c->ghist.as<u64>(), (size_t)pl.np * PRA_RADIX / 2, 0);
+        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (size_t)(pl.np + 1) / 2, 0);
+        {
+            Prof p(c, K_RADIX_HIST, 8.0 * n);
+            pra_launch_sort_hist(c->st, keys.as<u64>(), n, 1, pl, c->ghist.as<u32>());
+            check_launch("key_sort_hist");
+        }
+        pra_launch_sort_scan(c->st, c->ghist.as<u32>(), pl.np, c->gofs.as<u32>(), nullptr, (u32)c->nb, n);
+        check_launch("key_sort_scan");
+        for (int i = 0; i < pl.np; i++) {
+            u32 ep = next_sort_epoch(c);
+            Prof p(c, K_RADIX_SCATTER, 24.0 * n);
+            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, pl.p[i],
+                                    c->gofs.as<u32>() + (size_t)i * PRA_RADIX, c->lb_status.as<u64>(),
+                                    c->lb_ctr.as<u32>() + i, ep, c->dflags(), i == 0 ? nullptr : c->idxa.as<u32>(),
+                                    c->idxb.as<u32>());
+            check_launch("key_sort_scatter");
+            keys.swap(c->ktmp);
+            c->idxa.swap(c->idxb);
+        }
+        c->sort_tmp.reserve(n * w * 8, c->st);
+        {
+            Prof p(c, K_GATHER, 16.0 * n * w + 4.0 * n);
+            pra_launch_gather_rows(c->st, r.d(), c->idxa.as<u32>(), n, w, c->sort_tmp.as<u64>());
+            check_launch("gather_rows");
+        }
+        r.buf.swap(c->sort_tmp);
+    }
+    r.sorted = true;
+    if (c->nb > 1) {
+        pra_launch_key_bounds(c->st, keys.as<u64>(), n, bucket_shift, (u64)c->nb, bnd);
+        check_launch("key_bounds");
+    } else {
+        pra_launch_fill_u64(c->st, bnd, 1, 0);
+        pra_launch_fill_u64(c->st, bnd + 1, 1, n);
+    }
+}
+
+static int ceil_log2(u64 x)
+{
+    int b = 0;
+    while (b < 64 && (1ULL << b) < x) b++;
+    return b;
 }
 
 // remove rows equal (in all w columns) to their predecessor; rows sorted
@@ -466,6 +548,38 @@ static void unique_rows(pra_ctx* c, Rows& r, int w)
     u64* h = readback(c, c->tmp64b.as<u64>() + n, 1);
     r.buf.swap(c->sort_tmp);
     r.n = h[0];
+}
+
+// (Re)build the value dictionary from the key columns of every relation.
+static void build_dictionary(pra_ctx* c)
+{
+    u64 total = 0;
+    for (pra_rel* R : c->rels)
+        for (int v : {PRA_FULL, PRA_DELTA}) total += R->ver[v].n * (u64)R->arity;
+    if (total == 0) return;
+    Rows V;
+    V.buf.reserve(total * 8, c->st);
+    u64 off = 0;
+    for (pra_rel* R : c->rels)
+        for (int v : {PRA_FULL, PRA_DELTA}) {
+            Rows& r = R->ver[v];
+            pra_launch_collect_cols(c->st, r.d(), r.n, R->w, R->arity, V.d() + off);
+            off += r.n * (u64)R->arity;
+        }
+    check_launch("collect_cols");
+    V.n = total;
+    sort_rows(c, V, 1, 0, 1, false);
+    unique_rows(c, V, 1);
+    u64 cap = pow2_at_least(std::max<u64>(64, 2 * V.n));
+    c->dict.reserve(cap * 16, c->st);
+    pra_launch_fill_u64(c->st, c->dict.as<u64>(), cap * 2, 0);
+    pra_launch_dict_insert(c->st, V.d(), V.n, c->dict.as<u64>(), cap, c->dflags());
+    check_launch("dict_insert");
+    sync_check(c);
+    c->dict_cap = cap;
+    c->dict_n = V.n;
+    c->dict_bits = std::max(1, ceil_log2(V.n));
+    c->dict_state = 1;
 }
 
 static void ensure_sorted(pra_ctx* c, pra_rel* R, int v)
@@ -705,6 +819,17 @@ static void run_iteration(pra_scc* S)
         pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW, 24, 0);
         pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW + 24, 16, ~0ULL);
     }
+    // packed sort keys need the value dictionary (rebuilt after a miss, given up
+    // after repeated misses)
+    bool any_intern = false;
+    for (size_t oi = 0; oi < outs.size(); oi++) any_intern = any_intern || (out_intern[oi] && totals[oi]);
+    if (any_intern && c->dict_state == 0) build_dictionary(c);
+    const int bucket_bits = nb > 1 ? ceil_log2((u64)nb) : 0;
+    std::vector<char> packed(outs.size(), 0);
+    for (size_t oi = 0; oi < outs.size(); oi++)
+        packed[oi] = c->dict_state == 1 && out_intern[oi] && totals[oi] &&
+                     bucket_bits + outs[oi]->arity * c->dict_bits <= 64;
+    for (size_t oi = 0; oi < outs.size(); oi++) pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW + 2, 1, ~0ULL);
     u64* hpos = c->host_posadj((size_t)nra * nb);
     std::copy(posadj.begin(), posadj.end(), hpos);
     for (size_t oi = 0; oi < outs.size(); oi++) {
@@ -809,6 +934,15 @@ static void run_iteration(pra_scc* S)
             a.wn = R->w;
             a.intern = intern ? 1 : 0;
             a.tag = R->tag;
+            if (packed[oi]) {
+                R->nkeys.reserve(J * 8, c->st);
+                a.dict = c->dict.as<u64>();
+                a.dict_cap = c->dict_cap;
+                a.keys = R->nkeys.as<u64>();
+                a.kstat = c->stage2.as<u64>() + oi * SW + 1;
+                a.rank_bits = c->dict_bits;
+                a.bucket_shift = R->arity * c->dict_bits;
+            }
             Prof p(c, K_DEDUP_SCATTER, (double)J * (8.0 * cw + 8.0));
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
@@ -842,6 +976,13 @@ static void run_iteration(pra_scc* S)
         Rows& NT = R->ver[PRA_NEWT];
         R->bnd[PRA_DELTA].reserve((size_t)(nb + 1) * 8, c->st);
         u64* bnd = R->bnd[PRA_DELTA].as<u64>();
+        if (packed[oi] && st2[oi * SW + 3] == 0) {
+            sort_by_packed_key(c, NT, R->w, R->nkeys, st2[oi * SW + 1], st2[oi * SW + 2], R->arity * c->dict_bits, bnd);
+            bnd_ok[oi] = 1;
+            continue;
+        }
+        if (packed[oi] && c->dict_state == 1)
+            c->dict_state = ++c->dict_rebuilds > 3 ? 2 : 0;
         bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, &st2[oi * SW + 8], bnd);
         if (!bnd_ok[oi] && nb == 1) {
             pra_launch_fill_u64(c->st, bnd, 1, 0);
@@ -1036,7 +1177,7 @@ int pra_ctx_set_profiling(pra_ctx* ctx, int on)
     if (!ctx) return PRA_E_ARG;
     API_BEGIN(ctx)
     ctx->resolve_prof();
-    ctx->prof = on != 0;
+    ctx->prof_mask = (u32)on;
     if (on)
         for (auto& k : ctx->kst) { k.launches = 0; k.ms = 0; k.bytes = 0; }
     API_END

@@ -112,72 +112,6 @@ __global__ void k_scan_apply(const T* in, T* out, u64 n, const T* partials)
     }
 }
 
-// --------------------------------------------------------- radix sort ------
-
-__global__ void k_radix_hist(const u64* rows, u64 n, int w, DigitSpec ds, u32* hist, u32 nblocks)
-{
-    __shared__ u32 h[PRA_RADIX];
-    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
-    __syncthreads();
-    u64 base = (u64)blockIdx.x * PRA_TILE;
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        u64 idx = base + (u64)k * PRA_BLOCK + threadIdx.x;
-        if (idx < n) atomicAdd(&h[pra_digit(rows + idx * w, ds)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
-}
-
-// Stable scatter of one LSD pass.  Wave v of a workgroup owns the contiguous
-// rows [tile + v*TILE/WAVES, ...) processed 64 at a time; ranks inside a wave
-// come from ballot-based digit matching, ranks across waves from an LDS prefix.
-__global__ void k_radix_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* offs, u32 nblocks)
-{
-    __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
-    for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const u64 lt = pra_lanemask_lt();
-    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
-    u32 dg[PRA_ITEMS], lc[PRA_ITEMS];
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        u64 idx = base + (u64)k * 64 + lane;
-        bool valid = idx < n;
-        u32 d = valid ? pra_digit(in + idx * w, ds) : 0u;
-        u64 act = __ballot(valid);
-        u64 peers = pra_match<PRA_RADIX_BITS>(d, act);
-        dg[k] = d;
-        lc[k] = 0;
-        if (valid) {
-            u32 before = wh[wave][d];
-            lc[k] = before + (u32)__popcll(peers & lt);
-            if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
-        }
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < PRA_RADIX; d += PRA_BLOCK) {
-        u32 acc = offs[(u64)d * nblocks + blockIdx.x];
-        for (int v = 0; v < PRA_WAVES; v++) {
-            u32 t = wh[v][d];
-            wh[v][d] = acc;
-            acc += t;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        u64 idx = base + (u64)k * 64 + lane;
-        if (idx < n) {
-            u64 pos = (u64)wh[wave][dg[k]] + lc[k];
-            const u64* src = in + idx * w;
-            u64* dst = out + pos * w;
-            for (int c = 0; c < w; c++) dst[c] = src[c];
-        }
-    }
-}
-
 
 // ---------------------------------------------------- one-sweep LSD sort ---
 //
@@ -225,8 +159,9 @@ static_assert(PRA_BLOCK == PRA_RADIX, "look-back assigns one digit per thread");
 
 __device__ inline u64 st_pack(u32 epoch, u64 flag, u64 cnt) { return ((u64)epoch << 48) | (flag << 46) | cnt; }
 
+template <bool IDX>
 __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs, u64* status,
-                               u32* tile_ctr, u32 epoch, u32* flags)
+                               u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout)
 {
     __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
     __shared__ u32 s_base[PRA_RADIX];
@@ -300,6 +235,7 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
             const u64* src = in + idx * w;
             u64* dst = out + pos * w;
             for (int c = 0; c < w; c++) dst[c] = src[c];
+            if (IDX) iout[pos] = iin ? iin[idx] : (u32)idx;
         }
     }
 }
@@ -327,29 +263,76 @@ __global__ void k_col_orand2(const u64* rows, u64 n, int w, u64* out /* [0..15] 
     }
 }
 
-// bitwise OR and AND of each of the first kc columns (kc <= 16), used to skip
-// radix passes whose digit is the same for every row
-__global__ void k_col_orand(const u64* rows, u64 n, int w, int kc, u64* out)
+// ------------------------------------------------------ value dictionary ---
+//
+// An order-preserving dictionary of the values in the relations' key columns
+// (rank(a) < rank(b) iff a < b) lets a newt row's sort key
+// (bucket, col0, .., col[a-1]) be packed into one u64 of
+// bucket bits + a * rank bits, which the LSD sort then orders in a few passes.
+// Slots: [value, rank + 1]; rank + 1 == 0 marks an empty slot.
+
+__device__ inline bool pra_dict_rank(const u64* tab, u64 cap, u64 v, u64* rank)
 {
-    u64 o[16], a[16];
-    for (int c = 0; c < 16; c++) { o[c] = 0; a[c] = ~0ULL; }
+    u64 m = cap - 1, s = pra_mix64(v ^ 0x5851F42D4C957F2DULL) & m;
+    for (u64 q = 0; q < cap; q++) {
+        const u64* sl = tab + 2 * s;
+        u64 r = sl[1];
+        if (r == 0) return false;
+        if (sl[0] == v) {
+            *rank = r - 1;
+            return true;
+        }
+        s = (s + 1) & m;
+    }
+    return false;
+}
+
+__global__ void k_collect_cols(const u64* rows, u64 n, int w, int ncols, u64* out)
+{
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    u64 step = (u64)gridDim.x * blockDim.x;
-    for (; i < n; i += step) {
-        const u64* r = rows + i * w;
-        for (int c = 0; c < kc; c++) { o[c] |= r[c]; a[c] &= r[c]; }
-    }
-    for (int c = 0; c < kc; c++) {
-        u64 x = o[c], y = a[c];
-        for (int d = 32; d >= 1; d >>= 1) {
-            x |= __shfl_xor(x, d, 64);
-            y &= __shfl_xor(y, d, 64);
+    if (i >= n) return;
+    for (int c = 0; c < ncols; c++) out[i * ncols + c] = rows[i * w + c];
+}
+
+// values sorted and distinct: rank = position
+__global__ void k_dict_insert(const u64* vals, u64 n, u64* tab, u64 cap, u32* flags)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 v = vals[i];
+    u64 m = cap - 1, s = pra_mix64(v ^ 0x5851F42D4C957F2DULL) & m;
+    for (u64 q = 0; q < cap; q++) {
+        u64* sl = tab + 2 * s;
+        if (atomicCAS((unsigned long long*)(sl + 1), 0ULL, (unsigned long long)(i + 1)) == 0) {
+            sl[0] = v;
+            return;
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicOr((unsigned long long*)&out[c], (unsigned long long)x);
-            atomicAnd((unsigned long long*)&out[16 + c], (unsigned long long)y);
-        }
+        s = (s + 1) & m;
     }
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+// out[i] = in[idx[i]] for rows of w words
+__global__ void k_gather_rows(const u64* in, const u32* idx, u64 n, int w, u64* out)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* src = in + (u64)idx[i] * w;
+    u64* dst = out + i * w;
+    for (int c = 0; c < w; c++) dst[c] = src[c];
+}
+
+// bnd[s] = first position whose key has bucket (key >> shift) >= s
+__global__ void k_key_bounds(const u64* keys, u64 n, int shift, u64 nb, u64* bnd)
+{
+    u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nb) return;
+    u64 lo = 0, hi = n;
+    while (lo < hi) {
+        u64 mid = (lo + hi) >> 1;
+        if ((keys[mid] >> shift) < s) lo = mid + 1; else hi = mid;
+    }
+    bnd[s] = lo;
 }
 
 // bstart[s] = first row whose bucket is >= s, for s in [0, nb]
@@ -539,84 +522,6 @@ __global__ void k_gather_srcoff(const u64* off, const u64* bstart, u64 nb, u64* 
 
 // ---------------------------------------------------- dedup / newt ---------
 
-// Claim (or find) the slot of each candidate's key and record the earliest
-// arrival position (p+1; 0 marks a key already in full or delta).
-__global__ void k_dedup_claim(const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of,
-                              u32* flags)
-{
-    u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const u64* key = cand + p * cw;
-    const int stride = 2 + kw;
-    u64 h = pra_keyhash(key, kw);
-    u64 m = cap - 1, s = h & m;
-    for (u64 q = 0; q < cap; q++) {
-        u64* slot = table + s * stride;
-        u64 cur = slot[0];
-        if (cur == 0)
-            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
-        if (cur == 0 || cur == h) {
-            atomicMin((unsigned long long*)(slot + 1), (unsigned long long)(p + 1));
-            slot_of[p] = s;
-            return;
-        }
-        s = (s + 1) & m;
-    }
-    slot_of[p] = ~0ULL;
-    atomicOr(flags, PRA_FLAG_TABLE_FULL);
-}
-
-__global__ void k_dedup_decide(const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
-                               u32* keep)
-{
-    u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    u64 s = slot_of[p];
-    u32 k = 0;
-    if (s != ~0ULL) {
-        u64* slot = table + s * (2 + kw);
-        if (slot[1] == p + 1) {
-            const u64* key = cand + p * cw;
-            for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
-            k = 1;
-        }
-    }
-    keep[p] = k;
-}
-
-// Verify keys against their slots (a 64-bit hash collision between distinct
-// keys is reported, never silently merged), turn winners into "present"
-// entries, and count kept candidates per destination bucket per tile.
-__global__ void k_dedup_verify_hist(const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
-                                    const u32* keep, const u32* dest, u32 nb, u32* hist, u32 nblocks, u32* flags)
-{
-    __shared__ u32 h[PRA_RADIX];
-    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
-    __syncthreads();
-    u64 base = (u64)blockIdx.x * PRA_TILE;
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        u64 p = base + (u64)k * PRA_BLOCK + threadIdx.x;
-        if (p < n) {
-            u64 s = slot_of[p];
-            if (s != ~0ULL) {
-                u64* slot = table + s * (2 + kw);
-                const u64* key = cand + p * cw;
-                bool eq = true;
-                for (int c = 0; c < kw; c++)
-                    if (slot[2 + c] != key[c]) { eq = false; break; }
-                if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
-                if (keep[p]) {
-                    slot[1] = 0;
-                    atomicAdd(&h[dest[p]], 1u);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < nb; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
-}
-
 
 // Claim (or find) the slot of each candidate's key.  The first claimant
 // stores the key; every candidate then lowers the slot's arrival mark to
@@ -728,6 +633,8 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     }
     __syncthreads();
     const int kc = a.cw < 16 ? a.cw : 16;
+    u64 kor = 0, kand = ~0ULL;
+    u32 kmiss = 0;
     u64 o[16], an[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) { o[c] = 0; an[c] = ~0ULL; }
@@ -748,17 +655,45 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                 u64 key = ((u64)a.tag << 46) | ((u64)d << 28);
                 dst[a.cw] = key | (a.seqbase[d] + rank);
             }
+            if (a.dict) {
+                u64 K = (u64)d << a.bucket_shift;
+                for (int c = 0; c < a.cw; c++) {
+                    u64 r = 0;
+                    if (!pra_dict_rank(a.dict, a.dict_cap, src[c], &r)) kmiss = 1;
+                    K |= r << ((a.cw - 1 - c) * a.rank_bits);
+                }
+                a.keys[pos] = K;
+                kor |= K;
+                kand &= K;
+            }
         }
     }
-    for (int c = 0; c < kc; c++) {
-        u64 x = o[c], y = an[c];
+    if (a.dict) {
+#pragma unroll
         for (int dd = 32; dd >= 1; dd >>= 1) {
-            x |= __shfl_xor(x, dd, 64);
-            y &= __shfl_xor(y, dd, 64);
+            kor |= __shfl_xor(kor, dd, 64);
+            kand &= __shfl_xor(kand, dd, 64);
+            kmiss |= __shfl_xor(kmiss, dd, 64);
         }
         if (lane == 0) {
-            atomicOr((unsigned long long*)&a.orand[c], (unsigned long long)x);
-            atomicAnd((unsigned long long*)&a.orand[16 + c], (unsigned long long)y);
+            atomicOr((unsigned long long*)&a.kstat[0], (unsigned long long)kor);
+            atomicAnd((unsigned long long*)&a.kstat[1], (unsigned long long)kand);
+            if (kmiss) atomicOr((unsigned long long*)&a.kstat[2], 1ULL);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        if (c < kc) {
+            u64 x = o[c], y = an[c];
+#pragma unroll
+            for (int dd = 32; dd >= 1; dd >>= 1) {
+                x |= __shfl_xor(x, dd, 64);
+                y &= __shfl_xor(y, dd, 64);
+            }
+            if (lane == 0) {
+                atomicOr((unsigned long long*)&a.orand[c], (unsigned long long)x);
+                atomicAnd((unsigned long long*)&a.orand[16 + c], (unsigned long long)y);
+            }
         }
     }
 }
@@ -891,25 +826,6 @@ void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* partials,
 
 u32 pra_tiles(u64 n) { return (u32)((n + PRA_TILE - 1) / PRA_TILE); }
 
-void pra_launch_radix_hist(hipStream_t st, const u64* rows, u64 n, int w, DigitSpec ds, u32* hist, u32 nblocks)
-{
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, rows, n, w, ds, hist, nblocks);
-}
-
-void pra_launch_radix_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* offs,
-                              u32 nblocks)
-{
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, offs, nblocks);
-}
-
-void pra_launch_col_orand(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out)
-{
-    u64 g = (n + PRA_BLOCK - 1) / PRA_BLOCK;
-    if (g > 1024) g = 1024;
-    if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_col_orand, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, rows, n, w, kc, out);
-}
-
 void pra_launch_bucket_bounds(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart)
 {
     hipLaunchKernelGGL(k_bucket_bounds, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, rows, n, w, jcc, nb, bstart);
@@ -970,30 +886,6 @@ void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart,
     hipLaunchKernelGGL(k_gather_srcoff, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, off, bstart, nb, srcoff);
 }
 
-void pra_launch_dedup_claim(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of,
-                            u32* flags)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_dedup_claim, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, cap, slot_of,
-                       flags);
-}
-
-void pra_launch_dedup_decide(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
-                             u32* keep)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_dedup_decide, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
-                       keep);
-}
-
-void pra_launch_dedup_verify_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table,
-                                  const u64* slot_of, const u32* keep, const u32* dest, u32 nb, u32* hist,
-                                  u32 nblocks, u32* flags)
-{
-    hipLaunchKernelGGL(k_dedup_verify_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
-                       keep, dest, nb, hist, nblocks, flags);
-}
-
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
 {
     hipLaunchKernelGGL(k_dedup_scatter, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
@@ -1044,11 +936,38 @@ void pra_launch_sort_scan(hipStream_t st, const u32* ghist, int np, u32* gofs, u
 }
 
 void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs,
-                             u64* status, u32* tile_ctr, u32 epoch, u32* flags)
+                             u64* status, u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout)
 {
     u32 nt = pra_tiles(n);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, gofs, status, tile_ctr,
-                       epoch, flags);
+    if (iout)
+        hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, gofs, status,
+                           tile_ctr, epoch, flags, iin, iout);
+    else
+        hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, gofs, status,
+                           tile_ctr, epoch, flags, iin, iout);
+}
+
+void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_collect_cols, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, ncols, out);
+}
+
+void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u64 cap, u32* flags)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_dict_insert, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, vals, n, tab, cap, flags);
+}
+
+void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n, int w, u64* out)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, in, idx, n, w, out);
+}
+
+void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, int shift, u64 nb, u64* bnd)
+{
+    hipLaunchKernelGGL(k_key_bounds, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, keys, n, shift, nb, bnd);
 }
 
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out)

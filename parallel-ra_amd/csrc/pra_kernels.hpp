@@ -43,6 +43,11 @@ struct CopyExpandArgs {
 
 struct NewtScatterArgs {
     u64* orand;        // [0..15] OR, [16..31] AND of the written key columns
+    const u64* dict;   // value -> rank dictionary (null: no packed sort key)
+    u64 dict_cap;
+    u64* keys;         // packed sort key per written row
+    u64* kstat;        // [0] OR, [1] AND of the packed keys, [2] dictionary miss
+    int rank_bits, bucket_shift;
     const u64* cand;
     const u32* keep;
     const u32* dest;
@@ -61,10 +66,6 @@ u64 pra_scan_partials_count(u64 n);
 void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* partials, u64* total);
 void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* partials, u32* total);
 u32 pra_tiles(u64 n);
-void pra_launch_radix_hist(hipStream_t st, const u64* rows, u64 n, int w, DigitSpec ds, u32* hist, u32 nblocks);
-void pra_launch_radix_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* offs,
-                              u32 nblocks);
-void pra_launch_col_orand(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
 void pra_launch_bucket_bounds(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart);
 void pra_launch_group_flags(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64* flags);
 void pra_launch_group_compact(hipStream_t st, const u64* flags, const u64* gpos, u64 n, u64* gstart);
@@ -77,13 +78,6 @@ void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a);
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
 void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart, u64 nb, u64* srcoff);
-void pra_launch_dedup_claim(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of,
-                            u32* flags);
-void pra_launch_dedup_decide(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, const u64* slot_of,
-                             u32* keep);
-void pra_launch_dedup_verify_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table,
-                                  const u64* slot_of, const u32* keep, const u32* dest, u32 nb, u32* hist,
-                                  u32 nblocks, u32* flags);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
 void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
                            u64* count_out, int intern);
@@ -95,7 +89,11 @@ void pra_launch_table_rehash(hipStream_t st, const u64* old, u64 oldcap, u64* nt
 void pra_launch_sort_hist(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u32* ghist);
 void pra_launch_sort_scan(hipStream_t st, const u32* ghist, int np, u32* gofs, u64* bnd, u32 nb, u64 n);
 void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs,
-                             u64* status, u32* tile_ctr, u32 epoch, u32* flags);
+                             u64* status, u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout);
+void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out);
+void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u64 cap, u32* flags);
+void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n, int w, u64* out);
+void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, int shift, u64 nb, u64* bnd);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
 void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap,
                              u64* slot_of, u64 epoch, u32* flags);

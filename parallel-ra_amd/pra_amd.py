@@ -137,8 +137,12 @@ class Context:
     def set_join_threshold(self, t):
         self._chk(self.L.pra_ctx_set_join_threshold(self.c, int(t)), "set_join_threshold")
 
-    def set_profiling(self, on):
-        self._chk(self.L.pra_ctx_set_profiling(self.c, int(bool(on))), "set_profiling")
+    def set_profiling(self, on, mask=None):
+        m = (-1 if on else 0) if mask is None else int(mask)
+        self._chk(self.L.pra_ctx_set_profiling(self.c, m), "set_profiling")
+
+    def kernel_names(self):
+        return list(self.kernel_stats().keys())
 
     def kernel_stats(self):
         out = {}
