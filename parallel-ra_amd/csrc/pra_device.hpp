@@ -27,6 +27,7 @@ typedef uint32_t u32;
 #define PRA_FLAG_SORT_STALL 8u
 
 #define PRA_MAXPASS 40
+#define PRA_PART_WORDS 40   // per-block summary words of the newt scatter
 #define PRA_LOOKBACK_SPIN_LIMIT (1u << 22)
 
 // decoupled look-back status words: [63:48] epoch, [47:46] flag, [45:0] count

@@ -125,8 +125,8 @@ struct pra_rel {
     DBuf idx[2];
     u64 idx_cap[2] = {0, 0};
     bool idx_valid[2] = {false, false};
-    // packed sort keys of newt rows
-    DBuf nkeys;
+    // packed sort keys of newt rows and the offsets of their 8 byte digits
+    DBuf nkeys, kgofs;
     // bucket bounds per version
     DBuf bnd[2];
     bool bnd_valid[2] = {false, false};
@@ -197,7 +197,7 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb;
+    DBuf dict, ktmp, idxa, idxb, part, part_hist;
     u64 dict_cap = 0, dict_n = 0;
     int dict_bits = 0, dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -472,7 +472,8 @@ static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols, bool by_
 // Sort rows by their packed keys (bucket, rank0, .., rank[a-1]): LSD passes
 // over the varying key bytes move (key, row index) pairs, then one gather
 // moves the rows.  Writes the bucket bounds to `bnd` (nb + 1 words).
-static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, u64 kor, u64 kand, int bucket_shift, u64* bnd)
+static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32* kgofs, u64 kor, u64 kand,
+                               int bucket_shift, u64* bnd)
 {
     u64 n = r.n;
     u64 vary = kor ^ kand;
@@ -484,24 +485,13 @@ static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, u64 kor, 
         c->ktmp.reserve(n * 8, c->st);
         c->idxa.reserve(n * 4, c->st);
         c->idxb.reserve(n * 4, c->st);
-        PassList pl;
-        memset(&pl, 0, sizeof(pl));
-        pl.np = (int)passes.size();
-        for (size_t i = 0; i < passes.size(); i++) pl.p[i] = passes[i];
-        pra_launch_fill_u64(c->st, c->ghist.as<u64>(), (size_t)pl.np * PRA_RADIX / 2, 0);
-        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (size_t)(pl.np + 1) / 2, 0);
-        {
-            Prof p(c, K_RADIX_HIST, 8.0 * n);
-            pra_launch_sort_hist(c->st, keys.as<u64>(), n, 1, pl, c->ghist.as<u32>());
-            check_launch("key_sort_hist");
-        }
-        pra_launch_sort_scan(c->st, c->ghist.as<u32>(), pl.np, c->gofs.as<u32>(), nullptr, (u32)c->nb, n);
-        check_launch("key_sort_scan");
-        for (int i = 0; i < pl.np; i++) {
+        int np = (int)passes.size();
+        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (size_t)(np + 1) / 2, 0);
+        for (int i = 0; i < np; i++) {
             u32 ep = next_sort_epoch(c);
             Prof p(c, K_RADIX_SCATTER, 24.0 * n);
-            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, pl.p[i],
-                                    c->gofs.as<u32>() + (size_t)i * PRA_RADIX, c->lb_status.as<u64>(),
+            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, passes[i],
+                                    kgofs + (size_t)(passes[i].shift / 8) * PRA_RADIX, c->lb_status.as<u64>(),
                                     c->lb_ctr.as<u32>() + i, ep, c->dflags(), i == 0 ? nullptr : c->idxa.as<u32>(),
                                     c->idxb.as<u32>());
             check_launch("key_sort_scatter");
@@ -815,10 +805,6 @@ static void run_iteration(pra_scc* S)
     // per output relation: [0] new-tuple count, [8..23] OR, [24..39] AND of the key columns
     const size_t SW = 40;
     c->stage2.reserve(std::max<size_t>(1, outs.size()) * SW * 8, c->st);
-    for (size_t oi = 0; oi < outs.size(); oi++) {
-        pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW, 24, 0);
-        pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW + 24, 16, ~0ULL);
-    }
     // packed sort keys need the value dictionary (rebuilt after a miss, given up
     // after repeated misses)
     bool any_intern = false;
@@ -829,7 +815,6 @@ static void run_iteration(pra_scc* S)
     for (size_t oi = 0; oi < outs.size(); oi++)
         packed[oi] = c->dict_state == 1 && out_intern[oi] && totals[oi] &&
                      bucket_bits + outs[oi]->arity * c->dict_bits <= 64;
-    for (size_t oi = 0; oi < outs.size(); oi++) pra_launch_fill_u64(c->st, c->stage2.as<u64>() + oi * SW + 2, 1, ~0ULL);
     u64* hpos = c->host_posadj((size_t)nra * nb);
     std::copy(posadj.begin(), posadj.end(), hpos);
     for (size_t oi = 0; oi < outs.size(); oi++) {
@@ -934,18 +919,30 @@ static void run_iteration(pra_scc* S)
             a.wn = R->w;
             a.intern = intern ? 1 : 0;
             a.tag = R->tag;
+            c->part.reserve((size_t)nblocks * PRA_PART_WORDS * 8, c->st);
+            a.part = c->part.as<u64>();
             if (packed[oi]) {
                 R->nkeys.reserve(J * 8, c->st);
+                c->part_hist.reserve((size_t)nblocks * 8 * PRA_RADIX * 4, c->st);
                 a.dict = c->dict.as<u64>();
                 a.dict_cap = c->dict_cap;
                 a.keys = R->nkeys.as<u64>();
-                a.kstat = c->stage2.as<u64>() + oi * SW + 1;
+                a.part_hist = c->part_hist.as<u32>();
                 a.rank_bits = c->dict_bits;
                 a.bucket_shift = R->arity * c->dict_bits;
             }
             Prof p(c, K_DEDUP_SCATTER, (double)J * (8.0 * cw + 8.0));
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
+        }
+        c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
+        pra_launch_newt_reduce(c->st, c->part.as<u64>(), c->part_hist.as<u32>(), nblocks, packed[oi] ? 1 : 0,
+                               c->stage2.as<u64>() + oi * SW, c->ghist.as<u32>());
+        check_launch("newt_reduce");
+        if (packed[oi]) {
+            R->kgofs.reserve(8 * PRA_RADIX * 4, c->st);
+            pra_launch_sort_scan(c->st, c->ghist.as<u32>(), 8, R->kgofs.as<u32>(), nullptr, (u32)nb, 0);
+            check_launch("key_digit_offsets");
         }
         pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
                               c->stage2.as<u64>() + oi * SW, intern ? 1 : 0);
@@ -977,7 +974,8 @@ static void run_iteration(pra_scc* S)
         R->bnd[PRA_DELTA].reserve((size_t)(nb + 1) * 8, c->st);
         u64* bnd = R->bnd[PRA_DELTA].as<u64>();
         if (packed[oi] && st2[oi * SW + 3] == 0) {
-            sort_by_packed_key(c, NT, R->w, R->nkeys, st2[oi * SW + 1], st2[oi * SW + 2], R->arity * c->dict_bits, bnd);
+            sort_by_packed_key(c, NT, R->w, R->nkeys, R->kgofs.as<u32>(), st2[oi * SW + 1], st2[oi * SW + 2],
+                               R->arity * c->dict_bits, bnd);
             bnd_ok[oi] = 1;
             continue;
         }

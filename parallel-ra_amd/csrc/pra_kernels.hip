@@ -445,16 +445,29 @@ __global__ void k_join_count(const u64* outer, u64 n, int wo, int jcc, const u64
 // One thread per join pair: locate the outer row by binary search over the
 // exclusive offsets, build the projection of (outer ++ inner[jcc:]) (LEFT) or
 // (inner ++ outer[jcc:]) (RIGHT), and write it at its arrival position.
-__global__ void k_join_expand(JoinExpandArgs a)
+// largest i in [lo, hi] with off[i] <= x (off[lo] <= x holds)
+__device__ inline u64 pra_row_of(const u64* off, u64 lo, u64 hi, u64 x)
 {
-    u64 o = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= a.total) return;
-    u64 lo = 0, hi = a.n_outer;   // find largest i with off[i] <= o
     while (lo < hi) {
         u64 mid = (lo + hi + 1) >> 1;
-        if (a.off[mid] <= o) lo = mid; else hi = mid - 1;
+        if (off[mid] <= x) lo = mid; else hi = mid - 1;
     }
-    u64 i = lo;
+    return lo;
+}
+
+__global__ void k_join_expand(JoinExpandArgs a)
+{
+    __shared__ u64 s_lo, s_hi;
+    const u64 o0 = (u64)blockIdx.x * blockDim.x;
+    if (threadIdx.x == 0) {
+        u64 last = o0 + blockDim.x - 1 < a.total ? o0 + blockDim.x - 1 : a.total - 1;
+        s_lo = pra_row_of(a.off, 0, a.n_outer, o0);
+        s_hi = pra_row_of(a.off, s_lo, a.n_outer, last);
+    }
+    __syncthreads();
+    u64 o = o0 + threadIdx.x;
+    if (o >= a.total) return;
+    u64 i = pra_row_of(a.off, s_lo, s_hi, o);
     const u64* d = a.outer + i * a.wo;
     const u64* f = a.inner + (a.istart[i] + (o - a.off[i])) * a.wi;
     u64 s = a.nb == 1 ? 0 : pra_fnv(d, a.jcc) % a.nb;
@@ -632,9 +645,16 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     __syncthreads();
+    // per-block summaries: the packed key's OR / AND / dictionary miss and its
+    // byte-digit histograms (packed mode), or the key columns' OR / AND
+    __shared__ u32 khist[8 * PRA_RADIX];
+    __shared__ u64 red[PRA_WAVES][PRA_PART_WORDS];
+    const bool packed = a.dict != nullptr;
+    if (packed)
+        for (int i = threadIdx.x; i < 8 * PRA_RADIX; i += PRA_BLOCK) khist[i] = 0;
+    __syncthreads();
     const int kc = a.cw < 16 ? a.cw : 16;
-    u64 kor = 0, kand = ~0ULL;
-    u32 kmiss = 0;
+    u64 kor = 0, kand = ~0ULL, kmiss = 0;
     u64 o[16], an[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) { o[c] = 0; an[c] = ~0ULL; }
@@ -647,15 +667,12 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
             const u64* src = a.cand + p * a.cw;
             u64* dst = a.newt + pos * a.wn;
             for (int c = 0; c < a.cw; c++) dst[c] = src[c];
-#pragma unroll
-            for (int c = 0; c < 16; c++)
-                if (c < kc) { o[c] |= src[c]; an[c] &= src[c]; }
             if (a.intern) {
                 u64 rank = pos - a.offs[(u64)d * a.nblocks];
                 u64 key = ((u64)a.tag << 46) | ((u64)d << 28);
                 dst[a.cw] = key | (a.seqbase[d] + rank);
             }
-            if (a.dict) {
+            if (packed) {
                 u64 K = (u64)d << a.bucket_shift;
                 for (int c = 0; c < a.cw; c++) {
                     u64 r = 0;
@@ -665,37 +682,85 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                 a.keys[pos] = K;
                 kor |= K;
                 kand &= K;
+#pragma unroll
+                for (int b = 0; b < 8; b++) atomicAdd(&khist[b * PRA_RADIX + ((K >> (8 * b)) & 255)], 1u);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    if (c < kc) { o[c] |= src[c]; an[c] &= src[c]; }
             }
         }
     }
-    if (a.dict) {
 #pragma unroll
-        for (int dd = 32; dd >= 1; dd >>= 1) {
-            kor |= __shfl_xor(kor, dd, 64);
-            kand &= __shfl_xor(kand, dd, 64);
-            kmiss |= __shfl_xor(kmiss, dd, 64);
-        }
-        if (lane == 0) {
-            atomicOr((unsigned long long*)&a.kstat[0], (unsigned long long)kor);
-            atomicAnd((unsigned long long*)&a.kstat[1], (unsigned long long)kand);
-            if (kmiss) atomicOr((unsigned long long*)&a.kstat[2], 1ULL);
-        }
+    for (int dd = 32; dd >= 1; dd >>= 1) {
+        kor |= __shfl_xor(kor, dd, 64);
+        kand &= __shfl_xor(kand, dd, 64);
+        kmiss |= __shfl_xor(kmiss, dd, 64);
     }
+    if (!packed) {
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-        if (c < kc) {
-            u64 x = o[c], y = an[c];
+        for (int c = 0; c < 16; c++) {
+            if (c < kc) {
 #pragma unroll
-            for (int dd = 32; dd >= 1; dd >>= 1) {
-                x |= __shfl_xor(x, dd, 64);
-                y &= __shfl_xor(y, dd, 64);
-            }
-            if (lane == 0) {
-                atomicOr((unsigned long long*)&a.orand[c], (unsigned long long)x);
-                atomicAnd((unsigned long long*)&a.orand[16 + c], (unsigned long long)y);
+                for (int dd = 32; dd >= 1; dd >>= 1) {
+                    o[c] |= __shfl_xor(o[c], dd, 64);
+                    an[c] &= __shfl_xor(an[c], dd, 64);
+                }
             }
         }
     }
+    if (lane == 0) {
+        red[wave][0] = kor;
+        red[wave][1] = kand;
+        red[wave][2] = kmiss;
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            red[wave][8 + c] = c < kc ? o[c] : 0;
+            red[wave][24 + c] = c < kc ? an[c] : ~0ULL;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < PRA_PART_WORDS) {
+        const int j = threadIdx.x;
+        u64 v = red[0][j];
+        bool is_and = j == 1 || (j >= 24 && j < 40);
+        for (int w2 = 1; w2 < PRA_WAVES; w2++) v = is_and ? (v & red[w2][j]) : (v | red[w2][j]);
+        a.part[(u64)blockIdx.x * PRA_PART_WORDS + j] = v;
+    }
+    if (packed)
+        for (int i = threadIdx.x; i < 8 * PRA_RADIX; i += PRA_BLOCK)
+            a.part_hist[(u64)blockIdx.x * 8 * PRA_RADIX + i] = khist[i];
+}
+
+// Combine the per-block summaries of k_dedup_scatter: stat[1..3] = packed key
+// OR / AND / miss, stat[8..39] = key column OR / AND; ghist = byte-digit
+// histograms of the packed keys.
+__global__ void k_newt_reduce(const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat, u32* ghist)
+{
+    u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 8 * PRA_RADIX) {
+        if (!packed) return;
+        u32 acc = 0;
+        for (u32 b = 0; b < nblocks; b++) acc += part_hist[(u64)b * 8 * PRA_RADIX + t];
+        ghist[t] = acc;
+        return;
+    }
+    u32 j = t - 8 * PRA_RADIX;
+    if (j >= PRA_PART_WORDS || (j >= 3 && j < 8)) return;
+    bool is_and = j == 1 || (j >= 24 && j < 40);
+    u64 v = is_and ? ~0ULL : 0;
+    for (u32 b = 0; b < nblocks; b++) {
+        u64 x = part[(u64)b * PRA_PART_WORDS + j];
+        v = is_and ? (v & x) : (v | x);
+    }
+    if (j == 0)
+        stat[1] = v;
+    else if (j == 1)
+        stat[2] = v;
+    else if (j == 2)
+        stat[3] = v;
+    else
+        stat[j] = v;
 }
 
 // seq counters advance by the number of new tuples per bucket; the kept
@@ -992,4 +1057,12 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw
 {
     hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
                        epoch, keep, dest, nb, hist, nblocks, flags);
+}
+
+void pra_launch_newt_reduce(hipStream_t st, const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat,
+                            u32* ghist)
+{
+    u32 threads = 8 * PRA_RADIX + PRA_PART_WORDS;
+    hipLaunchKernelGGL(k_newt_reduce, dim3((threads + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0, st, part,
+                       part_hist, nblocks, packed, stat, ghist);
 }

@@ -46,7 +46,8 @@ struct NewtScatterArgs {
     const u64* dict;   // value -> rank dictionary (null: no packed sort key)
     u64 dict_cap;
     u64* keys;         // packed sort key per written row
-    u64* kstat;        // [0] OR, [1] AND of the packed keys, [2] dictionary miss
+    u64* part;         // per block: PRA_PART_WORDS summary words
+    u32* part_hist;    // per block: 8 x 256 byte-digit counts of the packed keys
     int rank_bits, bucket_shift;
     const u64* cand;
     const u32* keep;
@@ -100,3 +101,5 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, u64 n, int cw, int
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags);
+void pra_launch_newt_reduce(hipStream_t st, const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat,
+                            u32* ghist);
