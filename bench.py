@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["tc", "sg"], default="tc",
+                    help="tc: config 2 grid transitive closure (default); sg: config 3 same generation")
     ap.add_argument("--grid", type=int, default=150)
     ap.add_argument("--buckets", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=30)
@@ -58,17 +60,30 @@ def main():
               f"(the multi-GPU exchange is a design, not code).", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import pra_amd
-    from programs import grid_graph, grid_tc_size, tc_program
+    from programs import grid_graph, grid_tc_size, layered_dag, sg_program, tc_program
 
     n, nb = args.grid, args.buckets
-    edges = grid_graph(n, seed=150)
-    expect_tuples = grid_tc_size(n)
+    if args.workload == "tc":
+        edges = grid_graph(n, seed=150)
+        expect_tuples = grid_tc_size(n)
+        workload = f"tc_grid{n}_u64ids"
+
+        def build(c):
+            T, G, sfix, scopy = tc_program(c, edges)
+            return T, G, sfix, scopy
+    else:
+        edges = layered_dag(3)
+        expect_tuples = 1392703
+        workload = "same_generation_dag10k_l12"
+
+        def build(c):
+            h = sg_program(c, edges)
+            return h["SG"], h["E"], h["rec"], h["base"]
 
     def prepared():
         c = pra_amd.Context(nbuckets=nb, device=0, batch_size=1)
-        h = tc_program(c, edges)
+        h = build(c)
         c.synchronize()
         return c, h
 
@@ -137,7 +152,7 @@ def main():
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches_per_step": d["launches"] // args.steps,
                     "algorithmic_bytes_per_launch": round(bytes_per)}
 
-    cpu = None if args.no_cpu else cpu_baseline(edges, nb, args.cpu_iters)
+    cpu = None if (args.no_cpu or args.workload != "tc") else cpu_baseline(edges, nb, args.cpu_iters)
 
     out = {
         "metric": METRIC,
@@ -152,7 +167,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"tc_grid{n}_u64ids", "buckets": nb, "edges": int(len(edges)),
+        "config": {"workload": workload, "buckets": nb, "edges": int(len(edges)),
                    "tuples": expect_tuples, "fixed_point_iterations": iters,
                    "joined_tuples_per_step": joined // args.steps},
         "fixed_point_wall_s": total / args.steps,

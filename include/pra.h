@@ -116,6 +116,13 @@ int pra_scc_history(pra_scc* scc, uint64_t* out, uint64_t cap, uint64_t* n);
  * context in dependency order, first runnable in creation order. */
 int pra_execute(pra_ctx* ctx);
 
+/* Batched execution (no counterpart in the reference, which runs one program
+ * per MPI job): pra_execute on n independent contexts of the same device,
+ * taken in the given order by up to `concurrency` host worker threads; every
+ * context has its own stream, so instances of uneven sizes overlap on the GPU.
+ * rc[i] (optional) receives context i's status; returns the first failure. */
+int pra_execute_many(pra_ctx** ctxs, int n, int concurrency, int* rc);
+
 #ifdef __cplusplus
 }
 #endif

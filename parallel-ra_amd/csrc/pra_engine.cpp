@@ -13,12 +13,14 @@
 #include "pra_kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -44,19 +46,24 @@ static void check_launch(const char* what)
     if (e != hipSuccess) throw PraError(PRA_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device buffer in stream-ordered memory of the owning context's stream:
+// growth and release never synchronize the device, so contexts driven from
+// different host threads overlap.
 struct DBuf {
     void* p = nullptr;
     size_t cap = 0;
+    hipStream_t st = nullptr;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap), st(o.st) { o.p = nullptr; o.cap = 0; }
     DBuf& operator=(DBuf&& o) noexcept
     {
         if (this != &o) {
             release();
             p = o.p;
             cap = o.cap;
+            st = o.st;
             o.p = nullptr;
             o.cap = 0;
         }
@@ -65,31 +72,30 @@ struct DBuf {
     ~DBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, st);
         p = nullptr;
         cap = 0;
     }
-    // grow to at least `bytes`; `keep` preserves the first `used` bytes
-    void reserve(size_t bytes, hipStream_t st, size_t used = 0)
+    // grow to at least `bytes`, preserving the first `used` bytes
+    void reserve(size_t bytes, hipStream_t s, size_t used = 0)
     {
         if (bytes <= cap) return;
         size_t nc = std::max(bytes, cap + cap / 2);
         nc = std::max<size_t>(nc, 256);
         void* np = nullptr;
-        HIPCHK(hipMalloc(&np, nc));
-        if (p && used) HIPCHK(hipMemcpyAsync(np, p, used, hipMemcpyDeviceToDevice, st));
-        if (p) {
-            HIPCHK(hipStreamSynchronize(st));
-            HIPCHK(hipFree(p));
-        }
+        HIPCHK(hipMallocAsync(&np, nc, s));
+        if (p && used) HIPCHK(hipMemcpyAsync(np, p, used, hipMemcpyDeviceToDevice, s));
+        if (p) HIPCHK(hipFreeAsync(p, s));
         p = np;
         cap = nc;
+        st = s;
     }
     template <typename T> T* as() { return static_cast<T*>(p); }
     void swap(DBuf& o)
     {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
+        std::swap(st, o.st);
     }
 };
 
@@ -223,6 +229,14 @@ struct pra_ctx {
     ~pra_ctx()
     {
         resolve_prof();
+        // stream-ordered buffers are returned before their stream goes away
+        if (st) (void)hipStreamSynchronize(st);
+        seq.clear();
+        for (DBuf* b : {&partials, &hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
+                        &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
+                        &idxa, &idxb, &part, &part_hist})
+            b->release();
+        if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (host_stage) (void)hipHostFree(host_stage);
         if (host_pos) (void)hipHostFree(host_pos);
@@ -1441,6 +1455,33 @@ int pra_execute(pra_ctx* ctx)
     pra_scc* s;
     while ((s = one_runnable(ctx)) != nullptr) scc_run(s);
     API_END
+}
+
+int pra_execute_many(pra_ctx** ctxs, int n, int concurrency, int* rc)
+{
+    if (!ctxs || n < 0 || concurrency < 1) return PRA_E_ARG;
+    for (int i = 0; i < n; i++)
+        if (!ctxs[i]) return PRA_E_ARG;
+    std::vector<int> status(n, PRA_OK);
+    std::atomic<int> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            int i = next.fetch_add(1);
+            if (i >= n) return;
+            status[i] = pra_execute(ctxs[i]);
+        }
+    };
+    int nt = std::min(concurrency, std::max(n, 1));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    int first = PRA_OK;
+    for (int i = 0; i < n; i++) {
+        if (rc) rc[i] = status[i];
+        if (first == PRA_OK && status[i] != PRA_OK) first = status[i];
+    }
+    return first;
 }
 
 } // extern "C"

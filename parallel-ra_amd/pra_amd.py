@@ -28,7 +28,7 @@ EXPORTS = (
     "pra_rel_download", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
     "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_history",
-    "pra_execute",
+    "pra_execute", "pra_execute_many",
 )
 
 _vp = ctypes.c_void_p
@@ -83,6 +83,7 @@ def lib():
             "pra_scc_run": [_vp, _u64p, _u64, _u64p],
             "pra_scc_history": [_vp, _u64p, _u64, _u64p],
             "pra_execute": [_vp],
+            "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -249,3 +250,13 @@ class Context:
         buf = (_u64 * max(1, n.value))()
         self._chk(self.L.pra_scc_history(self.sccs[scc], buf, n.value, ctypes.byref(n)), "history")
         return [int(buf[i]) for i in range(n.value)]
+
+
+def execute_many(contexts, concurrency=4):
+    """Run independent contexts' fixed points concurrently (pra_execute_many);
+    returns the per-context status codes."""
+    n = len(contexts)
+    arr = (_vp * max(1, n))(*[c.c for c in contexts])
+    rc = (ctypes.c_int * max(1, n))()
+    lib().pra_execute_many(arr, n, int(concurrency), rc)
+    return [int(rc[i]) for i in range(n)]

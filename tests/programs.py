@@ -168,22 +168,30 @@ def layered_dag(seed, nvert=10000, nlayers=12, nedges=14000, lam=1.4):
 
 def random_graph(n, m, seed):
     """Uniform random directed graph with n vertices and m distinct edges, no
-    self-loops, uint64 vertex ids (the batched/served TC instances)."""
+    self-loops, uint64 vertex ids (the batched/served TC instances); edges are
+    kept in first-draw order."""
     rng = np.random.default_rng(seed)
-    seen = set()
-    out = []
-    while len(out) < m:
-        u = rng.integers(0, n, size=2 * (m - len(out)) + 16)
-        v = rng.integers(0, n, size=u.size)
-        for a, c in zip(u.tolist(), v.tolist()):
-            if a != c and (a, c) not in seen:
-                seen.add((a, c))
-                out.append((a, c))
-                if len(out) == m:
-                    break
+    k = m + m // 4 + 64
+    while True:
+        u = rng.integers(0, n, size=k)
+        v = rng.integers(0, n, size=k)
+        ok = u != v
+        u, v = u[ok], v[ok]
+        _, first = np.unique(u.astype(np.int64) * n + v, return_index=True)
+        if first.size >= m:
+            break
+        k *= 2
+    sel = np.sort(first)[:m]
     ids = distinct_u64(n, seed + 104729)
-    e = np.asarray(out, dtype=np.int64)
-    return with_rowid(ids[e])
+    return with_rowid(ids[np.stack([u[sel], v[sel]], axis=1)])
+
+
+def config4_instances(count=256, seed=4, nmin=1000, nmax=50000, density=1.3):
+    """Batched TC instances: vertex count log-uniform in [nmin, nmax], edge
+    count density x vertex count, one seed per instance."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count))).astype(np.int64)
+    return [(int(n), int(round(density * n)), seed * 100003 + i) for i, n in enumerate(ns)]
 
 
 # ------------------------------------------------------------ closed forms ---

@@ -147,3 +147,17 @@ def test_config2_grid150_properties():
     for bb in range(nb):
         s = np.sort(seq[bucket == bb])
         assert np.array_equal(s, np.arange(s.size, dtype=np.uint64))
+
+
+def test_config3_same_generation_full_size():
+    # 10,000-vertex 12-layer DAG, 14,000 uint64 edges, 8 virtual buckets:
+    # 1,392,703 sg tuples; every relation's rows (ids included) and deltas vs the oracle
+    edges = layered_dag(3)
+    print("[config3] same generation nb=8", flush=True)
+    o, ho, g, hg = both(8, lambda b: sg_program(b, edges))
+    for k in ["E", "I", "SG", "A", "B"]:
+        assert_same_rel(o, ho[k], g, hg[k], k)
+    for k in ["base", "filter", "rec"]:
+        assert o.history(ho[k]) == g.history(hg[k])
+    assert g.count(hg["SG"]) == 1392703
+    assert o.joined() == g.joined()
