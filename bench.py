@@ -47,8 +47,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["tc", "sg"], default="tc",
-                    help="tc: config 2 grid transitive closure (default); sg: config 3 same generation")
+    ap.add_argument("--workload", choices=["tc", "sg", "batch", "serve"], default="tc",
+                    help="tc: config 2 grid transitive closure (default); sg: config 3 same generation; "
+                         "batch: config 4 batched vs plain; serve: config 5 request stream")
+    ap.add_argument("--count", type=int, default=256, help="batch/serve: number of instances / requests")
+    ap.add_argument("--nmin", type=int, default=1000)
+    ap.add_argument("--nmax", type=int, default=50000)
+    ap.add_argument("--slots", type=int, default=4, help="batch/serve: concurrent instances")
+    ap.add_argument("--budget-gb", type=float, default=160.0, help="batch/serve: HBM admission budget")
+    ap.add_argument("--rates", default="50,200,1000", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--grid", type=int, default=150)
     ap.add_argument("--buckets", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=30)
@@ -62,6 +69,10 @@ def main():
 
     import pra_amd
     from programs import grid_graph, grid_tc_size, layered_dag, sg_program, tc_program
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     n, nb = args.grid, args.buckets
     if args.workload == "tc":
@@ -173,6 +184,128 @@ def main():
         "fixed_point_wall_s": total / args.steps,
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def est_bytes(n):
+    """device bytes a TC instance on n vertices with m = 1.3 n edges needs:
+    ~0.2 n^2 tuples x (24 B rows + 2 x 32 B table slots + scratch)"""
+    return int(0.2 * n * n * 120) + (64 << 20)
+
+
+def bench_batch(args):
+    """Config 4: independent TC instances, plain (one after another) vs batched
+    (pra_execute_many: up to --slots instances overlapping, admitted in waves
+    under the HBM budget); results compared per instance by device checksum."""
+    import pra_amd
+    from programs import config4_instances, random_graph, tc_program
+    inst = config4_instances(count=args.count, nmin=args.nmin, nmax=args.nmax)
+    graphs = [random_graph(n, m, sd) for n, m, sd in inst]
+    nb = args.buckets
+    log(f"batch: {len(graphs)} instances, n in [{min(n for n, _, _ in inst)}, {max(n for n, _, _ in inst)}]")
+
+    def make(e):
+        c = pra_amd.Context(nbuckets=nb, batch_size=1)
+        h = tc_program(c, e)
+        c.synchronize()
+        return c, h
+
+    # plain: one instance at a time
+    plain = []
+    t_plain = 0.0
+    for e in graphs:
+        c, (T, G, sfix, _) = make(e)
+        t0 = time.perf_counter()
+        c.execute()
+        c.synchronize()
+        t_plain += time.perf_counter() - t0
+        plain.append((c.count(T), c.joined(), c.checksum(T)))
+        c.close()
+    # batched: waves admitted by estimated bytes, largest first
+    order = sorted(range(len(graphs)), key=lambda i: -inst[i][0])
+    budget = int(args.budget_gb * (1 << 30))
+    batched = [None] * len(graphs)
+    t_batch = 0.0
+    pos = 0
+    while pos < len(order):
+        wave, used = [], 0
+        while pos < len(order) and (not wave or used + est_bytes(inst[order[pos]][0]) <= budget):
+            wave.append(order[pos])
+            used += est_bytes(inst[order[pos]][0])
+            pos += 1
+        ctxs = [make(graphs[i]) for i in wave]
+        t0 = time.perf_counter()
+        rc = pra_amd.execute_many([c for c, _ in ctxs], concurrency=args.slots)
+        for c, _ in ctxs:
+            c.synchronize()
+        t_batch += time.perf_counter() - t0
+        assert rc == [0] * len(ctxs), rc
+        for i, (c, (T, G, sfix, _)) in zip(wave, ctxs):
+            batched[i] = (c.count(T), c.joined(), c.checksum(T))
+            c.close()
+    exact = all(a == b for a, b in zip(plain, batched))
+    joined = sum(j for _, j, _ in plain)
+    log(f"plain {t_plain:.3f} s, batched {t_batch:.3f} s, bit-exact per instance: {exact}")
+    out = {
+        "metric": METRIC, "value": joined / t_batch, "unit": "joined tuples/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"tc_batch{len(graphs)}_n{args.nmin}-{args.nmax}", "buckets": nb,
+                   "slots": args.slots, "tuples": sum(t for t, _, _ in plain), "joined_tuples": joined},
+        "plain_value": joined / t_plain, "plain_s": t_plain, "batched_s": t_batch,
+        "batched_over_plain": t_plain / t_batch, "bit_exact_per_instance": exact,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Config 5: Poisson request streams of TC solves (sizes from the batched
+    distribution) into the serving loop; p50/p99 latency and aggregate joined
+    tuples/s per arrival rate; every result checked against a one-slot run."""
+    import numpy as np
+    import pra_amd
+    from programs import config4_instances, random_graph
+    inst = config4_instances(count=args.count, seed=5, nmin=args.nmin, nmax=args.nmax)
+    graphs = [random_graph(n, m, sd) for n, m, sd in inst]
+    budget = int(args.budget_gb * (1 << 30))
+    ref = None
+    rows = []
+    for rate in [float(r) for r in args.rates.split(",")]:
+        rng = np.random.default_rng(int(rate))
+        gaps = rng.exponential(1.0 / rate, size=len(graphs))
+        srv = pra_amd.Server(nbuckets=args.buckets, slots=args.slots, hbm_budget=budget)
+        t0 = time.perf_counter()
+        tickets = []
+        due = 0.0
+        for e, (n, _, _), gap in zip(graphs, inst, gaps):
+            due += gap
+            now = time.perf_counter() - t0
+            if due > now:
+                time.sleep(due - now)
+            tickets.append(srv.submit(e, est_bytes=est_bytes(n)))
+        res = [srv.wait(t) for t in tickets]
+        srv.close()
+        assert all(r["status"] == 0 for r in res)
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        joined = sum(r["joined"] for r in res)
+        got = [(r["tuples"], r["joined"]) for r in res]
+        if ref is None:
+            ref = got
+        rows.append({"rate": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3), "joined_tuples_per_s": joined / span,
+                     "results_equal_across_rates": got == ref})
+        log(f"rate {rate:7.1f}/s: p50 {rows[-1]['p50_ms']:.1f} ms p99 {rows[-1]['p99_ms']:.1f} ms "
+            f"{joined / span / 1e9:.3f} G joined/s")
+    out = {
+        "metric": METRIC, "value": max(r["joined_tuples_per_s"] for r in rows), "unit": "joined tuples/s",
+        "n_gpus": 1, "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"tc_serve{len(graphs)}_n{args.nmin}-{args.nmax}", "buckets": args.buckets,
+                   "slots": args.slots},
+        "serve": rows, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -84,6 +84,9 @@ int pra_rel_count(pra_rel* rel, int version, uint64_t* out);
 /* rows of a version, bucket-major and lexicographic inside each bucket (the
  * trie walk order of src/relation/shmap_relation_exp.cpp:77-112) */
 int pra_rel_download(pra_rel* rel, int version, uint64_t* rows, uint64_t cap, uint64_t* nrows);
+/* order-sensitive 64-bit checksum of a version's rows in download order:
+ * sum over i of hash(row i, i) (bit-exact comparison of large results) */
+int pra_rel_checksum(pra_rel* rel, int version, uint64_t* out);
 /* device pointer to the sorted rows of a version (valid until the next call
  * that changes the relation) */
 int pra_rel_device_rows(pra_rel* rel, int version, const uint64_t** rows, uint64_t* nrows);
@@ -122,6 +125,27 @@ int pra_execute(pra_ctx* ctx);
  * context has its own stream, so instances of uneven sizes overlap on the GPU.
  * rc[i] (optional) receives context i's status; returns the first failure. */
 int pra_execute_many(pra_ctx** ctxs, int n, int concurrency, int* rc);
+
+/* Serving (no counterpart in the reference): a request loop that solves
+ * transitive-closure requests -- the program of tests/transitive_closure.cpp
+ * on the given edge rows (m x 3 words: src, dst, row id) -- on one GPU.
+ * Requests are admitted in arrival order when one of `slots` workers is free
+ * and their estimated device bytes fit into `hbm_budget` beside the running
+ * ones (0 = no budget); each runs on its own context and stream, so admitted
+ * requests overlap on the device.  keep_rows != 0 keeps each result's rows on
+ * the host for pra_server_rows.  Times are seconds since server creation. */
+typedef struct pra_server pra_server;
+typedef struct {
+    int status;
+    uint64_t tuples, joined, iterations;
+    double t_submit, t_start, t_done;
+} pra_request_result;
+int pra_server_create(int device, int nbuckets, int slots, uint64_t hbm_budget, int keep_rows, pra_server** out);
+int pra_server_submit_tc(pra_server* srv, const uint64_t* edge_rows, uint64_t m, uint64_t est_bytes, uint64_t* ticket);
+int pra_server_wait(pra_server* srv, uint64_t ticket, pra_request_result* out);
+int pra_server_rows(pra_server* srv, uint64_t ticket, uint64_t* rows, uint64_t cap, uint64_t* nrows);
+/* drains the queue, joins the workers, frees the server */
+int pra_server_destroy(pra_server* srv);
 
 #ifdef __cplusplus
 }

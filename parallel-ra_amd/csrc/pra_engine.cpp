@@ -1303,6 +1303,21 @@ int pra_rel_device_rows(pra_rel* rel, int version, const uint64_t** rows, uint64
     API_END
 }
 
+int pra_rel_checksum(pra_rel* rel, int version, uint64_t* out)
+{
+    if (!rel || !out || !valid_ver(version)) return PRA_E_ARG;
+    API_BEGIN(rel->ctx)
+    pra_ctx* c = rel->ctx;
+    if (version != PRA_NEWT) ensure_sorted(c, rel, version);
+    Rows& r = rel->ver[version];
+    c->orand.reserve(32 * 8, c->st);
+    pra_launch_fill_u64(c->st, c->orand.as<u64>(), 1, 0);
+    pra_launch_rows_checksum(c->st, r.d(), r.n, rel->w, c->orand.as<u64>());
+    check_launch("rows_checksum");
+    *out = readback(c, c->orand.p, 1)[0];
+    API_END
+}
+
 int pra_rel_download(pra_rel* rel, int version, uint64_t* rows, uint64_t cap, uint64_t* nrows)
 {
     if (!rel || !nrows || !valid_ver(version)) return PRA_E_ARG;

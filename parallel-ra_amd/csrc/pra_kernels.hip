@@ -335,6 +335,24 @@ __global__ void k_key_bounds(const u64* keys, u64 n, int shift, u64 nb, u64* bnd
     bnd[s] = lo;
 }
 
+// order-sensitive checksum of rows: sum over i of hash(row i, i)
+__global__ void k_rows_checksum(const u64* rows, u64 n, int w, u64* out)
+{
+    __shared__ u64 red[PRA_WAVES];
+    u64 acc = 0;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 step = (u64)gridDim.x * blockDim.x;
+    for (; i < n; i += step) acc += pra_mix64(pra_keyhash(rows + i * w, w) ^ pra_mix64(i + 0x2545F4914F6CDD1DULL));
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 t = 0;
+        for (int v = 0; v < PRA_WAVES; v++) t += red[v];
+        atomicAdd((unsigned long long*)out, (unsigned long long)t);
+    }
+}
+
 // bstart[s] = first row whose bucket is >= s, for s in [0, nb]
 __global__ void k_bucket_bounds(const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart)
 {
@@ -1065,4 +1083,12 @@ void pra_launch_newt_reduce(hipStream_t st, const u64* part, const u32* part_his
     u32 threads = 8 * PRA_RADIX + PRA_PART_WORDS;
     hipLaunchKernelGGL(k_newt_reduce, dim3((threads + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0, st, part,
                        part_hist, nblocks, packed, stat, ghist);
+}
+
+void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out)
+{
+    u64 g = (n + PRA_BLOCK - 1) / PRA_BLOCK;
+    if (g > 1024) g = 1024;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_rows_checksum, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, rows, n, w, out);
 }

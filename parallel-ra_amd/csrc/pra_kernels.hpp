@@ -103,3 +103,4 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw
                                   u32 nblocks, u32* flags);
 void pra_launch_newt_reduce(hipStream_t st, const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat,
                             u32* ghist);
+void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);

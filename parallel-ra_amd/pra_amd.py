@@ -25,10 +25,11 @@ EXPORTS = (
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
     "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
-    "pra_rel_download", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
+    "pra_rel_download", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
     "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_history",
     "pra_execute", "pra_execute_many",
+    "pra_server_create", "pra_server_submit_tc", "pra_server_wait", "pra_server_rows", "pra_server_destroy",
 )
 
 _vp = ctypes.c_void_p
@@ -40,6 +41,12 @@ _lib = None
 
 class PraError(RuntimeError):
     pass
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("tuples", ctypes.c_uint64), ("joined", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint64), ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
 
 
 def build():
@@ -72,6 +79,7 @@ def lib():
             "pra_rel_count": [_vp, ctypes.c_int, _u64p],
             "pra_rel_download": [_vp, ctypes.c_int, _u64p, _u64, _u64p],
             "pra_rel_device_rows": [_vp, ctypes.c_int, ctypes.POINTER(_u64p), _u64p],
+            "pra_rel_checksum": [_vp, ctypes.c_int, _u64p],
             "pra_scc_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
             "pra_scc_destroy": [_vp],
             "pra_scc_add_relation": [_vp, _vp, ctypes.c_int],
@@ -84,6 +92,11 @@ def lib():
             "pra_scc_history": [_vp, _u64p, _u64, _u64p],
             "pra_execute": [_vp],
             "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],
+            "pra_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.POINTER(_vp)],
+            "pra_server_submit_tc": [_vp, _u64p, _u64, _u64, _u64p],
+            "pra_server_wait": [_vp, _u64, ctypes.POINTER(RequestResult)],
+            "pra_server_rows": [_vp, _u64, _u64p, _u64, _u64p],
+            "pra_server_destroy": [_vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -206,6 +219,11 @@ class Context:
         assert got.value == n
         return out[:n]
 
+    def checksum(self, rel, version=FULL):
+        v = _u64()
+        self._chk(self.L.pra_rel_checksum(self.rels[rel], version, ctypes.byref(v)), "checksum")
+        return int(v.value)
+
     # SCCs
     def scc_create(self, fixpoint, sid):
         h = _vp()
@@ -260,3 +278,52 @@ def execute_many(contexts, concurrency=4):
     rc = (ctypes.c_int * max(1, n))()
     lib().pra_execute_many(arr, n, int(concurrency), rc)
     return [int(rc[i]) for i in range(n)]
+
+
+class Server:
+    """Transitive-closure request server (pra_server_*): `slots` workers,
+    admission in arrival order under an HBM budget, overlapping requests."""
+
+    def __init__(self, nbuckets=1, slots=4, hbm_budget=0, keep_rows=False, device=0):
+        self.L = lib()
+        h = _vp()
+        rc = self.L.pra_server_create(int(device), int(nbuckets), int(slots), int(hbm_budget), int(bool(keep_rows)),
+                                      ctypes.byref(h))
+        if rc != 0:
+            raise PraError(f"pra_server_create failed ({rc})")
+        self.s = h
+        self._keep = {}
+
+    def submit(self, edges, est_bytes=0):
+        a = np.ascontiguousarray(np.asarray(edges, dtype=np.uint64).reshape(-1, 3))
+        t = _u64()
+        rc = self.L.pra_server_submit_tc(self.s, a.ctypes.data_as(_u64p), a.shape[0], int(est_bytes), ctypes.byref(t))
+        if rc != 0:
+            raise PraError(f"submit failed ({rc})")
+        return int(t.value)
+
+    def wait(self, ticket):
+        r = RequestResult()
+        rc = self.L.pra_server_wait(self.s, int(ticket), ctypes.byref(r))
+        if rc != 0:
+            raise PraError(f"wait failed ({rc})")
+        return {f: getattr(r, f) for f, _ in RequestResult._fields_}
+
+    def rows(self, ticket):
+        n = _u64()
+        if self.L.pra_server_rows(self.s, int(ticket), None, 0, ctypes.byref(n)) != 0:
+            raise PraError("rows unavailable")
+        out = np.zeros((max(1, n.value), 3), dtype=np.uint64)
+        self.L.pra_server_rows(self.s, int(ticket), out.ctypes.data_as(_u64p), n.value, ctypes.byref(n))
+        return out[:n.value]
+
+    def close(self):
+        if getattr(self, "s", None):
+            self.L.pra_server_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

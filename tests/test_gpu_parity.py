@@ -161,3 +161,33 @@ def test_config3_same_generation_full_size():
         assert o.history(ho[k]) == g.history(hg[k])
     assert g.count(hg["SG"]) == 1392703
     assert o.joined() == g.joined()
+
+
+def _mix64(x):
+    M = (1 << 64) - 1
+    x ^= x >> 33
+    x = (x * 0xff51afd7ed558ccd) & M
+    x ^= x >> 33
+    x = (x * 0xc4ceb9fe1a85ec53) & M
+    x ^= x >> 33
+    return x
+
+
+def _keyhash(k):
+    M = (1 << 64) - 1
+    h = 0x9E3779B97F4A7C15 ^ len(k)
+    for i, v in enumerate(k):
+        h = _mix64(h ^ ((v + 0x632BE59BD9B4E019 * (i + 1)) & M))
+    return h or 1
+
+
+def test_rows_checksum_definition():
+    g = gpu_ctx(3)
+    T, G, sfix, _ = tc_program(g, golden("g13236_edge_2_1_2"))
+    g.execute()
+    rows = g.rows(T).tolist()
+    M = (1 << 64) - 1
+    want = 0
+    for i, r in enumerate(rows):
+        want = (want + _mix64(_keyhash([int(x) for x in r]) ^ _mix64((i + 0x2545F4914F6CDD1D) & M))) & M
+    assert g.checksum(T) == want
