@@ -919,7 +919,6 @@ static void run_iteration(pra_scc* S)
         {
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
-            a.orand = c->stage2.as<u64>() + oi * SW + 8;
             a.cand = c->cand.as<u64>();
             a.keep = c->keep.as<u32>();
             a.dest = c->dest.as<u32>();
@@ -993,9 +992,11 @@ static void run_iteration(pra_scc* S)
             bnd_ok[oi] = 1;
             continue;
         }
+        // packed mode collects no column masks: a dictionary miss sorts every digit
+        const u64* masks = packed[oi] ? nullptr : &st2[oi * SW + 8];
         if (packed[oi] && c->dict_state == 1)
             c->dict_state = ++c->dict_rebuilds > 3 ? 2 : 0;
-        bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, &st2[oi * SW + 8], bnd);
+        bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, masks, bnd);
         if (!bnd_ok[oi] && nb == 1) {
             pra_launch_fill_u64(c->st, bnd, 1, 0);
             pra_launch_fill_u64(c->st, bnd + 1, 1, NT.n);

@@ -42,7 +42,6 @@ struct CopyExpandArgs {
 };
 
 struct NewtScatterArgs {
-    u64* orand;        // [0..15] OR, [16..31] AND of the written key columns
     const u64* dict;   // value -> rank dictionary (null: no packed sort key)
     u64 dict_cap;
     u64* keys;         // packed sort key per written row

@@ -191,3 +191,29 @@ def test_rows_checksum_definition():
     for i, r in enumerate(rows):
         want = (want + _mix64(_keyhash([int(x) for x in r]) ^ _mix64((i + 0x2545F4914F6CDD1D) & M))) & M
     assert g.checksum(T) == want
+
+
+def test_intern_ids_flowing_into_key_columns():
+    # the projection keeps the delta row's intern id as a key column, so new
+    # values appear every iteration: exercises the packed-sort-key fallback
+    edges = layered_dag(6, nvert=300, nlayers=6, nedges=420)
+
+    def build(b):
+        T = b.rel_create(2, 1, 257)
+        G = b.rel_create(2, 1, 256)
+        b.load_rows(G, edges, FULL)
+        fix = b.scc_create(True, 1)
+        b.scc_add_relation(fix, G, False)
+        b.scc_add_relation(fix, T, True)
+        b.add_join(fix, T, T, DELTA, G, FULL, [3, 2])
+        seed = b.scc_create(False, 2)
+        b.scc_add_relation(seed, G, False)
+        b.scc_add_relation(seed, T, True)
+        b.add_copy(seed, T, G, FULL, [1, 2])
+        b.add_dependency(seed, fix)
+        return T, fix
+
+    for nb in [1, 3]:
+        o, (To, fo), g, (Tg, fg) = both(nb, build)
+        assert_same_rel(o, To, g, Tg, "T")
+        assert o.history(fo) == g.history(fg)
