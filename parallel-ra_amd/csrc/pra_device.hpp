@@ -29,6 +29,7 @@ typedef uint32_t u32;
 #define PRA_MAXPASS 40
 #define PRA_PART_WORDS 40   // per-block summary words of the newt scatter
 #define PRA_LOOKBACK_SPIN_LIMIT (1u << 22)
+#define PRA_LOOKBACK 8       // predecessor tiles read per look-back step
 
 // decoupled look-back status words: [63:48] epoch, [47:46] flag, [45:0] count
 #define PRA_ST_AGG 1ULL

@@ -159,10 +159,50 @@ static_assert(PRA_BLOCK == PRA_RADIX, "look-back assigns one digit per thread");
 
 __device__ inline u64 st_pack(u32 epoch, u64 flag, u64 cnt) { return ((u64)epoch << 48) | (flag << 46) | cnt; }
 
-template <bool IDX>
+// Inclusive prefix of digit d over the tiles before `tile`: walks back over
+// the predecessors' published counts, PRA_LOOKBACK predecessors per step with
+// all their loads in flight, until an inclusive prefix is found.
+__device__ inline u64 pra_lookback(const u64* status, u32 tile, int d, u32 epoch, u32* flags)
+{
+    u64 excl = 0;
+    long long t = (long long)tile - 1;
+    u32 spins = 0;
+    while (t >= 0) {
+        u64 v[PRA_LOOKBACK];
+#pragma unroll
+        for (int j = 0; j < PRA_LOOKBACK; j++)
+            v[j] = t - j >= 0 ? __hip_atomic_load(status + (u64)(t - j) * PRA_RADIX + d, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : 0;
+        int used = 0;
+        bool done = false, stall = false;
+#pragma unroll
+        for (int j = 0; j < PRA_LOOKBACK; j++) {
+            if (done || stall || t - j < 0) continue;
+            u64 fl = (v[j] >> 46) & 3ULL;
+            if ((u32)(v[j] >> 48) != epoch || fl == 0) {
+                stall = true;
+                continue;
+            }
+            excl += v[j] & ((1ULL << 46) - 1);
+            used = j + 1;
+            if (fl == PRA_ST_INC) done = true;
+        }
+        if (done) break;
+        t -= used;
+        if (stall && ++spins > PRA_LOOKBACK_SPIN_LIMIT) {
+            atomicOr(flags, PRA_FLAG_SORT_STALL);
+            break;
+        }
+    }
+    return excl;
+}
+
+template <bool IDX, int ITEMS>
 __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs, u64* status,
                                u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout)
 {
+    constexpr u64 TILE = (u64)PRA_BLOCK * ITEMS;
     __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
     __shared__ u32 s_base[PRA_RADIX];
     __shared__ u32 s_tile;
@@ -172,10 +212,10 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
     const u32 tile = s_tile;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u64 lt = pra_lanemask_lt();
-    u64 base = (u64)tile * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
-    u32 dg[PRA_ITEMS], lc[PRA_ITEMS];
+    u64 base = (u64)tile * TILE + (u64)wave * (TILE / PRA_WAVES);
+    u32 dg[ITEMS], lc[ITEMS];
 #pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         u64 idx = base + (u64)k * 64 + lane;
         bool valid = idx < n;
         u32 d = valid ? pra_digit(in + idx * w, ds) : 0u;
@@ -204,23 +244,7 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
             __hip_atomic_store(my, st_pack(epoch, PRA_ST_INC, acc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(my, st_pack(epoch, PRA_ST_AGG, acc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            u32 t = tile - 1;
-            u32 spins = 0;
-            for (;;) {
-                u64 v = __hip_atomic_load(status + (u64)t * PRA_RADIX + d, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                u64 fl = (v >> 46) & 3ULL;
-                if ((u32)(v >> 48) != epoch || fl == 0) {
-                    if (++spins > PRA_LOOKBACK_SPIN_LIMIT) {
-                        atomicOr(flags, PRA_FLAG_SORT_STALL);
-                        break;
-                    }
-                    continue;
-                }
-                excl += v & ((1ULL << 46) - 1);
-                if (fl == PRA_ST_INC || t == 0) break;
-                t--;
-            }
+            excl = pra_lookback(status, tile, d, epoch, flags);
             __hip_atomic_store(my, st_pack(epoch, PRA_ST_INC, excl + acc), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -228,7 +252,7 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         u64 idx = base + (u64)k * 64 + lane;
         if (idx < n) {
             u64 pos = (u64)s_base[dg[k]] + wh[wave][dg[k]] + lc[k];
@@ -1018,16 +1042,19 @@ void pra_launch_sort_scan(hipStream_t st, const u32* ghist, int np, u32* gofs, u
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(PRA_BLOCK), 0, st, ghist, np, gofs, bnd, nb, n);
 }
 
+// rows are sorted in tiles of PRA_TILE; (key, index) pairs in tiles of 2 x PRA_TILE
 void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs,
                              u64* status, u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout)
 {
-    u32 nt = pra_tiles(n);
-    if (iout)
-        hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, gofs, status,
-                           tile_ctr, epoch, flags, iin, iout);
-    else
-        hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds, gofs, status,
-                           tile_ctr, epoch, flags, iin, iout);
+    if (iout) {
+        u32 nt = (u32)((n + 2 * PRA_TILE - 1) / (2 * PRA_TILE));
+        hipLaunchKernelGGL((k_sort_scatter<true, 2 * PRA_ITEMS>), dim3(nt ? nt : 1), dim3(PRA_BLOCK), 0, st, in, out, n,
+                           w, ds, gofs, status, tile_ctr, epoch, flags, iin, iout);
+    } else {
+        u32 nt = pra_tiles(n);
+        hipLaunchKernelGGL((k_sort_scatter<false, PRA_ITEMS>), dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds,
+                           gofs, status, tile_ctr, epoch, flags, iin, iout);
+    }
 }
 
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out)
