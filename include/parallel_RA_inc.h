@@ -437,9 +437,9 @@ class LIE
     mpi_comm mcomm;
     std::string app_name;
     u32 lie_relation_count = 0;
-    relation* lie_relations[64];
+    std::vector<relation*> lie_relations;
     u32 lie_sccs_count = 0;
-    RAM* lie_sccs[64];
+    std::vector<RAM*> lie_sccs;
     std::map<RAM*, std::set<RAM*>> taskgraph;
     std::vector<int> executed_scc_id;
     int loop_counter = 0;
@@ -478,8 +478,16 @@ public:
         if (ctx) pra_ctx_destroy(ctx);
     }
 
-    void add_relation(relation* rel) { lie_relations[lie_relation_count++] = rel; }
-    void add_scc(RAM* ra) { lie_sccs[lie_sccs_count++] = ra; }
+    void add_relation(relation* rel)
+    {
+        lie_relations.push_back(rel);
+        lie_relation_count++;
+    }
+    void add_scc(RAM* ra)
+    {
+        lie_sccs.push_back(ra);
+        lie_sccs_count++;
+    }
     void add_scc_dependance(RAM* src_task, RAM* destination_task) { taskgraph[src_task].insert(destination_task); }
     void set_comm(mpi_comm comm) { mcomm = comm; }
     void set_name(std::string name) { app_name = name; }
@@ -535,6 +543,17 @@ public:
             pra_scc_destroy(s);
             update_task_graph(task);
         }
+        /* PRA_DUMP_DIR: write every relation's full version as
+         * <dir>/<name>_full (+ .size), the shared-file layout of
+         * relation::parallel_IO (src/relation/balanced_hash_relation.cpp:109-235) */
+        const char* dump = getenv("PRA_DUMP_DIR");
+        if (dump && *dump)
+            for (u32 i = 0; i < lie_relation_count; i++) {
+                relation* r = lie_relations[i];
+                std::string name = r->get_debug_id().empty() ? "relation_" + std::to_string(i) : r->get_debug_id();
+                std::string path = std::string(dump) + "/" + name + "_full";
+                pra_facade::chk(ctx, pra_rel_write_bin(r->backend(), path.c_str(), PRA_FULL), "relation dump");
+            }
         return true;
     }
 

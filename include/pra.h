@@ -84,6 +84,11 @@ int pra_rel_count(pra_rel* rel, int version, uint64_t* out);
 /* rows of a version, bucket-major and lexicographic inside each bucket (the
  * trie walk order of src/relation/shmap_relation_exp.cpp:77-112) */
 int pra_rel_download(pra_rel* rel, int version, uint64_t* rows, uint64_t cap, uint64_t* nrows);
+/* write a version in the tsv_to_bin layout (<path> rows in download order,
+ * <path>.size "rows\ncols"); with one bucket per rank this is the shared-file
+ * dump of relation::parallel_IO (src/relation/balanced_hash_relation.cpp:109-235),
+ * which concatenates the ranks' trie-ordered rows in rank order */
+int pra_rel_write_bin(pra_rel* rel, const char* path, int version);
 /* order-sensitive 64-bit checksum of a version's rows in download order:
  * sum over i of hash(row i, i) (bit-exact comparison of large results) */
 int pra_rel_checksum(pra_rel* rel, int version, uint64_t* out);

@@ -208,7 +208,7 @@ static void umap_grow(umap *m)
 typedef struct {
     u64 *rows;
     u64 n, cap;
-    int sorted;
+    u64 nsorted;    /* rows[0, nsorted) are in lexicographic order */
 } rowvec;
 
 static void rv_push(rowvec *v, const u64 *row, int w)
@@ -219,13 +219,12 @@ static void rv_push(rowvec *v, const u64 *row, int w)
     }
     memcpy(v->rows + v->n * w, row, w * sizeof(u64));
     v->n++;
-    v->sorted = 0;
 }
 
 static void rv_free(rowvec *v)
 {
     free(v->rows);
-    v->rows = NULL; v->n = v->cap = 0; v->sorted = 1;
+    v->rows = NULL; v->n = v->cap = v->nsorted = 0;
 }
 
 /* Lexicographic row order over all columns: the order in which the nested
@@ -243,10 +242,27 @@ static int cmp_rows(const void *a, const void *b)
 
 static void rv_sort(rowvec *v, int w)
 {
-    if (v->sorted) return;
+    if (v->nsorted == v->n) return;
     g_cmp_w = w;
-    qsort(v->rows, v->n, w * sizeof(u64), cmp_rows);
-    v->sorted = 1;
+    u64 a = v->nsorted, n = v->n;
+    /* sort the rows appended since the last sort, then merge the two runs */
+    qsort(v->rows + a * w, n - a, w * sizeof(u64), cmp_rows);
+    if (a) {
+        /* merge from the back: the tail run moves to a temporary, the head
+         * run stays in place (rows of one version are distinct) */
+        u64 d = n - a;
+        u64 *t = (u64 *)xrealloc(NULL, d * w * sizeof(u64));
+        memcpy(t, v->rows + a * w, d * w * sizeof(u64));
+        u64 i = a, j = d, k = n;
+        while (j > 0) {
+            if (i > 0 && cmp_rows(v->rows + (i - 1) * w, t + (j - 1) * w) > 0)
+                memcpy(v->rows + (--k) * w, v->rows + (--i) * w, w * sizeof(u64));
+            else
+                memcpy(v->rows + (--k) * w, t + (--j) * w, w * sizeof(u64));
+        }
+        free(t);
+    }
+    v->nsorted = n;
 }
 
 /* first index whose first p columns are >= key */
@@ -345,7 +361,6 @@ static void ver_init(ora_ctx *c, relation_t *r, int v)
 {
     version *x = &r->v[v];
     x->b = (rowvec *)xcalloc(c->nb, sizeof(rowvec));
-    for (int i = 0; i < c->nb; i++) x->b[i].sorted = 1;
     tset_init(&x->rows, r->arity + 1);
     tset_init(&x->prefix, r->arity);
     x->count = 0;

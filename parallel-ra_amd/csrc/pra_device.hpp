@@ -25,6 +25,7 @@ typedef uint32_t u32;
 #define PRA_FLAG_TABLE_FULL 2u
 #define PRA_FLAG_INDEX_COLLISION 4u
 #define PRA_FLAG_SORT_STALL 8u
+#define PRA_FLAG_THRESHOLD 16u
 
 #define PRA_MAXPASS 40
 #define PRA_PART_WORDS 40   // per-block summary words of the newt scatter

@@ -21,6 +21,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -130,6 +131,8 @@ struct pra_rel {
     // join index (key -> first row, count) per version
     DBuf idx[2];
     u64 idx_cap[2] = {0, 0};
+    u64 idx_maxlen[2] = {0, 0};   // longest run of rows sharing a join key
+    u64 jhint = 0;                // candidates of the previous iteration producing this relation
     bool idx_valid[2] = {false, false};
     // packed sort keys of newt rows and the offsets of their 8 byte digits
     DBuf nkeys, kgofs;
@@ -203,7 +206,7 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, part, part_hist;
+    DBuf dict, ktmp, idxa, idxb, part, part_hist, posadj_dev, joined_dev, cpart, chist, rctr;
     u64 dict_cap = 0, dict_n = 0;
     int dict_bits = 0, dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -213,13 +216,15 @@ struct pra_ctx {
     size_t host_stage_cap = 0;
     u64* host_pos = nullptr;    // pinned
     size_t host_pos_cap = 0;
-    u64 joined = 0, iterations = 0;
+    u64 iterations = 0;
     std::string err;
     // profiling
     u32 prof_mask = 0;           // bit k: time kernel class k
     KStat kst[K_COUNT];
     struct Pending { int k; hipEvent_t a, b; double bytes; };
     std::vector<Pending> pending;
+    // (index into pending, bytes per candidate, output index) to fix up once counts are read back
+    std::vector<std::tuple<size_t, double, size_t>> per_candidate;
     std::vector<hipEvent_t> evpool;
 
     pra_ctx()
@@ -234,7 +239,7 @@ struct pra_ctx {
         seq.clear();
         for (DBuf* b : {&partials, &hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &part, &part_hist})
+                        &idxa, &idxb, &part, &part_hist, &posadj_dev, &joined_dev, &cpart, &chist, &rctr})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -330,6 +335,26 @@ struct Prof {
     ~Prof() { c->prof_end(k, a, bytes); }
 };
 
+// a timed launch whose byte count is (bytes per candidate) x (candidates of
+// output `oi`), known only after the iteration's readback
+struct ProfPerCand {
+    pra_ctx* c;
+    int k;
+    hipEvent_t a;
+    double per;
+    size_t oi;
+    ProfPerCand(pra_ctx* c_, int k_, double per_, size_t oi_) : c(c_), k(-1), a(nullptr), per(per_), oi(oi_)
+    {
+        k = c->prof_begin(k_, &a);
+    }
+    ~ProfPerCand()
+    {
+        if (k < 0) return;
+        c->prof_end(k, a, 0.0);
+        c->per_candidate.emplace_back(c->pending.size() - 1, per, oi);
+    }
+};
+
 // ----------------------------------------------------------------- scans ---
 
 static void scan_u64(pra_ctx* c, const u64* in, u64* out, u64 n, u64* total)
@@ -356,6 +381,9 @@ static void check_flags(u32 f)
         throw PraError(PRA_E_COLLISION, "64-bit key hash collision between distinct keys");
     if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
     if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort look-back did not complete");
+    if (f & PRA_FLAG_THRESHOLD)
+        throw PraError(PRA_E_UNSUPPORTED, "a source bucket produced more join outputs than the local-join "
+                                          "threshold; threshold rounds are not implemented on the device");
 }
 
 // Read back `words` u64 from device memory in one synchronization, together
@@ -621,6 +649,7 @@ static void ensure_index(pra_ctx* c, pra_rel* R, int v)
     u64 n = r.n;
     if (n == 0) {
         R->idx_cap[v] = 0;
+        R->idx_maxlen[v] = 0;
         R->idx_valid[v] = true;
         return;
     }
@@ -632,7 +661,15 @@ static void ensure_index(pra_ctx* c, pra_rel* R, int v)
     scan_u64(c, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), n, c->tmp64b.as<u64>() + n);
     pra_launch_group_compact(c->st, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), n, c->tmp64c.as<u64>());
     check_launch("group_compact");
-    u64 ng = readback(c, c->tmp64b.as<u64>() + n, 1)[0];
+    c->orand.reserve(32 * 8, c->st);
+    pra_launch_fill_u64(c->st, c->orand.as<u64>(), 1, 0);
+    pra_launch_group_maxlen(c->st, c->tmp64a.as<u64>(), c->tmp64b.as<u64>(), c->tmp64c.as<u64>(), n,
+                            c->orand.as<u64>());
+    check_launch("group_maxlen");
+    HIPCHK(hipMemcpyAsync(c->orand.as<u64>() + 1, c->tmp64b.as<u64>() + n, 8, hipMemcpyDeviceToDevice, c->st));
+    u64* hm = readback(c, c->orand.p, 2);
+    R->idx_maxlen[v] = hm[0];
+    u64 ng = hm[1];
     u64 cap = pow2_at_least(std::max<u64>(64, 2 * ng));
     R->idx[v].reserve(cap * 3 * 8, c->st);
     pra_launch_table_clear(c->st, R->idx[v].as<u64>(), cap, 3);
@@ -770,99 +807,112 @@ static void run_iteration(pra_scc* S)
         pra_launch_gather_srcoff(c->st, ra.off.as<u64>(), bs, (u64)nb, c->stage.as<u64>() + (size_t)r * (nb + 1));
         check_launch("gather_srcoff");
     }
-    std::vector<u64> srcoff((size_t)nra * (nb + 1));
-    {
-        u64* h = readback(c, c->stage.p, srcoff.size());
-        std::copy(h, h + srcoff.size(), srcoff.begin());
-    }
 
-    // 2. arrival order per output relation: source-major, RA-minor
-    //    (src/comm/all_to_allv_comm.cpp:64-142, src/RAM/RA_tasks.cpp:831-835)
+    // 2. output relations, their rule kinds, and candidate capacities: the
+    //    previous iteration's count with headroom, else an upper bound from the
+    //    outer sizes and the inner index's longest key run.  No host
+    //    synchronization: the arrival plan is computed on the device and an
+    //    output whose candidates exceed its capacity is re-run.
     std::vector<pra_rel*> outs;
     for (auto& ra : S->ras)
         if (std::find(outs.begin(), outs.end(), ra.out) == outs.end()) outs.push_back(ra.out);
-    std::vector<u64> totals(outs.size(), 0);
-    std::vector<u64> posadj((size_t)nra * nb);
-    for (size_t oi = 0; oi < outs.size(); oi++) {
-        u64 tot = 0;
-        for (int s = 0; s < nb; s++)
-            for (int r = 0; r < nra; r++) {
-                if (S->ras[r].out != outs[oi]) continue;
-                const u64* so = &srcoff[(size_t)r * (nb + 1)];
-                posadj[(size_t)r * nb + s] = tot - so[s];
-                u64 cnt = so[s + 1] - so[s];
-                if (S->ras[r].type == RA_JOIN && (long long)cnt > c->threshold)
-                    throw PraError(PRA_E_UNSUPPORTED,
-                                   "a source bucket produced more join outputs than the local-join threshold; "
-                                   "threshold rounds are not implemented on the device");
-                tot += cnt;
-            }
-        totals[oi] = tot;
-    }
-    for (int r = 0; r < nra; r++)
-        if (S->ras[r].type == RA_JOIN) c->joined += srcoff[(size_t)r * (nb + 1) + nb];
-
-    // 3+4. emit candidates and deduplicate, one output relation at a time
-    std::vector<int> out_intern(outs.size(), 1);
-    for (size_t oi = 0; oi < outs.size(); oi++) {
-        int type0 = -1;
+    const size_t no = outs.size();
+    std::vector<int> out_intern(no, 1);
+    std::vector<u64> ub(no, 0), cap(no, 0);
+    for (size_t oi = 0; oi < no; oi++) {
+        int type0 = -1, nr = 0;
         for (auto& ra : S->ras)
             if (ra.out == outs[oi]) {
+                nr++;
                 if (type0 < 0) type0 = is_intern_type(ra.type);
                 else if (type0 != (int)is_intern_type(ra.type))
                     throw PraError(PRA_E_UNSUPPORTED, "relation is the target of both acopy and join/copy rules");
+                ub[oi] += ra.type == RA_JOIN ? ra.n_outer * ra.inner->idx_maxlen[ra.inner_v] : ra.n_outer;
             }
+        if (nr > PRA_MAXRA) throw PraError(PRA_E_UNSUPPORTED, "too many rules produce one relation");
         out_intern[oi] = type0;
         pra_rel* R = outs[oi];
-        if (totals[oi]) ensure_member_table(c, R, type0 ? R->arity : R->w, totals[oi]);
+        cap[oi] = R->jhint ? std::max<u64>(2 * R->jhint, 65536) : std::min<u64>(ub[oi], 1ULL << 22);
+        cap[oi] = std::min(cap[oi], ub[oi]);
     }
-    // per output relation: [0] new-tuple count, [8..23] OR, [24..39] AND of the key columns
+    for (size_t i = 0; i < no; i++)
+        for (size_t j = i + 1; j < no; j++)
+            if (out_intern[i] && out_intern[j] && outs[i]->tag == outs[j]->tag)
+                throw PraError(PRA_E_UNSUPPORTED, "two interned relations of one SCC share an intern tag");
+
+    // per output relation: [0] new tuples, [1] packed-key OR, [2] AND, [3] dictionary miss,
+    // [4] candidates, [5] capacity overflow, [8..23] key-column OR, [24..39] AND
     const size_t SW = 40;
-    c->stage2.reserve(std::max<size_t>(1, outs.size()) * SW * 8, c->st);
+    c->stage2.reserve(std::max<size_t>(1, no) * SW * 8, c->st);
+    c->posadj_dev.reserve((size_t)std::max(1, nra) * nb * 8, c->st);
+    if (!c->joined_dev.p) {
+        c->joined_dev.reserve(8, c->st);
+        pra_launch_fill_u64(c->st, c->joined_dev.as<u64>(), 1, 0);
+    }
     // packed sort keys need the value dictionary (rebuilt after a miss, given up
     // after repeated misses)
     bool any_intern = false;
-    for (size_t oi = 0; oi < outs.size(); oi++) any_intern = any_intern || (out_intern[oi] && totals[oi]);
+    for (size_t oi = 0; oi < no; oi++) any_intern = any_intern || (out_intern[oi] && ub[oi]);
     if (any_intern && c->dict_state == 0) build_dictionary(c);
     const int bucket_bits = nb > 1 ? ceil_log2((u64)nb) : 0;
-    std::vector<char> packed(outs.size(), 0);
-    for (size_t oi = 0; oi < outs.size(); oi++)
-        packed[oi] = c->dict_state == 1 && out_intern[oi] && totals[oi] &&
+    std::vector<char> packed(no, 0);
+    for (size_t oi = 0; oi < no; oi++)
+        packed[oi] = c->dict_state == 1 && out_intern[oi] && ub[oi] &&
                      bucket_bits + outs[oi]->arity * c->dict_bits <= 64;
-    u64* hpos = c->host_posadj((size_t)nra * nb);
-    std::copy(posadj.begin(), posadj.end(), hpos);
-    for (size_t oi = 0; oi < outs.size(); oi++) {
+
+    // 3+4. per output: arrival plan, emit candidates, deduplicate, partition
+    //      and intern, summaries for the sort
+    auto run_output = [&](size_t oi) {
         pra_rel* R = outs[oi];
-        u64 J = totals[oi];
+        const u64 C = cap[oi];
         const bool intern = out_intern[oi] == 1;
         const int cw = intern ? R->arity : R->arity + 1;
         const int kw = cw;
         Rows& NT = R->ver[PRA_NEWT];
-        NT.n = 0;
-        if (J == 0) continue;
-        c->cand.reserve(J * cw * 8, c->st);
-        c->dest.reserve(J * 4, c->st);
-        c->slot_of.reserve(J * 8, c->st);
-        c->keep.reserve(J * 4, c->st);
+        u64* st2o = c->stage2.as<u64>() + oi * SW;
+        u64* ctl = st2o + 4;
+        ensure_member_table(c, R, kw, C);
+        c->cand.reserve(C * cw * 8, c->st);
+        c->dest.reserve(C * 4, c->st);
+        c->slot_of.reserve(C * 8, c->st);
+        c->keep.reserve(C * 4, c->st);
+        NT.buf.reserve(C * R->w * 8, c->st);
+        {
+            PlanArgs pa;
+            memset(&pa, 0, sizeof(pa));
+            pa.srcoff = c->stage.as<u64>();
+            pa.posadj = c->posadj_dev.as<u64>();
+            pa.ctl = ctl;
+            pa.joined = c->joined_dev.as<u64>();
+            pa.flags = c->dflags();
+            pa.cap = C;
+            pa.thresh = c->threshold;
+            pa.nb = nb;
+            for (int r = 0; r < nra; r++)
+                if (S->ras[r].out == R) {
+                    pa.ras[pa.nr] = r;
+                    pa.is_join[pa.nr] = S->ras[r].type == RA_JOIN;
+                    pa.nr++;
+                }
+            pra_launch_plan(c->st, pa);
+            check_launch("plan");
+        }
         for (int r = 0; r < nra; r++) {
             RADesc& ra = S->ras[r];
             if (ra.out != R) continue;
-            ra.posadj.reserve((size_t)nb * 8, c->st);
-            HIPCHK(hipMemcpyAsync(ra.posadj.p, hpos + (size_t)r * nb, (size_t)nb * 8, hipMemcpyHostToDevice, c->st));
             Rows& O = ra.outer->ver[ra.outer_v];
-            u64 tot_r = srcoff[(size_t)r * (nb + 1) + nb];
             if (ra.type == RA_JOIN) {
                 JoinExpandArgs a;
                 memset(&a, 0, sizeof(a));
+                a.ctl = ctl;
                 a.outer = O.d();
                 a.inner = ra.inner->ver[ra.inner_v].d();
                 a.off = ra.off.as<u64>();
                 a.istart = ra.istart.as<u64>();
-                a.posadj = ra.posadj.as<u64>();
+                a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
                 a.cand = c->cand.as<u64>();
                 a.dest = c->dest.as<u32>();
                 a.n_outer = ra.n_outer;
-                a.total = tot_r;
                 a.nb = (u64)nb;
                 a.wo = ra.outer->w;
                 a.wi = ra.inner->w;
@@ -871,15 +921,16 @@ static void run_iteration(pra_scc* S)
                 a.cw = cw;
                 a.out_jcc = R->jcc;
                 for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
-                Prof p(c, K_JOIN_EXPAND, (double)tot_r * (8.0 * ra.inner->w + 8.0 * cw));
-                pra_launch_join_expand(c->st, a);
+                ProfPerCand p(c, K_JOIN_EXPAND, 8.0 * ra.inner->w + 8.0 * cw, oi);
+                pra_launch_join_expand(c->st, a, std::min<u64>(C, ub[oi]));
                 check_launch("join_expand");
             } else {
                 CopyExpandArgs a;
                 memset(&a, 0, sizeof(a));
+                a.ctl = ctl;
                 a.rows = O.d();
                 a.off = ra.off.as<u64>();
-                a.posadj = ra.posadj.as<u64>();
+                a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
                 a.cand = c->cand.as<u64>();
                 a.dest = c->dest.as<u32>();
                 a.n = ra.n_outer;
@@ -889,7 +940,7 @@ static void run_iteration(pra_scc* S)
                 a.cw = cw;
                 a.out_jcc = R->jcc;
                 for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
-                Prof p(c, K_COPY_EXPAND, (double)ra.n_outer * 8.0 * ra.outer->w + (double)tot_r * 8.0 * cw);
+                Prof p(c, K_COPY_EXPAND, (double)ra.n_outer * 8.0 * (ra.outer->w + cw));
                 pra_launch_copy_expand(c->st, a);
                 check_launch("copy_expand");
             }
@@ -897,45 +948,44 @@ static void run_iteration(pra_scc* S)
         u64* tab = R->mt.as<u64>();
         const u64 epoch = ++c->dedup_epoch;
         {
-            Prof p(c, K_DEDUP_CLAIM, (double)J * 8.0 * (kw + 1));
-            pra_launch_dedup_claim2(c->st, c->cand.as<u64>(), J, cw, kw, tab, R->mt_cap, c->slot_of.as<u64>(), epoch,
-                                    c->dflags());
+            ProfPerCand p(c, K_DEDUP_CLAIM, 8.0 * (kw + 1), oi);
+            pra_launch_dedup_claim2(c->st, c->cand.as<u64>(), ctl, C, cw, kw, tab, R->mt_cap, c->slot_of.as<u64>(),
+                                    epoch, c->dflags());
             check_launch("dedup_claim");
         }
-        u32 nblocks = pra_tiles(J);
+        u32 nblocks = pra_tiles(C);
         c->hist.reserve((size_t)nb * nblocks * 4, c->st);
         c->offs.reserve((size_t)nb * nblocks * 4 + 8, c->st);
         {
-            Prof p(c, K_DEDUP_VERIFY, (double)J * (8.0 * kw + 16.0));
-            pra_launch_dedup_decide_hist(c->st, c->cand.as<u64>(), J, cw, kw, tab, c->slot_of.as<u64>(), epoch,
+            ProfPerCand p(c, K_DEDUP_VERIFY, 8.0 * kw + 16.0, oi);
+            pra_launch_dedup_decide_hist(c->st, c->cand.as<u64>(), ctl, cw, kw, tab, c->slot_of.as<u64>(), epoch,
                                          c->keep.as<u32>(), c->dest.as<u32>(), (u32)nb, c->hist.as<u32>(), nblocks,
                                          c->dflags());
             check_launch("dedup_decide_hist");
         }
         u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
         scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
-        NT.buf.reserve(J * R->w * 8, c->st);
         DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
+        c->part.reserve((size_t)nblocks * PRA_PART_WORDS * 8, c->st);
         {
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
+            a.ctl = ctl;
             a.cand = c->cand.as<u64>();
             a.keep = c->keep.as<u32>();
             a.dest = c->dest.as<u32>();
             a.offs = c->offs.as<u32>();
             a.seqbase = sq ? sq->as<u64>() : nullptr;
             a.newt = NT.d();
-            a.n = J;
             a.nblocks = nblocks;
             a.nb = (u32)nb;
             a.cw = cw;
             a.wn = R->w;
             a.intern = intern ? 1 : 0;
             a.tag = R->tag;
-            c->part.reserve((size_t)nblocks * PRA_PART_WORDS * 8, c->st);
             a.part = c->part.as<u64>();
             if (packed[oi]) {
-                R->nkeys.reserve(J * 8, c->st);
+                R->nkeys.reserve(C * 8, c->st);
                 c->part_hist.reserve((size_t)nblocks * 8 * PRA_RADIX * 4, c->st);
                 a.dict = c->dict.as<u64>();
                 a.dict_cap = c->dict_cap;
@@ -944,13 +994,29 @@ static void run_iteration(pra_scc* S)
                 a.rank_bits = c->dict_bits;
                 a.bucket_shift = R->arity * c->dict_bits;
             }
-            Prof p(c, K_DEDUP_SCATTER, (double)J * (8.0 * cw + 8.0));
+            ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
         }
         c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
-        pra_launch_newt_reduce(c->st, c->part.as<u64>(), c->part_hist.as<u32>(), nblocks, packed[oi] ? 1 : 0,
-                               c->stage2.as<u64>() + oi * SW, c->ghist.as<u32>());
+        {
+            const u32 nch = pra_newt_reduce_chunks(nblocks);
+            c->cpart.reserve((size_t)nch * PRA_PART_WORDS * 8, c->st);
+            if (packed[oi]) c->chist.reserve((size_t)nch * 8 * PRA_RADIX * 4, c->st);
+            NewtReduceArgs a;
+            memset(&a, 0, sizeof(a));
+            a.ctl = ctl;
+            a.part = c->part.as<u64>();
+            a.part_hist = packed[oi] ? c->part_hist.as<u32>() : nullptr;
+            a.cpart = c->cpart.as<u64>();
+            a.chist = packed[oi] ? c->chist.as<u32>() : nullptr;
+            a.ctr = c->rctr.as<u32>();
+            a.stat = st2o;
+            a.ghist = c->ghist.as<u32>();
+            a.nblocks = nblocks;
+            a.packed = packed[oi] ? 1 : 0;
+            pra_launch_newt_reduce(c->st, a);
+        }
         check_launch("newt_reduce");
         if (packed[oi]) {
             R->kgofs.reserve(8 * PRA_RADIX * 4, c->st);
@@ -958,24 +1024,42 @@ static void run_iteration(pra_scc* S)
             check_launch("key_digit_offsets");
         }
         pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
-                              c->stage2.as<u64>() + oi * SW, intern ? 1 : 0);
+                              st2o, intern ? 1 : 0, ctl);
         check_launch("seq_update");
-    }
+    };
+    for (size_t oi = 0; oi < no; oi++)
+        if (cap[oi]) run_output(oi);
 
-    // 5. new-tuple counts, key-column masks and error flags: the per-iteration
-    //    host readback
-    size_t no = outs.size();
-    std::vector<u64> st2(no * SW);
-    {
+    // 5. new-tuple counts, sort-key summaries and error flags: the
+    //    per-iteration host readback (repeated once for outputs whose
+    //    candidates overflowed their capacity, now sized exactly)
+    std::vector<u64> st2(no * SW, 0);
+    for (int pass = 0; pass < 2; pass++) {
         u64* h = readback(c, c->stage2.p, no * SW);
-        std::copy(h, h + no * SW, st2.begin());
+        std::vector<size_t> redo;
         for (size_t oi = 0; oi < no; oi++) {
-            pra_rel* R = outs[oi];
-            u64 cnt = totals[oi] ? st2[oi * SW] : 0;
-            R->ver[PRA_NEWT].n = cnt;
-            R->ver[PRA_NEWT].sorted = false;
-            R->mt_entries += cnt;
+            if (!cap[oi]) continue;
+            std::copy(h + oi * SW, h + (oi + 1) * SW, st2.begin() + oi * SW);
+            if (st2[oi * SW + 5]) redo.push_back(oi);
         }
+        if (redo.empty()) break;
+        if (pass == 1) throw PraError(PRA_E_HIP, "candidate capacity overflow after resizing");
+        for (size_t oi : redo) {
+            cap[oi] = st2[oi * SW + 4];
+            run_output(oi);
+        }
+    }
+    for (auto& pc : c->per_candidate)
+        if (std::get<0>(pc) < c->pending.size())
+            c->pending[std::get<0>(pc)].bytes = std::get<1>(pc) * (double)st2[std::get<2>(pc) * SW + 4];
+    c->per_candidate.clear();
+    for (size_t oi = 0; oi < no; oi++) {
+        pra_rel* R = outs[oi];
+        u64 cnt = cap[oi] ? st2[oi * SW] : 0;
+        R->jhint = cap[oi] ? st2[oi * SW + 4] : 0;
+        R->ver[PRA_NEWT].n = cnt;
+        R->ver[PRA_NEWT].sorted = false;
+        R->mt_entries += cnt;
     }
 
     // 6. sort newt into trie order, then delta -> full, newt -> delta
@@ -1133,6 +1217,8 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->budget = hbm_budget;
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
+        c->rctr.reserve(64, c->st);
+        pra_launch_fill_u64(c->st, c->rctr.as<u64>(), 8, 0);
         check_launch("init");
         HIPCHK(hipStreamSynchronize(c->st));
     } catch (const PraError& e) {
@@ -1174,8 +1260,9 @@ int pra_ctx_set_join_threshold(pra_ctx* ctx, int64_t t)
 int pra_ctx_joined_tuples(pra_ctx* ctx, uint64_t* out)
 {
     if (!ctx || !out) return PRA_E_ARG;
-    *out = ctx->joined;
-    return PRA_OK;
+    API_BEGIN(ctx)
+    *out = ctx->joined_dev.p ? readback(ctx, ctx->joined_dev.p, 1)[0] : 0;
+    API_END
 }
 
 int pra_ctx_iterations(pra_ctx* ctx, uint64_t* out)
@@ -1331,6 +1418,29 @@ int pra_rel_download(pra_rel* rel, int version, uint64_t* rows, uint64_t cap, ui
     if (m && rows) HIPCHK(hipMemcpyAsync(rows, r.d(), m * rel->w * 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     API_END
+}
+
+int pra_rel_write_bin(pra_rel* rel, const char* path, int version)
+{
+    if (!rel || !path || !valid_ver(version)) return PRA_E_ARG;
+    pra_ctx* c = rel->ctx;
+    uint64_t n = 0, got = 0;
+    int rc = pra_rel_count(rel, version, &n);
+    if (rc != PRA_OK) return rc;
+    std::vector<u64> buf((size_t)n * rel->w);
+    rc = pra_rel_download(rel, version, buf.data(), n, &got);
+    if (rc != PRA_OK) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(c, PRA_E_IO, std::string("cannot create ") + path);
+    size_t wrote = n ? fwrite(buf.data(), 8, buf.size(), f) : 0;
+    int closed = fclose(f);
+    if (wrote != buf.size() || closed != 0) return fail(c, PRA_E_IO, std::string("short write of ") + path);
+    std::string sp = std::string(path) + ".size";
+    FILE* m = fopen(sp.c_str(), "w");
+    if (!m) return fail(c, PRA_E_IO, "cannot create " + sp);
+    fprintf(m, "%llu\n%d", (unsigned long long)n, rel->w);
+    if (fclose(m) != 0) return fail(c, PRA_E_IO, "short write of " + sp);
+    return PRA_OK;
 }
 
 int pra_scc_create(pra_ctx* ctx, int fixpoint, int id, pra_scc** out)

@@ -429,6 +429,15 @@ __global__ void k_group_compact(const u64* flags, const u64* gpos, u64 n, u64* g
     if (flags[i]) gstart[gpos[i]] = i;
 }
 
+// longest group (largest number of rows sharing a key) of a sorted version
+__global__ void k_group_maxlen(const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    u64 len = gstart[gpos[i] + 1] - i;
+    atomicMax((unsigned long long*)maxlen, (unsigned long long)len);
+}
+
 // index slots: [hash, start row, row count]
 __global__ void k_index_insert(const u64* rows, int w, int jcc, const u64* gstart, u64 ng, u64* table, u64 cap,
                                u32* flags)
@@ -499,34 +508,39 @@ __device__ inline u64 pra_row_of(const u64* off, u64 lo, u64 hi, u64 x)
 
 __global__ void k_join_expand(JoinExpandArgs a)
 {
+    if (a.ctl[1]) return;
+    const u64 total = a.off[a.n_outer];
     __shared__ u64 s_lo, s_hi;
-    const u64 o0 = (u64)blockIdx.x * blockDim.x;
-    if (threadIdx.x == 0) {
-        u64 last = o0 + blockDim.x - 1 < a.total ? o0 + blockDim.x - 1 : a.total - 1;
-        s_lo = pra_row_of(a.off, 0, a.n_outer, o0);
-        s_hi = pra_row_of(a.off, s_lo, a.n_outer, last);
+    const u64 step = (u64)gridDim.x * blockDim.x;
+    for (u64 o0 = (u64)blockIdx.x * blockDim.x; o0 < total; o0 += step) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u64 last = o0 + blockDim.x - 1 < total ? o0 + blockDim.x - 1 : total - 1;
+            s_lo = pra_row_of(a.off, 0, a.n_outer, o0);
+            s_hi = pra_row_of(a.off, s_lo, a.n_outer, last);
+        }
+        __syncthreads();
+        u64 o = o0 + threadIdx.x;
+        if (o >= total) continue;
+        u64 i = pra_row_of(a.off, s_lo, s_hi, o);
+        const u64* d = a.outer + i * a.wo;
+        const u64* f = a.inner + (a.istart[i] + (o - a.off[i])) * a.wi;
+        u64 s = a.nb == 1 ? 0 : pra_fnv(d, a.jcc) % a.nb;
+        u64 pos = a.posadj[s] + o;
+        u64* dst = a.cand + pos * a.cw;
+        u64 tmp[PRA_MAXW];
+        for (int c = 0; c < a.cw; c++) {
+            int m = a.map[c];
+            u64 v;
+            if (a.left)
+                v = m < a.wo ? d[m] : f[a.jcc + (m - a.wo)];
+            else
+                v = m < a.wi ? f[m] : d[a.jcc + (m - a.wi)];
+            tmp[c] = v;
+            dst[c] = v;
+        }
+        a.dest[pos] = a.nb == 1 ? 0u : (u32)(pra_fnv(tmp, a.out_jcc) % a.nb);
     }
-    __syncthreads();
-    u64 o = o0 + threadIdx.x;
-    if (o >= a.total) return;
-    u64 i = pra_row_of(a.off, s_lo, s_hi, o);
-    const u64* d = a.outer + i * a.wo;
-    const u64* f = a.inner + (a.istart[i] + (o - a.off[i])) * a.wi;
-    u64 s = a.nb == 1 ? 0 : pra_fnv(d, a.jcc) % a.nb;
-    u64 pos = a.posadj[s] + o;
-    u64* dst = a.cand + pos * a.cw;
-    u64 tmp[PRA_MAXW];
-    for (int c = 0; c < a.cw; c++) {
-        int m = a.map[c];
-        u64 v;
-        if (a.left)
-            v = m < a.wo ? d[m] : f[a.jcc + (m - a.wo)];
-        else
-            v = m < a.wi ? f[m] : d[a.jcc + (m - a.wi)];
-        tmp[c] = v;
-        dst[c] = v;
-    }
-    a.dest[pos] = a.nb == 1 ? 0u : (u32)(pra_fnv(tmp, a.out_jcc) % a.nb);
 }
 
 __device__ inline bool pra_filter(int op, u64 x, u64 y)
@@ -553,6 +567,7 @@ __global__ void k_copy_count(const u64* rows, u64 n, int w, int op, u64* cnt)
 
 __global__ void k_copy_expand(CopyExpandArgs a)
 {
+    if (a.ctl[1]) return;
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     if (a.off[i + 1] == a.off[i]) return;
@@ -575,6 +590,31 @@ __global__ void k_gather_srcoff(const u64* off, const u64* bstart, u64 nb, u64* 
     if (s <= nb) srcoff[s] = off[bstart[s]];
 }
 
+// Arrival plan of one output relation, computed on the device: for source
+// bucket s and the RAs producing the relation (in RA order), the position of
+// their outputs in the arrival order source-major, RA-minor
+// (src/comm/all_to_allv_comm.cpp:64-142), the candidate count, whether it
+// exceeds the allocated capacity, the join pairs, and the local-join
+// threshold check of src/RA/parallel_join.cpp:196-201.
+__global__ void k_plan(PlanArgs a)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    u64 tot = 0;
+    for (int s = 0; s < a.nb; s++)
+        for (int j = 0; j < a.nr; j++) {
+            const u64* so = a.srcoff + (u64)a.ras[j] * (a.nb + 1);
+            a.posadj[(u64)a.ras[j] * a.nb + s] = tot - so[s];
+            u64 cnt = so[s + 1] - so[s];
+            if (a.is_join[j] && (long long)cnt > a.thresh) atomicOr(a.flags, PRA_FLAG_THRESHOLD);
+            tot += cnt;
+        }
+    a.ctl[0] = tot;
+    a.ctl[1] = tot > a.cap ? 1 : 0;
+    if (tot <= a.cap)
+        for (int j = 0; j < a.nr; j++)
+            if (a.is_join[j]) *a.joined += a.srcoff[(u64)a.ras[j] * (a.nb + 1) + a.nb];
+}
+
 // ---------------------------------------------------- dedup / newt ---------
 
 
@@ -582,9 +622,11 @@ __global__ void k_gather_srcoff(const u64* off, const u64* bstart, u64 nb, u64* 
 // stores the key; every candidate then lowers the slot's arrival mark to
 // (epoch << 40 | p+1), so the earliest arrival of this iteration wins, while a
 // mark from an earlier iteration (key already in full or delta) always wins.
-__global__ void k_dedup_claim2(const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap, u64* slot_of, u64 epoch,
-                               u32* flags)
+__global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, u64* table, u64 cap, u64* slot_of,
+                               u64 epoch, u32* flags)
 {
+    if (ctl[1]) return;
+    const u64 n = ctl[0];
     u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const u64* key = cand + p * cw;
@@ -618,9 +660,12 @@ __global__ void k_dedup_claim2(const u64* cand, u64 n, int cw, int kw, u64* tabl
 // Keep the winners, verify every key against its slot (a 64-bit hash
 // collision between distinct keys is reported, never merged) and count kept
 // candidates per destination bucket per tile.
-__global__ void k_dedup_decide_hist(const u64* cand, u64 n, int cw, int kw, const u64* table, const u64* slot_of,
-                                    u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist, u32 nblocks, u32* flags)
+__global__ void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
+                                    const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
+                                    u32 nblocks, u32* flags)
 {
+    if (ctl[1]) return;
+    const u64 n = ctl[0];
     __shared__ u32 h[PRA_RADIX];
     for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
     __syncthreads();
@@ -655,6 +700,10 @@ __global__ void k_dedup_decide_hist(const u64* cand, u64 n, int cw, int kw, cons
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
 __global__ void k_dedup_scatter(NewtScatterArgs a)
 {
+    if (a.ctl[1]) return;
+    a.n = a.ctl[0];
+    // blocks past the candidates hold nothing; the summary reduce skips them
+    if ((u64)blockIdx.x * PRA_TILE >= a.n) return;
     __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
     for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
     __syncthreads();
@@ -774,42 +823,67 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
             a.part_hist[(u64)blockIdx.x * 8 * PRA_RADIX + i] = khist[i];
 }
 
-// Combine the per-block summaries of k_dedup_scatter: stat[1..3] = packed key
-// OR / AND / miss, stat[8..39] = key column OR / AND; ghist = byte-digit
-// histograms of the packed keys.
-__global__ void k_newt_reduce(const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat, u32* ghist)
+// Block summaries of the scatter, reduced in two levels so that no thread
+// walks all blocks: blockIdx.y picks the quantity (y < 8: byte-digit
+// histogram y of the packed key, y == 8: the summary words), blockIdx.x an
+// equal share of the scatter blocks that held candidates (the first
+// ceil(n / PRA_TILE)).  Each share's result goes to `cpart` / `chist`; the
+// last block of each y to finish (counted in ctr[y], which it resets to zero
+// for the next launch) adds them up.  The fences order the share results
+// before the counter increment and the final reads after it.
+__global__ void k_newt_reduce(NewtReduceArgs a)
 {
-    u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 8 * PRA_RADIX) {
-        if (!packed) return;
+    if (a.ctl[1]) return;
+    const u32 y = blockIdx.y, t = threadIdx.x;
+    if (y < 8 && !a.packed) return;
+    const u64 n = a.ctl[0];
+    const u32 nact = (u32)min((n + PRA_TILE - 1) / PRA_TILE, (u64)a.nblocks);
+    const u32 share = (nact + gridDim.x - 1) / gridDim.x;
+    const u32 b0 = min(blockIdx.x * share, nact), b1 = min(b0 + share, nact);
+    const bool word = y == 8 && t < PRA_PART_WORDS && !(t >= 3 && t < 8);
+    const bool is_and = t == 1 || (t >= 24 && t < 40);
+    if (y < 8) {
         u32 acc = 0;
-        for (u32 b = 0; b < nblocks; b++) acc += part_hist[(u64)b * 8 * PRA_RADIX + t];
-        ghist[t] = acc;
-        return;
+        for (u32 b = b0; b < b1; b++) acc += a.part_hist[((u64)b * 8 + y) * PRA_RADIX + t];
+        a.chist[((u64)blockIdx.x * 8 + y) * PRA_RADIX + t] = acc;
+    } else if (word) {
+        u64 v = is_and ? ~0ULL : 0;
+        for (u32 b = b0; b < b1; b++) {
+            u64 x = a.part[(u64)b * PRA_PART_WORDS + t];
+            v = is_and ? (v & x) : (v | x);
+        }
+        a.cpart[(u64)blockIdx.x * PRA_PART_WORDS + t] = v;
     }
-    u32 j = t - 8 * PRA_RADIX;
-    if (j >= PRA_PART_WORDS || (j >= 3 && j < 8)) return;
-    bool is_and = j == 1 || (j >= 24 && j < 40);
-    u64 v = is_and ? ~0ULL : 0;
-    for (u32 b = 0; b < nblocks; b++) {
-        u64 x = part[(u64)b * PRA_PART_WORDS + j];
-        v = is_and ? (v & x) : (v | x);
+    __shared__ u32 last;
+    __threadfence();
+    __syncthreads();
+    if (t == 0) last = atomicAdd(a.ctr + y, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (y < 8) {
+        u32 acc = 0;
+        for (u32 c = 0; c < gridDim.x; c++) acc += a.chist[((u64)c * 8 + y) * PRA_RADIX + t];
+        a.ghist[y * PRA_RADIX + t] = acc;
+    } else if (word) {
+        u64 v = is_and ? ~0ULL : 0;
+        for (u32 c = 0; c < gridDim.x; c++) {
+            u64 x = a.cpart[(u64)c * PRA_PART_WORDS + t];
+            v = is_and ? (v & x) : (v | x);
+        }
+        // summary word 0/1/2 (key OR / AND / dictionary miss) -> stat 1/2/3,
+        // column masks keep their index
+        a.stat[t < 3 ? t + 1 : t] = v;
     }
-    if (j == 0)
-        stat[1] = v;
-    else if (j == 1)
-        stat[2] = v;
-    else if (j == 2)
-        stat[3] = v;
-    else
-        stat[j] = v;
+    if (t == 0) a.ctr[y] = 0;
 }
 
 // seq counters advance by the number of new tuples per bucket; the kept
 // total is reported to the host
 __global__ void k_seq_update(u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total, u64* count_out,
-                             int intern)
+                             int intern, const u64* ctl)
 {
+    if (ctl[1]) return;
     u32 d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d == 0) *count_out = *total;
     if (d >= nb || !intern) return;
@@ -970,10 +1044,17 @@ void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int 
                        wi, cnt, istart);
 }
 
-void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a)
+void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap)
 {
-    if (!a.total) return;
-    hipLaunchKernelGGL(k_join_expand, dim3(grid_for(a.total)), dim3(PRA_BLOCK), 0, st, a);
+    if (!cap) return;
+    u64 g = (cap + PRA_BLOCK - 1) / PRA_BLOCK;
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_join_expand, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_plan(hipStream_t st, const PlanArgs& a)
+{
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, a);
 }
 
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt)
@@ -999,10 +1080,10 @@ void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
 }
 
 void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
-                           u64* count_out, int intern)
+                           u64* count_out, int intern, const u64* ctl)
 {
     hipLaunchKernelGGL(k_seq_update, dim3(grid_for(nb)), dim3(PRA_BLOCK), 0, st, seqbase, offs, nblocks, nb, total,
-                       count_out, intern);
+                       count_out, intern, ctl);
 }
 
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
@@ -1047,12 +1128,12 @@ void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int
                              u64* status, u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout)
 {
     if (iout) {
-        u32 nt = (u32)((n + 2 * PRA_TILE - 1) / (2 * PRA_TILE));
-        hipLaunchKernelGGL((k_sort_scatter<true, 2 * PRA_ITEMS>), dim3(nt ? nt : 1), dim3(PRA_BLOCK), 0, st, in, out, n,
-                           w, ds, gofs, status, tile_ctr, epoch, flags, iin, iout);
+        u32 nt = pra_tiles(n);
+        hipLaunchKernelGGL((k_sort_scatter<true, PRA_ITEMS>), dim3(nt ? nt : 1), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds,
+                           gofs, status, tile_ctr, epoch, flags, iin, iout);
     } else {
         u32 nt = pra_tiles(n);
-        hipLaunchKernelGGL((k_sort_scatter<false, PRA_ITEMS>), dim3(nt), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds,
+        hipLaunchKernelGGL((k_sort_scatter<false, PRA_ITEMS>), dim3(nt ? nt : 1), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds,
                            gofs, status, tile_ctr, epoch, flags, iin, iout);
     }
 }
@@ -1088,28 +1169,35 @@ void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc
     hipLaunchKernelGGL(k_col_orand2, dim3((unsigned)g, (unsigned)kc), dim3(PRA_BLOCK), 0, st, rows, n, w, out);
 }
 
-void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap,
-                             u64* slot_of, u64 epoch, u32* flags)
+void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u64 capn, int cw, int kw, u64* table,
+                             u64 cap, u64* slot_of, u64 epoch, u32* flags)
 {
-    if (!n) return;
-    hipLaunchKernelGGL(k_dedup_claim2, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, cap, slot_of,
-                       epoch, flags);
+    if (!capn) return;
+    hipLaunchKernelGGL(k_dedup_claim2, dim3(grid_for(capn)), dim3(PRA_BLOCK), 0, st, cand, ctl, cw, kw, table, cap,
+                       slot_of, epoch, flags);
 }
 
-void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, const u64* table,
+void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags)
 {
-    hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, n, cw, kw, table, slot_of,
+    hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, ctl, cw, kw, table, slot_of,
                        epoch, keep, dest, nb, hist, nblocks, flags);
 }
 
-void pra_launch_newt_reduce(hipStream_t st, const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat,
-                            u32* ghist)
+// shares of the first reduce level: about sqrt(nblocks), so that both levels
+// read a similar number of summaries per thread
+u32 pra_newt_reduce_chunks(u32 nblocks)
 {
-    u32 threads = 8 * PRA_RADIX + PRA_PART_WORDS;
-    hipLaunchKernelGGL(k_newt_reduce, dim3((threads + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0, st, part,
-                       part_hist, nblocks, packed, stat, ghist);
+    u32 g = 1;
+    while (g * g < nblocks && g < 128) g++;
+    return g;
+}
+
+void pra_launch_newt_reduce(hipStream_t st, NewtReduceArgs a)
+{
+    static_assert(PRA_RADIX == PRA_BLOCK && PRA_PART_WORDS <= PRA_BLOCK, "one thread per bin / word");
+    hipLaunchKernelGGL(k_newt_reduce, dim3(pra_newt_reduce_chunks(a.nblocks), 9), dim3(PRA_BLOCK), 0, st, a);
 }
 
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out)
@@ -1118,4 +1206,10 @@ void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64
     if (g > 1024) g = 1024;
     if (g == 0) g = 1;
     hipLaunchKernelGGL(k_rows_checksum, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, rows, n, w, out);
+}
+
+void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_group_maxlen, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, flags, gpos, gstart, n, maxlen);
 }

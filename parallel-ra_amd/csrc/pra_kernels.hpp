@@ -15,6 +15,7 @@
 #define PRA_F_FALSE 7
 
 struct JoinExpandArgs {
+    const u64* ctl;      // [0] candidates of the output, [1] capacity overflow
     const u64* outer;
     const u64* inner;
     const u64* off;      // n_outer + 1 exclusive offsets
@@ -30,6 +31,7 @@ struct JoinExpandArgs {
 };
 
 struct CopyExpandArgs {
+    const u64* ctl;
     const u64* rows;
     const u64* off;
     const u64* posadj;
@@ -41,7 +43,23 @@ struct CopyExpandArgs {
     int map[PRA_MAXW];
 };
 
+#define PRA_MAXRA 64
+
+struct PlanArgs {
+    const u64* srcoff;   // per RA: nb + 1 output offsets at source bucket bounds
+    u64* posadj;         // per RA: nb arrival adjustments
+    u64* ctl;            // [0] candidates, [1] overflow
+    u64* joined;
+    u32* flags;
+    u64 cap;
+    long long thresh;
+    int nb, nr;
+    int ras[PRA_MAXRA];
+    int is_join[PRA_MAXRA];
+};
+
 struct NewtScatterArgs {
+    const u64* ctl;
     const u64* dict;   // value -> rank dictionary (null: no packed sort key)
     u64 dict_cap;
     u64* keys;         // packed sort key per written row
@@ -74,13 +92,15 @@ void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, co
                              u64 cap, u32* flags);
 void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
                            const u64* inner, int wi, u64* cnt, u64* istart);
-void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a);
+void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap);
+void pra_launch_plan(hipStream_t st, const PlanArgs& a);
+void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
 void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart, u64 nb, u64* srcoff);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
 void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
-                           u64* count_out, int intern);
+                           u64* count_out, int intern, const u64* ctl);
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
                                   u32* flags);
 void pra_launch_table_verify_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap,
@@ -95,11 +115,23 @@ void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u6
 void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n, int w, u64* out);
 void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, int shift, u64 nb, u64* bnd);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
-void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, u64 n, int cw, int kw, u64* table, u64 cap,
-                             u64* slot_of, u64 epoch, u32* flags);
-void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, u64 n, int cw, int kw, const u64* table,
+void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u64 capn, int cw, int kw, u64* table,
+                             u64 cap, u64* slot_of, u64 epoch, u32* flags);
+void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags);
-void pra_launch_newt_reduce(hipStream_t st, const u64* part, const u32* part_hist, u32 nblocks, int packed, u64* stat,
-                            u32* ghist);
+struct NewtReduceArgs {
+    const u64* ctl;
+    const u64* part;      // scatter block summaries
+    const u32* part_hist; // scatter block digit histograms
+    u64* cpart;           // per chunk: PRA_PART_WORDS words
+    u32* chist;           // per chunk: 8 x 256 digit counts
+    u32* ctr;             // 9 arrival counters, zero between launches
+    u64* stat;            // output summary words
+    u32* ghist;           // output digit histograms
+    u32 nblocks;
+    int packed;
+};
+u32 pra_newt_reduce_chunks(u32 nblocks);
+void pra_launch_newt_reduce(hipStream_t st, NewtReduceArgs a);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);

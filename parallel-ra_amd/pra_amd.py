@@ -25,7 +25,7 @@ EXPORTS = (
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
     "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
-    "pra_rel_download", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
+    "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
     "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_history",
     "pra_execute", "pra_execute_many",
@@ -78,6 +78,7 @@ def lib():
             "pra_rel_load_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
             "pra_rel_count": [_vp, ctypes.c_int, _u64p],
             "pra_rel_download": [_vp, ctypes.c_int, _u64p, _u64, _u64p],
+            "pra_rel_write_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
             "pra_rel_device_rows": [_vp, ctypes.c_int, ctypes.POINTER(_u64p), _u64p],
             "pra_rel_checksum": [_vp, ctypes.c_int, _u64p],
             "pra_scc_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
@@ -218,6 +219,9 @@ class Context:
                                           ctypes.byref(got)), "download")
         assert got.value == n
         return out[:n]
+
+    def write_bin(self, rel, path, version=FULL):
+        self._chk(self.L.pra_rel_write_bin(self.rels[rel], str(path).encode(), version), "write_bin")
 
     def checksum(self, rel, version=FULL):
         v = _u64()

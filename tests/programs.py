@@ -288,3 +288,44 @@ def sg_bruteforce(pairs):
         if not new:
             return sg
         sg |= new
+
+
+# ------------------------------------------------- facade program runs ------
+
+def run_facade_program(binary, data_name, root, nb, dump_dir=None, args=(), timeout=600):
+    """Run a reference program built against the facade the way its sources
+    expect: cwd = <root>/tests, with <root>/data/<data_name> pointing at the
+    fixture copy under tests/golden.  Returns the completed process."""
+    import subprocess
+    os.makedirs(os.path.join(root, "tests"), exist_ok=True)
+    os.makedirs(os.path.join(root, "data"), exist_ok=True)
+    link = os.path.join(root, "data", data_name)
+    if not os.path.exists(link):
+        os.symlink(os.path.join(GOLDEN, data_name), link)
+    env = dict(os.environ, PRA_BUCKETS=str(nb))
+    if dump_dir:
+        os.makedirs(dump_dir, exist_ok=True)
+        env["PRA_DUMP_DIR"] = str(dump_dir)
+    return subprocess.run([binary, *args], cwd=os.path.join(root, "tests"), env=env, capture_output=True,
+                          text=True, timeout=timeout)
+
+
+def dump_digest(dump_dir):
+    """{relation: [rows, sha256 of the dumped bytes]} of a PRA_DUMP_DIR dump."""
+    import hashlib
+    out = {}
+    for f in sorted(os.listdir(dump_dir)):
+        if not f.endswith("_full"):
+            continue
+        with open(os.path.join(dump_dir, f + ".size")) as s:
+            rows = int(s.read().split()[0])
+        with open(os.path.join(dump_dir, f), "rb") as d:
+            out[f[:-len("_full")]] = [rows, hashlib.sha256(d.read()).hexdigest()]
+    return out
+
+
+def total_facts(stdout):
+    """the last "Total facts across all relations N" line (lie.cpp:128-150)"""
+    import re
+    m = re.findall(r"Total facts across all relations (\d+)", stdout)
+    return int(m[-1]) if m else None

@@ -56,3 +56,22 @@ def test_same_generation_program_on_gpu(tmp_path):
     s = sizes(r.stdout)
     assert s["sg_2_1_2"] == o.count(h["SG"])
     assert s["edge_2_1_2"] == 2800
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BUILD, "worstcase-110-terms-4-m")),
+                    reason="reference worstcase program not built")
+def test_worstcase_program_on_gpu(tmp_path):
+    # every relation's full version, ids included, bit-exact against the
+    # oracle's run of the same unchanged program at 3 buckets
+    import json
+    from programs import dump_digest, run_facade_program, total_facts
+    expect = json.load(open(os.path.join(GOLDEN, "worstcase-110-terms-4-m.k3.json")))
+    print("[facade] worstcase-110-terms-4-m nb=3", flush=True)
+    r = run_facade_program(os.path.join(BUILD, "worstcase-110-terms-4-m"), "worstcase-110-terms-4-m",
+                           str(tmp_path / "run"), 3, str(tmp_path / "dump"), timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    print("\n".join(l for l in r.stdout.splitlines() if "batches to the fixed point" in l), flush=True)
+    assert total_facts(r.stdout) == 6476108
+    got = dump_digest(str(tmp_path / "dump"))
+    bad = sorted(k for k in expect["relations"] if got.get(k) != expect["relations"][k])
+    assert not bad, [(k, got.get(k), expect["relations"][k]) for k in bad[:5]]
