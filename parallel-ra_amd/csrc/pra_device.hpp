@@ -28,7 +28,8 @@ typedef uint32_t u32;
 #define PRA_FLAG_THRESHOLD 16u
 
 #define PRA_MAXPASS 40
-#define PRA_PART_WORDS 40   // per-block summary words of the newt scatter
+#define PRA_PART_WORDS 40   // summary words of the newt scatter
+#define PRA_NREP 16         // replicas the scatter blocks fold their summaries into
 #define PRA_LOOKBACK_SPIN_LIMIT (1u << 22)
 #define PRA_LOOKBACK 8       // predecessor tiles read per look-back step
 
@@ -38,7 +39,11 @@ typedef uint32_t u32;
 
 // FNV-1a-64 over the 8 little-endian bytes of each word, exactly the
 // reference's tuple_hash (src/hash/hash.h:11-30).
-__host__ __device__ inline u64 pra_fnv(const u64* t, int n)
+__host__ // summary word j of the newt scatter combines by AND (packed key AND, key
+// column ANDs) or by OR (everything else)
+__device__ inline bool pra_summary_is_and(int j) { return j == 1 || (j >= 24 && j < 40); }
+
+__device__ inline u64 pra_fnv(const u64* t, int n)
 {
     u64 h = 14695981039346656037ULL;
     for (int i = 0; i < n; i++) {

@@ -206,7 +206,7 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, part, part_hist, posadj_dev, joined_dev, cpart, chist, rctr;
+    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_hist, rep_words;
     u64 dict_cap = 0, dict_n = 0;
     int dict_bits = 0, dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -239,7 +239,7 @@ struct pra_ctx {
         seq.clear();
         for (DBuf* b : {&partials, &hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &part, &part_hist, &posadj_dev, &joined_dev, &cpart, &chist, &rctr})
+                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_hist, &rep_words})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -966,7 +966,6 @@ static void run_iteration(pra_scc* S)
         u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
         scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
         DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
-        c->part.reserve((size_t)nblocks * PRA_PART_WORDS * 8, c->st);
         {
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
@@ -983,14 +982,13 @@ static void run_iteration(pra_scc* S)
             a.wn = R->w;
             a.intern = intern ? 1 : 0;
             a.tag = R->tag;
-            a.part = c->part.as<u64>();
+            a.rep_words = c->rep_words.as<u64>();
             if (packed[oi]) {
                 R->nkeys.reserve(C * 8, c->st);
-                c->part_hist.reserve((size_t)nblocks * 8 * PRA_RADIX * 4, c->st);
                 a.dict = c->dict.as<u64>();
                 a.dict_cap = c->dict_cap;
                 a.keys = R->nkeys.as<u64>();
-                a.part_hist = c->part_hist.as<u32>();
+                a.rep_hist = c->rep_hist.as<u32>();
                 a.rank_bits = c->dict_bits;
                 a.bucket_shift = R->arity * c->dict_bits;
             }
@@ -998,31 +996,10 @@ static void run_iteration(pra_scc* S)
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
         }
-        c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
-        {
-            const u32 nch = pra_newt_reduce_chunks(nblocks);
-            c->cpart.reserve((size_t)nch * PRA_PART_WORDS * 8, c->st);
-            if (packed[oi]) c->chist.reserve((size_t)nch * 8 * PRA_RADIX * 4, c->st);
-            NewtReduceArgs a;
-            memset(&a, 0, sizeof(a));
-            a.ctl = ctl;
-            a.part = c->part.as<u64>();
-            a.part_hist = packed[oi] ? c->part_hist.as<u32>() : nullptr;
-            a.cpart = c->cpart.as<u64>();
-            a.chist = packed[oi] ? c->chist.as<u32>() : nullptr;
-            a.ctr = c->rctr.as<u32>();
-            a.stat = st2o;
-            a.ghist = c->ghist.as<u32>();
-            a.nblocks = nblocks;
-            a.packed = packed[oi] ? 1 : 0;
-            pra_launch_newt_reduce(c->st, a);
-        }
-        check_launch("newt_reduce");
-        if (packed[oi]) {
-            R->kgofs.reserve(8 * PRA_RADIX * 4, c->st);
-            pra_launch_sort_scan(c->st, c->ghist.as<u32>(), 8, R->kgofs.as<u32>(), nullptr, (u32)nb, 0);
-            check_launch("key_digit_offsets");
-        }
+        R->kgofs.reserve(8 * PRA_RADIX * 4, c->st);
+        pra_launch_newt_fold(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>(), ctl, packed[oi] ? 1 : 0, st2o,
+                             R->kgofs.as<u32>());
+        check_launch("newt_fold");
         pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
                               st2o, intern ? 1 : 0, ctl);
         check_launch("seq_update");
@@ -1217,8 +1194,9 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->budget = hbm_budget;
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
-        c->rctr.reserve(64, c->st);
-        pra_launch_fill_u64(c->st, c->rctr.as<u64>(), 8, 0);
+        c->rep_hist.reserve((size_t)PRA_NREP * 8 * PRA_RADIX * 4, c->st);
+        c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
+        pra_launch_newt_rep_init(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>());
         check_launch("init");
         HIPCHK(hipStreamSynchronize(c->st));
     } catch (const PraError& e) {

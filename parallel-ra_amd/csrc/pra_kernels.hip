@@ -811,71 +811,65 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     __syncthreads();
+    // fold into replica blockIdx.x % PRA_NREP of the summaries (identity
+    // values are not sent); k_newt_fold combines and resets the replicas
+    const u32 rep = blockIdx.x % PRA_NREP;
     if (threadIdx.x < PRA_PART_WORDS) {
         const int j = threadIdx.x;
-        u64 v = red[0][j];
-        bool is_and = j == 1 || (j >= 24 && j < 40);
-        for (int w2 = 1; w2 < PRA_WAVES; w2++) v = is_and ? (v & red[w2][j]) : (v | red[w2][j]);
-        a.part[(u64)blockIdx.x * PRA_PART_WORDS + j] = v;
+        if (j < 3 || j >= 8) {
+            u64 v = red[0][j];
+            bool is_and = pra_summary_is_and(j);
+            for (int w2 = 1; w2 < PRA_WAVES; w2++) v = is_and ? (v & red[w2][j]) : (v | red[w2][j]);
+            unsigned long long* dst = (unsigned long long*)(a.rep_words + (u64)rep * PRA_PART_WORDS + j);
+            if (is_and && v != ~0ULL) atomicAnd(dst, (unsigned long long)v);
+            if (!is_and && v != 0) atomicOr(dst, (unsigned long long)v);
+        }
     }
     if (packed)
         for (int i = threadIdx.x; i < 8 * PRA_RADIX; i += PRA_BLOCK)
-            a.part_hist[(u64)blockIdx.x * 8 * PRA_RADIX + i] = khist[i];
+            if (khist[i]) atomicAdd(a.rep_hist + (u64)rep * 8 * PRA_RADIX + i, khist[i]);
 }
 
-// Block summaries of the scatter, reduced in two levels so that no thread
-// walks all blocks: blockIdx.y picks the quantity (y < 8: byte-digit
-// histogram y of the packed key, y == 8: the summary words), blockIdx.x an
-// equal share of the scatter blocks that held candidates (the first
-// ceil(n / PRA_TILE)).  Each share's result goes to `cpart` / `chist`; the
-// last block of each y to finish (counted in ctr[y], which it resets to zero
-// for the next launch) adds them up.  The fences order the share results
-// before the counter increment and the final reads after it.
-__global__ void k_newt_reduce(NewtReduceArgs a)
+// Combine the PRA_NREP replicas of the scatter summaries and reset them to
+// their identity values for the next scatter.  Block y < 8: byte digit y of
+// the packed key, whose exclusive digit offsets go to kgofs (the one-sweep
+// sort's global histogram scan); block 8: summary words -> stat[1..3]
+// (packed key OR / AND / dictionary miss) and stat[8..39] (key column OR /
+// AND).
+__global__ void k_newt_fold(u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat, u32* kgofs)
 {
-    if (a.ctl[1]) return;
-    const u32 y = blockIdx.y, t = threadIdx.x;
-    if (y < 8 && !a.packed) return;
-    const u64 n = a.ctl[0];
-    const u32 nact = (u32)min((n + PRA_TILE - 1) / PRA_TILE, (u64)a.nblocks);
-    const u32 share = (nact + gridDim.x - 1) / gridDim.x;
-    const u32 b0 = min(blockIdx.x * share, nact), b1 = min(b0 + share, nact);
-    const bool word = y == 8 && t < PRA_PART_WORDS && !(t >= 3 && t < 8);
-    const bool is_and = t == 1 || (t >= 24 && t < 40);
+    if (ctl[1]) return;
+    __shared__ u32 lds[PRA_WAVES + 1];
+    const u32 y = blockIdx.x, t = threadIdx.x;
     if (y < 8) {
-        u32 acc = 0;
-        for (u32 b = b0; b < b1; b++) acc += a.part_hist[((u64)b * 8 + y) * PRA_RADIX + t];
-        a.chist[((u64)blockIdx.x * 8 + y) * PRA_RADIX + t] = acc;
-    } else if (word) {
-        u64 v = is_and ? ~0ULL : 0;
-        for (u32 b = b0; b < b1; b++) {
-            u64 x = a.part[(u64)b * PRA_PART_WORDS + t];
-            v = is_and ? (v & x) : (v | x);
+        if (!packed) return;
+        u32 v = 0;
+        for (int r = 0; r < PRA_NREP; r++) {
+            u32* p = rep_hist + ((u64)r * 8 + y) * PRA_RADIX + t;
+            v += *p;
+            *p = 0;
         }
-        a.cpart[(u64)blockIdx.x * PRA_PART_WORDS + t] = v;
-    }
-    __shared__ u32 last;
-    __threadfence();
-    __syncthreads();
-    if (t == 0) last = atomicAdd(a.ctr + y, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (y < 8) {
-        u32 acc = 0;
-        for (u32 c = 0; c < gridDim.x; c++) acc += a.chist[((u64)c * 8 + y) * PRA_RADIX + t];
-        a.ghist[y * PRA_RADIX + t] = acc;
-    } else if (word) {
+        u32 tot;
+        kgofs[y * PRA_RADIX + t] = block_exclusive_scan<u32>(v, lds, &tot);
+    } else if (t < PRA_PART_WORDS && !(t >= 3 && t < 8)) {
+        const bool is_and = pra_summary_is_and(t);
         u64 v = is_and ? ~0ULL : 0;
-        for (u32 c = 0; c < gridDim.x; c++) {
-            u64 x = a.cpart[(u64)c * PRA_PART_WORDS + t];
-            v = is_and ? (v & x) : (v | x);
+        for (int r = 0; r < PRA_NREP; r++) {
+            u64* p = rep_words + (u64)r * PRA_PART_WORDS + t;
+            v = is_and ? (v & *p) : (v | *p);
+            *p = is_and ? ~0ULL : 0;
         }
-        // summary word 0/1/2 (key OR / AND / dictionary miss) -> stat 1/2/3,
-        // column masks keep their index
-        a.stat[t < 3 ? t + 1 : t] = v;
+        stat[t < 3 ? t + 1 : t] = v;
     }
-    if (t == 0) a.ctr[y] = 0;
+}
+
+// identity values of the summary replicas (all counts 0, OR words 0, AND
+// words all ones)
+__global__ void k_newt_rep_init(u32* rep_hist, u64* rep_words)
+{
+    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < PRA_NREP * 8 * PRA_RADIX) rep_hist[i] = 0;
+    if (i < PRA_NREP * PRA_PART_WORDS) rep_words[i] = pra_summary_is_and(i % PRA_PART_WORDS) ? ~0ULL : 0;
 }
 
 // seq counters advance by the number of new tuples per bucket; the kept
@@ -1185,19 +1179,17 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                        epoch, keep, dest, nb, hist, nblocks, flags);
 }
 
-// shares of the first reduce level: about sqrt(nblocks), so that both levels
-// read a similar number of summaries per thread
-u32 pra_newt_reduce_chunks(u32 nblocks)
-{
-    u32 g = 1;
-    while (g * g < nblocks && g < 128) g++;
-    return g;
-}
-
-void pra_launch_newt_reduce(hipStream_t st, NewtReduceArgs a)
+void pra_launch_newt_fold(hipStream_t st, u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat,
+                          u32* kgofs)
 {
     static_assert(PRA_RADIX == PRA_BLOCK && PRA_PART_WORDS <= PRA_BLOCK, "one thread per bin / word");
-    hipLaunchKernelGGL(k_newt_reduce, dim3(pra_newt_reduce_chunks(a.nblocks), 9), dim3(PRA_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_newt_fold, dim3(9), dim3(PRA_BLOCK), 0, st, rep_hist, rep_words, ctl, packed, stat, kgofs);
+}
+
+void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words)
+{
+    hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * 8 * PRA_RADIX + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0,
+                       st, rep_hist, rep_words);
 }
 
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out)

@@ -63,8 +63,8 @@ struct NewtScatterArgs {
     const u64* dict;   // value -> rank dictionary (null: no packed sort key)
     u64 dict_cap;
     u64* keys;         // packed sort key per written row
-    u64* part;         // per block: PRA_PART_WORDS summary words
-    u32* part_hist;    // per block: 8 x 256 byte-digit counts of the packed keys
+    u64* rep_words;    // PRA_NREP x PRA_PART_WORDS summary words (OR / AND)
+    u32* rep_hist;     // PRA_NREP x 8 x 256 byte-digit counts of the packed keys
     int rank_bits, bucket_shift;
     const u64* cand;
     const u32* keep;
@@ -120,18 +120,7 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags);
-struct NewtReduceArgs {
-    const u64* ctl;
-    const u64* part;      // scatter block summaries
-    const u32* part_hist; // scatter block digit histograms
-    u64* cpart;           // per chunk: PRA_PART_WORDS words
-    u32* chist;           // per chunk: 8 x 256 digit counts
-    u32* ctr;             // 9 arrival counters, zero between launches
-    u64* stat;            // output summary words
-    u32* ghist;           // output digit histograms
-    u32 nblocks;
-    int packed;
-};
-u32 pra_newt_reduce_chunks(u32 nblocks);
-void pra_launch_newt_reduce(hipStream_t st, NewtReduceArgs a);
+void pra_launch_newt_fold(hipStream_t st, u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat,
+                          u32* kgofs);
+void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
