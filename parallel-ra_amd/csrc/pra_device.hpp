@@ -43,17 +43,23 @@ __host__ // summary word j of the newt scatter combines by AND (packed key AND, 
 // column ANDs) or by OR (everything else)
 __device__ inline bool pra_summary_is_and(int j) { return j == 1 || (j >= 24 && j < 40); }
 
+#define PRA_FNV_OFFSET 14695981039346656037ULL
+
+// one u64 word (8 little-endian bytes) of FNV-1a-64
+__device__ inline u64 pra_fnv_word(u64 h, u64 w)
+{
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        h ^= (w >> (8 * b)) & 255ULL;
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
 __device__ inline u64 pra_fnv(const u64* t, int n)
 {
-    u64 h = 14695981039346656037ULL;
-    for (int i = 0; i < n; i++) {
-        u64 w = t[i];
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            h ^= (w >> (8 * b)) & 255ULL;
-            h *= 1099511628211ULL;
-        }
-    }
+    u64 h = PRA_FNV_OFFSET;
+    for (int i = 0; i < n; i++) h = pra_fnv_word(h, t[i]);
     return h;
 }
 

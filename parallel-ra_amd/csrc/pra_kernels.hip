@@ -528,7 +528,9 @@ __global__ void k_join_expand(JoinExpandArgs a)
         u64 s = a.nb == 1 ? 0 : pra_fnv(d, a.jcc) % a.nb;
         u64 pos = a.posadj[s] + o;
         u64* dst = a.cand + pos * a.cw;
-        u64 tmp[PRA_MAXW];
+        // the destination bucket hashes the output's first out_jcc columns
+        // as they are written
+        u64 h = PRA_FNV_OFFSET;
         for (int c = 0; c < a.cw; c++) {
             int m = a.map[c];
             u64 v;
@@ -536,10 +538,10 @@ __global__ void k_join_expand(JoinExpandArgs a)
                 v = m < a.wo ? d[m] : f[a.jcc + (m - a.wo)];
             else
                 v = m < a.wi ? f[m] : d[a.jcc + (m - a.wi)];
-            tmp[c] = v;
             dst[c] = v;
+            if (c < a.out_jcc) h = pra_fnv_word(h, v);
         }
-        a.dest[pos] = a.nb == 1 ? 0u : (u32)(pra_fnv(tmp, a.out_jcc) % a.nb);
+        a.dest[pos] = a.nb == 1 ? 0u : (u32)(h % a.nb);
     }
 }
 
@@ -575,12 +577,13 @@ __global__ void k_copy_expand(CopyExpandArgs a)
     u64 s = a.nb == 1 ? 0 : pra_fnv(r, a.in_jcc) % a.nb;
     u64 pos = a.posadj[s] + a.off[i];
     u64* dst = a.cand + pos * a.cw;
-    u64 tmp[PRA_MAXW];
+    u64 h = PRA_FNV_OFFSET;
     for (int c = 0; c < a.cw; c++) {
-        tmp[c] = r[a.map[c]];
-        dst[c] = tmp[c];
+        u64 v = r[a.map[c]];
+        dst[c] = v;
+        if (c < a.out_jcc) h = pra_fnv_word(h, v);
     }
-    a.dest[pos] = a.nb == 1 ? 0u : (u32)(pra_fnv(tmp, a.out_jcc) % a.nb);
+    a.dest[pos] = a.nb == 1 ? 0u : (u32)(h % a.nb);
 }
 
 // srcoff[s] = off[bstart[s]] for s in [0, nb]
@@ -698,6 +701,7 @@ __global__ void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw, int
 // Stable partition of the kept candidates by destination bucket; JOIN-type
 // rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
+template <bool PACKED>
 __global__ void k_dedup_scatter(NewtScatterArgs a)
 {
     if (a.ctl[1]) return;
@@ -740,15 +744,16 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     // byte-digit histograms (packed mode), or the key columns' OR / AND
     __shared__ u32 khist[8 * PRA_RADIX];
     __shared__ u64 red[PRA_WAVES][PRA_PART_WORDS];
-    const bool packed = a.dict != nullptr;
+    constexpr bool packed = PACKED;
     if (packed)
         for (int i = threadIdx.x; i < 8 * PRA_RADIX; i += PRA_BLOCK) khist[i] = 0;
     __syncthreads();
     const int kc = a.cw < 16 ? a.cw : 16;
+    constexpr int NC = PACKED ? 1 : 16;   // key columns summarised (unpacked)
     u64 kor = 0, kand = ~0ULL, kmiss = 0;
-    u64 o[16], an[16];
+    u64 o[NC], an[NC];
 #pragma unroll
-    for (int c = 0; c < 16; c++) { o[c] = 0; an[c] = ~0ULL; }
+    for (int c = 0; c < NC; c++) { o[c] = 0; an[c] = ~0ULL; }
 #pragma unroll
     for (int k = 0; k < PRA_ITEMS; k++) {
         u64 p = base + (u64)k * 64 + lane;
@@ -777,7 +782,7 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                 for (int b = 0; b < 8; b++) atomicAdd(&khist[b * PRA_RADIX + ((K >> (8 * b)) & 255)], 1u);
             } else {
 #pragma unroll
-                for (int c = 0; c < 16; c++)
+                for (int c = 0; c < NC; c++)
                     if (c < kc) { o[c] |= src[c]; an[c] &= src[c]; }
             }
         }
@@ -790,7 +795,7 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     }
     if (!packed) {
 #pragma unroll
-        for (int c = 0; c < 16; c++) {
+        for (int c = 0; c < NC; c++) {
             if (c < kc) {
 #pragma unroll
                 for (int dd = 32; dd >= 1; dd >>= 1) {
@@ -806,8 +811,8 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         red[wave][2] = kmiss;
 #pragma unroll
         for (int c = 0; c < 16; c++) {
-            red[wave][8 + c] = c < kc ? o[c] : 0;
-            red[wave][24 + c] = c < kc ? an[c] : ~0ULL;
+            red[wave][8 + c] = (!packed && c < kc) ? o[c < NC ? c : 0] : 0;
+            red[wave][24 + c] = (!packed && c < kc) ? an[c < NC ? c : 0] : ~0ULL;
         }
     }
     __syncthreads();
@@ -1070,7 +1075,10 @@ void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart,
 
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
 {
-    hipLaunchKernelGGL(k_dedup_scatter, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
+    if (a.dict)
+        hipLaunchKernelGGL(k_dedup_scatter<true>, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_dedup_scatter<false>, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
 }
 
 void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
