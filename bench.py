@@ -42,6 +42,17 @@ def cpu_baseline(edges, nb, iters):
                       f"{j} joined tuples in {dt:.2f} s"}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload`, from the calibrated
+    FETCH_SIZE / WRITE_SIZE counter runs recorded in profiles/pmc_traffic.json
+    (None when that workload / kernel was not profiled)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
+            return json.load(f)[workload][kernel]["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,7 +170,8 @@ def main():
         bytes_per = d["bytes"] / max(1, d["launches"])
         achieved = bytes_per / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(workload, dom),
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches_per_step": d["launches"] // args.steps,
                     "algorithmic_bytes_per_launch": round(bytes_per)}
 
