@@ -29,6 +29,7 @@ typedef uint32_t u32;
 
 #define PRA_MAXPASS 40
 #define PRA_PART_WORDS 40   // summary words of the newt scatter
+#define PRA_SMALL_SCAN 8192  // scans up to this many elements run in one block
 #define PRA_NREP 16         // replicas the scatter blocks fold their summaries into
 #define PRA_LOOKBACK_SPIN_LIMIT (1u << 22)
 #define PRA_LOOKBACK 8       // predecessor tiles read per look-back step

@@ -139,6 +139,10 @@ struct pra_rel {
     // bucket bounds per version
     DBuf bnd[2];
     bool bnd_valid[2] = {false, false};
+    // set while an SCC runs: the relation is produced by one of its rules
+    // (changes every iteration), and its full version is read by one of them
+    // (so it is kept sorted by merging each delta into it)
+    bool scc_changing = false, merge_full = false;
     void touched(int v)
     {
         if (v == PRA_DELTA || v == PRA_FULL) {
@@ -158,6 +162,8 @@ struct RADesc {
     // per-iteration state
     DBuf cnt, off, istart, posadj;
     u64 n_outer = 0;
+    bool active = false;   // outer (and inner) non-empty this iteration
+    bool sorted_probe = false;   // inner changes inside the SCC: binary search, no hash index
     pra_rel* outer = nullptr;
     int outer_v = 0;
     pra_rel* inner = nullptr;
@@ -173,11 +179,11 @@ struct KStat {
 
 enum {
     K_JOIN_COUNT, K_JOIN_EXPAND, K_COPY_EXPAND, K_DEDUP_CLAIM, K_DEDUP_DECIDE, K_DEDUP_VERIFY, K_DEDUP_SCATTER,
-    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_COUNT
+    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_MERGE, K_COUNT
 };
 static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand", "dedup_claim", "dedup_decide",
                                       "dedup_verify_hist", "dedup_scatter", "radix_hist", "radix_scatter",
-                                      "table_build", "index_build", "scan", "gather_rows"};
+                                      "table_build", "index_build", "scan", "gather_rows", "merge_full"};
 
 } // namespace
 
@@ -749,6 +755,38 @@ static void append_rows(pra_ctx* c, pra_rel* R, int v, const u64* src, u64 n)
     R->touched(v);
 }
 
+// full <- full u delta keeping full sorted: both are sorted by (bucket,
+// columns) and disjoint, so each delta row's place is a binary search inside
+// its bucket and every row moves once (falls back to an append when either
+// side is unsorted)
+static void merge_into_full(pra_ctx* c, pra_rel* R)
+{
+    Rows& D = R->ver[PRA_DELTA];
+    Rows& F = R->ver[PRA_FULL];
+    if (D.n == 0) return;
+    if (F.n == 0 || !F.sorted || !D.sorted) {
+        bool sorted = F.n == 0 && D.sorted;
+        append_rows(c, R, PRA_FULL, D.d(), D.n);
+        F.sorted = sorted;
+        return;
+    }
+    const u64* fb = bucket_bounds(c, R, PRA_FULL);
+    const u64* db = bucket_bounds(c, R, PRA_DELTA);
+    u64 n = F.n + D.n;
+    c->tmp64a.reserve(D.n * 8, c->st);
+    c->sort_tmp.reserve(n * R->w * 8, c->st);
+    {
+        Prof p(c, K_MERGE, 16.0 * R->w * n);
+        pra_launch_merge(c->st, F.d(), F.n, fb, D.d(), D.n, db, (u64)c->nb, R->w, c->tmp64a.as<u64>(),
+                         c->sort_tmp.as<u64>());
+        check_launch("merge");
+    }
+    F.buf.swap(c->sort_tmp);
+    F.n = n;
+    F.sorted = true;
+    R->touched(PRA_FULL);
+}
+
 // ------------------------------------------------------------ iteration ---
 
 static bool is_intern_type(int type) { return type != RA_ACOPY; }
@@ -760,7 +798,8 @@ static void run_iteration(pra_scc* S)
     const int nb = c->nb;
     const int nra = (int)S->ras.size();
 
-    // 0. inputs ready: sorted versions, join indexes
+    // 0. inputs ready: sorted versions, join indexes.  Rules whose outer (or
+    //    inner) version is empty emit nothing and launch nothing.
     for (auto& ra : S->ras) {
         if (ra.type == RA_JOIN) {
             if (ra.v0 == PRA_DELTA) {
@@ -768,35 +807,51 @@ static void run_iteration(pra_scc* S)
             } else {
                 ra.outer = ra.in1; ra.outer_v = ra.v1; ra.inner = ra.in0; ra.inner_v = PRA_FULL; ra.left = 0;
             }
+            ra.active = ra.outer->ver[ra.outer_v].n > 0 && ra.inner->ver[ra.inner_v].n > 0;
+            ra.sorted_probe = ra.inner->scc_changing;
+            if (!ra.active) continue;
             ensure_sorted(c, ra.outer, ra.outer_v);
-            ensure_index(c, ra.inner, ra.inner_v);
+            if (ra.sorted_probe) {
+                ensure_sorted(c, ra.inner, ra.inner_v);
+                bucket_bounds(c, ra.inner, ra.inner_v);
+            } else {
+                ensure_index(c, ra.inner, ra.inner_v);
+            }
         } else {
             ra.outer = ra.in0; ra.outer_v = ra.v0; ra.inner = nullptr; ra.left = 1;
-            ensure_sorted(c, ra.outer, ra.outer_v);
+            ra.active = ra.outer->ver[ra.outer_v].n > 0;
+            if (ra.active) ensure_sorted(c, ra.outer, ra.outer_v);
         }
     }
 
-    // 1. per-RA output counts and per-source offsets
+    // 1. per-RA output counts and per-source offsets (rows of inactive rules
+    //    stay zero)
     c->stage.reserve((size_t)std::max(1, nra) * (nb + 1) * 8, c->st);
+    pra_launch_fill_u64(c->st, c->stage.as<u64>(), (u64)std::max(1, nra) * (nb + 1), 0);
     for (int r = 0; r < nra; r++) {
         RADesc& ra = S->ras[r];
         Rows& O = ra.outer->ver[ra.outer_v];
         u64 n = O.n;
-        ra.n_outer = n;
+        ra.n_outer = ra.active ? n : 0;
+        if (!ra.active) continue;
         ra.cnt.reserve(std::max<u64>(1, n) * 8, c->st);
         ra.off.reserve((n + 1) * 8, c->st);
         ra.istart.reserve(std::max<u64>(1, n) * 8, c->st);
         if (ra.type == RA_JOIN) {
             Rows& I = ra.inner->ver[ra.inner_v];
-            if (I.n == 0 || ra.inner->idx_cap[ra.inner_v] == 0) {
+            Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + 16.0 * n);
+            if (ra.sorted_probe) {
+                pra_launch_join_count_sorted(c->st, O.d(), n, ra.outer->w, ra.in0->jcc, I.d(), ra.inner->w,
+                                             ra.inner->bnd[ra.inner_v].as<u64>(), (u64)nb, ra.cnt.as<u64>(),
+                                             ra.istart.as<u64>());
+            } else if (ra.inner->idx_cap[ra.inner_v] == 0) {
                 pra_launch_fill_u64(c->st, ra.cnt.as<u64>(), n, 0);
             } else {
-                Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + 16.0 * n);
                 pra_launch_join_count(c->st, O.d(), n, ra.outer->w, ra.in0->jcc, ra.inner->idx[ra.inner_v].as<u64>(),
                                       ra.inner->idx_cap[ra.inner_v], I.d(), ra.inner->w, ra.cnt.as<u64>(),
                                       ra.istart.as<u64>());
-                check_launch("join_count");
             }
+            check_launch("join_count");
         } else {
             int op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
             pra_launch_copy_count(c->st, O.d(), n, ra.outer->w, op, ra.cnt.as<u64>());
@@ -827,7 +882,15 @@ static void run_iteration(pra_scc* S)
                 if (type0 < 0) type0 = is_intern_type(ra.type);
                 else if (type0 != (int)is_intern_type(ra.type))
                     throw PraError(PRA_E_UNSUPPORTED, "relation is the target of both acopy and join/copy rules");
-                ub[oi] += ra.type == RA_JOIN ? ra.n_outer * ra.inner->idx_maxlen[ra.inner_v] : ra.n_outer;
+                if (!ra.active) continue;
+                if (ra.type != RA_JOIN)
+                    ub[oi] += ra.n_outer;
+                else if (ra.sorted_probe)   // no run-length bound without an index: outer x inner
+                    ub[oi] += ra.inner->ver[ra.inner_v].n > (~0ULL >> 8) / ra.n_outer
+                                  ? (~0ULL >> 8)
+                                  : ra.n_outer * ra.inner->ver[ra.inner_v].n;
+                else
+                    ub[oi] += ra.n_outer * ra.inner->idx_maxlen[ra.inner_v];
             }
         if (nr > PRA_MAXRA) throw PraError(PRA_E_UNSUPPORTED, "too many rules produce one relation");
         out_intern[oi] = type0;
@@ -899,7 +962,7 @@ static void run_iteration(pra_scc* S)
         }
         for (int r = 0; r < nra; r++) {
             RADesc& ra = S->ras[r];
-            if (ra.out != R) continue;
+            if (ra.out != R || !ra.active) continue;
             Rows& O = ra.outer->ver[ra.outer_v];
             if (ra.type == RA_JOIN) {
                 JoinExpandArgs a;
@@ -1039,8 +1102,18 @@ static void run_iteration(pra_scc* S)
         R->mt_entries += cnt;
     }
 
-    // 6. sort newt into trie order, then delta -> full, newt -> delta
-    //    (src/RAM/RA_tasks.cpp:1044-1058)
+    // 6. delta -> full, sort newt into trie order, newt -> delta
+    //    (src/RAM/RA_tasks.cpp:1044-1058).  The old delta is folded into full
+    //    first: the newt sort writes the next delta's bucket bounds.
+    for (pra_rel* R : S->rels) {
+        Rows& D = R->ver[PRA_DELTA];
+        if (R->merge_full)
+            merge_into_full(c, R);
+        else
+            append_rows(c, R, PRA_FULL, D.d(), D.n);
+        D.n = 0;
+        R->touched(PRA_DELTA);
+    }
     std::vector<char> bnd_ok(no, 0);
     for (size_t oi = 0; oi < no; oi++) {
         pra_rel* R = outs[oi];
@@ -1066,7 +1139,6 @@ static void run_iteration(pra_scc* S)
     }
     for (pra_rel* R : S->rels) {
         Rows& D = R->ver[PRA_DELTA];
-        append_rows(c, R, PRA_FULL, D.d(), D.n);
         D.buf.swap(R->ver[PRA_NEWT].buf);
         D.n = R->ver[PRA_NEWT].n;
         D.sorted = true;
@@ -1124,6 +1196,12 @@ static void delta_to_full(pra_ctx* c, pra_rel* R)
 static void scc_run(pra_scc* S)
 {
     pra_ctx* c = S->ctx;
+    for (pra_rel* R : S->rels) R->scc_changing = R->merge_full = false;
+    for (auto& ra : S->ras) ra.out->scc_changing = true;
+    for (auto& ra : S->ras) {
+        if (ra.v0 == PRA_FULL) ra.in0->merge_full = ra.in0->scc_changing;
+        if (ra.type == RA_JOIN && ra.v1 == PRA_FULL) ra.in1->merge_full = ra.in1->scc_changing;
+    }
     for (size_t i = 0; i < S->rels.size(); i++)
         if (S->status[i]) full_to_delta(c, S->rels[i]);
     if (!S->fixpoint) {
@@ -1137,7 +1215,10 @@ static void scc_run(pra_scc* S)
             S->hist.push_back(d);
         } while (d != 0);
     }
-    for (pra_rel* R : S->rels) delta_to_full(c, R);
+    for (pra_rel* R : S->rels) {
+        delta_to_full(c, R);
+        R->scc_changing = R->merge_full = false;
+    }
     sync_check(c);
     S->done = true;
 }

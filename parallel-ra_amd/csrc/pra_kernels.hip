@@ -493,6 +493,88 @@ __global__ void k_join_count(const u64* outer, u64 n, int wo, int jcc, const u64
     istart[i] = st;
 }
 
+// lexicographic comparison of the first n words of two rows
+__device__ inline int pra_row_cmp(const u64* a, const u64* b, int n)
+{
+    for (int x = 0; x < n; x++)
+        if (a[x] != b[x]) return a[x] < b[x] ? -1 : 1;
+    return 0;
+}
+
+// Join count against an inner version kept sorted by (bucket, columns): the
+// rows matching an outer key form one run inside the key's bucket, found by
+// two binary searches (used for inner relations that change every iteration,
+// where a hash index would be rebuilt each time).
+__global__ void k_join_count_sorted(const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
+                                    const u64* ibnd, u64 nb, u64* cnt, u64* istart)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* key = outer + i * wo;
+    u64 b = nb == 1 ? 0 : pra_fnv(key, jcc) % nb;
+    u64 lo = ibnd[b], hi = ibnd[b + 1];
+    u64 a = lo, z = hi;   // first row >= key
+    while (a < z) {
+        u64 m = (a + z) >> 1;
+        if (pra_row_cmp(inner + m * wi, key, jcc) < 0) a = m + 1; else z = m;
+    }
+    u64 first = a;
+    z = hi;               // first row > key
+    while (a < z) {
+        u64 m = (a + z) >> 1;
+        if (pra_row_cmp(inner + m * wi, key, jcc) <= 0) a = m + 1; else z = m;
+    }
+    cnt[i] = a - first;
+    istart[i] = first;
+}
+
+// Merge of a sorted delta into a sorted full (both ordered by (bucket, all w
+// columns), disjoint).  rank[j] = number of full rows before delta row j,
+// searched inside delta row j's bucket.
+__global__ void k_merge_rank(const u64* D, u64 nd, const u64* dbnd, const u64* F, const u64* fbnd, u64 nb, int w,
+                             u64* rank)
+{
+    u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nd) return;
+    u64 b = 0, bh = nb;   // bucket of row j: last b with dbnd[b] <= j
+    while (b + 1 < bh) {
+        u64 m = (b + bh) >> 1;
+        if (dbnd[m] <= j) b = m; else bh = m;
+    }
+    const u64* d = D + j * w;
+    u64 a = fbnd[b], z = fbnd[b + 1];
+    while (a < z) {
+        u64 m = (a + z) >> 1;
+        if (pra_row_cmp(F + m * w, d, w) < 0) a = m + 1; else z = m;
+    }
+    rank[j] = a;
+}
+
+// Place every row of the merge: delta row j at j + rank[j], full row i after
+// the delta rows ranked at or before it (rank is non-decreasing).
+__global__ void k_merge_scatter(const u64* F, u64 nf, const u64* D, u64 nd, const u64* rank, int w, u64* out)
+{
+    u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf + nd) return;
+    const u64* src;
+    u64 pos;
+    if (t < nf) {
+        u64 a = 0, z = nd;   // delta rows with rank <= t
+        while (a < z) {
+            u64 m = (a + z) >> 1;
+            if (rank[m] <= t) a = m + 1; else z = m;
+        }
+        src = F + t * w;
+        pos = t + a;
+    } else {
+        u64 j = t - nf;
+        src = D + j * w;
+        pos = j + rank[j];
+    }
+    u64* dst = out + pos * w;
+    for (int c = 0; c < w; c++) dst[c] = src[c];
+}
+
 // One thread per join pair: locate the outer row by binary search over the
 // exclusive offsets, build the projection of (outer ++ inner[jcc:]) (LEFT) or
 // (inner ++ outer[jcc:]) (RIGHT), and write it at its arrival position.
@@ -984,9 +1066,31 @@ void pra_launch_table_clear(hipStream_t st, u64* t, u64 cap, int stride)
 
 u64 pra_scan_partials_count(u64 n) { return (n + PRA_TILE - 1) / PRA_TILE; }
 
+// one block scans inputs of up to PRA_SMALL_SCAN elements in chunks of
+// PRA_BLOCK, carrying the running total
+template <typename T>
+__global__ void k_scan_small(const T* in, T* out, u64 n, T* total)
+{
+    __shared__ T lds[PRA_WAVES + 1];
+    T carry = 0;
+    for (u64 off = 0; off < n; off += PRA_BLOCK) {
+        u64 i = off + threadIdx.x;
+        T v = i < n ? in[i] : 0;
+        T tot;
+        T ex = block_exclusive_scan<T>(v, lds, &tot);
+        if (i < n) out[i] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
 template <typename T>
 static void scan_impl(hipStream_t st, const T* in, T* out, u64 n, T* partials, T* total)
 {
+    if (n <= PRA_SMALL_SCAN) {
+        hipLaunchKernelGGL(k_scan_small<T>, dim3(1), dim3(PRA_BLOCK), 0, st, in, out, n, total);
+        return;
+    }
     u64 nb = pra_scan_partials_count(n);
     if (nb == 0) nb = 1;
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(PRA_BLOCK), 0, st, in, n, partials);
@@ -1198,6 +1302,22 @@ void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words)
 {
     hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * 8 * PRA_RADIX + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0,
                        st, rep_hist, rep_words);
+}
+
+void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
+                                  const u64* ibnd, u64 nb, u64* cnt, u64* istart)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_join_count_sorted, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, outer, n, wo, jcc, inner, wi,
+                       ibnd, nb, cnt, istart);
+}
+
+void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
+                      u64 nb, int w, u64* rank, u64* out)
+{
+    if (!nd) return;
+    hipLaunchKernelGGL(k_merge_rank, dim3(grid_for(nd)), dim3(PRA_BLOCK), 0, st, D, nd, dbnd, F, fbnd, nb, w, rank);
+    hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(nf + nd)), dim3(PRA_BLOCK), 0, st, F, nf, D, nd, rank, w, out);
 }
 
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out)

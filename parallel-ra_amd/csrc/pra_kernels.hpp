@@ -124,3 +124,7 @@ void pra_launch_newt_fold(hipStream_t st, u32* rep_hist, u64* rep_words, const u
                           u32* kgofs);
 void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
+void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
+                                  const u64* ibnd, u64 nb, u64* cnt, u64* istart);
+void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
+                      u64 nb, int w, u64* rank, u64* out);
