@@ -329,3 +329,22 @@ def total_facts(stdout):
     import re
     m = re.findall(r"Total facts across all relations (\d+)", stdout)
     return int(m[-1]) if m else None
+
+
+def nonlinear_tc_program(b, edges):
+    """Doubling transitive closure written with a reversed index relation:
+         Tr(y, x)  <- acopy of T(x, y)              (re-keyed on y, id kept)
+         T(x, z)   <- Tr(y, x) DELTA JOIN T(y, z) FULL   on y
+         T(x, z)   <- Tr(y, x) FULL  JOIN T(y, z) DELTA  on y  (RIGHT join)
+    T's full version is both a join inner and changes every iteration; Tr's
+    full version is the outer of the RIGHT join.  Returns (T, Tr, scc)."""
+    T = b.rel_create(2, 1, 410)
+    Tr = b.rel_create(2, 1, 411)
+    b.load_rows(T, edges, FULL)
+    s = b.scc_create(True, 1)
+    b.scc_add_relation(s, T, True)
+    b.scc_add_relation(s, Tr, False)
+    b.add_acopy(s, Tr, T, DELTA, [1, 0, 2])
+    b.add_join(s, T, Tr, DELTA, T, FULL, [1, 3])
+    b.add_join(s, T, Tr, FULL, T, DELTA, [1, 3])
+    return T, Tr, s

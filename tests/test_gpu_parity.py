@@ -10,7 +10,8 @@ import pytest
 
 import oracle as O
 from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
-                      grid_tc_size, layered_dag, random_graph, sg_bruteforce, sg_program, tc_program, with_rowid)
+                      grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_program,
+                      with_rowid)
 
 pytestmark = pytest.mark.gpu
 
@@ -217,3 +218,16 @@ def test_intern_ids_flowing_into_key_columns():
         o, (To, fo), g, (Tg, fg) = both(nb, build)
         assert_same_rel(o, To, g, Tg, "T")
         assert o.history(fo) == g.history(fg)
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+def test_nonlinear_tc_parity(nb):
+    # joins whose inner full version changes every iteration (binary-search
+    # probes, merge-maintained full) and a FULL x DELTA (RIGHT) join
+    for seed in range(2):
+        edges = random_graph(400, 700, 40 + seed)
+        print(f"[nonlinear tc] nb={nb} seed={seed}", flush=True)
+        o, (To, Tro, so), g, (Tg, Trg, sg) = both(nb, lambda b: nonlinear_tc_program(b, edges))
+        assert_same_rel(o, To, g, Tg, "T")
+        assert_same_rel(o, Tro, g, Trg, "Tr")
+        assert o.history(so) == g.history(sg)

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
-                      grid_tc_size, layered_dag, sg_bruteforce, sg_program, tc_join_pairs, tc_pairs_bruteforce,
+                      grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_join_pairs, tc_pairs_bruteforce,
                       tc_program, with_rowid)
 
 TAG_T = 257
@@ -230,6 +230,21 @@ def test_same_generation_bruteforce():
             o.execute(1)
             got = set(map(tuple, o.rows(h["SG"])[:, :2].tolist()))
             assert got == want
+
+
+def test_nonlinear_tc_reaches_the_closure():
+    # RIGHT joins (FULL x DELTA), joins against a changing full inner and an
+    # acopy-maintained index relation still compute the transitive closure
+    for seed in range(3):
+        edges = random_graph(50, 90, seed)
+        want = tc_pairs_bruteforce([tuple(r) for r in edges[:, :2].tolist()])
+        for nb in [1, 3]:
+            o = O.Oracle(nb)
+            T, Tr, s = nonlinear_tc_program(o, edges)
+            o.execute(1)
+            got = set(map(tuple, o.rows(T)[:, :2].tolist()))
+            assert got == want
+            assert set(map(tuple, o.rows(Tr)[:, :2].tolist())) == {(y, x) for x, y in want}
 
 
 def test_acopy_keeps_ids_and_dedups_on_all_columns():
