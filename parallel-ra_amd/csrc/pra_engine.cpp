@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -221,6 +222,9 @@ struct pra_ctx {
     u64* host_stage = nullptr;  // pinned
     size_t host_stage_cap = 0;
     u64* host_pos = nullptr;    // pinned
+    u64* mailbox = nullptr;     // pinned, written by the publish kernel
+    size_t mailbox_cap = 0;
+    hipEvent_t pub_ev = nullptr;
     size_t host_pos_cap = 0;
     u64 iterations = 0;
     std::string err;
@@ -251,6 +255,8 @@ struct pra_ctx {
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (host_stage) (void)hipHostFree(host_stage);
         if (host_pos) (void)hipHostFree(host_pos);
+        if (mailbox) (void)hipHostFree(mailbox);
+        if (pub_ev) (void)hipEventDestroy(pub_ev);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -312,6 +318,18 @@ struct pra_ctx {
             host_stage_cap = c;
         }
         return host_stage;
+    }
+    u64* mailbox_buf(size_t words)
+    {
+        if (words > mailbox_cap) {
+            if (mailbox) HIPCHK(hipHostFree(mailbox));
+            mailbox = nullptr;
+            size_t c = std::max<size_t>(words, 1024);
+            HIPCHK(hipHostMalloc((void**)&mailbox, c * sizeof(u64), hipHostMallocCoherent));
+            memset(mailbox, 0, c * sizeof(u64));
+            mailbox_cap = c;
+        }
+        return mailbox;
     }
     u64* host_posadj(size_t words)
     {
@@ -407,6 +425,33 @@ static u64* readback(pra_ctx* c, const void* src, size_t words)
 
 static void sync_check(pra_ctx* c) { readback(c, nullptr, 0); }
 
+// The per-iteration readback: one kernel copies the summary words and the
+// error flags into pinned host memory (no copy-engine transfers), and the host
+// polls an event recorded behind it instead of sleeping in a blocking
+// synchronization.  Completion of the event (the end of the kernel, with its
+// system-scope release) is what makes the words valid to read; after a while
+// the host falls back to a blocking wait.
+static u64* publish_readback(pra_ctx* c, const u64* src, size_t words)
+{
+    u64* h = c->mailbox_buf(words + 1);
+    pra_launch_publish(c->st, src, words, c->flags_buf.as<u32>(), h);
+    check_launch("publish");
+    if (!c->pub_ev) HIPCHK(hipEventCreateWithFlags(&c->pub_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->pub_ev, c->st));
+    auto t0 = std::chrono::steady_clock::now();
+    for (u32 k = 0;; k++) {
+        hipError_t q = hipEventQuery(c->pub_ev);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) HIPCHK(q);
+        if ((k & 63) == 63 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+            HIPCHK(hipEventSynchronize(c->pub_ev));
+            break;
+        }
+    }
+    check_flags((u32)h[words]);
+    return h;
+}
+
 // ------------------------------------------------------------------ sort ---
 
 // look-back status words for `n` rows, zero-initialised once per allocation
@@ -420,7 +465,10 @@ static void prepare_lookback(pra_ctx* c, u64 n)
         c->lb_zeroed = words;
         c->sort_epoch = 0;
     }
-    c->lb_ctr.reserve(PRA_MAXPASS * 4 + 8, c->st);
+    if (!c->lb_ctr.p) {   // per-pass tile counters; each sort pass returns its own to zero
+        c->lb_ctr.reserve(PRA_MAXPASS * 4 + 8, c->st);
+        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (PRA_MAXPASS * 4 + 8) / 8, 0);
+    }
     c->ghist.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
     c->gofs.reserve(PRA_MAXPASS * PRA_RADIX * 4, c->st);
 }
@@ -474,7 +522,6 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
         for (size_t i = 0; i < np; i++) pl.p[i] = passes[first + i];
         bool last_chunk = first + np == passes.size();
         pra_launch_fill_u64(c->st, c->ghist.as<u64>(), np * PRA_RADIX / 2, 0);
-        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (np + 1) / 2, 0);
         {
             Prof p(c, K_RADIX_HIST, 8.0 * n * keycols);
             pra_launch_sort_hist(c->st, r.d(), n, w, pl, c->ghist.as<u32>());
@@ -534,7 +581,6 @@ static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32
         c->idxa.reserve(n * 4, c->st);
         c->idxb.reserve(n * 4, c->st);
         int np = (int)passes.size();
-        pra_launch_fill_u64(c->st, c->lb_ctr.as<u64>(), (size_t)(np + 1) / 2, 0);
         for (int i = 0; i < np; i++) {
             u32 ep = next_sort_epoch(c);
             Prof p(c, K_RADIX_SCATTER, 24.0 * n);
@@ -824,10 +870,9 @@ static void run_iteration(pra_scc* S)
         }
     }
 
-    // 1. per-RA output counts and per-source offsets (rows of inactive rules
-    //    stay zero)
+    // 1. per-RA output counts and their exclusive offsets (the arrival plan
+    //    reads them at the source-bucket bounds)
     c->stage.reserve((size_t)std::max(1, nra) * (nb + 1) * 8, c->st);
-    pra_launch_fill_u64(c->st, c->stage.as<u64>(), (u64)std::max(1, nra) * (nb + 1), 0);
     for (int r = 0; r < nra; r++) {
         RADesc& ra = S->ras[r];
         Rows& O = ra.outer->ver[ra.outer_v];
@@ -858,9 +903,7 @@ static void run_iteration(pra_scc* S)
             check_launch("copy_count");
         }
         scan_u64(c, ra.cnt.as<u64>(), ra.off.as<u64>(), n, ra.off.as<u64>() + n);
-        const u64* bs = bucket_bounds(c, ra.outer, ra.outer_v);
-        pra_launch_gather_srcoff(c->st, ra.off.as<u64>(), bs, (u64)nb, c->stage.as<u64>() + (size_t)r * (nb + 1));
-        check_launch("gather_srcoff");
+        bucket_bounds(c, ra.outer, ra.outer_v);
     }
 
     // 2. output relations, their rule kinds, and candidate capacities: the
@@ -951,12 +994,15 @@ static void run_iteration(pra_scc* S)
             pa.cap = C;
             pa.thresh = c->threshold;
             pa.nb = nb;
-            for (int r = 0; r < nra; r++)
-                if (S->ras[r].out == R) {
-                    pa.ras[pa.nr] = r;
-                    pa.is_join[pa.nr] = S->ras[r].type == RA_JOIN;
-                    pa.nr++;
-                }
+            for (int r = 0; r < nra; r++) {
+                RADesc& ra = S->ras[r];
+                if (ra.out != R || !ra.active) continue;
+                pa.ras[pa.nr] = r;
+                pa.is_join[pa.nr] = ra.type == RA_JOIN;
+                pa.off[pa.nr] = ra.off.as<u64>();
+                pa.bst[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>();
+                pa.nr++;
+            }
             pra_launch_plan(c->st, pa);
             check_launch("plan");
         }
@@ -1075,7 +1121,7 @@ static void run_iteration(pra_scc* S)
     //    candidates overflowed their capacity, now sized exactly)
     std::vector<u64> st2(no * SW, 0);
     for (int pass = 0; pass < 2; pass++) {
-        u64* h = readback(c, c->stage2.p, no * SW);
+        u64* h = publish_readback(c, c->stage2.as<u64>(), no * SW);
         std::vector<size_t> redo;
         for (size_t oi = 0; oi < no; oi++) {
             if (!cap[oi]) continue;

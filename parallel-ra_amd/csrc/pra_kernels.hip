@@ -206,7 +206,13 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
     __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
     __shared__ u32 s_base[PRA_RADIX];
     __shared__ u32 s_tile;
-    if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    if (threadIdx.x == 0) {
+        // tiles are handed out in launch order; the block taking the last one
+        // returns the counter to zero for the next launch
+        u32 t = atomicAdd(tile_ctr, 1u);
+        if (t == gridDim.x - 1) atomicExch(tile_ctr, 0u);
+        s_tile = t;
+    }
     for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
     __syncthreads();
     const u32 tile = s_tile;
@@ -669,25 +675,27 @@ __global__ void k_copy_expand(CopyExpandArgs a)
 }
 
 // srcoff[s] = off[bstart[s]] for s in [0, nb]
-__global__ void k_gather_srcoff(const u64* off, const u64* bstart, u64 nb, u64* srcoff)
-{
-    u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s <= nb) srcoff[s] = off[bstart[s]];
-}
-
 // Arrival plan of one output relation, computed on the device: for source
 // bucket s and the RAs producing the relation (in RA order), the position of
 // their outputs in the arrival order source-major, RA-minor
 // (src/comm/all_to_allv_comm.cpp:64-142), the candidate count, whether it
 // exceeds the allocated capacity, the join pairs, and the local-join
-// threshold check of src/RA/parallel_join.cpp:196-201.
+// threshold check of src/RA/parallel_join.cpp:196-201.  The block first
+// gathers each RA's output offset at every source-bucket bound.
 __global__ void k_plan(PlanArgs a)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int nb1 = a.nb + 1;
+    for (int e = threadIdx.x; e < a.nr * nb1; e += blockDim.x) {
+        int j = e / nb1, s = e % nb1;
+        a.srcoff[(u64)a.ras[j] * nb1 + s] = a.off[j][a.bst[j][s]];
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     u64 tot = 0;
     for (int s = 0; s < a.nb; s++)
         for (int j = 0; j < a.nr; j++) {
-            const u64* so = a.srcoff + (u64)a.ras[j] * (a.nb + 1);
+            const u64* so = a.srcoff + (u64)a.ras[j] * nb1;
             a.posadj[(u64)a.ras[j] * a.nb + s] = tot - so[s];
             u64 cnt = so[s + 1] - so[s];
             if (a.is_join[j] && (long long)cnt > a.thresh) atomicOr(a.flags, PRA_FLAG_THRESHOLD);
@@ -697,7 +705,15 @@ __global__ void k_plan(PlanArgs a)
     a.ctl[1] = tot > a.cap ? 1 : 0;
     if (tot <= a.cap)
         for (int j = 0; j < a.nr; j++)
-            if (a.is_join[j]) *a.joined += a.srcoff[(u64)a.ras[j] * (a.nb + 1) + a.nb];
+            if (a.is_join[j]) *a.joined += a.srcoff[(u64)a.ras[j] * nb1 + a.nb];
+}
+
+// Copy `words` summary words and then the error flags into pinned host
+// memory; the host reads them once the kernel has completed.
+__global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host)
+{
+    for (u64 i = threadIdx.x; i < words; i += blockDim.x) host[i] = src[i];
+    if (threadIdx.x == 0) host[words] = *flags;
 }
 
 // ---------------------------------------------------- dedup / newt ---------
@@ -1157,7 +1173,12 @@ void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap)
 
 void pra_launch_plan(hipStream_t st, const PlanArgs& a)
 {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host)
+{
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(PRA_BLOCK), 0, st, src, words, flags, host);
 }
 
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt)
@@ -1172,10 +1193,6 @@ void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)
     hipLaunchKernelGGL(k_copy_expand, dim3(grid_for(a.n)), dim3(PRA_BLOCK), 0, st, a);
 }
 
-void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart, u64 nb, u64* srcoff)
-{
-    hipLaunchKernelGGL(k_gather_srcoff, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, off, bstart, nb, srcoff);
-}
 
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
 {

@@ -46,7 +46,7 @@ struct CopyExpandArgs {
 #define PRA_MAXRA 64
 
 struct PlanArgs {
-    const u64* srcoff;   // per RA: nb + 1 output offsets at source bucket bounds
+    u64* srcoff;         // per RA: nb + 1 output offsets at source bucket bounds (written here)
     u64* posadj;         // per RA: nb arrival adjustments
     u64* ctl;            // [0] candidates, [1] overflow
     u64* joined;
@@ -56,6 +56,8 @@ struct PlanArgs {
     int nb, nr;
     int ras[PRA_MAXRA];
     int is_join[PRA_MAXRA];
+    const u64* off[PRA_MAXRA];   // per listed RA: exclusive output offsets over its outer rows
+    const u64* bst[PRA_MAXRA];   // per listed RA: first outer row of each source bucket (nb + 1)
 };
 
 struct NewtScatterArgs {
@@ -94,10 +96,10 @@ void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int 
                            const u64* inner, int wi, u64* cnt, u64* istart);
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap);
 void pra_launch_plan(hipStream_t st, const PlanArgs& a);
+void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
 void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
-void pra_launch_gather_srcoff(hipStream_t st, const u64* off, const u64* bstart, u64 nb, u64* srcoff);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
 void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
                            u64* count_out, int intern, const u64* ctl);
