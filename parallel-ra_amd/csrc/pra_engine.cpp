@@ -82,7 +82,7 @@ struct DBuf {
     void reserve(size_t bytes, hipStream_t s, size_t used = 0)
     {
         if (bytes <= cap) return;
-        size_t nc = std::max(bytes, cap + cap / 2);
+        size_t nc = std::max(bytes, 2 * cap);
         nc = std::max<size_t>(nc, 256);
         void* np = nullptr;
         HIPCHK(hipMallocAsync(&np, nc, s));
@@ -1315,6 +1315,13 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         HIPCHK(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) return PRA_E_ARG;
         HIPCHK(hipSetDevice(device));
+        // freed stream-ordered allocations stay in the device's pool for
+        // reuse instead of going back to the driver at every synchronization
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+            uint64_t keep = ~0ULL;
+            HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+        }
         HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
         c->device = device;
         c->nb = nbuckets;
