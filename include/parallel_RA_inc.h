@@ -32,7 +32,11 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
+#include <algorithm>
+#include <fstream>
 #include <iostream>
 #include <map>
 #include <set>
@@ -83,6 +87,30 @@ inline void die(pra_ctx* ctx, int rc, const char* what)
 inline void chk(pra_ctx* ctx, int rc, const char* what)
 {
     if (rc != PRA_OK) die(ctx, rc, what);
+}
+
+inline bool file_exists(const std::string& p) { return access(p.c_str(), F_OK) == 0; }
+
+/* a relation file as the reference writes it: rows (when any) and the size
+ * record "rows\ncols" */
+inline void write_rows_file(const std::string& path, const u64* rows, size_t words, long long size_rows, int size_cols,
+                            bool data)
+{
+    if (data) {
+        FILE* f = fopen(path.c_str(), "wb");
+        if (!f || fwrite(rows, 8, words, f) != words) {
+            std::cout << path << " Wrong IO" << std::endl;
+            MPI_Abort(MPI_COMM_WORLD, -1);
+        }
+        fclose(f);
+    }
+    FILE* m = fopen((path + ".size").c_str(), "w");
+    if (!m) {
+        std::cout << path << ".size Wrong IO" << std::endl;
+        MPI_Abort(MPI_COMM_WORLD, -1);
+    }
+    fprintf(m, "%lld\n%d", size_rows, size_cols);
+    fclose(m);
 }
 
 inline int env_int(const char* name, int dflt)
@@ -181,10 +209,12 @@ public:
     std::string get_filename() { return filename; }
     void set_filename(std::string f) { filename = f; }
     void set_initailization_type(int x) { initailization_type = x; }
-    void set_restart_flag(bool) {}
-    void set_offset_io(bool) {}
-    void set_share_io(bool) {}
-    void set_separate_io(bool) {}
+    /* IO modes of src/relation/balanced_hash_relation.h, set by LIE::execute */
+    bool restart_flag = false, offset_io = false, share_io = false, separate_io = false;
+    void set_restart_flag(bool b) { restart_flag = b; }
+    void set_offset_io(bool b) { offset_io = b; }
+    void set_share_io(bool b) { share_io = b; }
+    void set_separate_io(bool b) { separate_io = b; }
 
     /* counts of this relation's versions */
     int get_full_element_count()
@@ -217,16 +247,74 @@ public:
 
     /* backend side: created by LIE::execute */
     pra_rel* backend() { return handle; }
-    void initialize_relation_on(pra_ctx* ctx)
+
+    /* one relation file into a version (tsv_to_bin layout) */
+    void load_file(pra_ctx* ctx, const std::string& path, int version)
+    {
+        int rc = pra_rel_load_bin(handle, path.c_str(), version == DELTA ? PRA_DELTA : PRA_FULL);
+        if (rc != PRA_OK) {
+            std::cout << path << " Wrong IO: " << pra_last_error(ctx) << std::endl;
+            MPI_Abort(MPI_COMM_WORLD, -1);
+        }
+    }
+
+    /* relation::initialize_relation (src/relation/balanced_hash_relation.cpp:658-793):
+     * on a restart with per-rank files, rank r reads <file>_<r>; here the
+     * buckets of all ranks are read */
+    void initialize_relation_on(pra_ctx* ctx, int nbuckets)
     {
         if (handle) return;
         pra_facade::chk(ctx, pra_rel_create(ctx, arity, join_column_count, intern_tag, is_canonical, &handle),
                         "relation create");
-        if (initailization_type == DELTA || initailization_type == FULL) {
-            int rc = pra_rel_load_bin(handle, filename.c_str(), initailization_type == DELTA ? PRA_DELTA : PRA_FULL);
-            if (rc != PRA_OK) {
-                std::cout << filename << " Wrong IO: " << pra_last_error(ctx) << std::endl;
-                MPI_Abort(MPI_COMM_WORLD, -1);
+        if (initailization_type != DELTA && initailization_type != FULL) return;
+        if (restart_flag && separate_io) {
+            for (int b = 0; b < nbuckets; b++) load_file(ctx, filename + "_" + std::to_string(b), initailization_type);
+        } else {
+            load_file(ctx, filename, initailization_type);
+        }
+    }
+
+    /* the delta of a restart (src/lie/lie.cpp:304-333): <dir>/<name>_delta, or
+     * <dir>/<name>_delta_<rank> per rank; missing files are skipped */
+    void load_restart_delta(pra_ctx* ctx, const std::string& dir, bool separate, int nbuckets)
+    {
+        std::string base = dir + "/" + debug_id + "_delta";
+        if (separate) {
+            for (int b = 0; b < nbuckets; b++) {
+                std::string p = base + "_" + std::to_string(b);
+                if (pra_facade::file_exists(p + ".size")) load_file(ctx, p, DELTA);
+            }
+        } else if (pra_facade::file_exists(base + ".size")) {
+            load_file(ctx, base, DELTA);
+        }
+    }
+
+    /* relation::parallel_IO (src/relation/balanced_hash_relation.cpp:109-430):
+     * <dir>/<name>_full and _delta, either one file of all ranks' rows in rank
+     * order with "rows\narity+1" (written only when non-empty), or per rank
+     * <name>_full_<r> (data when non-empty) with the size record the reference
+     * writes for separate files, "bytes/24\n3" */
+    void parallel_IO(const std::string& dir, bool separate, int nbuckets)
+    {
+        const int w = (int)arity + 1;
+        for (int version : {FULL, DELTA}) {
+            std::string base = dir + "/" + debug_id + (version == FULL ? "_full" : "_delta");
+            std::vector<u64> all = rows(version);
+            size_t n = all.size() / w;
+            if (!separate) {
+                if (n) pra_facade::write_rows_file(base, all.data(), all.size(), (long long)n, w, true);
+                continue;
+            }
+            std::vector<std::vector<u64>> per(nbuckets);
+            for (size_t i = 0; i < n; i++) {
+                const u64* r = all.data() + i * w;
+                u64 b = tuple_hash(r, join_column_count) % (u64)nbuckets;
+                per[b].insert(per[b].end(), r, r + w);
+            }
+            for (int b = 0; b < nbuckets; b++) {
+                size_t bytes = per[b].size() * 8;
+                pra_facade::write_rows_file(base + "_" + std::to_string(b), per[b].data(), per[b].size(),
+                                            (long long)(int)(bytes / 24), 3, bytes > 0);
             }
         }
     }
@@ -443,8 +531,11 @@ class LIE
     std::map<RAM*, std::set<RAM*>> taskgraph;
     std::vector<int> executed_scc_id;
     int loop_counter = 0;
-    std::string output_dir;
+    std::string output_dir, restart_dir_name;
     int cp_iteration = 0;
+    bool enable_io = false, separate_io = false, share_io = false, offset_io = false, restart_flag = false;
+    bool all_to_all_meta_data_dump = false;
+    int nbuckets = 1;
     pra_ctx* ctx = nullptr;
 
     /* first SCC in add order that no remaining SCC points to (src/lie/lie.cpp:27-60) */
@@ -496,50 +587,107 @@ public:
     void set_batch_time(double bt) { batch_time = bt; }
     double get_batch_time() { return batch_time; }
 
-    /* checkpoint / restart controls: accepted, checkpoints are not written */
+    /* checkpoint / restart controls (src/lie/lie.h:65-110) */
     void set_cp_iteration(int it) { cp_iteration = it; }
     void set_output_dir(std::string o) { output_dir = o; }
     void set_executed_scc_id(std::vector<int> id) { executed_scc_id = id; }
     std::vector<int> get_executed_scc_id() { return executed_scc_id; }
     void set_loop_counter(int c) { loop_counter = c; }
     int get_loop_counter() { return loop_counter; }
-    void enable_separate_io() {}
-    void enable_all_to_all_dump() {}
-    void enable_offset_io() {}
-    void enable_share_io() {}
-    void set_restart_dir_name(std::string) {}
-    void set_restart_flag(bool) {}
-    void enable_IO() {}
-    void write_checkpoint_dump(int, std::vector<int>, int = 0) {}
+    void enable_separate_io() { separate_io = true; }
+    /* the reference's per-iteration send-count record (a2a_<rank>) is not
+     * written: this backend has no sender-side buffers to count */
+    void enable_all_to_all_dump() { all_to_all_meta_data_dump = true; }
+    void enable_offset_io() { offset_io = true; }
+    void enable_share_io() { share_io = true; }
+    void set_restart_dir_name(std::string path) { restart_dir_name = path; }
+    void set_restart_flag(bool flag) { restart_flag = flag; }
+    void enable_IO() { enable_io = true; }
+
+    /* LIE::create_checkpoint_dump (src/lie/lie.cpp:182-193) */
+    void create_checkpoint_dump(int loop, int scc_id)
+    {
+        std::string dir = output_dir + "/checkpoint-" + std::to_string(scc_id) + "-" + std::to_string(loop);
+        mkdir(dir.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    }
+
+    /* LIE::write_checkpoint_dump (src/lie/lie.cpp:154-178): scc_metadata (the
+     * executed SCC ids, one per line) and every relation's full and delta */
+    void write_checkpoint_dump(int loop, std::vector<int> executed, int scc_id = 0)
+    {
+        std::string dir = output_dir + "/checkpoint-" + std::to_string(scc_id) + "-" + std::to_string(loop);
+        FILE* fp = fopen((dir + "/scc_metadata").c_str(), "w");
+        if (!fp) {
+            std::cout << dir << "/scc_metadata Wrong IO" << std::endl;
+            MPI_Abort(MPI_COMM_WORLD, -1);
+        }
+        for (int id : executed) fprintf(fp, "%d\n", id);
+        fclose(fp);
+        for (u32 i = 0; i < lie_relation_count; i++) lie_relations[i]->parallel_IO(dir, separate_io, nbuckets);
+    }
 
     pra_ctx* backend() { return ctx; }
 
-    /* LIE::execute (src/lie/lie.cpp:197-448) */
+    /* LIE::execute (src/lie/lie.cpp:197-448), one SCC at a time in steps of
+     * one batch so that checkpoints are written where the reference writes
+     * them (every cp_iteration loops of an SCC). */
     bool execute()
     {
-        int nb = pra_facade::env_int("PRA_BUCKETS", 1);
+        nbuckets = pra_facade::env_int("PRA_BUCKETS", 1);
         int dev = pra_facade::env_int("PRA_DEVICE", 0);
-        int rc = pra_ctx_create(dev, nb, 0, &ctx);
+        int rc = pra_ctx_create(dev, nbuckets, 0, &ctx);
         if (rc != PRA_OK) pra_facade::die(nullptr, rc, "GPU context");
         pra_facade::chk(ctx, pra_ctx_set_batch_size(ctx, batch_size), "set_batch_size");
-        for (u32 i = 0; i < lie_relation_count; i++) lie_relations[i]->initialize_relation_on(ctx);
+        for (u32 i = 0; i < lie_relation_count; i++) {
+            relation* r = lie_relations[i];
+            r->set_restart_flag(restart_flag);
+            r->set_share_io(share_io);
+            r->set_separate_io(separate_io);
+            r->set_offset_io(offset_io);
+            r->initialize_relation_on(ctx, nbuckets);
+        }
+        if (enable_io) mkdir(output_dir.c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
         std::cout << "----------------- Initialization Complete ---------------------" << std::endl << std::endl;
 
+        if (restart_flag) {
+            std::ifstream meta(restart_dir_name + "/scc_metadata");
+            int a;
+            while (meta >> a) executed_scc_id.push_back(a);
+        }
+        const bool io = enable_io && cp_iteration > 0;
         RAM* task;
         while ((task = one_runnable_tasks()) != nullptr) {
-            std::cout << "-------------------Executing SCC " << task->get_id() << "------------------" << std::endl;
+            const int id = task->get_id();
+            if (std::find(executed_scc_id.begin(), executed_scc_id.end(), id) != executed_scc_id.end()) {
+                update_task_graph(task);
+                continue;
+            }
+            std::cout << "-------------------Executing SCC " << id << "------------------" << std::endl;
             pra_scc* s = task->build_on(ctx);
+            if (restart_flag) {
+                relation** rels = task->get_RAM_relations();
+                for (u32 i = 0; i < task->get_ram_relation_count(); i++)
+                    rels[i]->load_restart_delta(ctx, restart_dir_name, separate_io, nbuckets);
+            }
+            pra_facade::chk(ctx, pra_scc_begin(s, restart_flag ? 0 : 1), "SCC start");
             double t0 = MPI_Wtime();
-            uint64_t nbatches = 0;
-            pra_facade::chk(ctx, pra_scc_run(s, nullptr, 0, &nbatches), "SCC execute");
+            int loop = 0;
+            uint64_t delta = 0;
+            do {
+                if (io && loop % cp_iteration == 0) create_checkpoint_dump(loop, id);
+                pra_facade::chk(ctx, pra_scc_step(s, &delta), "SCC execute");
+                if (task->get_iteration_count() == 1 || delta == 0) executed_scc_id.push_back(id);
+                if (io && loop % cp_iteration == 0) write_checkpoint_dump(loop, executed_scc_id, id);
+                loop++;
+            } while (task->get_iteration_count() != 1 && delta != 0);
+            pra_facade::chk(ctx, pra_scc_end(s), "SCC finish");
             double t1 = MPI_Wtime();
             if (task->get_iteration_count() == 1)
                 print_all_relation_size();
             else
-                std::cout << "SCC " << task->get_id() << ": " << nbatches << " batches to the fixed point in "
-                          << (t1 - t0) << " s" << std::endl;
-            executed_scc_id.push_back(task->get_id());
-            loop_counter += (int)nbatches;
+                std::cout << "SCC " << id << ": " << loop << " batches to the fixed point in " << (t1 - t0) << " s"
+                          << std::endl;
+            loop_counter = loop;
             pra_scc_destroy(s);
             update_task_graph(task);
         }

@@ -117,6 +117,18 @@ int pra_scc_add_dependency(pra_scc* before, pra_scc* after);
  * receives the global delta count after every batch (RAM::check_for_fixed_point,
  * src/RAM/RA_tasks.cpp:1072-1085). */
 int pra_scc_run(pra_scc* scc, uint64_t* delta_hist, uint64_t cap, uint64_t* nbatches);
+/* The same SCC run in steps, so a caller can act between batches (the
+ * facade writes checkpoints there, src/lie/lie.cpp:353-423):
+ * pra_scc_begin = the SCC prologue of LIE::execute (status-true relations
+ * move full to delta unless move_full_to_delta is 0, as after a restart that
+ * loaded the deltas, src/lie/lie.cpp:295-333); pra_scc_step = one batch
+ * (RAM::execute_in_batches, src/RAM/RA_tasks.cpp:1121-1269; one iteration for
+ * an SCC created with fixpoint 0), *delta = the global delta after it;
+ * pra_scc_end = delta folded into full (RAM::insert_delta_in_full,
+ * src/lie/lie.cpp:432). */
+int pra_scc_begin(pra_scc* scc, int move_full_to_delta);
+int pra_scc_step(pra_scc* scc, uint64_t* delta);
+int pra_scc_end(pra_scc* scc);
 /* number of batches run by the SCC and their delta counts */
 int pra_scc_history(pra_scc* scc, uint64_t* out, uint64_t cap, uint64_t* n);
 

@@ -33,6 +33,9 @@ int ora_scc_add_copy(ora_ctx *c, int scc, int out, int in, int v, const int *map
 int ora_scc_add_acopy(ora_ctx *c, int scc, int out, int in, int v, const int *map, int n);
 int ora_scc_add_copy_filter(ora_ctx *c, int scc, int out, int in, int v, const int *map, int n, int op);
 int ora_scc_run(ora_ctx *c, int scc, int batch_size);
+int ora_scc_begin(ora_ctx *c, int scc, int move_status);
+u64 ora_scc_step(ora_ctx *c, int scc, int batch_size);
+int ora_scc_end(ora_ctx *c, int scc);
 u64 ora_rel_count(ora_ctx *c, int rel, int version);
 u64 ora_rel_rows(ora_ctx *c, int rel, int version, u64 *out, u64 cap);
 int ora_scc_history(ora_ctx *c, int scc, u64 *out, int cap);
@@ -49,7 +52,7 @@ struct pra_rel {
 };
 struct pra_scc {
     pra_ctx *c;
-    int id, ran;
+    int id, ran, running;
 };
 
 static int fail(pra_ctx *c, int code, const char *msg)
@@ -243,5 +246,33 @@ int pra_scc_run(pra_scc *s, uint64_t *delta_hist, uint64_t cap, uint64_t *nbatch
     s->ran = 1;
     int n = ora_scc_history(s->c->o, s->id, delta_hist, delta_hist ? (int)cap : 0);
     if (nbatches) *nbatches = (uint64_t)n;
+    return PRA_OK;
+}
+
+int pra_scc_begin(pra_scc *s, int move_full_to_delta)
+{
+    if (!s) return PRA_E_ARG;
+    if (s->ran || s->running) return fail(s->c, PRA_E_STATE, "SCC already started");
+    ora_scc_begin(s->c->o, s->id, move_full_to_delta);
+    s->running = 1;
+    return PRA_OK;
+}
+
+int pra_scc_step(pra_scc *s, uint64_t *delta)
+{
+    if (!s) return PRA_E_ARG;
+    if (!s->running) return fail(s->c, PRA_E_STATE, "SCC not started");
+    u64 d = ora_scc_step(s->c->o, s->id, (int)s->c->batch);
+    if (delta) *delta = d;
+    return PRA_OK;
+}
+
+int pra_scc_end(pra_scc *s)
+{
+    if (!s) return PRA_E_ARG;
+    if (!s->running) return fail(s->c, PRA_E_STATE, "SCC not started");
+    ora_scc_end(s->c->o, s->id);
+    s->running = 0;
+    s->ran = 1;
     return PRA_OK;
 }

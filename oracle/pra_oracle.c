@@ -831,26 +831,49 @@ int ora_add_dependency(ora_ctx *c, int before, int after)
 
 /* Run one SCC the way LIE::execute does: status-true relations move full to
  * delta, then either one iteration or batches until the global delta is empty,
- * then delta is folded into full (src/lie/lie.cpp:286-441). */
-int ora_scc_run(ora_ctx *c, int scc, int batch_size)
+ * then delta is folded into full (src/lie/lie.cpp:286-441).  The three steps
+ * are also callable one at a time. */
+int ora_scc_begin(ora_ctx *c, int scc, int move_status)
 {
     scc_t *sc = &c->scc[scc];
-    for (int i = 0; i < sc->nrel; i++)
-        if (sc->status[i]) rel_full_to_delta(c, &c->rel[sc->rel[i]]);
-    if (!sc->fixpoint) {
-        run_iteration(c, sc);
-        hist_push(sc, scc_delta_sum(c, sc));
-    } else {
-        u64 d;
-        do {
-            for (int b = 0; b < batch_size && !limit_hit(c); b++) run_iteration(c, sc);
-            d = scc_delta_sum(c, sc);
-            hist_push(sc, d);
-        } while (d != 0 && !limit_hit(c));
-    }
+    if (move_status)
+        for (int i = 0; i < sc->nrel; i++)
+            if (sc->status[i]) rel_full_to_delta(c, &c->rel[sc->rel[i]]);
+    return 0;
+}
+
+/* one batch (one iteration for a non-fixpoint SCC); returns the global delta */
+u64 ora_scc_step(ora_ctx *c, int scc, int batch_size)
+{
+    scc_t *sc = &c->scc[scc];
+    int iters = sc->fixpoint ? batch_size : 1;
+    for (int b = 0; b < iters && !limit_hit(c); b++) run_iteration(c, sc);
+    u64 d = scc_delta_sum(c, sc);
+    hist_push(sc, d);
+    return d;
+}
+
+int ora_scc_end(ora_ctx *c, int scc)
+{
+    scc_t *sc = &c->scc[scc];
     for (int i = 0; i < sc->nrel; i++) rel_delta_to_full(c, &c->rel[sc->rel[i]]);
     sc->done = 1;
     return 0;
+}
+
+int ora_scc_run(ora_ctx *c, int scc, int batch_size)
+{
+    scc_t *sc = &c->scc[scc];
+    ora_scc_begin(c, scc, 1);
+    if (!sc->fixpoint) {
+        ora_scc_step(c, scc, batch_size);
+    } else {
+        u64 d;
+        do {
+            d = ora_scc_step(c, scc, batch_size);
+        } while (d != 0 && !limit_hit(c));
+    }
+    return ora_scc_end(c, scc);
 }
 
 int ora_execute(ora_ctx *c, int batch_size)

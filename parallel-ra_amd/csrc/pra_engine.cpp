@@ -194,7 +194,7 @@ struct pra_scc {
     std::vector<pra_rel*> rels;
     std::vector<int> status;
     std::vector<RADesc> ras;
-    bool done = false;
+    bool done = false, running = false;
     std::vector<u64> hist;
 };
 
@@ -1239,34 +1239,63 @@ static void delta_to_full(pra_ctx* c, pra_rel* R)
     R->touched(PRA_DELTA);
 }
 
-static void scc_run(pra_scc* S)
+// SCC prologue (src/lie/lie.cpp:286-302): note which relations the rules
+// produce and which full versions they read in order, then move full to delta
+// for the relations added with status true (skipped when a restart loaded the
+// deltas from a checkpoint instead)
+static void scc_begin(pra_scc* S, bool move_status)
 {
     pra_ctx* c = S->ctx;
+    if (S->done || S->running) throw PraError(PRA_E_STATE, "SCC already started");
     for (pra_rel* R : S->rels) R->scc_changing = R->merge_full = false;
     for (auto& ra : S->ras) ra.out->scc_changing = true;
     for (auto& ra : S->ras) {
         if (ra.v0 == PRA_FULL) ra.in0->merge_full = ra.in0->scc_changing;
         if (ra.type == RA_JOIN && ra.v1 == PRA_FULL) ra.in1->merge_full = ra.in1->scc_changing;
     }
-    for (size_t i = 0; i < S->rels.size(); i++)
-        if (S->status[i]) full_to_delta(c, S->rels[i]);
-    if (!S->fixpoint) {
-        run_iteration(S);
-        S->hist.push_back(delta_sum(S));
-    } else {
-        u64 d;
-        do {
-            for (u32 b = 0; b < c->batch_size; b++) run_iteration(S);
-            d = delta_sum(S);
-            S->hist.push_back(d);
-        } while (d != 0);
-    }
+    if (move_status)
+        for (size_t i = 0; i < S->rels.size(); i++)
+            if (S->status[i]) full_to_delta(c, S->rels[i]);
+    S->running = true;
+}
+
+// one batch: a single iteration for an SCC that runs once, else batch_size
+// iterations; records and returns the global delta after it
+// (RAM::execute_in_batches, src/RAM/RA_tasks.cpp:1121-1269)
+static u64 scc_step(pra_scc* S)
+{
+    pra_ctx* c = S->ctx;
+    if (!S->running) throw PraError(PRA_E_STATE, "SCC not started");
+    u32 iters = S->fixpoint ? c->batch_size : 1;
+    for (u32 b = 0; b < iters; b++) run_iteration(S);
+    u64 d = delta_sum(S);
+    S->hist.push_back(d);
+    return d;
+}
+
+// SCC epilogue: delta folded into full for every relation of the SCC
+static void scc_end(pra_scc* S)
+{
+    pra_ctx* c = S->ctx;
+    if (!S->running) throw PraError(PRA_E_STATE, "SCC not started");
     for (pra_rel* R : S->rels) {
         delta_to_full(c, R);
         R->scc_changing = R->merge_full = false;
     }
     sync_check(c);
+    S->running = false;
     S->done = true;
+}
+
+static void scc_run(pra_scc* S)
+{
+    scc_begin(S, true);
+    if (!S->fixpoint)
+        scc_step(S);
+    else
+        while (scc_step(S) != 0) {
+        }
+    scc_end(S);
 }
 
 static pra_scc* one_runnable(pra_ctx* c)
@@ -1674,6 +1703,34 @@ int pra_scc_run(pra_scc* scc, uint64_t* delta_hist, uint64_t cap, uint64_t* nbat
     size_t nb = scc->hist.size() - before;
     if (nbatches) *nbatches = nb;
     for (size_t i = 0; delta_hist && i < nb && i < cap; i++) delta_hist[i] = scc->hist[before + i];
+    API_END
+}
+
+int pra_scc_begin(pra_scc* scc, int move_full_to_delta)
+{
+    if (!scc) return PRA_E_ARG;
+    API_BEGIN(scc->ctx)
+    HIPCHK(hipSetDevice(scc->ctx->device));
+    scc_begin(scc, move_full_to_delta != 0);
+    API_END
+}
+
+int pra_scc_step(pra_scc* scc, uint64_t* delta)
+{
+    if (!scc) return PRA_E_ARG;
+    API_BEGIN(scc->ctx)
+    HIPCHK(hipSetDevice(scc->ctx->device));
+    u64 d = scc_step(scc);
+    if (delta) *delta = d;
+    API_END
+}
+
+int pra_scc_end(pra_scc* scc)
+{
+    if (!scc) return PRA_E_ARG;
+    API_BEGIN(scc->ctx)
+    HIPCHK(hipSetDevice(scc->ctx->device));
+    scc_end(scc);
     API_END
 }
 

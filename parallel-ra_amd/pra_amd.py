@@ -27,7 +27,8 @@ EXPORTS = (
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
     "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
-    "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_history",
+    "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_begin", "pra_scc_step",
+    "pra_scc_end", "pra_scc_history",
     "pra_execute", "pra_execute_many",
     "pra_server_create", "pra_server_submit_tc", "pra_server_wait", "pra_server_rows", "pra_server_destroy",
 )
@@ -90,6 +91,9 @@ def lib():
             "pra_scc_add_copy_filter": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int, ctypes.c_int],
             "pra_scc_add_dependency": [_vp, _vp],
             "pra_scc_run": [_vp, _u64p, _u64, _u64p],
+            "pra_scc_begin": [_vp, ctypes.c_int],
+            "pra_scc_step": [_vp, _u64p],
+            "pra_scc_end": [_vp],
             "pra_scc_history": [_vp, _u64p, _u64, _u64p],
             "pra_execute": [_vp],
             "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],

@@ -348,3 +348,45 @@ def nonlinear_tc_program(b, edges):
     b.add_join(s, T, Tr, DELTA, T, FULL, [1, 3])
     b.add_join(s, T, Tr, FULL, T, DELTA, [1, 3])
     return T, Tr, s
+
+
+def run_tc_checkpointed(exe, root, nb=2, cp=1, restart_from="checkpoint-1-5"):
+    """The reference TC program's checkpoint mode on data/g13236 (a checkpoint
+    every `cp` loops into <root>/out), then its restart mode from
+    out/<restart_from> into <root>/out2 with the final relations dumped to
+    <root>/restart_dump.  Returns (out, restart_dump)."""
+    import shutil
+    import subprocess
+    g5955 = os.path.join(root, "data", "g5955")
+    os.makedirs(g5955, exist_ok=True)
+    os.makedirs(os.path.join(root, "tests"), exist_ok=True)
+    for suffix in ("", ".size"):
+        shutil.copyfile(os.path.join(GOLDEN, "empty_path_2_1_2" + suffix), os.path.join(g5955, "path_2_1_2" + suffix))
+    out = os.path.join(root, "out")
+    env = dict(os.environ, PRA_BUCKETS=str(nb))
+    r = subprocess.run([exe, os.path.join(GOLDEN, "g13236_edge_2_1_2"), out, str(cp)], cwd=os.path.join(root, "tests"),
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert "T: {2}. (573 total facts)" in r.stdout
+    dump = os.path.join(root, "restart_dump")
+    os.makedirs(dump, exist_ok=True)
+    r = subprocess.run([exe, "--restart", os.path.join(out, restart_from), os.path.join(root, "out2"), str(cp)],
+                       cwd=os.path.join(root, "tests"), env=dict(env, PRA_DUMP_DIR=dump), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert "T: {2}. (573 total facts)" in r.stdout, r.stdout[-2000:]
+    return out, dump
+
+
+def checkpoint_digest(out):
+    """{checkpoint dir: {file: [bytes, sha256]}} of a checkpoint output directory"""
+    import hashlib
+    res = {}
+    for d in sorted(os.listdir(out)):
+        files = {}
+        for f in sorted(os.listdir(os.path.join(out, d))):
+            with open(os.path.join(out, d, f), "rb") as fh:
+                data = fh.read()
+            files[f] = [len(data), hashlib.sha256(data).hexdigest()]
+        res[d] = files
+    return res

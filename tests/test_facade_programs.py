@@ -38,3 +38,35 @@ def test_worstcase_program_on_oracle(tmp_path):
     assert r.returncode == 0, r.stderr
     assert total_facts(r.stdout) == REFERENCE_TOTAL
     assert dump_digest(str(tmp_path / "dump")) == json.load(open(EXPECT))["relations"]
+
+
+REF_TC = "/root/reference/tests/transitive_closure.cpp"
+CKPT = os.path.join(GOLDEN, "tc_g13236_k2_checkpoints.json")
+
+
+def _size_record(path):
+    with open(path + ".size") as f:
+        return [int(x) for x in f.read().split()]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TC), reason="reference tree absent")
+def test_tc_checkpoint_restart_on_oracle(tmp_path):
+    # The reference at 2 ranks with a checkpoint every loop writes
+    # checkpoint-1-5 holding |T full| = 534 and |T delta| = 19 and, restarted
+    # from it, reaches 573 (measured on the reference itself).
+    from programs import checkpoint_digest, dump_digest, run_tc_checkpointed
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    exe = ge.build_reference_program(REF_TC, str(tmp_path / "tc_oracle"), libdir=ge.ORACLE, lib="pra_abi_oracle")
+    out, dump = run_tc_checkpointed(exe, str(tmp_path))
+    ck = os.path.join(out, "checkpoint-1-5")
+    full = [_size_record(os.path.join(ck, f"T_full_{b}")) for b in (0, 1)]
+    delta = [_size_record(os.path.join(ck, f"T_delta_{b}")) for b in (0, 1)]
+    assert sum(r for r, _ in full) == 534 and sum(r for r, _ in delta) == 19
+    assert all(c == 3 for _, c in full + delta)
+    assert open(os.path.join(ck, "scc_metadata")).read() == "2\n"
+    # the copy SCC's own checkpoint and one per fixed-point loop (10 batches)
+    assert sorted(os.listdir(out)) == ["checkpoint-1-%d" % i for i in range(10)] + ["checkpoint-2-0"]
+    e = json.load(open(CKPT))
+    assert checkpoint_digest(out) == e["checkpoints"]
+    assert dump_digest(dump) == e["restart_dump"]

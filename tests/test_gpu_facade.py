@@ -75,3 +75,20 @@ def test_worstcase_program_on_gpu(tmp_path):
     got = dump_digest(str(tmp_path / "dump"))
     bad = sorted(k for k in expect["relations"] if got.get(k) != expect["relations"][k])
     assert not bad, [(k, got.get(k), expect["relations"][k]) for k in bad[:5]]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BUILD, "TC_reference")), reason="reference TC not built")
+def test_tc_checkpoint_restart_on_gpu(tmp_path):
+    # every checkpoint file (per-rank full/delta rows with ids, size records,
+    # scc_metadata) byte-identical to the oracle's run, and the restart from
+    # checkpoint-1-5 reaching the same relations
+    import json
+    from programs import checkpoint_digest, dump_digest, run_tc_checkpointed
+    e = json.load(open(os.path.join(GOLDEN, "tc_g13236_k2_checkpoints.json")))
+    print("[facade] TC_reference checkpoint + restart nb=2", flush=True)
+    out, dump = run_tc_checkpointed(os.path.join(BUILD, "TC_reference"), str(tmp_path))
+    got = checkpoint_digest(out)
+    assert sorted(got) == sorted(e["checkpoints"])
+    for d in e["checkpoints"]:
+        assert got[d] == e["checkpoints"][d], d
+    assert dump_digest(dump) == e["restart_dump"]
