@@ -215,6 +215,7 @@ def bench_batch(args):
     inst = config4_instances(count=args.count, nmin=args.nmin, nmax=args.nmax)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     nb = args.buckets
+    log(f"batch: {sum(m for _, m, _ in inst)} edges generated;")
     log(f"batch: {len(graphs)} instances, n in [{min(n for n, _, _ in inst)}, {max(n for n, _, _ in inst)}]")
 
     def make(e):
@@ -226,7 +227,7 @@ def bench_batch(args):
     # plain: one instance at a time
     plain = []
     t_plain = 0.0
-    for e in graphs:
+    for k, e in enumerate(graphs):
         c, (T, G, sfix, _) = make(e)
         t0 = time.perf_counter()
         c.execute()
@@ -234,6 +235,8 @@ def bench_batch(args):
         t_plain += time.perf_counter() - t0
         plain.append((c.count(T), c.joined(), c.checksum(T)))
         c.close()
+        if (k + 1) % 32 == 0:
+            log(f"  plain: {k + 1}/{len(graphs)} instances, {t_plain:.2f} s")
     # batched: waves admitted by estimated bytes, largest first
     order = sorted(range(len(graphs)), key=lambda i: -inst[i][0])
     budget = int(args.budget_gb * (1 << 30))
@@ -252,10 +255,13 @@ def bench_batch(args):
         for c, _ in ctxs:
             c.synchronize()
         t_batch += time.perf_counter() - t0
-        assert rc == [0] * len(ctxs), rc
+        if rc != [0] * len(ctxs):
+            bad = [(wave[k], inst[wave[k]][0], rc[k], c.last_error()) for k, (c, _) in enumerate(ctxs) if rc[k]]
+            raise RuntimeError(f"batched instances failed: {bad}")
         for i, (c, (T, G, sfix, _)) in zip(wave, ctxs):
             batched[i] = (c.count(T), c.joined(), c.checksum(T))
             c.close()
+        log(f"  batched: wave of {len(wave)} done, {pos}/{len(order)} instances, {t_batch:.2f} s")
     exact = all(a == b for a, b in zip(plain, batched))
     joined = sum(j for _, j, _ in plain)
     log(f"plain {t_plain:.3f} s, batched {t_batch:.3f} s, bit-exact per instance: {exact}")

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -48,17 +49,116 @@ static void check_launch(const char* what)
     if (e != hipSuccess) throw PraError(PRA_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Device buffer in stream-ordered memory of the owning context's stream:
+// Blocks of at least 64 MiB are kept in a process-wide cache when released
+// instead of going back to the driver, and handed to later requests of the
+// same power-of-two size class (or the next one) on any stream of the device:
+// returning tens of GB to the driver, and first use of fresh device memory,
+// cost far more than the work such blocks hold.  An event recorded where a
+// block was released orders its reuse.  Out of memory flushes the cache.
+namespace {
+constexpr size_t kBigBlock = 64ULL << 20;
+struct BigFree {
+    void* p;
+    size_t size;
+    int device;
+    hipEvent_t ready;
+};
+std::mutex g_big_mu;
+std::vector<BigFree> g_big_free;
+
+size_t big_class(size_t bytes)
+{
+    size_t c = kBigBlock;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+void big_flush(int dev)
+{
+    std::vector<BigFree> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_big_mu);
+        for (size_t i = 0; i < g_big_free.size();)
+            if (g_big_free[i].device == dev) {
+                drop.push_back(g_big_free[i]);
+                g_big_free.erase(g_big_free.begin() + i);
+            } else {
+                i++;
+            }
+    }
+    for (auto& b : drop) {
+        (void)hipEventSynchronize(b.ready);
+        (void)hipEventDestroy(b.ready);
+        (void)hipFree(b.p);
+    }
+}
+
+void* big_get(size_t bytes, hipStream_t st, size_t* got)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    const size_t cls = big_class(bytes);
+    BigFree hit{nullptr, 0, dev, nullptr};
+    {
+        std::lock_guard<std::mutex> lk(g_big_mu);
+        long best = -1;
+        for (size_t i = 0; i < g_big_free.size(); i++) {
+            const BigFree& b = g_big_free[i];
+            if (b.device == dev && b.size >= cls && b.size <= 2 * cls &&
+                (best < 0 || b.size < g_big_free[(size_t)best].size))
+                best = (long)i;
+        }
+        if (best >= 0) {
+            hit = g_big_free[(size_t)best];
+            g_big_free.erase(g_big_free.begin() + best);
+        }
+    }
+    if (hit.p) {
+        HIPCHK(hipStreamWaitEvent(st, hit.ready, 0));
+        HIPCHK(hipEventDestroy(hit.ready));
+        *got = hit.size;
+        return hit.p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, cls) != hipSuccess) {
+        (void)hipGetLastError();
+        big_flush(dev);
+        HIPCHK(hipMalloc(&p, cls));
+    }
+    *got = cls;
+    return p;
+}
+
+void big_put(void* p, size_t size, hipStream_t st)
+{
+    int dev = 0;
+    hipEvent_t ev = nullptr;
+    if (hipGetDevice(&dev) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(ev, st) != hipSuccess) {
+        // cannot order the reuse: give the block back to the driver instead
+        (void)hipStreamSynchronize(st);
+        if (ev) (void)hipEventDestroy(ev);
+        (void)hipFree(p);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_big_mu);
+    g_big_free.push_back(BigFree{p, size, dev, ev});
+}
+} // namespace
+
+// Device buffer owned by one context and ordered on its stream: small blocks
+// come from the stream-ordered pool, large ones from the block cache above;
 // growth and release never synchronize the device, so contexts driven from
 // different host threads overlap.
 struct DBuf {
     void* p = nullptr;
     size_t cap = 0;
     hipStream_t st = nullptr;
+    bool big = false;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap), st(o.st) { o.p = nullptr; o.cap = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap), st(o.st), big(o.big) { o.p = nullptr; o.cap = 0; }
     DBuf& operator=(DBuf&& o) noexcept
     {
         if (this != &o) {
@@ -66,6 +166,7 @@ struct DBuf {
             p = o.p;
             cap = o.cap;
             st = o.st;
+            big = o.big;
             o.p = nullptr;
             o.cap = 0;
         }
@@ -74,9 +175,15 @@ struct DBuf {
     ~DBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFreeAsync(p, st);
+        if (p) {
+            if (big)
+                big_put(p, cap, st);
+            else
+                (void)hipFreeAsync(p, st);
+        }
         p = nullptr;
         cap = 0;
+        big = false;
     }
     // grow to at least `bytes`, preserving the first `used` bytes
     void reserve(size_t bytes, hipStream_t s, size_t used = 0)
@@ -85,12 +192,27 @@ struct DBuf {
         size_t nc = std::max(bytes, 2 * cap);
         nc = std::max<size_t>(nc, 256);
         void* np = nullptr;
-        HIPCHK(hipMallocAsync(&np, nc, s));
+        bool nbig = nc >= kBigBlock;
+        if (nbig) {
+            np = big_get(nc, s, &nc);
+        } else if (hipMallocAsync(&np, nc, s) != hipSuccess) {
+            int dev = 0;
+            (void)hipGetLastError();
+            HIPCHK(hipGetDevice(&dev));
+            big_flush(dev);
+            HIPCHK(hipMallocAsync(&np, nc, s));
+        }
         if (p && used) HIPCHK(hipMemcpyAsync(np, p, used, hipMemcpyDeviceToDevice, s));
-        if (p) HIPCHK(hipFreeAsync(p, s));
+        if (p) {
+            if (big)
+                big_put(p, cap, s);
+            else
+                HIPCHK(hipFreeAsync(p, s));
+        }
         p = np;
         cap = nc;
         st = s;
+        big = nbig;
     }
     template <typename T> T* as() { return static_cast<T*>(p); }
     void swap(DBuf& o)
@@ -98,6 +220,7 @@ struct DBuf {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
         std::swap(st, o.st);
+        std::swap(big, o.big);
     }
 };
 
@@ -213,7 +336,7 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_hist, rep_words;
+    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_hist, rep_words, rctl, ttab, kp;
     u64 dict_cap = 0, dict_n = 0;
     int dict_bits = 0, dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -249,7 +372,7 @@ struct pra_ctx {
         seq.clear();
         for (DBuf* b : {&partials, &hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_hist, &rep_words})
+                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_hist, &rep_words, &rctl, &ttab, &kp})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -405,9 +528,6 @@ static void check_flags(u32 f)
         throw PraError(PRA_E_COLLISION, "64-bit key hash collision between distinct keys");
     if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
     if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort look-back did not complete");
-    if (f & PRA_FLAG_THRESHOLD)
-        throw PraError(PRA_E_UNSUPPORTED, "a source bucket produced more join outputs than the local-join "
-                                          "threshold; threshold rounds are not implemented on the device");
 }
 
 // Read back `words` u64 from device memory in one synchronization, together
@@ -833,6 +953,189 @@ static void merge_into_full(pra_ctx* c, pra_rel* R)
     R->touched(PRA_FULL);
 }
 
+// One threshold round of an output relation: the rules emitting (RA index
+// order) and, per listed rule, the outer rows [lo, hi) of every source.
+struct RoundSpec {
+    std::vector<int> ras;
+    std::vector<u64> lo, hi;   // per listed rule, nb each
+    DBuf* bounds = nullptr;    // the same on the device: per listed rule lo[nb] then hi[nb]
+    u64 nt_base = 0;           // newt rows written by earlier rounds
+    u64 cap = 0;               // candidates of this round
+};
+
+static u64 read_word(pra_ctx* c, const u64* p) { return readback(c, p, 1)[0]; }
+
+// Where the reference's local_join call over outer rows [start, end) of rule r
+// stops (src/RA/parallel_join.cpp:157-230): its outputs are deduplicated per
+// call, and the call returns after the first row through which it emitted
+// more than the threshold.  Returns that row relative to start, or end-start.
+static u64 find_split(pra_ctx* c, pra_scc* S, int r, u64 start, u64 end, int cw)
+{
+    RADesc& ra = S->ras[r];
+    const int nb = c->nb;
+    const u64 n = end - start;
+    const u64 base = read_word(c, ra.off.as<u64>() + start);
+    const u64 raw = read_word(c, ra.off.as<u64>() + end) - base;
+    c->rctl.reserve((size_t)(nb + 4) * 8, c->st);
+    u64* tctl = c->rctl.as<u64>();        // [0] candidates, [1] overflow
+    u64* tout = tctl + 2;                  // split row
+    u64* tposadj = tctl + 4;               // per source: -base
+    pra_launch_fill_u64(c->st, tctl, 1, raw);
+    pra_launch_fill_u64(c->st, tctl + 1, 1, 0);
+    pra_launch_fill_u64(c->st, tposadj, (u64)nb, (u64)0 - base);
+    c->cand.reserve(raw * cw * 8, c->st);
+    c->dest.reserve(raw * 4, c->st);
+    c->slot_of.reserve(raw * 8, c->st);
+    c->keep.reserve(raw * 4, c->st);
+    {
+        JoinExpandArgs a;
+        memset(&a, 0, sizeof(a));
+        a.ctl = tctl;
+        a.outer = ra.outer->ver[ra.outer_v].d() + start * ra.outer->w;
+        a.inner = ra.inner->ver[ra.inner_v].d();
+        a.off = ra.off.as<u64>() + start;
+        a.istart = ra.istart.as<u64>() + start;
+        a.posadj = tposadj;
+        a.cand = c->cand.as<u64>();
+        a.dest = c->dest.as<u32>();
+        a.n_outer = n;
+        a.nb = (u64)nb;
+        a.wo = ra.outer->w;
+        a.wi = ra.inner->w;
+        a.jcc = ra.in0->jcc;
+        a.left = ra.left;
+        a.cw = cw;
+        a.out_jcc = ra.out->jcc;
+        for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+        pra_launch_join_expand(c->st, a, raw);
+        check_launch("split_expand");
+    }
+    // first occurrences inside the call: a fresh table keyed on whole outputs
+    const u64 tcap = pow2_at_least(std::max<u64>(1024, 2 * raw));
+    c->ttab.reserve(tcap * (2 + cw) * 8, c->st);
+    pra_launch_table_clear(c->st, c->ttab.as<u64>(), tcap, 2 + cw);
+    pra_launch_dedup_claim2(c->st, c->cand.as<u64>(), tctl, raw, cw, cw, c->ttab.as<u64>(), tcap,
+                            c->slot_of.as<u64>(), 1, c->dflags());
+    const u32 nblocks = pra_tiles(raw);
+    c->hist.reserve((size_t)nb * nblocks * 4, c->st);
+    pra_launch_dedup_decide_hist(c->st, c->cand.as<u64>(), tctl, cw, cw, c->ttab.as<u64>(), c->slot_of.as<u64>(), 1,
+                                 c->keep.as<u32>(), c->dest.as<u32>(), (u32)nb, c->hist.as<u32>(), nblocks,
+                                 c->dflags());
+    check_launch("split_dedup");
+    c->kp.reserve((raw + 1) * 4, c->st);
+    scan_u32(c, c->keep.as<u32>(), c->kp.as<u32>(), raw, c->kp.as<u32>() + raw);
+    pra_launch_find_split(c->st, ra.off.as<u64>() + start, n, c->kp.as<u32>(), c->threshold, tout);
+    check_launch("find_split");
+    return read_word(c, tout);
+}
+
+// The round loop of RAM::execute_in_batches for one output relation whose
+// join calls exceed the local-join threshold (src/RAM/RA_tasks.cpp:1121-1269):
+// round k emits, per source and rule, the rows from where that call stopped
+// in round k-1 to where it stops now; copies and finished calls would only
+// re-emit tuples that already arrived, so they emit in the first round only.
+// Interning and first-arrival dedup run per round, rounds in order.  Returns
+// the new tuples of all rounds.
+template <class F>
+static u64 run_rounds(pra_scc* S, pra_rel* R, F&& run_round, u64* st2o_dev)
+{
+    pra_ctx* c = S->ctx;
+    const int nb = c->nb;
+    const int cw = R->arity;   // rounds happen for join outputs, which are interned
+    struct Call {
+        int r;
+        std::vector<u64> bst, start;
+        std::vector<char> done;
+    };
+    std::vector<Call> joins;
+    std::vector<int> copies;
+    for (int r = 0; r < (int)S->ras.size(); r++) {
+        RADesc& ra = S->ras[r];
+        if (ra.out != R || !ra.active) continue;
+        if (ra.type != RA_JOIN) {
+            copies.push_back(r);
+            continue;
+        }
+        Call cl;
+        cl.r = r;
+        u64* hb = readback(c, ra.outer->bnd[ra.outer_v].p, (size_t)nb + 1);
+        cl.bst.assign(hb, hb + nb + 1);
+        cl.start.assign(cl.bst.begin(), cl.bst.begin() + nb);
+        cl.done.resize(nb);
+        for (int s = 0; s < nb; s++) cl.done[s] = cl.start[s] == cl.bst[s + 1];
+        joins.push_back(std::move(cl));
+    }
+    DBuf bounds;
+    u64 total = 0;
+    for (int round = 0;; round++) {
+        // (rule, lo, hi) of this round, in rule order
+        std::vector<std::tuple<int, std::vector<u64>, std::vector<u64>>> rules;
+        u64 cap = 0;
+        for (auto& cl : joins) {
+            RADesc& ra = S->ras[cl.r];
+            std::vector<u64> lo(cl.start), hi(cl.start);
+            bool any = false;
+            for (int s = 0; s < nb; s++) {
+                if (cl.done[s]) continue;
+                u64 end = cl.bst[s + 1];
+                u64 o0 = read_word(c, ra.off.as<u64>() + cl.start[s]);
+                u64 stop = end;
+                if ((long long)(read_word(c, ra.off.as<u64>() + end) - o0) > c->threshold)
+                    stop = std::min(end, cl.start[s] + find_split(c, S, cl.r, cl.start[s], end, cw) + 1);
+                hi[s] = stop;
+                cap += read_word(c, ra.off.as<u64>() + stop) - o0;
+                any = true;
+            }
+            if (any) rules.emplace_back(cl.r, lo, hi);
+        }
+        if (round == 0)
+            for (int r : copies) {
+                RADesc& ra = S->ras[r];
+                u64* hb = readback(c, ra.outer->bnd[ra.outer_v].p, (size_t)nb + 1);
+                std::vector<u64> lo(hb, hb + nb), hi(hb + 1, hb + nb + 1);
+                cap += read_word(c, ra.off.as<u64>() + ra.n_outer);
+                rules.emplace_back(r, lo, hi);
+            }
+        if (rules.empty()) break;
+        std::sort(rules.begin(), rules.end(),
+                  [](const auto& a, const auto& b) { return std::get<0>(a) < std::get<0>(b); });
+        RoundSpec rs;
+        rs.bounds = &bounds;
+        rs.nt_base = total;
+        rs.cap = cap;
+        std::vector<u64> dev;
+        for (auto& t : rules) {
+            rs.ras.push_back(std::get<0>(t));
+            rs.lo.insert(rs.lo.end(), std::get<1>(t).begin(), std::get<1>(t).end());
+            rs.hi.insert(rs.hi.end(), std::get<2>(t).begin(), std::get<2>(t).end());
+            dev.insert(dev.end(), std::get<1>(t).begin(), std::get<1>(t).end());
+            dev.insert(dev.end(), std::get<2>(t).begin(), std::get<2>(t).end());
+        }
+        bounds.reserve(dev.size() * 8, c->st);
+        HIPCHK(hipMemcpyAsync(bounds.p, dev.data(), dev.size() * 8, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (cap) {
+            run_round(&rs);
+            u64 cnt = publish_readback(c, st2o_dev, 8)[0];
+            total += cnt;
+            R->mt_entries += cnt;
+        }
+        bool all = true;
+        for (auto& cl : joins) {
+            for (auto& t : rules)
+                if (std::get<0>(t) == cl.r)
+                    for (int s = 0; s < nb; s++)
+                        if (!cl.done[s]) {
+                            cl.start[s] = std::get<2>(t)[s];
+                            cl.done[s] = cl.start[s] == cl.bst[s + 1];
+                        }
+            for (int s = 0; s < nb; s++) all = all && cl.done[s];
+        }
+        if (all) break;
+    }
+    return total;
+}
+
 // ------------------------------------------------------------ iteration ---
 
 static bool is_intern_type(int type) { return type != RA_ACOPY; }
@@ -872,7 +1175,7 @@ static void run_iteration(pra_scc* S)
 
     // 1. per-RA output counts and their exclusive offsets (the arrival plan
     //    reads them at the source-bucket bounds)
-    c->stage.reserve((size_t)std::max(1, nra) * (nb + 1) * 8, c->st);
+    c->stage.reserve((size_t)std::max(1, nra) * 2 * nb * 8, c->st);
     for (int r = 0; r < nra; r++) {
         RADesc& ra = S->ras[r];
         Rows& O = ra.outer->ver[ra.outer_v];
@@ -968,9 +1271,12 @@ static void run_iteration(pra_scc* S)
 
     // 3+4. per output: arrival plan, emit candidates, deduplicate, partition
     //      and intern, summaries for the sort
-    auto run_output = [&](size_t oi) {
+    // rs == nullptr: the iteration's whole emission; else one threshold round
+    auto run_output = [&](size_t oi, const RoundSpec* rs) {
         pra_rel* R = outs[oi];
-        const u64 C = cap[oi];
+        const u64 C = rs ? rs->cap : cap[oi];
+        const bool pk = packed[oi] && !rs;
+        const u64 nt_base = rs ? rs->nt_base : 0;
         const bool intern = out_intern[oi] == 1;
         const int cw = intern ? R->arity : R->arity + 1;
         const int kw = cw;
@@ -982,7 +1288,7 @@ static void run_iteration(pra_scc* S)
         c->dest.reserve(C * 4, c->st);
         c->slot_of.reserve(C * 8, c->st);
         c->keep.reserve(C * 4, c->st);
-        NT.buf.reserve(C * R->w * 8, c->st);
+        NT.buf.reserve((nt_base + C) * R->w * 8, c->st, nt_base * R->w * 8);
         {
             PlanArgs pa;
             memset(&pa, 0, sizeof(pa));
@@ -994,14 +1300,29 @@ static void run_iteration(pra_scc* S)
             pa.cap = C;
             pa.thresh = c->threshold;
             pa.nb = nb;
-            for (int r = 0; r < nra; r++) {
-                RADesc& ra = S->ras[r];
-                if (ra.out != R || !ra.active) continue;
-                pa.ras[pa.nr] = r;
-                pa.is_join[pa.nr] = ra.type == RA_JOIN;
-                pa.off[pa.nr] = ra.off.as<u64>();
-                pa.bst[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>();
-                pa.nr++;
+            pa.rounds_check = rs ? 0 : 1;
+            pa.count_joined = rs ? 0 : 1;
+            if (rs) {
+                for (size_t k = 0; k < rs->ras.size(); k++) {
+                    RADesc& ra = S->ras[rs->ras[k]];
+                    pa.ras[pa.nr] = rs->ras[k];
+                    pa.is_join[pa.nr] = ra.type == RA_JOIN;
+                    pa.off[pa.nr] = ra.off.as<u64>();
+                    pa.lo[pa.nr] = rs->bounds->as<u64>() + k * 2 * nb;
+                    pa.hi[pa.nr] = rs->bounds->as<u64>() + k * 2 * nb + nb;
+                    pa.nr++;
+                }
+            } else {
+                for (int r = 0; r < nra; r++) {
+                    RADesc& ra = S->ras[r];
+                    if (ra.out != R || !ra.active) continue;
+                    pa.ras[pa.nr] = r;
+                    pa.is_join[pa.nr] = ra.type == RA_JOIN;
+                    pa.off[pa.nr] = ra.off.as<u64>();
+                    pa.lo[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>();
+                    pa.hi[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>() + 1;
+                    pa.nr++;
+                }
             }
             pra_launch_plan(c->st, pa);
             check_launch("plan");
@@ -1009,19 +1330,29 @@ static void run_iteration(pra_scc* S)
         for (int r = 0; r < nra; r++) {
             RADesc& ra = S->ras[r];
             if (ra.out != R || !ra.active) continue;
+            size_t k = 0;   // position in the round's rule list
+            if (rs) {
+                while (k < rs->ras.size() && rs->ras[k] != r) k++;
+                if (k == rs->ras.size()) continue;
+            }
             Rows& O = ra.outer->ver[ra.outer_v];
             if (ra.type == RA_JOIN) {
+                // a round emits one window of rows per source; otherwise all rows at once
+                const int nwin = rs ? nb : 1;
+                for (int s = 0; s < nwin; s++) {
+                u64 lo = rs ? rs->lo[k * nb + s] : 0, hi = rs ? rs->hi[k * nb + s] : ra.n_outer;
+                if (hi <= lo) continue;
                 JoinExpandArgs a;
                 memset(&a, 0, sizeof(a));
                 a.ctl = ctl;
-                a.outer = O.d();
+                a.outer = O.d() + lo * ra.outer->w;
                 a.inner = ra.inner->ver[ra.inner_v].d();
-                a.off = ra.off.as<u64>();
-                a.istart = ra.istart.as<u64>();
+                a.off = ra.off.as<u64>() + lo;
+                a.istart = ra.istart.as<u64>() + lo;
                 a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
                 a.cand = c->cand.as<u64>();
                 a.dest = c->dest.as<u32>();
-                a.n_outer = ra.n_outer;
+                a.n_outer = hi - lo;
                 a.nb = (u64)nb;
                 a.wo = ra.outer->w;
                 a.wi = ra.inner->w;
@@ -1033,6 +1364,7 @@ static void run_iteration(pra_scc* S)
                 ProfPerCand p(c, K_JOIN_EXPAND, 8.0 * ra.inner->w + 8.0 * cw, oi);
                 pra_launch_join_expand(c->st, a, std::min<u64>(C, ub[oi]));
                 check_launch("join_expand");
+                }
             } else {
                 CopyExpandArgs a;
                 memset(&a, 0, sizeof(a));
@@ -1084,7 +1416,7 @@ static void run_iteration(pra_scc* S)
             a.dest = c->dest.as<u32>();
             a.offs = c->offs.as<u32>();
             a.seqbase = sq ? sq->as<u64>() : nullptr;
-            a.newt = NT.d();
+            a.newt = NT.d() + nt_base * R->w;
             a.nblocks = nblocks;
             a.nb = (u32)nb;
             a.cw = cw;
@@ -1092,7 +1424,7 @@ static void run_iteration(pra_scc* S)
             a.intern = intern ? 1 : 0;
             a.tag = R->tag;
             a.rep_words = c->rep_words.as<u64>();
-            if (packed[oi]) {
+            if (pk) {
                 R->nkeys.reserve(C * 8, c->st);
                 a.dict = c->dict.as<u64>();
                 a.dict_cap = c->dict_cap;
@@ -1106,7 +1438,7 @@ static void run_iteration(pra_scc* S)
             check_launch("dedup_scatter");
         }
         R->kgofs.reserve(8 * PRA_RADIX * 4, c->st);
-        pra_launch_newt_fold(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>(), ctl, packed[oi] ? 1 : 0, st2o,
+        pra_launch_newt_fold(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>(), ctl, pk ? 1 : 0, st2o,
                              R->kgofs.as<u32>());
         check_launch("newt_fold");
         pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
@@ -1114,7 +1446,7 @@ static void run_iteration(pra_scc* S)
         check_launch("seq_update");
     };
     for (size_t oi = 0; oi < no; oi++)
-        if (cap[oi]) run_output(oi);
+        if (cap[oi]) run_output(oi, nullptr);
 
     // 5. new-tuple counts, sort-key summaries and error flags: the
     //    per-iteration host readback (repeated once for outputs whose
@@ -1126,15 +1458,25 @@ static void run_iteration(pra_scc* S)
         for (size_t oi = 0; oi < no; oi++) {
             if (!cap[oi]) continue;
             std::copy(h + oi * SW, h + (oi + 1) * SW, st2.begin() + oi * SW);
-            if (st2[oi * SW + 5]) redo.push_back(oi);
+            if (st2[oi * SW + 5] == 1) redo.push_back(oi);
         }
         if (redo.empty()) break;
         if (pass == 1) throw PraError(PRA_E_HIP, "candidate capacity overflow after resizing");
         for (size_t oi : redo) {
             cap[oi] = st2[oi * SW + 4];
-            run_output(oi);
+            run_output(oi, nullptr);
         }
     }
+    // outputs with a join call over the local-join threshold: emitted in
+    // rounds (src/RAM/RA_tasks.cpp:1121-1269, src/RA/parallel_join.cpp:157-230)
+    std::vector<char> in_rounds(no, 0);
+    for (size_t oi = 0; oi < no; oi++)
+        if (cap[oi] && st2[oi * SW + 5] == 2) {
+            st2[oi * SW] = run_rounds(S, outs[oi], [&](const RoundSpec* rs) { run_output(oi, rs); },
+                                      c->stage2.as<u64>() + oi * SW);
+            st2[oi * SW + 5] = 0;
+            in_rounds[oi] = 1;
+        }
     for (auto& pc : c->per_candidate)
         if (std::get<0>(pc) < c->pending.size())
             c->pending[std::get<0>(pc)].bytes = std::get<1>(pc) * (double)st2[std::get<2>(pc) * SW + 4];
@@ -1145,7 +1487,7 @@ static void run_iteration(pra_scc* S)
         R->jhint = cap[oi] ? st2[oi * SW + 4] : 0;
         R->ver[PRA_NEWT].n = cnt;
         R->ver[PRA_NEWT].sorted = false;
-        R->mt_entries += cnt;
+        if (!in_rounds[oi]) R->mt_entries += cnt;   // rounds count their own
     }
 
     // 6. delta -> full, sort newt into trie order, newt -> delta
@@ -1166,15 +1508,16 @@ static void run_iteration(pra_scc* S)
         Rows& NT = R->ver[PRA_NEWT];
         R->bnd[PRA_DELTA].reserve((size_t)(nb + 1) * 8, c->st);
         u64* bnd = R->bnd[PRA_DELTA].as<u64>();
-        if (packed[oi] && st2[oi * SW + 3] == 0) {
+        if (packed[oi] && !in_rounds[oi] && st2[oi * SW + 3] == 0) {
             sort_by_packed_key(c, NT, R->w, R->nkeys, R->kgofs.as<u32>(), st2[oi * SW + 1], st2[oi * SW + 2],
                                R->arity * c->dict_bits, bnd);
             bnd_ok[oi] = 1;
             continue;
         }
-        // packed mode collects no column masks: a dictionary miss sorts every digit
-        const u64* masks = packed[oi] ? nullptr : &st2[oi * SW + 8];
-        if (packed[oi] && c->dict_state == 1)
+        // packed mode collects no column masks (a dictionary miss sorts every
+        // digit), and a round's masks cover that round only
+        const u64* masks = (packed[oi] || in_rounds[oi]) ? nullptr : &st2[oi * SW + 8];
+        if (packed[oi] && !in_rounds[oi] && c->dict_state == 1)
             c->dict_state = ++c->dict_rebuilds > 3 ? 2 : 0;
         bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, masks, bnd);
         if (!bnd_ok[oi] && nb == 1) {
@@ -1344,11 +1687,17 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         HIPCHK(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) return PRA_E_ARG;
         HIPCHK(hipSetDevice(device));
-        // freed stream-ordered allocations stay in the device's pool for
-        // reuse instead of going back to the driver at every synchronization
+        // freed stream-ordered allocations stay in the device's pool for reuse
+        // (instead of going back to the driver at every synchronization) up to
+        // a quarter of the device memory, so that many contexts in a row do
+        // not hoard fragments until allocations fail
         hipMemPool_t pool = nullptr;
-        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
-            uint64_t keep = ~0ULL;
+        size_t free_b = 0, total_b = 0;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool &&
+            hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            uint64_t keep = total_b / 4;
+            const char* env = getenv("PRA_POOL_RELEASE_GB");   // bytes the pool keeps, in GiB
+            if (env && *env) keep = (uint64_t)(atof(env) * (double)(1ULL << 30));
             HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
         }
         HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));

@@ -597,10 +597,10 @@ __device__ inline u64 pra_row_of(const u64* off, u64 lo, u64 hi, u64 x)
 __global__ void k_join_expand(JoinExpandArgs a)
 {
     if (a.ctl[1]) return;
-    const u64 total = a.off[a.n_outer];
+    const u64 first = a.off[0], total = a.off[a.n_outer];   // first > 0 for a window of rows
     __shared__ u64 s_lo, s_hi;
     const u64 step = (u64)gridDim.x * blockDim.x;
-    for (u64 o0 = (u64)blockIdx.x * blockDim.x; o0 < total; o0 += step) {
+    for (u64 o0 = first + (u64)blockIdx.x * blockDim.x; o0 < total; o0 += step) {
         __syncthreads();
         if (threadIdx.x == 0) {
             u64 last = o0 + blockDim.x - 1 < total ? o0 + blockDim.x - 1 : total - 1;
@@ -674,38 +674,59 @@ __global__ void k_copy_expand(CopyExpandArgs a)
     a.dest[pos] = a.nb == 1 ? 0u : (u32)(h % a.nb);
 }
 
-// srcoff[s] = off[bstart[s]] for s in [0, nb]
 // Arrival plan of one output relation, computed on the device: for source
 // bucket s and the RAs producing the relation (in RA order), the position of
 // their outputs in the arrival order source-major, RA-minor
 // (src/comm/all_to_allv_comm.cpp:64-142), the candidate count, whether it
-// exceeds the allocated capacity, the join pairs, and the local-join
-// threshold check of src/RA/parallel_join.cpp:196-201.  The block first
-// gathers each RA's output offset at every source-bucket bound.
+// exceeds the allocated capacity, and the join pairs.  RA j emits the outer
+// rows [lo[j][s], hi[j][s]) of source s (whole buckets, or one round's rows).
+// With rounds_check, a join call with more than `thresh` outputs from one
+// source (src/RA/parallel_join.cpp:196-201, src/RAM/RA_tasks.cpp:312) marks
+// the output for threshold rounds (ctl[1] = 2) instead of emitting.
 __global__ void k_plan(PlanArgs a)
 {
-    const int nb1 = a.nb + 1;
-    for (int e = threadIdx.x; e < a.nr * nb1; e += blockDim.x) {
-        int j = e / nb1, s = e % nb1;
-        a.srcoff[(u64)a.ras[j] * nb1 + s] = a.off[j][a.bst[j][s]];
+    const int nb = a.nb;
+    for (int e = threadIdx.x; e < a.nr * nb; e += blockDim.x) {
+        int j = e / nb, s = e % nb;
+        u64 lo = a.off[j][a.lo[j][s]], hi = a.off[j][a.hi[j][s]];
+        a.srcoff[(u64)a.ras[j] * 2 * nb + s] = lo;
+        a.srcoff[(u64)a.ras[j] * 2 * nb + nb + s] = hi - lo;
     }
     __threadfence_block();
     __syncthreads();
     if (threadIdx.x != 0) return;
-    u64 tot = 0;
-    for (int s = 0; s < a.nb; s++)
+    u64 tot = 0, jtot = 0;
+    bool rounds = false;
+    for (int s = 0; s < nb; s++)
         for (int j = 0; j < a.nr; j++) {
-            const u64* so = a.srcoff + (u64)a.ras[j] * nb1;
-            a.posadj[(u64)a.ras[j] * a.nb + s] = tot - so[s];
-            u64 cnt = so[s + 1] - so[s];
-            if (a.is_join[j] && (long long)cnt > a.thresh) atomicOr(a.flags, PRA_FLAG_THRESHOLD);
+            const u64* so = a.srcoff + (u64)a.ras[j] * 2 * nb;
+            a.posadj[(u64)a.ras[j] * nb + s] = tot - so[s];
+            u64 cnt = so[nb + s];
+            if (a.is_join[j]) {
+                if (a.rounds_check && (long long)cnt > a.thresh) rounds = true;
+                jtot += cnt;
+            }
             tot += cnt;
         }
     a.ctl[0] = tot;
-    a.ctl[1] = tot > a.cap ? 1 : 0;
-    if (tot <= a.cap)
-        for (int j = 0; j < a.nr; j++)
-            if (a.is_join[j]) *a.joined += a.srcoff[(u64)a.ras[j] * nb1 + a.nb];
+    a.ctl[1] = rounds ? 2 : tot > a.cap ? 1 : 0;
+    if (a.count_joined && (rounds || tot <= a.cap)) *a.joined += jtot;
+}
+
+// Split of one threshold round: the first outer row i (of n, offsets off
+// relative to base = off[0]) such that the call's distinct outputs through
+// row i, kp[off[i + 1] - base] (exclusive prefix of first-occurrence flags),
+// exceed thresh; n when there is none.
+__global__ void k_find_split(const u64* off, u64 n, const u32* kp, long long thresh, u64* out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const u64 base = off[0];
+    u64 a = 0, z = n;
+    while (a < z) {
+        u64 m = (a + z) >> 1;
+        if ((long long)kp[off[m + 1] - base] > thresh) z = m; else a = m + 1;
+    }
+    *out = a;
 }
 
 // Copy `words` summary words and then the error flags into pinned host
@@ -1174,6 +1195,11 @@ void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap)
 void pra_launch_plan(hipStream_t st, const PlanArgs& a)
 {
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out)
+{
+    hipLaunchKernelGGL(k_find_split, dim3(1), dim3(1), 0, st, off, n, kp, thresh, out);
 }
 
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host)

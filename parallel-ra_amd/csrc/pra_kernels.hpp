@@ -46,7 +46,7 @@ struct CopyExpandArgs {
 #define PRA_MAXRA 64
 
 struct PlanArgs {
-    u64* srcoff;         // per RA: nb + 1 output offsets at source bucket bounds (written here)
+    u64* srcoff;         // per RA: nb first offsets and nb counts per source (written here)
     u64* posadj;         // per RA: nb arrival adjustments
     u64* ctl;            // [0] candidates, [1] overflow
     u64* joined;
@@ -57,7 +57,10 @@ struct PlanArgs {
     int ras[PRA_MAXRA];
     int is_join[PRA_MAXRA];
     const u64* off[PRA_MAXRA];   // per listed RA: exclusive output offsets over its outer rows
-    const u64* bst[PRA_MAXRA];   // per listed RA: first outer row of each source bucket (nb + 1)
+    const u64* lo[PRA_MAXRA];    // per listed RA: first outer row emitted from each source (nb)
+    const u64* hi[PRA_MAXRA];    // per listed RA: end of the rows emitted from each source (nb)
+    int rounds_check;            // mark threshold rounds instead of emitting
+    int count_joined;            // add the join pairs to *joined
 };
 
 struct NewtScatterArgs {
@@ -97,6 +100,7 @@ void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int 
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap);
 void pra_launch_plan(hipStream_t st, const PlanArgs& a);
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
+void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out);
 void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);

@@ -139,6 +139,10 @@ class Context:
             msg = self.L.pra_last_error(self.c)
             raise PraError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
+    def last_error(self):
+        msg = self.L.pra_last_error(self.c)
+        return msg.decode() if msg else ""
+
     def close(self):
         if getattr(self, "c", None):
             self.L.pra_ctx_destroy(self.c)

@@ -231,3 +231,29 @@ def test_nonlinear_tc_parity(nb):
         assert_same_rel(o, To, g, Tg, "T")
         assert_same_rel(o, Tro, g, Trg, "Tr")
         assert o.history(so) == g.history(sg)
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+def test_threshold_rounds_parity(nb):
+    # join calls over the local-join threshold split into rounds, as the
+    # reference's local_join / execute_in_batches do: ids, per-batch deltas and
+    # join pairs equal to the oracle's, which restates the rounds exactly
+    cases = [("g13236 tc", golden("g13236_edge_2_1_2"), tc_program, [0, 3, 17]),
+             ("random tc", random_graph(300, 520, 7), tc_program, [17, 200]),
+             ("same generation", layered_dag(5, nvert=300, nlayers=6, nedges=420), sg_program, [9, 60])]
+    for name, edges, prog, thresholds in cases:
+        for t in thresholds:
+            print(f"[threshold] {name} nb={nb} threshold={t}", flush=True)
+
+            def build(b):
+                b.set_join_threshold(t)
+                h = prog(b, edges)
+                return h if isinstance(h, dict) else {"T": h[0], "G": h[1], "fix": h[2], "seed": h[3]}
+            o, ho, g, hg = both(nb, build)
+            for k in ho:
+                if isinstance(ho[k], int) and k not in ("fix", "seed", "base", "filter", "rec"):
+                    assert_same_rel(o, ho[k], g, hg[k], f"{name} t={t} {k}")
+            for k in ("fix", "seed", "base", "filter", "rec"):
+                if k in ho:
+                    assert o.history(ho[k]) == g.history(hg[k]), (name, t, k)
+            assert o.joined() == g.joined()
