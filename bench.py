@@ -201,9 +201,11 @@ def main():
 
 
 def est_bytes(n):
-    """device bytes a TC instance on n vertices with m = 1.3 n edges needs:
-    ~0.2 n^2 tuples x (24 B rows + 2 x 32 B table slots + scratch)"""
-    return int(0.2 * n * n * 120) + (64 << 20)
+    """device bytes a TC instance on n vertices with m = 1.3 n edges needs at
+    its peak: ~0.2 n^2 tuples x (up to 4 x 32 B of membership table slots
+    while it grows, 2 x 24 B of full log and sort space rounded up to
+    power-of-two blocks, scratch)"""
+    return int(0.2 * n * n * 200) + (64 << 20)
 
 
 def bench_batch(args):

@@ -51,17 +51,19 @@ static void check_launch(const char* what)
 
 // Blocks of at least 64 MiB are kept in a process-wide cache when released
 // instead of going back to the driver, and handed to later requests of the
-// same power-of-two size class (or the next one) on any stream of the device:
-// returning tens of GB to the driver, and first use of fresh device memory,
-// cost far more than the work such blocks hold.  An event recorded where a
-// block was released orders its reuse.  Out of memory flushes the cache.
+// same power-of-two size class (or the next one): returning tens of GB to the
+// driver, and first use of fresh device memory, cost far more than the work
+// such blocks hold.  A block released while its stream still runs stays with
+// that stream (its reuse there is ordered by the stream itself); when the
+// owning context retires its stream (after synchronizing it) the block
+// becomes free for every stream.  Out of memory flushes the free blocks.
 namespace {
 constexpr size_t kBigBlock = 64ULL << 20;
 struct BigFree {
     void* p;
     size_t size;
     int device;
-    hipEvent_t ready;
+    hipStream_t owner;   // stream that released it while running, or null when idle
 };
 std::mutex g_big_mu;
 std::vector<BigFree> g_big_free;
@@ -73,24 +75,21 @@ size_t big_class(size_t bytes)
     return c;
 }
 
+// give the idle blocks of a device back to the driver
 void big_flush(int dev)
 {
     std::vector<BigFree> drop;
     {
         std::lock_guard<std::mutex> lk(g_big_mu);
         for (size_t i = 0; i < g_big_free.size();)
-            if (g_big_free[i].device == dev) {
+            if (g_big_free[i].device == dev && !g_big_free[i].owner) {
                 drop.push_back(g_big_free[i]);
                 g_big_free.erase(g_big_free.begin() + i);
             } else {
                 i++;
             }
     }
-    for (auto& b : drop) {
-        (void)hipEventSynchronize(b.ready);
-        (void)hipEventDestroy(b.ready);
-        (void)hipFree(b.p);
-    }
+    for (auto& b : drop) (void)hipFree(b.p);
 }
 
 void* big_get(size_t bytes, hipStream_t st, size_t* got)
@@ -98,26 +97,21 @@ void* big_get(size_t bytes, hipStream_t st, size_t* got)
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     const size_t cls = big_class(bytes);
-    BigFree hit{nullptr, 0, dev, nullptr};
     {
         std::lock_guard<std::mutex> lk(g_big_mu);
         long best = -1;
         for (size_t i = 0; i < g_big_free.size(); i++) {
             const BigFree& b = g_big_free[i];
-            if (b.device == dev && b.size >= cls && b.size <= 2 * cls &&
+            if (b.device == dev && (!b.owner || b.owner == st) && b.size >= cls && b.size <= 2 * cls &&
                 (best < 0 || b.size < g_big_free[(size_t)best].size))
                 best = (long)i;
         }
         if (best >= 0) {
-            hit = g_big_free[(size_t)best];
+            BigFree b = g_big_free[(size_t)best];
             g_big_free.erase(g_big_free.begin() + best);
+            *got = b.size;
+            return b.p;
         }
-    }
-    if (hit.p) {
-        HIPCHK(hipStreamWaitEvent(st, hit.ready, 0));
-        HIPCHK(hipEventDestroy(hit.ready));
-        *got = hit.size;
-        return hit.p;
     }
     void* p = nullptr;
     if (hipMalloc(&p, cls) != hipSuccess) {
@@ -132,17 +126,18 @@ void* big_get(size_t bytes, hipStream_t st, size_t* got)
 void big_put(void* p, size_t size, hipStream_t st)
 {
     int dev = 0;
-    hipEvent_t ev = nullptr;
-    if (hipGetDevice(&dev) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventRecord(ev, st) != hipSuccess) {
-        // cannot order the reuse: give the block back to the driver instead
-        (void)hipStreamSynchronize(st);
-        if (ev) (void)hipEventDestroy(ev);
-        (void)hipFree(p);
-        return;
-    }
+    (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> lk(g_big_mu);
-    g_big_free.push_back(BigFree{p, size, dev, ev});
+    g_big_free.push_back(BigFree{p, size, dev, st});
+}
+
+// the stream's work is complete and it is about to be destroyed: its blocks
+// are idle from now on
+void big_retire(hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_big_mu);
+    for (auto& b : g_big_free)
+        if (b.owner == st) b.owner = nullptr;
 }
 } // namespace
 
@@ -376,6 +371,7 @@ struct pra_ctx {
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
+        if (st) big_retire(st);
         if (host_stage) (void)hipHostFree(host_stage);
         if (host_pos) (void)hipHostFree(host_pos);
         if (mailbox) (void)hipHostFree(mailbox);
