@@ -350,6 +350,56 @@ def nonlinear_tc_program(b, edges):
     return T, Tr, s
 
 
+TWO_RULE_TAG = 5
+
+
+def two_rule_program(b):
+    """Two join rules into one output, one iteration, one bucket:
+         O(z, x) <- A(y, x) DELTA JOIN B(y, z) FULL   (rule 0)
+         O(z, x) <- A(y, x) DELTA JOIN C(y, z) FULL   (rule 1)
+    A = (1,10) (2,10) (3,30); B = (1,100) (2,100) (3,300); C = (1,7) (2,7) (3,9).
+    Rule 0 emits (100,10) (100,10) (300,30) and rule 1 (7,10) (7,10) (9,30), the
+    second of each pair a duplicate inside its own call.  Returns O."""
+    rows = lambda pairs: np.array([[k, v, 900 + i] for i, (k, v) in enumerate(pairs)], dtype=np.uint64)
+    A = b.rel_create(2, 1, 1)
+    B = b.rel_create(2, 1, 2)
+    C = b.rel_create(2, 1, 3)
+    O = b.rel_create(2, 1, TWO_RULE_TAG)
+    b.load_rows(A, rows([(1, 10), (2, 10), (3, 30)]), FULL)
+    b.load_rows(B, rows([(1, 100), (2, 100), (3, 300)]), FULL)
+    b.load_rows(C, rows([(1, 7), (2, 7), (3, 9)]), FULL)
+    s = b.scc_create(False, 1)
+    b.scc_add_relation(s, A, True)
+    b.scc_add_relation(s, B, False)
+    b.scc_add_relation(s, C, False)
+    b.scc_add_relation(s, O, False)
+    b.add_join(s, O, A, DELTA, B, FULL, [3, 1])
+    b.add_join(s, O, A, DELTA, C, FULL, [3, 1])
+    return O
+
+
+def two_rule_expected(order):
+    """O's rows at one bucket when its tuples were interned in `order`
+    (id = tag << 46 | bucket << 28 | seq), in download order."""
+    seq = {t: i for i, t in enumerate(order)}
+    return np.array([[z, x, (TWO_RULE_TAG << 46) | seq[(z, x)]] for z, x in sorted(order)], dtype=np.uint64)
+
+
+# Hand-derived interning orders of two_rule_program (src/RAM/RA_tasks.cpp:312,
+# 1121-1269; src/RA/parallel_join.cpp:157-230).  Without a split the arrival
+# order is rule 0's outputs then rule 1's.  A call stops after the outer row
+# through which its distinct outputs exceed the threshold, so with threshold 0
+# every round takes one row of A from each rule: round 1 interns (100,10) then
+# (7,10), round 2 re-emits them (dropped as already in newt), round 3 interns
+# (300,30) then (9,30).  With threshold 1 the duplicate rows do not count, so
+# each call passes 1 only at its last row: no second round, the plain order.
+TWO_RULE_ORDERS = {
+    None: [(100, 10), (300, 30), (7, 10), (9, 30)],
+    0: [(100, 10), (7, 10), (300, 30), (9, 30)],
+    1: [(100, 10), (300, 30), (7, 10), (9, 30)],
+}
+
+
 def run_tc_checkpointed(exe, root, nb=2, cp=1, restart_from="checkpoint-1-5"):
     """The reference TC program's checkpoint mode on data/g13236 (a checkpoint
     every `cp` loops into <root>/out), then its restart mode from

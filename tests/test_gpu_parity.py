@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
                       grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_program,
-                      with_rowid)
+                      two_rule_expected, two_rule_program, TWO_RULE_ORDERS, with_rowid)
 
 pytestmark = pytest.mark.gpu
 
@@ -257,3 +257,15 @@ def test_threshold_rounds_parity(nb):
                 if k in ho:
                     assert o.history(ho[k]) == g.history(hg[k]), (name, t, k)
             assert o.joined() == g.joined()
+
+
+@pytest.mark.parametrize("threshold", [None, 0, 1])
+def test_threshold_rounds_hand_derived(threshold):
+    # the interning order derived by hand from the reference's round loop
+    print(f"[threshold hand] threshold={threshold}", flush=True)
+    g = gpu_ctx(1)
+    if threshold is not None:
+        g.set_join_threshold(threshold)
+    out = two_rule_program(g)
+    g.execute()
+    assert np.array_equal(g.rows(out), two_rule_expected(TWO_RULE_ORDERS[threshold]))

@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
                       grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_join_pairs, tc_pairs_bruteforce,
-                      tc_program, with_rowid)
+                      tc_program, two_rule_expected, two_rule_program, TWO_RULE_ORDERS, with_rowid)
 
 TAG_T = 257
 
@@ -217,6 +217,18 @@ def test_threshold_rounds_reorder_interning():
     o2, T2, *_ = run_tc(3, edges, threshold=3)
     assert sorted(map(tuple, o1.rows(T1)[:, :2].tolist())) == sorted(map(tuple, o2.rows(T2)[:, :2].tolist()))
     assert not np.array_equal(o1.rows(T1), o2.rows(T2))
+
+
+@pytest.mark.parametrize("threshold", [None, 0, 1])
+def test_threshold_rounds_hand_derived(threshold):
+    # interning order of a two-rule program with and without split join
+    # calls, derived by hand from the reference's round loop
+    o = O.Oracle(1)
+    if threshold is not None:
+        o.set_join_threshold(threshold)
+    out = two_rule_program(o)
+    o.execute(1)
+    assert np.array_equal(o.rows(out), two_rule_expected(TWO_RULE_ORDERS[threshold]))
 
 
 def test_same_generation_bruteforce():
