@@ -16,6 +16,10 @@ typedef uint32_t u32;
 #define PRA_WAVES (PRA_BLOCK / 64)
 #define PRA_ITEMS 8
 #define PRA_TILE (PRA_BLOCK * PRA_ITEMS)
+// packed-key sort bins: rows per bin sorted in LDS by one block
+#define PRA_BIN_BLOCK 1024
+#define PRA_BIN_ITEMS 8
+#define PRA_BIN_CAP (PRA_BIN_BLOCK * PRA_BIN_ITEMS)
 #define PRA_RADIX_BITS 8
 #define PRA_RADIX (1 << PRA_RADIX_BITS)
 #define PRA_MAXW 32
@@ -26,6 +30,7 @@ typedef uint32_t u32;
 #define PRA_FLAG_INDEX_COLLISION 4u
 #define PRA_FLAG_SORT_STALL 8u
 #define PRA_FLAG_THRESHOLD 16u
+#define PRA_FLAG_BIN_OVERFLOW 32u
 
 #define PRA_MAXPASS 40
 #define PRA_PART_WORDS 40   // summary words of the newt scatter

@@ -298,11 +298,12 @@ struct KStat {
 
 enum {
     K_JOIN_COUNT, K_JOIN_EXPAND, K_COPY_EXPAND, K_DEDUP_CLAIM, K_DEDUP_DECIDE, K_DEDUP_VERIFY, K_DEDUP_SCATTER,
-    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_MERGE, K_COUNT
+    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_MERGE, K_BIN_SORT, K_COUNT
 };
 static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand", "dedup_claim", "dedup_decide",
                                       "dedup_verify_hist", "dedup_scatter", "radix_hist", "radix_scatter",
-                                      "table_build", "index_build", "scan", "gather_rows", "merge_full"};
+                                      "table_build", "index_build", "scan", "gather_rows", "merge_full",
+                                      "bin_sort"};
 
 } // namespace
 
@@ -333,7 +334,7 @@ struct pra_ctx {
     // order-preserving value dictionary for packed sort keys
     DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_hist, rep_words, rctl, ttab, kp;
     u64 dict_cap = 0, dict_n = 0;
-    int dict_bits = 0, dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
+    int dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
     u32 sort_epoch = 0;
     u64 dedup_epoch = 0;
@@ -524,6 +525,7 @@ static void check_flags(u32 f)
         throw PraError(PRA_E_COLLISION, "64-bit key hash collision between distinct keys");
     if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
     if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort look-back did not complete");
+    if (f & PRA_FLAG_BIN_OVERFLOW) throw PraError(PRA_E_HIP, "sort bin exceeds its capacity");
 }
 
 // Read back `words` u64 from device memory in one synchronization, together
@@ -683,11 +685,67 @@ static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols, bool by_
 // Sort rows by their packed keys (bucket, rank0, .., rank[a-1]): LSD passes
 // over the varying key bytes move (key, row index) pairs, then one gather
 // moves the rows.  Writes the bucket bounds to `bnd` (nb + 1 words).
+static int ceil_log2(u64 x);
+
+// Packed sort key of an interned relation's newt rows: bucket * D^arity + the
+// column ranks as digits of radix D (the dictionary size), which orders rows
+// by (bucket, col0, col1, ..), shifted left by ls so that its top bit is bit
+// 63.  bmul = D^arity.  False when the key range does not fit 64 bits.
+static bool packed_layout(u64 nb, u64 D, int arity, u64* bmul, int* ls)
+{
+    unsigned __int128 m = 1;
+    for (int i = 0; i < arity; i++) {
+        m *= D;
+        if (m >> 64) return false;
+    }
+    unsigned __int128 range = m * nb;
+    if (range >> 64) return false;
+    int bits = ceil_log2((u64)range);
+    *bmul = (u64)m;
+    *ls = bits ? 64 - bits : 0;
+    return true;
+}
+
+// Sort newt rows by their packed keys (all distinct).  When the largest bin of
+// the key's top byte fits one block, one one-sweep pass groups the keys by
+// that byte and k_bin_sort orders every bin in LDS, gathers the rows and
+// writes the bucket bounds; otherwise every varying byte gets a one-sweep
+// pass, followed by a row gather and a bounds search.
 static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32* kgofs, u64 kor, u64 kand,
-                               int bucket_shift, u64* bnd)
+                               u64 maxbin, u64 bmul, int ls, u64* bnd)
 {
     u64 n = r.n;
     u64 vary = kor ^ kand;
+    const u32* top_ofs = kgofs + 7 * PRA_RADIX;
+    if (n > 1 && maxbin <= PRA_BIN_CAP) {
+        const u32* idx = nullptr;
+        if ((vary >> 56) & (PRA_RADIX - 1)) {
+            prepare_lookback(c, n);
+            c->ktmp.reserve(n * 8, c->st);
+            c->idxb.reserve(n * 4, c->st);
+            u32 ep = next_sort_epoch(c);
+            Prof p(c, K_RADIX_SCATTER, 24.0 * n);
+            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, DigitSpec{0, 56, 0, 0, 0}, top_ofs,
+                                    c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags(), nullptr,
+                                    c->idxb.as<u32>());
+            check_launch("key_sort_scatter");
+            keys.swap(c->ktmp);
+            idx = c->idxb.as<u32>();
+        }
+        u32 bytes = 0;
+        for (int b = 0; b < 7; b++)
+            if ((vary >> (8 * b)) & (PRA_RADIX - 1)) bytes |= 1u << b;
+        c->sort_tmp.reserve(n * w * 8, c->st);
+        {
+            Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
+            pra_launch_bin_sort(c->st, keys.as<u64>(), idx, n, top_ofs, bytes, r.d(), w, c->sort_tmp.as<u64>(), bmul,
+                                ls, (u64)c->nb, bnd, c->dflags());
+            check_launch("bin_sort");
+        }
+        r.buf.swap(c->sort_tmp);
+        r.sorted = true;
+        return;
+    }
     std::vector<DigitSpec> passes;
     for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
         if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{0, sh, 0, 0, 0});
@@ -718,7 +776,7 @@ static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32
     }
     r.sorted = true;
     if (c->nb > 1) {
-        pra_launch_key_bounds(c->st, keys.as<u64>(), n, bucket_shift, (u64)c->nb, bnd);
+        pra_launch_key_bounds(c->st, keys.as<u64>(), n, bmul, ls, (u64)c->nb, bnd);
         check_launch("key_bounds");
     } else {
         pra_launch_fill_u64(c->st, bnd, 1, 0);
@@ -778,7 +836,6 @@ static void build_dictionary(pra_ctx* c)
     sync_check(c);
     c->dict_cap = cap;
     c->dict_n = V.n;
-    c->dict_bits = std::max(1, ceil_log2(V.n));
     c->dict_state = 1;
 }
 
@@ -1246,7 +1303,8 @@ static void run_iteration(pra_scc* S)
                 throw PraError(PRA_E_UNSUPPORTED, "two interned relations of one SCC share an intern tag");
 
     // per output relation: [0] new tuples, [1] packed-key OR, [2] AND, [3] dictionary miss,
-    // [4] candidates, [5] capacity overflow, [8..23] key-column OR, [24..39] AND
+    // [4] candidates, [5] capacity overflow, [6] largest top-byte bin of the packed keys,
+    // [8..23] key-column OR, [24..39] AND
     const size_t SW = 40;
     c->stage2.reserve(std::max<size_t>(1, no) * SW * 8, c->st);
     c->posadj_dev.reserve((size_t)std::max(1, nra) * nb * 8, c->st);
@@ -1259,11 +1317,12 @@ static void run_iteration(pra_scc* S)
     bool any_intern = false;
     for (size_t oi = 0; oi < no; oi++) any_intern = any_intern || (out_intern[oi] && ub[oi]);
     if (any_intern && c->dict_state == 0) build_dictionary(c);
-    const int bucket_bits = nb > 1 ? ceil_log2((u64)nb) : 0;
     std::vector<char> packed(no, 0);
+    std::vector<u64> pk_mul(no, 0);
+    std::vector<int> pk_ls(no, 0);
     for (size_t oi = 0; oi < no; oi++)
         packed[oi] = c->dict_state == 1 && out_intern[oi] && ub[oi] &&
-                     bucket_bits + outs[oi]->arity * c->dict_bits <= 64;
+                     packed_layout((u64)nb, c->dict_n, outs[oi]->arity, &pk_mul[oi], &pk_ls[oi]);
 
     // 3+4. per output: arrival plan, emit candidates, deduplicate, partition
     //      and intern, summaries for the sort
@@ -1426,8 +1485,8 @@ static void run_iteration(pra_scc* S)
                 a.dict_cap = c->dict_cap;
                 a.keys = R->nkeys.as<u64>();
                 a.rep_hist = c->rep_hist.as<u32>();
-                a.rank_bits = c->dict_bits;
-                a.bucket_shift = R->arity * c->dict_bits;
+                a.rank_radix = c->dict_n;
+                a.key_lshift = pk_ls[oi];
             }
             ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
             pra_launch_dedup_scatter(c->st, a);
@@ -1506,7 +1565,7 @@ static void run_iteration(pra_scc* S)
         u64* bnd = R->bnd[PRA_DELTA].as<u64>();
         if (packed[oi] && !in_rounds[oi] && st2[oi * SW + 3] == 0) {
             sort_by_packed_key(c, NT, R->w, R->nkeys, R->kgofs.as<u32>(), st2[oi * SW + 1], st2[oi * SW + 2],
-                               R->arity * c->dict_bits, bnd);
+                               st2[oi * SW + 6], pk_mul[oi], pk_ls[oi], bnd);
             bnd_ok[oi] = 1;
             continue;
         }

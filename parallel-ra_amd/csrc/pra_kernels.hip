@@ -297,8 +297,9 @@ __global__ void k_col_orand2(const u64* rows, u64 n, int w, u64* out /* [0..15] 
 //
 // An order-preserving dictionary of the values in the relations' key columns
 // (rank(a) < rank(b) iff a < b) lets a newt row's sort key
-// (bucket, col0, .., col[a-1]) be packed into one u64 of
-// bucket bits + a * rank bits, which the LSD sort then orders in a few passes.
+// (bucket, col0, .., col[a-1]) be packed into one u64: the bucket and the
+// column ranks as digits of radix D (the dictionary size), which the sort
+// then orders in a few byte passes.
 // Slots: [value, rank + 1]; rank + 1 == 0 marks an empty slot.
 
 __device__ inline bool pra_dict_rank(const u64* tab, u64 cap, u64 v, u64* rank)
@@ -352,15 +353,167 @@ __global__ void k_gather_rows(const u64* in, const u32* idx, u64 n, int w, u64* 
     for (int c = 0; c < w; c++) dst[c] = src[c];
 }
 
-// bnd[s] = first position whose key has bucket (key >> shift) >= s
-__global__ void k_key_bounds(const u64* keys, u64 n, int shift, u64 nb, u64* bnd)
+// Second stage of the packed-key sort.  The one-sweep pass on the key's top
+// byte has grouped the (key, row index) pairs into 256 bins (bin b = rows
+// whose left-aligned key has top byte b, at [bofs[b], bofs[b + 1])); block b
+// orders its bin by the key's lower varying bytes (bit p of `bytes` = byte p
+// varies, p < 7) with stable LSD passes in LDS, then gathers the rows into
+// `out` in that order.  Keys are distinct, so the result is the total order.
+// A bin holds at most PRA_BIN_CAP pairs; the host checks the largest bin
+// before it launches this kernel (a larger one raises PRA_FLAG_BIN_OVERFLOW
+// and is left unsorted).  Bucket s starts at the first key >= (s * bmul) <<
+// ls (the key is bucket * bmul + column ranks, shifted left by ls); the block
+// whose bin holds that value counts its keys below it.  idx == nullptr: the
+// pairs are in row order.
+__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* keys, const u32* idx, u64 n, const u32* bofs,
+                                                            u32 bytes, const u64* rows, int w, u64* out, u64 bmul,
+                                                            int ls, u64 nb, u64* bnd, u32* flags)
+{
+    constexpr int IT = PRA_BIN_ITEMS;
+    constexpr int WAVES = PRA_BIN_BLOCK / 64;
+    constexpr u32 PER_WAVE = PRA_BIN_CAP / WAVES;
+    __shared__ u64 skey[PRA_BIN_CAP];
+    __shared__ u32 sidx[PRA_BIN_CAP];
+    __shared__ u32 wh[WAVES][PRA_RADIX];
+    __shared__ u32 wsum[PRA_RADIX / 64];
+    __shared__ u32 below;
+    const u32 b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) {
+        bnd[0] = 0;
+        bnd[nb] = n;
+    }
+    const u64 start = bofs[b], end = b + 1 < PRA_RADIX ? bofs[b + 1] : n;
+    if (end <= start) {
+        for (u64 s = 1 + threadIdx.x; s < nb; s += PRA_BIN_BLOCK)
+            if ((((s * bmul) << ls) >> 56) == b) bnd[s] = start;
+        return;
+    }
+    if (end - start > PRA_BIN_CAP) {
+        if (threadIdx.x == 0) atomicOr(flags, PRA_FLAG_BIN_OVERFLOW);
+        return;
+    }
+    const u32 m = (u32)(end - start);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u64 lt = pra_lanemask_lt();
+    u64 key[IT];
+    u32 id[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+        key[k] = 0;
+        id[k] = 0;
+        if (i < m) {
+            key[k] = keys[start + i];
+            id[k] = idx ? idx[start + i] : (u32)(start + i);
+        }
+    }
+#pragma unroll 1
+    for (int p = 0; p < 7; p++) {
+        if (!((bytes >> p) & 1u)) continue;
+        for (int i = threadIdx.x; i < WAVES * PRA_RADIX; i += PRA_BIN_BLOCK) (&wh[0][0])[i] = 0;
+        __syncthreads();
+        // stable rank of every pair among the pairs of its digit: wave-local
+        // counts in element order, then a scan over waves and digits
+        u32 dg[IT], lc[IT];
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            bool valid = i < m;
+            u32 d = valid ? (u32)(key[k] >> (8 * p)) & 255u : 0u;
+            u64 peers = pra_match<PRA_RADIX_BITS>(d, __ballot(valid));
+            dg[k] = d;
+            lc[k] = 0;
+            if (valid) {
+                u32 before = wh[wave][d];
+                lc[k] = before + (u32)__popcll(peers & lt);
+                if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
+            }
+        }
+        __syncthreads();
+        const int d = threadIdx.x;   // digit of this thread in the scan (threads < PRA_RADIX)
+        u32 acc = 0;
+        if (d < PRA_RADIX)
+            for (int v = 0; v < WAVES; v++) {
+                u32 t = wh[v][d];
+                wh[v][d] = acc;
+                acc += t;
+            }
+        u32 x = acc;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            u32 y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (d < PRA_RADIX && lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (d < PRA_RADIX) {
+            u32 base = x - acc;
+            for (int v = 0; v < wave; v++) base += wsum[v];
+            for (int v = 0; v < WAVES; v++) wh[v][d] += base;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            if (i < m) {
+                u32 pos = wh[wave][dg[k]] + lc[k];
+                skey[pos] = key[k];
+                sidx[pos] = id[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            if (i < m) {
+                key[k] = skey[i];
+                id[k] = sidx[i];
+            }
+        }
+        __syncthreads();
+    }
+    for (u64 s = 1; s < nb; s++) {
+        const u64 T = (s * bmul) << ls;
+        if ((T >> 56) != b) continue;
+        if (threadIdx.x == 0) below = 0;
+        __syncthreads();
+        u32 cnt = 0;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            if (i < m && key[k] < T) cnt++;
+        }
+        if (cnt) atomicAdd(&below, cnt);
+        __syncthreads();
+        if (threadIdx.x == 0) bnd[s] = start + below;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+        if (i < m) {
+            const u64* src = rows + (u64)id[k] * w;
+            u64* dst = out + (start + i) * w;
+            for (int c = 0; c < w; c++) dst[c] = src[c];
+        }
+    }
+}
+
+// bnd[s] = first position whose key is >= (s * bmul) << ls (the key is
+// bucket * bmul + column ranks, shifted left by ls)
+__global__ void k_key_bounds(const u64* keys, u64 n, u64 bmul, int ls, u64 nb, u64* bnd)
 {
     u64 s = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nb) return;
+    if (s == nb) {
+        bnd[s] = n;
+        return;
+    }
+    const u64 T = (s * bmul) << ls;
     u64 lo = 0, hi = n;
     while (lo < hi) {
         u64 mid = (lo + hi) >> 1;
-        if ((keys[mid] >> shift) < s) lo = mid + 1; else hi = mid;
+        if (keys[mid] < T) lo = mid + 1; else hi = mid;
     }
     bnd[s] = lo;
 }
@@ -888,12 +1041,13 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                 dst[a.cw] = key | (a.seqbase[d] + rank);
             }
             if (packed) {
-                u64 K = (u64)d << a.bucket_shift;
+                u64 K = d;
                 for (int c = 0; c < a.cw; c++) {
                     u64 r = 0;
                     if (!pra_dict_rank(a.dict, a.dict_cap, src[c], &r)) kmiss = 1;
-                    K |= r << ((a.cw - 1 - c) * a.rank_bits);
+                    K = K * a.rank_radix + r;
                 }
+                K <<= a.key_lshift;
                 a.keys[pos] = K;
                 kor |= K;
                 kand &= K;
@@ -957,9 +1111,9 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
 // Combine the PRA_NREP replicas of the scatter summaries and reset them to
 // their identity values for the next scatter.  Block y < 8: byte digit y of
 // the packed key, whose exclusive digit offsets go to kgofs (the one-sweep
-// sort's global histogram scan); block 8: summary words -> stat[1..3]
-// (packed key OR / AND / dictionary miss) and stat[8..39] (key column OR /
-// AND).
+// sort's global histogram scan; block 7 also writes its largest digit count
+// to stat[6]); block 8: summary words -> stat[1..3] (packed key OR / AND /
+// dictionary miss) and stat[8..39] (key column OR / AND).
 __global__ void k_newt_fold(u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat, u32* kgofs)
 {
     if (ctl[1]) return;
@@ -975,6 +1129,15 @@ __global__ void k_newt_fold(u32* rep_hist, u64* rep_words, const u64* ctl, int p
         }
         u32 tot;
         kgofs[y * PRA_RADIX + t] = block_exclusive_scan<u32>(v, lds, &tot);
+        if (y == 7) {
+            // the largest top-byte bin, which decides how the keys are sorted
+            __shared__ u32 mx;
+            if (t == 0) mx = 0;
+            __syncthreads();
+            atomicMax(&mx, v);
+            __syncthreads();
+            if (t == 0) stat[6] = mx;
+        }
     } else if (t < PRA_PART_WORDS && !(t >= 3 && t < 8)) {
         const bool is_and = pra_summary_is_and(t);
         u64 v = is_and ? ~0ULL : 0;
@@ -1287,6 +1450,13 @@ void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int
     }
 }
 
+void pra_launch_bin_sort(hipStream_t st, const u64* keys, const u32* idx, u64 n, const u32* bofs, u32 bytes,
+                         const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
+{
+    hipLaunchKernelGGL(k_bin_sort, dim3(PRA_RADIX), dim3(PRA_BIN_BLOCK), 0, st, keys, idx, n, bofs, bytes, rows, w, out,
+                       bmul, ls, nb, bnd, flags);
+}
+
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out)
 {
     if (!n) return;
@@ -1305,9 +1475,9 @@ void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n
     hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, in, idx, n, w, out);
 }
 
-void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, int shift, u64 nb, u64* bnd)
+void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int ls, u64 nb, u64* bnd)
 {
-    hipLaunchKernelGGL(k_key_bounds, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, keys, n, shift, nb, bnd);
+    hipLaunchKernelGGL(k_key_bounds, dim3(grid_for(nb + 1)), dim3(PRA_BLOCK), 0, st, keys, n, bmul, ls, nb, bnd);
 }
 
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out)

@@ -70,7 +70,9 @@ struct NewtScatterArgs {
     u64* keys;         // packed sort key per written row
     u64* rep_words;    // PRA_NREP x PRA_PART_WORDS summary words (OR / AND)
     u32* rep_hist;     // PRA_NREP x 8 x 256 byte-digit counts of the packed keys
-    int rank_bits, bucket_shift;
+    u64 rank_radix;    // packed key = bucket, then each column's rank, in this radix
+    int key_lshift;    // packed key shifted left so that its top bit is bit 63
+    int pad2;
     const u64* cand;
     const u32* keep;
     const u32* dest;
@@ -119,7 +121,9 @@ void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out);
 void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u64 cap, u32* flags);
 void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n, int w, u64* out);
-void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, int shift, u64 nb, u64* bnd);
+void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int ls, u64 nb, u64* bnd);
+void pra_launch_bin_sort(hipStream_t st, const u64* keys, const u32* idx, u64 n, const u32* bofs, u32 bytes,
+                         const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
 void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u64 capn, int cw, int kw, u64* table,
                              u64 cap, u64* slot_of, u64 epoch, u32* flags);
