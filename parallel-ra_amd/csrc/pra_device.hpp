@@ -31,6 +31,7 @@ typedef uint32_t u32;
 #define PRA_FLAG_SORT_STALL 8u
 #define PRA_FLAG_THRESHOLD 16u
 #define PRA_FLAG_BIN_OVERFLOW 32u
+#define PRA_FLAG_SCAN_OVERFLOW 64u
 
 #define PRA_MAXPASS 40
 #define PRA_PART_WORDS 40   // summary words of the newt scatter

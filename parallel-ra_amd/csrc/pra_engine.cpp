@@ -328,7 +328,7 @@ struct pra_ctx {
     std::vector<std::pair<pra_scc*, pra_scc*>> deps;
     std::map<u32, std::unique_ptr<DBuf>> seq;  // per intern tag: nb counters
     // scratch
-    DBuf partials, hist, offs, sort_tmp, flags_buf, stage, stage2, orand, tmp64a, tmp64b, tmp64c;
+    DBuf hist, offs, sort_tmp, flags_buf, stage, stage2, orand, tmp64a, tmp64b, tmp64c;
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
@@ -366,7 +366,7 @@ struct pra_ctx {
         // stream-ordered buffers are returned before their stream goes away
         if (st) (void)hipStreamSynchronize(st);
         seq.clear();
-        for (DBuf* b : {&partials, &hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
+        for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
                         &idxa, &idxb, &posadj_dev, &joined_dev, &rep_hist, &rep_words, &rctl, &ttab, &kp})
             b->release();
@@ -501,21 +501,32 @@ struct ProfPerCand {
 
 // ----------------------------------------------------------------- scans ---
 
+static void prepare_lookback(pra_ctx* c, u64 n);
+static u32 next_sort_epoch(pra_ctx* c);
+
+// exclusive scans on the context stream (one-pass look-back above
+// PRA_SMALL_SCAN elements, over the sort's status words)
 static void scan_u64(pra_ctx* c, const u64* in, u64* out, u64 n, u64* total)
 {
     Prof p(c, K_SCAN, 16.0 * n);
-    u64 np = std::max<u64>(1, pra_scan_partials_count(n));
-    c->partials.reserve(np * 8, c->st);
-    pra_scan_u64(c->st, in, out, n, c->partials.as<u64>(), total);
+    u32 ep = 0;
+    if (n > PRA_SMALL_SCAN) {
+        prepare_lookback(c, n);
+        ep = next_sort_epoch(c);
+    }
+    pra_scan_u64(c->st, in, out, n, c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags(), total);
     check_launch("scan_u64");
 }
 
 static void scan_u32(pra_ctx* c, const u32* in, u32* out, u64 n, u32* total)
 {
     Prof p(c, K_SCAN, 8.0 * n);
-    u64 np = std::max<u64>(1, pra_scan_partials_count(n));
-    c->partials.reserve(np * 8, c->st);
-    pra_scan_u32(c->st, in, out, n, c->partials.as<u32>(), total);
+    u32 ep = 0;
+    if (n > PRA_SMALL_SCAN) {
+        prepare_lookback(c, n);
+        ep = next_sort_epoch(c);
+    }
+    pra_scan_u32(c->st, in, out, n, c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags(), total);
     check_launch("scan_u32");
 }
 
@@ -526,6 +537,7 @@ static void check_flags(u32 f)
     if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
     if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort look-back did not complete");
     if (f & PRA_FLAG_BIN_OVERFLOW) throw PraError(PRA_E_HIP, "sort bin exceeds its capacity");
+    if (f & PRA_FLAG_SCAN_OVERFLOW) throw PraError(PRA_E_HIP, "prefix sum exceeds 2^46");
 }
 
 // Read back `words` u64 from device memory in one synchronization, together
@@ -682,9 +694,6 @@ static void sort_rows(pra_ctx* c, Rows& r, int w, int jcc, int keycols, bool by_
     sort_rows_masked(c, r, w, jcc, keycols, m, nullptr, by_bucket);
 }
 
-// Sort rows by their packed keys (bucket, rank0, .., rank[a-1]): LSD passes
-// over the varying key bytes move (key, row index) pairs, then one gather
-// moves the rows.  Writes the bucket bounds to `bnd` (nb + 1 words).
 static int ceil_log2(u64 x);
 
 // Packed sort key of an interned relation's newt rows: bucket * D^arity + the
@@ -706,17 +715,17 @@ static bool packed_layout(u64 nb, u64 D, int arity, u64* bmul, int* ls)
     return true;
 }
 
-// Sort newt rows by their packed keys (all distinct).  When the largest bin of
-// the key's top byte fits one block, one one-sweep pass groups the keys by
-// that byte and k_bin_sort orders every bin in LDS, gathers the rows and
-// writes the bucket bounds; otherwise every varying byte gets a one-sweep
-// pass, followed by a row gather and a bounds search.
-static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32* kgofs, u64 kor, u64 kand,
+// Sort newt rows by their packed keys (all distinct).  `top_ofs` holds the
+// exclusive offsets of the keys' top-byte digits.  When the largest top-byte
+// bin fits one block, one one-sweep pass groups the keys by that byte and
+// k_bin_sort orders every bin in LDS, gathers the rows and writes the bucket
+// bounds (nb + 1 words); otherwise every varying byte gets a one-sweep pass,
+// followed by a row gather and a bounds search.
+static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32* top_ofs, u64 kor, u64 kand,
                                u64 maxbin, u64 bmul, int ls, u64* bnd)
 {
     u64 n = r.n;
     u64 vary = kor ^ kand;
-    const u32* top_ofs = kgofs + 7 * PRA_RADIX;
     if (n > 1 && maxbin <= PRA_BIN_CAP) {
         const u32* idx = nullptr;
         if ((vary >> 56) & (PRA_RADIX - 1)) {
@@ -755,11 +764,23 @@ static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32
         c->idxa.reserve(n * 4, c->st);
         c->idxb.reserve(n * 4, c->st);
         int np = (int)passes.size();
+        PassList pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.np = np;
+        for (int i = 0; i < np; i++) pl.p[i] = passes[i];
+        pra_launch_fill_u64(c->st, c->ghist.as<u64>(), np * PRA_RADIX / 2, 0);
+        {
+            Prof p(c, K_RADIX_HIST, 8.0 * n);
+            pra_launch_sort_hist(c->st, keys.as<u64>(), n, 1, pl, c->ghist.as<u32>());
+            check_launch("sort_hist");
+        }
+        pra_launch_sort_scan(c->st, c->ghist.as<u32>(), np, c->gofs.as<u32>(), nullptr, (u32)c->nb, n);
+        check_launch("sort_scan");
         for (int i = 0; i < np; i++) {
             u32 ep = next_sort_epoch(c);
             Prof p(c, K_RADIX_SCATTER, 24.0 * n);
             pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, passes[i],
-                                    kgofs + (size_t)(passes[i].shift / 8) * PRA_RADIX, c->lb_status.as<u64>(),
+                                    c->gofs.as<u32>() + (size_t)i * PRA_RADIX, c->lb_status.as<u64>(),
                                     c->lb_ctr.as<u32>() + i, ep, c->dflags(), i == 0 ? nullptr : c->idxa.as<u32>(),
                                     c->idxb.as<u32>());
             check_launch("key_sort_scatter");
@@ -1492,13 +1513,25 @@ static void run_iteration(pra_scc* S)
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
         }
-        R->kgofs.reserve(8 * PRA_RADIX * 4, c->st);
-        pra_launch_newt_fold(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>(), ctl, pk ? 1 : 0, st2o,
-                             R->kgofs.as<u32>());
-        check_launch("newt_fold");
-        pra_launch_seq_update(c->st, sq ? sq->as<u64>() : nullptr, c->offs.as<u32>(), nblocks, (u32)nb, total_dev,
-                              st2o, intern ? 1 : 0, ctl);
-        check_launch("seq_update");
+        R->kgofs.reserve(PRA_RADIX * 4, c->st);
+        {
+            NewtFoldArgs f;
+            memset(&f, 0, sizeof(f));
+            f.rep_hist = c->rep_hist.as<u32>();
+            f.rep_words = c->rep_words.as<u64>();
+            f.ctl = ctl;
+            f.stat = st2o;
+            f.kgofs = R->kgofs.as<u32>();
+            f.seqbase = sq ? sq->as<u64>() : nullptr;
+            f.offs = c->offs.as<u32>();
+            f.total = total_dev;
+            f.nblocks = nblocks;
+            f.nb = (u32)nb;
+            f.packed = pk ? 1 : 0;
+            f.intern = intern ? 1 : 0;
+            pra_launch_newt_fold(c->st, f);
+            check_launch("newt_fold");
+        }
     };
     for (size_t oi = 0; oi < no; oi++)
         if (cap[oi]) run_output(oi, nullptr);
@@ -1761,7 +1794,7 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->budget = hbm_budget;
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
-        c->rep_hist.reserve((size_t)PRA_NREP * 8 * PRA_RADIX * 4, c->st);
+        c->rep_hist.reserve((size_t)PRA_NREP * PRA_RADIX * 4, c->st);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
         pra_launch_newt_rep_init(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>());
         check_launch("init");

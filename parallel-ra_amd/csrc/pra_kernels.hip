@@ -59,60 +59,6 @@ __device__ inline T block_exclusive_scan(T v, T* lds, T* total)
     return res;
 }
 
-// ---- device-wide exclusive scan: reduce tiles, scan tile sums, apply ------
-
-template <typename T>
-__global__ void k_scan_reduce(const T* in, u64 n, T* partials)
-{
-    __shared__ T lds[PRA_WAVES + 1];
-    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)threadIdx.x * PRA_ITEMS;
-    T s = 0;
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++)
-        if (base + k < n) s += in[base + k];
-    T tot;
-    block_exclusive_scan<T>(s, lds, &tot);
-    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
-}
-
-template <typename T>
-__global__ void k_scan_partials(T* partials, u64 np, T* total)
-{
-    __shared__ T lds[PRA_WAVES + 1];
-    T carry = 0;
-    for (u64 off = 0; off < np; off += PRA_BLOCK) {
-        u64 i = off + threadIdx.x;
-        T v = i < np ? partials[i] : 0;
-        T tot;
-        T ex = block_exclusive_scan<T>(v, lds, &tot);
-        if (i < np) partials[i] = ex + carry;
-        carry += tot;
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
-}
-
-template <typename T>
-__global__ void k_scan_apply(const T* in, T* out, u64 n, const T* partials)
-{
-    __shared__ T lds[PRA_WAVES + 1];
-    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)threadIdx.x * PRA_ITEMS;
-    T v[PRA_ITEMS];
-    T s = 0;
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        v[k] = base + k < n ? in[base + k] : 0;
-        s += v[k];
-    }
-    T tot;
-    T ex = block_exclusive_scan<T>(s, lds, &tot) + partials[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        if (base + k < n) out[base + k] = ex;
-        ex += v[k];
-    }
-}
-
-
 // ---------------------------------------------------- one-sweep LSD sort ---
 //
 // k_sort_hist counts the digits of every pass in one read of the rows;
@@ -302,9 +248,11 @@ __global__ void k_col_orand2(const u64* rows, u64 n, int w, u64* out /* [0..15] 
 // then orders in a few byte passes.
 // Slots: [value, rank + 1]; rank + 1 == 0 marks an empty slot.
 
+__device__ inline u64 pra_dict_slot(u64 v, u64 cap) { return pra_mix64(v ^ 0x5851F42D4C957F2DULL) & (cap - 1); }
+
 __device__ inline bool pra_dict_rank(const u64* tab, u64 cap, u64 v, u64* rank)
 {
-    u64 m = cap - 1, s = pra_mix64(v ^ 0x5851F42D4C957F2DULL) & m;
+    u64 m = cap - 1, s = pra_dict_slot(v, cap);
     for (u64 q = 0; q < cap; q++) {
         const u64* sl = tab + 2 * s;
         u64 r = sl[1];
@@ -331,7 +279,7 @@ __global__ void k_dict_insert(const u64* vals, u64 n, u64* tab, u64 cap, u32* fl
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     u64 v = vals[i];
-    u64 m = cap - 1, s = pra_mix64(v ^ 0x5851F42D4C957F2DULL) & m;
+    u64 m = cap - 1, s = pra_dict_slot(v, cap);
     for (u64 q = 0; q < cap; q++) {
         u64* sl = tab + 2 * s;
         if (atomicCAS((unsigned long long*)(sl + 1), 0ULL, (unsigned long long)(i + 1)) == 0) {
@@ -986,11 +934,12 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u64 lt = pra_lanemask_lt();
     u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
-    u32 dg[PRA_ITEMS], lc[PRA_ITEMS];
+    u32 dg[PRA_ITEMS], lc[PRA_ITEMS], kept = 0;
 #pragma unroll
     for (int k = 0; k < PRA_ITEMS; k++) {
         u64 p = base + (u64)k * 64 + lane;
         bool valid = p < a.n && a.keep[p];
+        kept |= (valid ? 1u : 0u) << k;
         u32 d = valid ? a.dest[p] : 0u;
         u64 act = __ballot(valid);
         u64 peers = pra_match<PRA_RADIX_BITS>(d, act);
@@ -1014,11 +963,11 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     __syncthreads();
     // per-block summaries: the packed key's OR / AND / dictionary miss and its
     // byte-digit histograms (packed mode), or the key columns' OR / AND
-    __shared__ u32 khist[8 * PRA_RADIX];
+    __shared__ u32 khist[PRA_RADIX];
     __shared__ u64 red[PRA_WAVES][PRA_PART_WORDS];
     constexpr bool packed = PACKED;
     if (packed)
-        for (int i = threadIdx.x; i < 8 * PRA_RADIX; i += PRA_BLOCK) khist[i] = 0;
+        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) khist[i] = 0;
     __syncthreads();
     const int kc = a.cw < 16 ? a.cw : 16;
     constexpr int NC = PACKED ? 1 : 16;   // key columns summarised (unpacked)
@@ -1026,10 +975,45 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     u64 o[NC], an[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) { o[c] = 0; an[c] = ~0ULL; }
+    // packed keys: the column ranks of all items are looked up column by
+    // column with every item's first probe in flight at once (a dictionary
+    // slot resolves at its first probe unless its chain is longer)
+    u64 K[PACKED ? PRA_ITEMS : 1];
+    if (packed) {
+#pragma unroll
+        for (int k = 0; k < PRA_ITEMS; k++) K[k] = dg[k];
+        for (int c = 0; c < a.cw; c++) {
+            u64 v[PRA_ITEMS], e[PRA_ITEMS], r1[PRA_ITEMS];
+#pragma unroll
+            for (int k = 0; k < PRA_ITEMS; k++) {
+                v[k] = e[k] = r1[k] = 0;
+                if ((kept >> k) & 1u) v[k] = a.cand[(base + (u64)k * 64 + lane) * a.cw + c];
+            }
+#pragma unroll
+            for (int k = 0; k < PRA_ITEMS; k++)
+                if ((kept >> k) & 1u) {
+                    const u64* sl = a.dict + 2 * pra_dict_slot(v[k], a.dict_cap);
+                    e[k] = sl[0];
+                    r1[k] = sl[1];
+                }
+#pragma unroll
+            for (int k = 0; k < PRA_ITEMS; k++)
+                if ((kept >> k) & 1u) {
+                    u64 r = 0;
+                    if (r1[k] != 0 && e[k] == v[k])
+                        r = r1[k] - 1;
+                    else if (!pra_dict_rank(a.dict, a.dict_cap, v[k], &r))
+                        kmiss = 1;
+                    K[k] = K[k] * a.rank_radix + r;
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < PRA_ITEMS; k++) K[k] <<= a.key_lshift;
+    }
 #pragma unroll
     for (int k = 0; k < PRA_ITEMS; k++) {
         u64 p = base + (u64)k * 64 + lane;
-        if (p < a.n && a.keep[p]) {
+        if ((kept >> k) & 1u) {
             u32 d = dg[k];
             u64 pos = (u64)wh[wave][d] + lc[k];
             const u64* src = a.cand + p * a.cw;
@@ -1041,18 +1025,11 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                 dst[a.cw] = key | (a.seqbase[d] + rank);
             }
             if (packed) {
-                u64 K = d;
-                for (int c = 0; c < a.cw; c++) {
-                    u64 r = 0;
-                    if (!pra_dict_rank(a.dict, a.dict_cap, src[c], &r)) kmiss = 1;
-                    K = K * a.rank_radix + r;
-                }
-                K <<= a.key_lshift;
-                a.keys[pos] = K;
-                kor |= K;
-                kand &= K;
-#pragma unroll
-                for (int b = 0; b < 8; b++) atomicAdd(&khist[b * PRA_RADIX + ((K >> (8 * b)) & 255)], 1u);
+                const u64 Kk = K[PACKED ? k : 0];
+                a.keys[pos] = Kk;
+                kor |= Kk;
+                kand &= Kk;
+                atomicAdd(&khist[Kk >> 56], 1u);
             } else {
 #pragma unroll
                 for (int c = 0; c < NC; c++)
@@ -1104,49 +1081,55 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     if (packed)
-        for (int i = threadIdx.x; i < 8 * PRA_RADIX; i += PRA_BLOCK)
-            if (khist[i]) atomicAdd(a.rep_hist + (u64)rep * 8 * PRA_RADIX + i, khist[i]);
+        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK)
+            if (khist[i]) atomicAdd(a.rep_hist + (u64)rep * PRA_RADIX + i, khist[i]);
 }
 
 // Combine the PRA_NREP replicas of the scatter summaries and reset them to
-// their identity values for the next scatter.  Block y < 8: byte digit y of
-// the packed key, whose exclusive digit offsets go to kgofs (the one-sweep
-// sort's global histogram scan; block 7 also writes its largest digit count
-// to stat[6]); block 8: summary words -> stat[1..3] (packed key OR / AND /
-// dictionary miss) and stat[8..39] (key column OR / AND).
-__global__ void k_newt_fold(u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat, u32* kgofs)
+// their identity values for the next scatter.  Block 0 (packed keys): the
+// top-byte digit counts, whose exclusive offsets go to kgofs and whose
+// largest count goes to stat[6].  Block 1: summary words -> stat[1..3]
+// (packed key OR / AND / dictionary miss) and stat[8..39] (key column OR /
+// AND).  Block 2: the kept total -> stat[0], and the seq counters advance by
+// the new tuples of each bucket (bucket d's rows start at offs[d * nblocks]).
+__global__ void k_newt_fold(NewtFoldArgs a)
 {
-    if (ctl[1]) return;
+    if (a.ctl[1]) return;
     __shared__ u32 lds[PRA_WAVES + 1];
+    __shared__ u32 mx;
     const u32 y = blockIdx.x, t = threadIdx.x;
-    if (y < 8) {
-        if (!packed) return;
+    if (y == 0) {
+        if (!a.packed) return;
         u32 v = 0;
         for (int r = 0; r < PRA_NREP; r++) {
-            u32* p = rep_hist + ((u64)r * 8 + y) * PRA_RADIX + t;
+            u32* p = a.rep_hist + (u64)r * PRA_RADIX + t;
             v += *p;
             *p = 0;
         }
         u32 tot;
-        kgofs[y * PRA_RADIX + t] = block_exclusive_scan<u32>(v, lds, &tot);
-        if (y == 7) {
-            // the largest top-byte bin, which decides how the keys are sorted
-            __shared__ u32 mx;
-            if (t == 0) mx = 0;
-            __syncthreads();
-            atomicMax(&mx, v);
-            __syncthreads();
-            if (t == 0) stat[6] = mx;
+        a.kgofs[t] = block_exclusive_scan<u32>(v, lds, &tot);
+        if (t == 0) mx = 0;
+        __syncthreads();
+        atomicMax(&mx, v);
+        __syncthreads();
+        if (t == 0) a.stat[6] = mx;
+    } else if (y == 1) {
+        if (t < PRA_PART_WORDS && !(t >= 3 && t < 8)) {
+            const bool is_and = pra_summary_is_and(t);
+            u64 v = is_and ? ~0ULL : 0;
+            for (int r = 0; r < PRA_NREP; r++) {
+                u64* p = a.rep_words + (u64)r * PRA_PART_WORDS + t;
+                v = is_and ? (v & *p) : (v | *p);
+                *p = is_and ? ~0ULL : 0;
+            }
+            a.stat[t < 3 ? t + 1 : t] = v;
         }
-    } else if (t < PRA_PART_WORDS && !(t >= 3 && t < 8)) {
-        const bool is_and = pra_summary_is_and(t);
-        u64 v = is_and ? ~0ULL : 0;
-        for (int r = 0; r < PRA_NREP; r++) {
-            u64* p = rep_words + (u64)r * PRA_PART_WORDS + t;
-            v = is_and ? (v & *p) : (v | *p);
-            *p = is_and ? ~0ULL : 0;
-        }
-        stat[t < 3 ? t + 1 : t] = v;
+    } else {
+        if (t == 0) a.stat[0] = *a.total;
+        if (t >= a.nb || !a.intern) return;
+        u32 start = a.offs[(u64)t * a.nblocks];
+        u32 end = t + 1 < a.nb ? a.offs[(u64)(t + 1) * a.nblocks] : *a.total;
+        a.seqbase[t] += (u64)(end - start);
     }
 }
 
@@ -1155,22 +1138,8 @@ __global__ void k_newt_fold(u32* rep_hist, u64* rep_words, const u64* ctl, int p
 __global__ void k_newt_rep_init(u32* rep_hist, u64* rep_words)
 {
     u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < PRA_NREP * 8 * PRA_RADIX) rep_hist[i] = 0;
+    if (i < PRA_NREP * PRA_RADIX) rep_hist[i] = 0;
     if (i < PRA_NREP * PRA_PART_WORDS) rep_words[i] = pra_summary_is_and(i % PRA_PART_WORDS) ? ~0ULL : 0;
-}
-
-// seq counters advance by the number of new tuples per bucket; the kept
-// total is reported to the host
-__global__ void k_seq_update(u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total, u64* count_out,
-                             int intern, const u64* ctl)
-{
-    if (ctl[1]) return;
-    u32 d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d == 0) *count_out = *total;
-    if (d >= nb || !intern) return;
-    u32 start = offs[(u64)d * nblocks];
-    u32 end = d + 1 < nb ? offs[(u64)(d + 1) * nblocks] : *total;
-    seqbase[d] += (u64)(end - start);
 }
 
 // ----------------------------------------- membership table maintenance -----
@@ -1264,48 +1233,152 @@ void pra_launch_table_clear(hipStream_t st, u64* t, u64 cap, int stride)
     hipLaunchKernelGGL(k_table_clear, dim3((unsigned)(g ? g : 1)), dim3(PRA_BLOCK), 0, st, t, cap, stride);
 }
 
-u64 pra_scan_partials_count(u64 n) { return (n + PRA_TILE - 1) / PRA_TILE; }
 
-// one block scans inputs of up to PRA_SMALL_SCAN elements in chunks of
-// PRA_BLOCK, carrying the running total
+// one block scans inputs of up to PRA_SMALL_SCAN elements, every thread
+// holding a run of consecutive elements in registers
 template <typename T>
 __global__ void k_scan_small(const T* in, T* out, u64 n, T* total)
 {
+    constexpr int PER = PRA_SMALL_SCAN / PRA_BLOCK;
     __shared__ T lds[PRA_WAVES + 1];
-    T carry = 0;
-    for (u64 off = 0; off < n; off += PRA_BLOCK) {
-        u64 i = off + threadIdx.x;
-        T v = i < n ? in[i] : 0;
-        T tot;
-        T ex = block_exclusive_scan<T>(v, lds, &tot);
-        if (i < n) out[i] = ex + carry;
-        carry += tot;
+    const u64 base = (u64)threadIdx.x * PER;
+    T v[PER];
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        sum += v[k];
     }
-    if (threadIdx.x == 0 && total) *total = carry;
+    T tot;
+    T ex = block_exclusive_scan<T>(sum, lds, &tot);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+// Exclusive prefix of `tile` over its predecessors' published values (digit
+// 0 of the status words), walked by the 64 lanes of one wave: each step reads
+// the 64 nearest unconsumed predecessors at once and adds the ready ones up to
+// the nearest inclusive prefix.  Every lane returns the same value.
+__device__ inline u64 pra_lookback_wave(const u64* status, u32 tile, u32 epoch, u32* flags)
+{
+    constexpr u64 VAL = (1ULL << 46) - 1;
+    const int lane = __lane_id();
+    u64 excl = 0;
+    long long t = (long long)tile - 1;
+    u32 spins = 0;
+    while (t >= 0) {
+        long long j = t - lane;
+        u64 v = j >= 0 ? __hip_atomic_load(status + (u64)j * PRA_RADIX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : st_pack(epoch, PRA_ST_INC, 0);
+        u64 fl = (v >> 46) & 3ULL;
+        bool ready = (u32)(v >> 48) == epoch && fl != 0;
+        u64 notready = __ballot(!ready);
+        u64 inc = __ballot(ready && fl == PRA_ST_INC);
+        int first_nr = notready ? __ffsll((unsigned long long)notready) - 1 : 64;
+        int first_inc = inc ? __ffsll((unsigned long long)inc) - 1 : 64;
+        bool stop = first_inc < first_nr;
+        int take = stop ? first_inc + 1 : first_nr;
+        u64 x = lane < take ? (v & VAL) : 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+        excl += x;
+        if (stop) break;
+        t -= take;
+        if (take == 0 && ++spins > PRA_LOOKBACK_SPIN_LIMIT) {
+            if (lane == 0) atomicOr(flags, PRA_FLAG_SORT_STALL);
+            break;
+        }
+    }
+    return excl;
+}
+
+// One-pass exclusive scan: tiles of PRA_TILE elements are taken in ticket
+// order; each publishes its sum, finds the sum of the tiles before it by
+// decoupled look-back (digit 0 of the sort's status words), publishes its
+// inclusive prefix and writes its outputs.  Prefixes must stay below 2^46
+// (PRA_FLAG_SCAN_OVERFLOW otherwise).
+template <typename T>
+__global__ void k_scan_lookback(const T* in, T* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
+                                T* total)
+{
+    __shared__ T lds[PRA_WAVES + 1];
+    __shared__ u32 s_tile;
+    __shared__ u64 s_excl;
+    if (threadIdx.x == 0) {
+        u32 t = atomicAdd(tile_ctr, 1u);
+        if (t == gridDim.x - 1) atomicExch(tile_ctr, 0u);
+        s_tile = t;
+    }
+    __syncthreads();
+    const u32 tile = s_tile;
+    const u64 base = (u64)tile * PRA_TILE + (u64)threadIdx.x * PRA_ITEMS;
+    T v[PRA_ITEMS];
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        sum += v[k];
+    }
+    T tot;
+    T ex = block_exclusive_scan<T>(sum, lds, &tot);
+    if (threadIdx.x < 64) {
+        constexpr u64 LIM = 1ULL << 46;
+        u64* my = status + (u64)tile * PRA_RADIX;
+        const u64 agg = (u64)tot;
+        u64 excl = 0;
+        if (threadIdx.x == 0) {
+            if (agg >= LIM) atomicOr(flags, PRA_FLAG_SCAN_OVERFLOW);
+            __hip_atomic_store(my, st_pack(epoch, tile == 0 ? PRA_ST_INC : PRA_ST_AGG, agg & (LIM - 1)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tile != 0) {
+            excl = pra_lookback_wave(status, tile, epoch, flags);
+            if (threadIdx.x == 0) {
+                if (excl + agg >= LIM) atomicOr(flags, PRA_FLAG_SCAN_OVERFLOW);
+                __hip_atomic_store(my, st_pack(epoch, PRA_ST_INC, (excl + agg) & (LIM - 1)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (threadIdx.x == 0) {
+            s_excl = excl;
+            if (tile == gridDim.x - 1 && total) *total = (T)(excl + agg);
+        }
+    }
+    __syncthreads();
+    T run = (T)s_excl + ex;
+#pragma unroll
+    for (int k = 0; k < PRA_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
 }
 
 template <typename T>
-static void scan_impl(hipStream_t st, const T* in, T* out, u64 n, T* partials, T* total)
+static void scan_impl(hipStream_t st, const T* in, T* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
+                      T* total)
 {
     if (n <= PRA_SMALL_SCAN) {
         hipLaunchKernelGGL(k_scan_small<T>, dim3(1), dim3(PRA_BLOCK), 0, st, in, out, n, total);
         return;
     }
-    u64 nb = pra_scan_partials_count(n);
-    if (nb == 0) nb = 1;
-    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(PRA_BLOCK), 0, st, in, n, partials);
-    hipLaunchKernelGGL(k_scan_partials<T>, dim3(1), dim3(PRA_BLOCK), 0, st, partials, nb, total);
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(PRA_BLOCK), 0, st, in, out, n, partials);
+    hipLaunchKernelGGL(k_scan_lookback<T>, dim3(pra_tiles(n)), dim3(PRA_BLOCK), 0, st, in, out, n, status, tile_ctr,
+                       epoch, flags, total);
 }
 
-void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* partials, u64* total)
+void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
+                  u64* total)
 {
-    scan_impl<u64>(st, in, out, n, partials, total);
+    scan_impl<u64>(st, in, out, n, status, tile_ctr, epoch, flags, total);
 }
 
-void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* partials, u32* total)
+void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
+                  u32* total)
 {
-    scan_impl<u32>(st, in, out, n, partials, total);
+    scan_impl<u32>(st, in, out, n, status, tile_ctr, epoch, flags, total);
 }
 
 u32 pra_tiles(u64 n) { return (u32)((n + PRA_TILE - 1) / PRA_TILE); }
@@ -1389,13 +1462,6 @@ void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
         hipLaunchKernelGGL(k_dedup_scatter<true>, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
     else
         hipLaunchKernelGGL(k_dedup_scatter<false>, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
-}
-
-void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
-                           u64* count_out, int intern, const u64* ctl)
-{
-    hipLaunchKernelGGL(k_seq_update, dim3(grid_for(nb)), dim3(PRA_BLOCK), 0, st, seqbase, offs, nblocks, nb, total,
-                       count_out, intern, ctl);
 }
 
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
@@ -1504,16 +1570,15 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                        epoch, keep, dest, nb, hist, nblocks, flags);
 }
 
-void pra_launch_newt_fold(hipStream_t st, u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat,
-                          u32* kgofs)
+void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a)
 {
     static_assert(PRA_RADIX == PRA_BLOCK && PRA_PART_WORDS <= PRA_BLOCK, "one thread per bin / word");
-    hipLaunchKernelGGL(k_newt_fold, dim3(9), dim3(PRA_BLOCK), 0, st, rep_hist, rep_words, ctl, packed, stat, kgofs);
+    hipLaunchKernelGGL(k_newt_fold, dim3(3), dim3(PRA_BLOCK), 0, st, a);
 }
 
 void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words)
 {
-    hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * 8 * PRA_RADIX + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0,
+    hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * PRA_RADIX + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0,
                        st, rep_hist, rep_words);
 }
 

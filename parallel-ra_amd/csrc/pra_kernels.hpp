@@ -63,13 +63,27 @@ struct PlanArgs {
     int count_joined;            // add the join pairs to *joined
 };
 
+// Summaries of one newt scatter (k_newt_fold)
+struct NewtFoldArgs {
+    u32* rep_hist;
+    u64* rep_words;
+    const u64* ctl;
+    u64* stat;         // per-output summary words read back by the host
+    u32* kgofs;        // top-byte digit offsets of the packed keys
+    u64* seqbase;      // per-bucket seq counters (interned outputs)
+    const u32* offs;   // scanned per-tile destination histograms
+    const u32* total;  // kept candidates
+    u32 nblocks, nb;
+    int packed, intern;
+};
+
 struct NewtScatterArgs {
     const u64* ctl;
     const u64* dict;   // value -> rank dictionary (null: no packed sort key)
     u64 dict_cap;
     u64* keys;         // packed sort key per written row
     u64* rep_words;    // PRA_NREP x PRA_PART_WORDS summary words (OR / AND)
-    u32* rep_hist;     // PRA_NREP x 8 x 256 byte-digit counts of the packed keys
+    u32* rep_hist;     // PRA_NREP x 256 top-byte digit counts of the packed keys
     u64 rank_radix;    // packed key = bucket, then each column's rank, in this radix
     int key_lshift;    // packed key shifted left so that its top bit is bit 63
     int pad2;
@@ -87,9 +101,13 @@ struct NewtScatterArgs {
 
 void pra_launch_fill_u64(hipStream_t st, u64* p, u64 n, u64 v);
 void pra_launch_table_clear(hipStream_t st, u64* t, u64 cap, int stride);
-u64 pra_scan_partials_count(u64 n);
-void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* partials, u64* total);
-void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u32* partials, u32* total);
+// exclusive scans; inputs above PRA_SMALL_SCAN elements take a one-pass
+// look-back scan over `status` (PRA_RADIX words per PRA_TILE elements, as the
+// sort's) with its own self-resetting tile counter
+void pra_scan_u64(hipStream_t st, const u64* in, u64* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
+                  u64* total);
+void pra_scan_u32(hipStream_t st, const u32* in, u32* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
+                  u32* total);
 u32 pra_tiles(u64 n);
 void pra_launch_bucket_bounds(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64 nb, u64* bstart);
 void pra_launch_group_flags(hipStream_t st, const u64* rows, u64 n, int w, int jcc, u64* flags);
@@ -107,8 +125,6 @@ void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, 
 void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
-void pra_launch_seq_update(hipStream_t st, u64* seqbase, const u32* offs, u32 nblocks, u32 nb, const u32* total,
-                           u64* count_out, int intern, const u64* ctl);
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
                                   u32* flags);
 void pra_launch_table_verify_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap,
@@ -130,8 +146,7 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags);
-void pra_launch_newt_fold(hipStream_t st, u32* rep_hist, u64* rep_words, const u64* ctl, int packed, u64* stat,
-                          u32* kgofs);
+void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
 void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
 void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
