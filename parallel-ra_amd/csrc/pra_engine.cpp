@@ -253,8 +253,10 @@ struct pra_rel {
     u64 idx_maxlen[2] = {0, 0};   // longest run of rows sharing a join key
     u64 jhint = 0;                // candidates of the previous iteration producing this relation
     bool idx_valid[2] = {false, false};
-    // packed sort keys of newt rows and the offsets of their 8 byte digits
-    DBuf nkeys, kgofs;
+    // packed sort keys of newt rows, the offsets of their top-byte bins, and
+    // the bins of (key, newt position) pairs the scatter fills
+    DBuf nkeys, kgofs, bin_keys, bin_idx, bin_cur;
+    u32 bin_cap = 0;
     // bucket bounds per version
     DBuf bnd[2];
     bool bnd_valid[2] = {false, false};
@@ -332,7 +334,7 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_hist, rep_words, rctl, ttab, kp;
+    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_words, rctl, ttab, kp;
     u64 dict_cap = 0, dict_n = 0;
     int dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -368,7 +370,7 @@ struct pra_ctx {
         seq.clear();
         for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_hist, &rep_words, &rctl, &ttab, &kp})
+                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_words, &rctl, &ttab, &kp})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -715,40 +717,30 @@ static bool packed_layout(u64 nb, u64 D, int arity, u64* bmul, int* ls)
     return true;
 }
 
-// Sort newt rows by their packed keys (all distinct).  `top_ofs` holds the
-// exclusive offsets of the keys' top-byte digits.  When the largest top-byte
-// bin fits one block, one one-sweep pass groups the keys by that byte and
-// k_bin_sort orders every bin in LDS, gathers the rows and writes the bucket
-// bounds (nb + 1 words); otherwise every varying byte gets a one-sweep pass,
-// followed by a row gather and a bounds search.
-static void sort_by_packed_key(pra_ctx* c, Rows& r, int w, DBuf& keys, const u32* top_ofs, u64 kor, u64 kand,
-                               u64 maxbin, u64 bmul, int ls, u64* bnd)
+// Sort a relation's newt rows by their packed keys (all distinct; R->nkeys in
+// newt order, R->kgofs the exclusive offsets of the keys' top-byte bins, and
+// the bins of (key, position) pairs filled by the newt scatter).  When the
+// largest bin fits its capacity and one block, k_bin_sort orders every bin in
+// LDS, gathers the rows and writes the bucket bounds (nb + 1 words);
+// otherwise every varying byte gets a one-sweep pass over R->nkeys, followed
+// by a row gather and a bounds search.
+static void sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 maxbin, u64 bmul, int ls, u64* bnd)
 {
+    Rows& r = R->ver[PRA_NEWT];
+    const int w = R->w;
+    DBuf& keys = R->nkeys;
+    const u32* top_ofs = R->kgofs.as<u32>();
     u64 n = r.n;
     u64 vary = kor ^ kand;
-    if (n > 1 && maxbin <= PRA_BIN_CAP) {
-        const u32* idx = nullptr;
-        if ((vary >> 56) & (PRA_RADIX - 1)) {
-            prepare_lookback(c, n);
-            c->ktmp.reserve(n * 8, c->st);
-            c->idxb.reserve(n * 4, c->st);
-            u32 ep = next_sort_epoch(c);
-            Prof p(c, K_RADIX_SCATTER, 24.0 * n);
-            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, DigitSpec{0, 56, 0, 0, 0}, top_ofs,
-                                    c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags(), nullptr,
-                                    c->idxb.as<u32>());
-            check_launch("key_sort_scatter");
-            keys.swap(c->ktmp);
-            idx = c->idxb.as<u32>();
-        }
+    if (n > 1 && maxbin <= std::min<u64>(R->bin_cap, PRA_BIN_CAP)) {
         u32 bytes = 0;
         for (int b = 0; b < 7; b++)
             if ((vary >> (8 * b)) & (PRA_RADIX - 1)) bytes |= 1u << b;
         c->sort_tmp.reserve(n * w * 8, c->st);
         {
             Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
-            pra_launch_bin_sort(c->st, keys.as<u64>(), idx, n, top_ofs, bytes, r.d(), w, c->sort_tmp.as<u64>(), bmul,
-                                ls, (u64)c->nb, bnd, c->dflags());
+            pra_launch_bin_sort(c->st, R->bin_keys.as<u64>(), R->bin_idx.as<u32>(), R->bin_cap, n, top_ofs, bytes,
+                                r.d(), w, c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags());
             check_launch("bin_sort");
         }
         r.buf.swap(c->sort_tmp);
@@ -1501,11 +1493,26 @@ static void run_iteration(pra_scc* S)
             a.tag = R->tag;
             a.rep_words = c->rep_words.as<u64>();
             if (pk) {
+                // bins of twice the average top-byte share (a fuller bin sends
+                // the sort to its byte-pass fallback)
+                const u32 bcap = (u32)std::min<u64>(PRA_BIN_CAP, pow2_at_least(std::max<u64>(256, 2 * C / 256 + 1)));
+                if (bcap > R->bin_cap) {
+                    R->bin_keys.reserve((size_t)PRA_RADIX * bcap * 8, c->st);
+                    R->bin_idx.reserve((size_t)PRA_RADIX * bcap * 4, c->st);
+                    R->bin_cap = bcap;
+                }
+                if (!R->bin_cur.p) {
+                    R->bin_cur.reserve(PRA_RADIX * 4, c->st);
+                    pra_launch_fill_u64(c->st, R->bin_cur.as<u64>(), PRA_RADIX / 2, 0);
+                }
+                a.bin_cap = R->bin_cap;
+                a.bin_keys = R->bin_keys.as<u64>();
+                a.bin_idx = R->bin_idx.as<u32>();
+                a.bin_cur = R->bin_cur.as<u32>();
                 R->nkeys.reserve(C * 8, c->st);
                 a.dict = c->dict.as<u64>();
                 a.dict_cap = c->dict_cap;
                 a.keys = R->nkeys.as<u64>();
-                a.rep_hist = c->rep_hist.as<u32>();
                 a.rank_radix = c->dict_n;
                 a.key_lshift = pk_ls[oi];
             }
@@ -1517,7 +1524,7 @@ static void run_iteration(pra_scc* S)
         {
             NewtFoldArgs f;
             memset(&f, 0, sizeof(f));
-            f.rep_hist = c->rep_hist.as<u32>();
+            f.bin_cur = pk ? R->bin_cur.as<u32>() : nullptr;
             f.rep_words = c->rep_words.as<u64>();
             f.ctl = ctl;
             f.stat = st2o;
@@ -1597,8 +1604,7 @@ static void run_iteration(pra_scc* S)
         R->bnd[PRA_DELTA].reserve((size_t)(nb + 1) * 8, c->st);
         u64* bnd = R->bnd[PRA_DELTA].as<u64>();
         if (packed[oi] && !in_rounds[oi] && st2[oi * SW + 3] == 0) {
-            sort_by_packed_key(c, NT, R->w, R->nkeys, R->kgofs.as<u32>(), st2[oi * SW + 1], st2[oi * SW + 2],
-                               st2[oi * SW + 6], pk_mul[oi], pk_ls[oi], bnd);
+            sort_by_packed_key(c, R, st2[oi * SW + 1], st2[oi * SW + 2], st2[oi * SW + 6], pk_mul[oi], pk_ls[oi], bnd);
             bnd_ok[oi] = 1;
             continue;
         }
@@ -1794,9 +1800,8 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->budget = hbm_budget;
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
-        c->rep_hist.reserve((size_t)PRA_NREP * PRA_RADIX * 4, c->st);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
-        pra_launch_newt_rep_init(c->st, c->rep_hist.as<u32>(), c->rep_words.as<u64>());
+        pra_launch_newt_rep_init(c->st, c->rep_words.as<u64>());
         check_launch("init");
         HIPCHK(hipStreamSynchronize(c->st));
     } catch (const PraError& e) {

@@ -301,21 +301,21 @@ __global__ void k_gather_rows(const u64* in, const u32* idx, u64 n, int w, u64* 
     for (int c = 0; c < w; c++) dst[c] = src[c];
 }
 
-// Second stage of the packed-key sort.  The one-sweep pass on the key's top
-// byte has grouped the (key, row index) pairs into 256 bins (bin b = rows
-// whose left-aligned key has top byte b, at [bofs[b], bofs[b + 1])); block b
-// orders its bin by the key's lower varying bytes (bit p of `bytes` = byte p
-// varies, p < 7) with stable LSD passes in LDS, then gathers the rows into
-// `out` in that order.  Keys are distinct, so the result is the total order.
-// A bin holds at most PRA_BIN_CAP pairs; the host checks the largest bin
+// Packed-key sort of the newt rows.  The newt scatter has put every row's
+// (key, newt position) pair into the bin of the key's top byte (bin b holds
+// its pairs, in no particular order, at bin_keys / bin_idx + b * bin_cap, and
+// its rows go to [bofs[b], bofs[b + 1]) of the sorted order); block b orders
+// its bin by the key's lower varying bytes (bit p of `bytes` = byte p varies,
+// p < 7) with LSD passes in LDS, then gathers the rows into `out` in that
+// order.  Keys are distinct, so the result is the total order.  A bin holds
+// at most min(bin_cap, PRA_BIN_CAP) pairs; the host checks the largest bin
 // before it launches this kernel (a larger one raises PRA_FLAG_BIN_OVERFLOW
 // and is left unsorted).  Bucket s starts at the first key >= (s * bmul) <<
 // ls (the key is bucket * bmul + column ranks, shifted left by ls); the block
-// whose bin holds that value counts its keys below it.  idx == nullptr: the
-// pairs are in row order.
-__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* keys, const u32* idx, u64 n, const u32* bofs,
-                                                            u32 bytes, const u64* rows, int w, u64* out, u64 bmul,
-                                                            int ls, u64 nb, u64* bnd, u32* flags)
+// whose bin holds that value counts its keys below it.
+__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys, const u32* bin_idx, u32 bin_cap,
+                                                            u64 n, const u32* bofs, u32 bytes, const u64* rows, int w,
+                                                            u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
 {
     constexpr int IT = PRA_BIN_ITEMS;
     constexpr int WAVES = PRA_BIN_BLOCK / 64;
@@ -336,11 +336,13 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* keys, con
             if ((((s * bmul) << ls) >> 56) == b) bnd[s] = start;
         return;
     }
-    if (end - start > PRA_BIN_CAP) {
+    if (end - start > PRA_BIN_CAP || end - start > bin_cap) {
         if (threadIdx.x == 0) atomicOr(flags, PRA_FLAG_BIN_OVERFLOW);
         return;
     }
     const u32 m = (u32)(end - start);
+    const u64* keys = bin_keys + (u64)b * bin_cap;
+    const u32* idx = bin_idx + (u64)b * bin_cap;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u64 lt = pra_lanemask_lt();
     u64 key[IT];
@@ -351,8 +353,8 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* keys, con
         key[k] = 0;
         id[k] = 0;
         if (i < m) {
-            key[k] = keys[start + i];
-            id[k] = idx ? idx[start + i] : (u32)(start + i);
+            key[k] = keys[i];
+            id[k] = idx[i];
         }
     }
 #pragma unroll 1
@@ -979,6 +981,7 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     // column with every item's first probe in flight at once (a dictionary
     // slot resolves at its first probe unless its chain is longer)
     u64 K[PACKED ? PRA_ITEMS : 1];
+    u32 lr[PACKED ? PRA_ITEMS : 1];
     if (packed) {
 #pragma unroll
         for (int k = 0; k < PRA_ITEMS; k++) K[k] = dg[k];
@@ -1007,8 +1010,18 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                     K[k] = K[k] * a.rank_radix + r;
                 }
         }
+        // every row's (key, newt position) goes to the bin of the key's top
+        // byte: a rank inside the block's share of the bin, then one counter
+        // reservation per bin and block
 #pragma unroll
-        for (int k = 0; k < PRA_ITEMS; k++) K[k] <<= a.key_lshift;
+        for (int k = 0; k < PRA_ITEMS; k++) {
+            K[k] <<= a.key_lshift;
+            if ((kept >> k) & 1u) lr[k] = atomicAdd(&khist[K[k] >> 56], 1u);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK)
+            if (khist[i]) khist[i] = atomicAdd(a.bin_cur + i, khist[i]);
+        __syncthreads();
     }
 #pragma unroll
     for (int k = 0; k < PRA_ITEMS; k++) {
@@ -1029,7 +1042,11 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
                 a.keys[pos] = Kk;
                 kor |= Kk;
                 kand &= Kk;
-                atomicAdd(&khist[Kk >> 56], 1u);
+                const u32 bin = (u32)(Kk >> 56), slot = khist[bin] + lr[PACKED ? k : 0];
+                if (slot < a.bin_cap) {
+                    a.bin_keys[(u64)bin * a.bin_cap + slot] = Kk;
+                    a.bin_idx[(u64)bin * a.bin_cap + slot] = (u32)pos;
+                }
             } else {
 #pragma unroll
                 for (int c = 0; c < NC; c++)
@@ -1080,15 +1097,12 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
             if (!is_and && v != 0) atomicOr(dst, (unsigned long long)v);
         }
     }
-    if (packed)
-        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK)
-            if (khist[i]) atomicAdd(a.rep_hist + (u64)rep * PRA_RADIX + i, khist[i]);
 }
 
 // Combine the PRA_NREP replicas of the scatter summaries and reset them to
 // their identity values for the next scatter.  Block 0 (packed keys): the
-// top-byte digit counts, whose exclusive offsets go to kgofs and whose
-// largest count goes to stat[6].  Block 1: summary words -> stat[1..3]
+// top-byte bin counters, whose exclusive offsets go to kgofs and whose
+// largest count goes to stat[6]; the counters return to zero.  Block 1: summary words -> stat[1..3]
 // (packed key OR / AND / dictionary miss) and stat[8..39] (key column OR /
 // AND).  Block 2: the kept total -> stat[0], and the seq counters advance by
 // the new tuples of each bucket (bucket d's rows start at offs[d * nblocks]).
@@ -1100,12 +1114,8 @@ __global__ void k_newt_fold(NewtFoldArgs a)
     const u32 y = blockIdx.x, t = threadIdx.x;
     if (y == 0) {
         if (!a.packed) return;
-        u32 v = 0;
-        for (int r = 0; r < PRA_NREP; r++) {
-            u32* p = a.rep_hist + (u64)r * PRA_RADIX + t;
-            v += *p;
-            *p = 0;
-        }
+        u32 v = a.bin_cur[t];
+        a.bin_cur[t] = 0;
         u32 tot;
         a.kgofs[t] = block_exclusive_scan<u32>(v, lds, &tot);
         if (t == 0) mx = 0;
@@ -1135,10 +1145,9 @@ __global__ void k_newt_fold(NewtFoldArgs a)
 
 // identity values of the summary replicas (all counts 0, OR words 0, AND
 // words all ones)
-__global__ void k_newt_rep_init(u32* rep_hist, u64* rep_words)
+__global__ void k_newt_rep_init(u64* rep_words)
 {
     u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < PRA_NREP * PRA_RADIX) rep_hist[i] = 0;
     if (i < PRA_NREP * PRA_PART_WORDS) rep_words[i] = pra_summary_is_and(i % PRA_PART_WORDS) ? ~0ULL : 0;
 }
 
@@ -1516,11 +1525,11 @@ void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int
     }
 }
 
-void pra_launch_bin_sort(hipStream_t st, const u64* keys, const u32* idx, u64 n, const u32* bofs, u32 bytes,
-                         const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
+void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
+                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
 {
-    hipLaunchKernelGGL(k_bin_sort, dim3(PRA_RADIX), dim3(PRA_BIN_BLOCK), 0, st, keys, idx, n, bofs, bytes, rows, w, out,
-                       bmul, ls, nb, bnd, flags);
+    hipLaunchKernelGGL(k_bin_sort, dim3(PRA_RADIX), dim3(PRA_BIN_BLOCK), 0, st, bin_keys, bin_idx, bin_cap, n, bofs,
+                       bytes, rows, w, out, bmul, ls, nb, bnd, flags);
 }
 
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out)
@@ -1576,10 +1585,10 @@ void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a)
     hipLaunchKernelGGL(k_newt_fold, dim3(3), dim3(PRA_BLOCK), 0, st, a);
 }
 
-void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words)
+void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words)
 {
-    hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * PRA_RADIX + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK), 0,
-                       st, rep_hist, rep_words);
+    hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * PRA_PART_WORDS + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK),
+                       0, st, rep_words);
 }
 
 void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,

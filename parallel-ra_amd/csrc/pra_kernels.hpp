@@ -65,7 +65,7 @@ struct PlanArgs {
 
 // Summaries of one newt scatter (k_newt_fold)
 struct NewtFoldArgs {
-    u32* rep_hist;
+    u32* bin_cur;      // top-byte bin counters of the packed keys (reset here)
     u64* rep_words;
     const u64* ctl;
     u64* stat;         // per-output summary words read back by the host
@@ -83,10 +83,12 @@ struct NewtScatterArgs {
     u64 dict_cap;
     u64* keys;         // packed sort key per written row
     u64* rep_words;    // PRA_NREP x PRA_PART_WORDS summary words (OR / AND)
-    u32* rep_hist;     // PRA_NREP x 256 top-byte digit counts of the packed keys
     u64 rank_radix;    // packed key = bucket, then each column's rank, in this radix
     int key_lshift;    // packed key shifted left so that its top bit is bit 63
-    int pad2;
+    u32 bin_cap;       // (key, row) pairs per top-byte bin
+    u64* bin_keys;     // 256 bins of bin_cap packed keys
+    u32* bin_idx;      // 256 bins of bin_cap newt positions
+    u32* bin_cur;      // 256 bin fill counters (all rows, also those past bin_cap)
     const u64* cand;
     const u32* keep;
     const u32* dest;
@@ -138,8 +140,8 @@ void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int 
 void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u64 cap, u32* flags);
 void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n, int w, u64* out);
 void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int ls, u64 nb, u64* bnd);
-void pra_launch_bin_sort(hipStream_t st, const u64* keys, const u32* idx, u64 n, const u32* bofs, u32 bytes,
-                         const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags);
+void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
+                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
 void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u64 capn, int cw, int kw, u64* table,
                              u64 cap, u64* slot_of, u64 epoch, u32* flags);
@@ -147,7 +149,7 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags);
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
-void pra_launch_newt_rep_init(hipStream_t st, u32* rep_hist, u64* rep_words);
+void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
 void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
                                   const u64* ibnd, u64 nb, u64* cnt, u64* istart);
