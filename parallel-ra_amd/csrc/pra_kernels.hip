@@ -369,10 +369,13 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys,
         for (int k = 0; k < IT; k++) {
             u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
             bool valid = i < m;
+            dg[k] = 0;
+            lc[k] = 0;
+            // a wave skips the ranking of item rows wholly past the bin
+            if ((u32)wave * PER_WAVE + (u32)k * 64 >= m) continue;
             u32 d = valid ? (u32)(key[k] >> (8 * p)) & 255u : 0u;
             u64 peers = pra_match<PRA_RADIX_BITS>(d, __ballot(valid));
             dg[k] = d;
-            lc[k] = 0;
             if (valid) {
                 u32 before = wh[wave][d];
                 lc[k] = before + (u32)__popcll(peers & lt);
@@ -697,23 +700,58 @@ __device__ inline u64 pra_row_of(const u64* off, u64 lo, u64 hi, u64 x)
     return lo;
 }
 
+// Row of output o0 for the whole block: the largest i in [0, n] with off[i]
+// <= o0 (off[0] <= o0 holds), found with PRA_BLOCK probes per step, so the
+// range shrinks PRA_BLOCK-fold per dependent load.
+__device__ inline u64 pra_block_row_of(const u64* off, u64 n, u64 o0, u32* s_cnt)
+{
+    u64 lo = 0, hi = n + 1;   // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const u64 span = hi - lo;
+        const u64 idx = lo + (span * threadIdx.x) / PRA_BLOCK;
+        const bool pred = off[idx] <= o0;   // true for a prefix of the threads
+        if (threadIdx.x == 0) *s_cnt = 0;
+        __syncthreads();
+        const u64 b = __ballot(pred);
+        if ((threadIdx.x & 63) == 0 && b) atomicAdd(s_cnt, (u32)__popcll(b));
+        __syncthreads();
+        const u32 cnt = *s_cnt;
+        __syncthreads();
+        const u64 nlo = lo + (span * (cnt - 1)) / PRA_BLOCK;
+        hi = cnt < PRA_BLOCK ? lo + (span * cnt) / PRA_BLOCK : hi;
+        lo = nlo;
+    }
+    return lo;
+}
+
 __global__ void k_join_expand(JoinExpandArgs a)
 {
     if (a.ctl[1]) return;
     const u64 first = a.off[0], total = a.off[a.n_outer];   // first > 0 for a window of rows
-    __shared__ u64 s_lo, s_hi;
+    __shared__ u64 s_off[PRA_BLOCK + 1];
+    __shared__ u32 s_cnt;
     const u64 step = (u64)gridDim.x * blockDim.x;
     for (u64 o0 = first + (u64)blockIdx.x * blockDim.x; o0 < total; o0 += step) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            u64 last = o0 + blockDim.x - 1 < total ? o0 + blockDim.x - 1 : total - 1;
-            s_lo = pra_row_of(a.off, 0, a.n_outer, o0);
-            s_hi = pra_row_of(a.off, s_lo, a.n_outer, last);
-        }
+        // the block's first row, then the offsets of the next PRA_BLOCK rows
+        // in LDS, where each output finds its row
+        const u64 r0 = pra_block_row_of(a.off, a.n_outer, o0, &s_cnt);
+        s_off[threadIdx.x] = r0 + threadIdx.x <= a.n_outer ? a.off[r0 + threadIdx.x] : ~0ULL;
+        if (threadIdx.x == 0) s_off[PRA_BLOCK] = r0 + PRA_BLOCK <= a.n_outer ? a.off[r0 + PRA_BLOCK] : ~0ULL;
         __syncthreads();
         u64 o = o0 + threadIdx.x;
         if (o >= total) continue;
-        u64 i = pra_row_of(a.off, s_lo, s_hi, o);
+        u64 i;
+        if (o < s_off[PRA_BLOCK]) {
+            u32 lo = 0, hi = PRA_BLOCK;   // largest j in [lo, hi] with s_off[j] <= o
+            while (lo < hi) {
+                u32 mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= o) lo = mid; else hi = mid - 1;
+            }
+            i = r0 + lo;
+        } else {
+            i = pra_row_of(a.off, r0 + PRA_BLOCK, a.n_outer, o);
+        }
         const u64* d = a.outer + i * a.wo;
         const u64* f = a.inner + (a.istart[i] + (o - a.off[i])) * a.wi;
         u64 s = a.nb == 1 ? 0 : pra_fnv(d, a.jcc) % a.nb;
@@ -885,19 +923,22 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
 // Keep the winners, verify every key against its slot (a 64-bit hash
 // collision between distinct keys is reported, never merged) and count kept
 // candidates per destination bucket per tile.
-__global__ void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
-                                    const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
-                                    u32 nblocks, u32* flags)
+// One block of PRA_DECIDE_BLOCK threads per PRA_TILE candidates: each
+// candidate is a chain of dependent loads, so many waves per tile hide them.
+__global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw,
+                                                                        int kw, const u64* table, const u64* slot_of,
+                                                                        u64 epoch, u32* keep, const u32* dest, u32 nb,
+                                                                        u32* hist, u32 nblocks, u32* flags)
 {
     if (ctl[1]) return;
     const u64 n = ctl[0];
     __shared__ u32 h[PRA_RADIX];
-    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) h[i] = 0;
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
     __syncthreads();
     u64 base = (u64)blockIdx.x * PRA_TILE;
 #pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
-        u64 p = base + (u64)k * PRA_BLOCK + threadIdx.x;
+    for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
+        u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
         if (p < n) {
             u64 s = slot_of[p];
             u32 kp = 0;
@@ -917,7 +958,7 @@ __global__ void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw, int
         }
     }
     __syncthreads();
-    for (u32 i = threadIdx.x; i < nb; i += PRA_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
+    for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
 }
 
 // Stable partition of the kept candidates by destination bucket; JOIN-type
@@ -1575,8 +1616,8 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags)
 {
-    hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_BLOCK), 0, st, cand, ctl, cw, kw, table, slot_of,
-                       epoch, keep, dest, nb, hist, nblocks, flags);
+    hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_DECIDE_BLOCK), 0, st, cand, ctl, cw, kw, table,
+                       slot_of, epoch, keep, dest, nb, hist, nblocks, flags);
 }
 
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a)

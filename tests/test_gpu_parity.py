@@ -269,3 +269,26 @@ def test_threshold_rounds_hand_derived(threshold):
     out = two_rule_program(g)
     g.execute()
     assert np.array_equal(g.rows(out), two_rule_expected(TWO_RULE_ORDERS[threshold]))
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+def test_skewed_keys_take_the_byte_pass_sort(nb):
+    # every tuple the copy and the join produce shares its first column (a
+    # hub, then a sink), so all packed sort keys fall into one top-byte bin,
+    # which overflows its capacity: the newt sort takes the one-sweep byte
+    # passes instead of the bin sort, with the same result as the oracle
+    print(f"[skewed bins] nb={nb}", flush=True)
+    hub, sink = 1_000_000, 2_000_000
+    edges = with_rowid([(x, hub) for x in range(3000)] + [(hub, sink)])
+    g = gpu_ctx(nb)
+    g.set_profiling(True)
+    Tg, Gg, fg, cg = tc_program(g, edges)
+    g.execute()
+    o = O.Oracle(nb)
+    To, Go, fo, co = tc_program(o, edges)
+    o.execute(1)
+    assert_same_rel(o, To, g, Tg, "T")
+    assert o.history(fo) == g.history(fg)
+    assert o.count(To) == 3000 + 1 + 3000
+    ks = g.kernel_stats()
+    assert ks["bin_sort"]["launches"] == 0 and ks["radix_scatter"]["launches"] > 0
