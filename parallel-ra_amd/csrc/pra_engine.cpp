@@ -264,6 +264,8 @@ struct pra_rel {
     // (changes every iteration), and its full version is read by one of them
     // (so it is kept sorted by merging each delta into it)
     bool scc_changing = false, merge_full = false;
+    // set while an SCC runs: a rule of it reads the relation's full version
+    bool full_read = false;
     void touched(int v)
     {
         if (v == PRA_DELTA || v == PRA_FULL) {
@@ -563,13 +565,20 @@ static void sync_check(pra_ctx* c) { readback(c, nullptr, 0); }
 // synchronization.  Completion of the event (the end of the kernel, with its
 // system-scope release) is what makes the words valid to read; after a while
 // the host falls back to a blocking wait.
-static u64* publish_readback(pra_ctx* c, const u64* src, size_t words)
+static u64* publish_begin(pra_ctx* c, const u64* src, size_t words)
 {
     u64* h = c->mailbox_buf(words + 1);
     pra_launch_publish(c->st, src, words, c->flags_buf.as<u32>(), h);
     check_launch("publish");
     if (!c->pub_ev) HIPCHK(hipEventCreateWithFlags(&c->pub_ev, hipEventDisableTiming));
     HIPCHK(hipEventRecord(c->pub_ev, c->st));
+    return h;
+}
+
+// waits for the summary of publish_begin (work enqueued after it keeps the
+// device busy meanwhile) and checks the error flags
+static u64* publish_wait(pra_ctx* c, u64* h, size_t words)
+{
     auto t0 = std::chrono::steady_clock::now();
     for (u32 k = 0;; k++) {
         hipError_t q = hipEventQuery(c->pub_ev);
@@ -582,6 +591,11 @@ static u64* publish_readback(pra_ctx* c, const u64* src, size_t words)
     }
     check_flags((u32)h[words]);
     return h;
+}
+
+static u64* publish_readback(pra_ctx* c, const u64* src, size_t words)
+{
+    return publish_wait(c, publish_begin(c, src, words), words);
 }
 
 // ------------------------------------------------------------------ sort ---
@@ -1547,8 +1561,19 @@ static void run_iteration(pra_scc* S)
     //    per-iteration host readback (repeated once for outputs whose
     //    candidates overflowed their capacity, now sized exactly)
     std::vector<u64> st2(no * SW, 0);
+    // deltas of relations whose full version no rule of the SCC reads go to
+    // their full logs while the host waits for the first readback
+    std::vector<char> early(S->rels.size(), 0);
     for (int pass = 0; pass < 2; pass++) {
-        u64* h = publish_readback(c, c->stage2.as<u64>(), no * SW);
+        u64* h = publish_begin(c, c->stage2.as<u64>(), no * SW);
+        if (pass == 0)
+            for (size_t i = 0; i < S->rels.size(); i++) {
+                pra_rel* R = S->rels[i];
+                if (R->full_read || R->merge_full || !R->ver[PRA_DELTA].n) continue;
+                append_rows(c, R, PRA_FULL, R->ver[PRA_DELTA].d(), R->ver[PRA_DELTA].n);
+                early[i] = 1;
+            }
+        h = publish_wait(c, h, no * SW);
         std::vector<size_t> redo;
         for (size_t oi = 0; oi < no; oi++) {
             if (!cap[oi]) continue;
@@ -1588,11 +1613,12 @@ static void run_iteration(pra_scc* S)
     // 6. delta -> full, sort newt into trie order, newt -> delta
     //    (src/RAM/RA_tasks.cpp:1044-1058).  The old delta is folded into full
     //    first: the newt sort writes the next delta's bucket bounds.
-    for (pra_rel* R : S->rels) {
+    for (size_t i = 0; i < S->rels.size(); i++) {
+        pra_rel* R = S->rels[i];
         Rows& D = R->ver[PRA_DELTA];
         if (R->merge_full)
             merge_into_full(c, R);
-        else
+        else if (!early[i])
             append_rows(c, R, PRA_FULL, D.d(), D.n);
         D.n = 0;
         R->touched(PRA_DELTA);
@@ -1684,12 +1710,15 @@ static void scc_begin(pra_scc* S, bool move_status)
 {
     pra_ctx* c = S->ctx;
     if (S->done || S->running) throw PraError(PRA_E_STATE, "SCC already started");
-    for (pra_rel* R : S->rels) R->scc_changing = R->merge_full = false;
+    for (pra_rel* R : S->rels) R->scc_changing = R->merge_full = R->full_read = false;
     for (auto& ra : S->ras) ra.out->scc_changing = true;
     for (auto& ra : S->ras) {
-        if (ra.v0 == PRA_FULL) ra.in0->merge_full = ra.in0->scc_changing;
-        if (ra.type == RA_JOIN && ra.v1 == PRA_FULL) ra.in1->merge_full = ra.in1->scc_changing;
+        if (ra.v0 == PRA_FULL) ra.in0->full_read = true;
+        if (ra.type == RA_JOIN && ra.v1 == PRA_FULL) ra.in1->full_read = true;
     }
+    for (auto& ra : S->ras)
+        for (pra_rel* R : {ra.in0, ra.in1, ra.out})
+            if (R) R->merge_full = R->full_read && R->scc_changing;
     if (move_status)
         for (size_t i = 0; i < S->rels.size(); i++)
             if (S->status[i]) full_to_delta(c, S->rels[i]);
@@ -1717,7 +1746,7 @@ static void scc_end(pra_scc* S)
     if (!S->running) throw PraError(PRA_E_STATE, "SCC not started");
     for (pra_rel* R : S->rels) {
         delta_to_full(c, R);
-        R->scc_changing = R->merge_full = false;
+        R->scc_changing = R->merge_full = R->full_read = false;
     }
     sync_check(c);
     S->running = false;
