@@ -909,11 +909,16 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
             }
         }
         if (mine || cur == h) {
-            // a mark from an earlier iteration (key already present) or an
-            // earlier arrival of this one already beats ours
-            u64 old = slot[1];
-            if (old > mark && (mine || old >= (epoch << 40)))
+            // the claimant lowers the mark at once; a later candidate first
+            // reads it: a mark from an earlier iteration (key already present)
+            // or an earlier arrival of this one already beats ours
+            if (mine) {
                 atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
+            } else {
+                u64 old = slot[1];
+                if (old > mark && old >= (epoch << 40))
+                    atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
+            }
             slot_of[p] = s;
             return;
         }
