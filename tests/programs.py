@@ -350,6 +350,90 @@ def nonlinear_tc_program(b, edges):
     return T, Tr, s
 
 
+def random_program(b, seed, nbase=2, nderived=4, dom=9):
+    """A random two-SCC program over small relations (values in [0, dom)):
+    SCC 1 (runs once) seeds the derived relations from the base ones with
+    copies, filters and joins over FULL versions; SCC 2 (to a fixed point)
+    applies random LEFT (DELTA x FULL) and RIGHT (FULL x DELTA) joins, copies,
+    filters and acopies.  Every derived relation is the target of either acopy
+    rules or join/copy/filter rules, never both; joined relations share their
+    join-column count; values in data columns only come from data columns
+    (ids never feed new tuples), so the fixed point is finite.  Returns the
+    relation handles, the SCC handles and a description of the rules."""
+    rng = random.Random(seed)
+    specs = []
+    for i in range(nbase + nderived):
+        arity = rng.choice([1, 2, 2, 3])
+        jcc = 1 if arity == 1 or rng.random() < 0.7 else 2
+        specs.append((arity, jcc))
+    rels = [b.rel_create(a, j, 600 + i) for i, (a, j) in enumerate(specs)]
+    for i, (a, _) in enumerate(specs):
+        n = rng.randint(0 if i >= nbase else 8, 40)
+        tuples = sorted({tuple(rng.randrange(dom) for _ in range(a)) for _ in range(n)})
+        if tuples:
+            b.load_rows(rels[i], np.array([list(t) + [7000 * (i + 1) + k] for k, t in enumerate(tuples)],
+                                          dtype=np.uint64), FULL)
+    derived = list(range(nbase, nbase + nderived))
+    acopy_target = {d: rng.random() < 0.3 for d in derived}
+    rules = []
+
+    def data_proj(a0, a1, jcc, n_out, right_cols=True):
+        cols = list(range(a0)) + ([a0 + 1 + j for j in range(a1 - jcc)] if right_cols else [])
+        return [rng.choice(cols) for _ in range(n_out)]
+
+    def add_rules(scc, sources, versions):
+        for d in derived:
+            ad, jd = specs[d]
+            if acopy_target[d]:
+                cands = [r for r in sources if specs[r][0] == ad]
+                if not cands or versions == "full":
+                    continue
+                for _ in range(rng.randint(1, 2)):
+                    src = rng.choice(cands)
+                    perm = list(range(ad))
+                    rng.shuffle(perm)
+                    b.add_acopy(scc, rels[d], rels[src], DELTA, perm + [ad])
+                    rules.append(("acopy", d, src, perm))
+                continue
+            for _ in range(rng.randint(1, 3)):
+                kind = rng.choice(["join", "join", "copy", "filter"])
+                src = rng.choice(sources)
+                a0, j0 = specs[src]
+                if kind == "join":
+                    partners = [r for r in range(len(specs)) if specs[r][1] == j0]
+                    other = rng.choice(partners)
+                    a1 = specs[other][0]
+                    proj = data_proj(a0, a1, j0, ad)
+                    if versions == "full":
+                        b.add_join(scc, rels[d], rels[src], FULL, rels[other], FULL, proj)
+                    elif rng.random() < 0.7:
+                        b.add_join(scc, rels[d], rels[src], DELTA, rels[other], FULL, proj)
+                    else:
+                        b.add_join(scc, rels[d], rels[other], FULL, rels[src], DELTA,
+                                   data_proj(a1, a0, j0, ad))
+                    rules.append((kind, d, src, other))
+                elif kind == "filter" and a0 >= 2:
+                    op = rng.randrange(6)
+                    b.add_copy_filter(scc, rels[d], rels[src], FULL if versions == "full" else DELTA,
+                                      [rng.randrange(a0) for _ in range(ad)], op)
+                    rules.append((kind, d, src, op))
+                else:
+                    b.add_copy(scc, rels[d], rels[src], FULL if versions == "full" else DELTA,
+                               [rng.randrange(a0) for _ in range(ad)])
+                    rules.append(("copy", d, src))
+
+    s1 = b.scc_create(False, 1)
+    for i in range(len(specs)):
+        b.scc_add_relation(s1, rels[i], False)
+    add_rules(s1, list(range(nbase)), "full")
+    s2 = b.scc_create(True, 2)
+    for i in range(len(specs)):
+        b.scc_add_relation(s2, rels[i], i >= nbase)
+    add_rules(s2, derived, "delta")
+    b.add_dependency(s1, s2)
+    return {"rels": rels, "sccs": [s1, s2], "rules": rules}
+
+
 TWO_RULE_TAG = 5
 
 

@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
                       grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_join_pairs, tc_pairs_bruteforce,
-                      tc_program, two_rule_expected, two_rule_program, TWO_RULE_ORDERS, with_rowid)
+                      tc_program, two_rule_expected, two_rule_program, TWO_RULE_ORDERS, with_rowid, random_program)
 
 TAG_T = 257
 
@@ -292,3 +292,23 @@ def test_empty_inputs():
     assert o.count(T) == 0
     assert o.history(sfix) == [0]
     assert o.history(scopy) == [0]
+
+
+def test_random_programs_bucket_and_batch_invariance():
+    # the same relation contents (data columns, as sets) for any bucket count
+    # and batch size; at one bucket the ids do not depend on the batch size
+    for seed in range(40):
+        ref = None
+        for nb, batch in [(1, 1), (3, 1), (5, 3)]:
+            o = O.Oracle(nb)
+            h = random_program(o, seed)
+            o.execute(batch)
+            sets = [sorted(map(tuple, o.rows(r)[:, :-1].tolist())) for r in h["rels"]]
+            if ref is None:
+                ref = sets
+                ids1 = [o.rows(r) for r in h["rels"]]
+            assert sets == ref, (seed, nb, batch)
+        o = O.Oracle(1)
+        h = random_program(o, seed)
+        o.execute(4)
+        assert all(np.array_equal(o.rows(r), x) for r, x in zip(h["rels"], ids1)), seed
