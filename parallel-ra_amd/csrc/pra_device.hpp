@@ -18,6 +18,10 @@ typedef uint32_t u32;
 #define PRA_TILE (PRA_BLOCK * PRA_ITEMS)
 // packed-key sort bins: rows per bin sorted in LDS by one block
 #define PRA_BIN_BLOCK 1024
+// one-block LDS sort of small inputs: rows, key words (bucket + columns), threads
+#define PRA_SMALL_SORT 2048
+#define PRA_SMALL_SORT_KW 8
+#define PRA_SMALL_SORT_BLOCK 1024
 // threads of the dedup decision per PRA_TILE candidates
 #define PRA_DECIDE_BLOCK 1024
 #define PRA_BIN_ITEMS 8

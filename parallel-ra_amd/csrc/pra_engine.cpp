@@ -302,12 +302,13 @@ struct KStat {
 
 enum {
     K_JOIN_COUNT, K_JOIN_EXPAND, K_COPY_EXPAND, K_DEDUP_CLAIM, K_DEDUP_DECIDE, K_DEDUP_VERIFY, K_DEDUP_SCATTER,
-    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_MERGE, K_BIN_SORT, K_COUNT
+    K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_MERGE, K_BIN_SORT, K_SMALL_SORT,
+    K_COUNT
 };
 static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand", "dedup_claim", "dedup_decide",
                                       "dedup_verify_hist", "dedup_scatter", "radix_hist", "radix_scatter",
                                       "table_build", "index_build", "scan", "gather_rows", "merge_full",
-                                      "bin_sort"};
+                                      "bin_sort", "small_sort"};
 
 } // namespace
 
@@ -656,6 +657,22 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
     if (passes.empty()) {
         r.sorted = true;
         return false;
+    }
+    // small inputs with several varying bytes: one block sorts them in LDS
+    const bool bucketed = c->nb > 1 && by_bucket;
+    const int kw = keycols + (bucketed ? 1 : 0);
+    if (n <= PRA_SMALL_SORT && kw <= PRA_SMALL_SORT_KW && passes.size() >= 2) {
+        c->sort_tmp.reserve(n * w * 8, c->st);
+        u64* bout = (bnd && bucketed && bucket_last) ? bnd : nullptr;
+        {
+            Prof p(c, K_SMALL_SORT, 8.0 * n * kw + 16.0 * n * w);
+            pra_launch_small_sort(c->st, r.d(), n, w, jcc, keycols, (u64)c->nb, bucketed ? 1 : 0,
+                                  c->sort_tmp.as<u64>(), bout);
+            check_launch("small_sort");
+        }
+        r.buf.swap(c->sort_tmp);
+        r.sorted = true;
+        return bout != nullptr;
     }
     prepare_lookback(c, n);
     c->sort_tmp.reserve(n * w * 8, c->st);
@@ -1027,10 +1044,14 @@ static void merge_into_full(pra_ctx* c, pra_rel* R)
                          c->sort_tmp.as<u64>());
         check_launch("merge");
     }
+    // the merged bucket s holds full's and delta's rows of bucket s
+    pra_launch_add_bounds(c->st, R->bnd[PRA_FULL].as<u64>(), db, (u64)c->nb + 1);
+    check_launch("add_bounds");
     F.buf.swap(c->sort_tmp);
     F.n = n;
     F.sorted = true;
     R->touched(PRA_FULL);
+    R->bnd_valid[PRA_FULL] = true;
 }
 
 // One threshold round of an output relation: the rules emitting (RA index

@@ -216,6 +216,67 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
     }
 }
 
+// One block sorts n <= PRA_SMALL_SORT rows by (bucket, col0 .. col[kc-1])
+// (the bucket FNV(row[0:jcc]) % nb leads when `bucketed`; columns compare as
+// unsigned words) with a bitonic network over row indices in LDS, then
+// gathers the rows into `out`.  Unstable: rows with equal keys are equal.
+// With bnd (bucketed), bnd[s] = first row of bucket s, bnd[nb] = n.
+__global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort(const u64* rows, u64 n, int w, int jcc, int kc,
+                                                                     u64 nb, int bucketed, u64* out, u64* bnd)
+{
+    __shared__ u64 key[PRA_SMALL_SORT * PRA_SMALL_SORT_KW];
+    __shared__ u32 idx[PRA_SMALL_SORT];
+    const int kw = kc + (bucketed ? 1 : 0);
+    const u32 m = (u32)n;
+    for (u32 i = threadIdx.x; i < m; i += PRA_SMALL_SORT_BLOCK) {
+        const u64* r = rows + (u64)i * w;
+        u64* k = key + (u64)i * kw;
+        int o = 0;
+        if (bucketed) k[o++] = pra_fnv(r, jcc) % nb;
+        for (int c = 0; c < kc; c++) k[o++] = r[c];
+    }
+    u32 N = 2;
+    while (N < m) N <<= 1;
+    for (u32 i = threadIdx.x; i < N; i += PRA_SMALL_SORT_BLOCK) idx[i] = i;   // i >= m: past every row
+    __syncthreads();
+    for (u32 k = 2; k <= N; k <<= 1)
+        for (u32 j = k >> 1; j > 0; j >>= 1) {
+            for (u32 i = threadIdx.x; i < N; i += PRA_SMALL_SORT_BLOCK) {
+                const u32 l = i ^ j;
+                if (l <= i) continue;
+                const u32 a = idx[i], b = idx[l];
+                bool gt;
+                if (a >= m || b >= m) {
+                    gt = a >= m && (b < m || a > b);
+                } else {
+                    const u64* ka = key + (u64)a * kw;
+                    const u64* kb = key + (u64)b * kw;
+                    int c = 0;
+                    while (c < kw && ka[c] == kb[c]) c++;
+                    gt = c < kw && ka[c] > kb[c];
+                }
+                if (((i & k) == 0) == gt) {
+                    idx[i] = b;
+                    idx[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    for (u32 i = threadIdx.x; i < m; i += PRA_SMALL_SORT_BLOCK) {
+        const u64* src = rows + (u64)idx[i] * w;
+        u64* dst = out + (u64)i * w;
+        for (int c = 0; c < w; c++) dst[c] = src[c];
+    }
+    if (bnd && bucketed) {
+        for (u32 i = threadIdx.x; i <= m; i += PRA_SMALL_SORT_BLOCK) {
+            // buckets (prev, cur] start at row i
+            const u64 prev = i == 0 ? 0 : key[(u64)idx[i - 1] * kw] + 1;
+            const u64 cur = i == m ? nb : key[(u64)idx[i] * kw];
+            for (u64 s = prev; s <= cur && s <= nb; s++) bnd[s] = i;
+        }
+    }
+}
+
 // bitwise OR and AND of column blockIdx.y over all rows, used to skip radix
 // passes whose digit is the same for every row
 __global__ void k_col_orand2(const u64* rows, u64 n, int w, u64* out /* [0..15] or, [16..31] and */)
@@ -1581,6 +1642,13 @@ void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx
                        bytes, rows, w, out, bmul, ls, nb, bnd, flags);
 }
 
+void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jcc, int kc, u64 nb, int bucketed,
+                           u64* out, u64* bnd)
+{
+    hipLaunchKernelGGL(k_small_sort, dim3(1), dim3(PRA_SMALL_SORT_BLOCK), 0, st, rows, n, w, jcc, kc, nb, bucketed, out,
+                       bnd);
+}
+
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out)
 {
     if (!n) return;
@@ -1654,6 +1722,18 @@ void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, con
     if (!nd) return;
     hipLaunchKernelGGL(k_merge_rank, dim3(grid_for(nd)), dim3(PRA_BLOCK), 0, st, D, nd, dbnd, F, fbnd, nb, w, rank);
     hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(nf + nd)), dim3(PRA_BLOCK), 0, st, F, nf, D, nd, rank, w, out);
+}
+
+// bucket bounds of a merge of two bucket-partitioned versions: a[s] += b[s]
+__global__ void k_add_bounds(u64* a, const u64* b, u64 n)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] += b[i];
+}
+
+void pra_launch_add_bounds(hipStream_t st, u64* a, const u64* b, u64 n)
+{
+    hipLaunchKernelGGL(k_add_bounds, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, a, b, n);
 }
 
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out)
