@@ -22,8 +22,9 @@ typedef uint32_t u32;
 #define PRA_SMALL_SORT 2048
 #define PRA_SMALL_SORT_KW 8
 #define PRA_SMALL_SORT_BLOCK 1024
-// threads of the dedup decision per PRA_TILE candidates
+// threads of the dedup decision and of the newt scatter per PRA_TILE candidates
 #define PRA_DECIDE_BLOCK 1024
+#define PRA_SCATTER_BLOCK 1024
 #define PRA_BIN_ITEMS 8
 #define PRA_BIN_CAP (PRA_BIN_BLOCK * PRA_BIN_ITEMS)
 #define PRA_RADIX_BITS 8

@@ -1034,21 +1034,23 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
 // rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
 template <bool PACKED>
-__global__ void k_dedup_scatter(NewtScatterArgs a)
+__global__ __launch_bounds__(PRA_SCATTER_BLOCK) void k_dedup_scatter(NewtScatterArgs a)
 {
+    // PRA_TILE candidates per block: SB threads of IT items, SW waves
+    constexpr int SB = PRA_SCATTER_BLOCK, SW = SB / 64, IT = PRA_TILE / SB;
     if (a.ctl[1]) return;
     a.n = a.ctl[0];
     // blocks past the candidates hold nothing; the summary reduce skips them
     if ((u64)blockIdx.x * PRA_TILE >= a.n) return;
-    __shared__ u32 wh[PRA_WAVES][PRA_RADIX];
-    for (int i = threadIdx.x; i < PRA_WAVES * PRA_RADIX; i += PRA_BLOCK) (&wh[0][0])[i] = 0;
+    __shared__ u32 wh[SW][PRA_RADIX];
+    for (int i = threadIdx.x; i < SW * PRA_RADIX; i += SB) (&wh[0][0])[i] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u64 lt = pra_lanemask_lt();
-    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / PRA_WAVES);
-    u32 dg[PRA_ITEMS], lc[PRA_ITEMS], kept = 0;
+    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / SW);
+    u32 dg[IT], lc[IT], kept = 0;
 #pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
+    for (int k = 0; k < IT; k++) {
         u64 p = base + (u64)k * 64 + lane;
         bool valid = p < a.n && a.keep[p];
         kept |= (valid ? 1u : 0u) << k;
@@ -1064,9 +1066,9 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     __syncthreads();
-    for (u32 d = threadIdx.x; d < a.nb; d += PRA_BLOCK) {
+    for (u32 d = threadIdx.x; d < a.nb; d += SB) {
         u32 acc = a.offs[(u64)d * a.nblocks + blockIdx.x];
-        for (int v = 0; v < PRA_WAVES; v++) {
+        for (int v = 0; v < SW; v++) {
             u32 t = wh[v][d];
             wh[v][d] = acc;
             acc += t;
@@ -1076,13 +1078,15 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     // per-block summaries: the packed key's OR / AND / dictionary miss and its
     // byte-digit histograms (packed mode), or the key columns' OR / AND
     __shared__ u32 khist[PRA_RADIX];
-    __shared__ u64 red[PRA_WAVES][PRA_PART_WORDS];
+    __shared__ u64 red[SW][PRA_PART_WORDS];
     constexpr bool packed = PACKED;
     if (packed)
-        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK) khist[i] = 0;
+        for (int i = threadIdx.x; i < PRA_RADIX; i += SB) khist[i] = 0;
     __syncthreads();
-    const int kc = a.cw < 16 ? a.cw : 16;
-    constexpr int NC = PACKED ? 1 : 16;   // key columns summarised (unpacked)
+    // key columns summarised (unpacked): the first 8; later ones report the
+    // identity values (OR 0, AND all ones), which the host reads as "varies"
+    constexpr int NC = PACKED ? 1 : 8;
+    const int kc = a.cw < NC ? a.cw : NC;
     u64 kor = 0, kand = ~0ULL, kmiss = 0;
     u64 o[NC], an[NC];
 #pragma unroll
@@ -1090,27 +1094,27 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
     // packed keys: the column ranks of all items are looked up column by
     // column with every item's first probe in flight at once (a dictionary
     // slot resolves at its first probe unless its chain is longer)
-    u64 K[PACKED ? PRA_ITEMS : 1];
-    u32 lr[PACKED ? PRA_ITEMS : 1];
+    u64 K[PACKED ? IT : 1];
+    u32 lr[PACKED ? IT : 1];
     if (packed) {
 #pragma unroll
-        for (int k = 0; k < PRA_ITEMS; k++) K[k] = dg[k];
+        for (int k = 0; k < IT; k++) K[k] = dg[k];
         for (int c = 0; c < a.cw; c++) {
-            u64 v[PRA_ITEMS], e[PRA_ITEMS], r1[PRA_ITEMS];
+            u64 v[IT], e[IT], r1[IT];
 #pragma unroll
-            for (int k = 0; k < PRA_ITEMS; k++) {
+            for (int k = 0; k < IT; k++) {
                 v[k] = e[k] = r1[k] = 0;
                 if ((kept >> k) & 1u) v[k] = a.cand[(base + (u64)k * 64 + lane) * a.cw + c];
             }
 #pragma unroll
-            for (int k = 0; k < PRA_ITEMS; k++)
+            for (int k = 0; k < IT; k++)
                 if ((kept >> k) & 1u) {
                     const u64* sl = a.dict + 2 * pra_dict_slot(v[k], a.dict_cap);
                     e[k] = sl[0];
                     r1[k] = sl[1];
                 }
 #pragma unroll
-            for (int k = 0; k < PRA_ITEMS; k++)
+            for (int k = 0; k < IT; k++)
                 if ((kept >> k) & 1u) {
                     u64 r = 0;
                     if (r1[k] != 0 && e[k] == v[k])
@@ -1124,17 +1128,17 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         // byte: a rank inside the block's share of the bin, then one counter
         // reservation per bin and block
 #pragma unroll
-        for (int k = 0; k < PRA_ITEMS; k++) {
+        for (int k = 0; k < IT; k++) {
             K[k] <<= a.key_lshift;
             if ((kept >> k) & 1u) lr[k] = atomicAdd(&khist[K[k] >> 56], 1u);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_BLOCK)
+        for (int i = threadIdx.x; i < PRA_RADIX; i += SB)
             if (khist[i]) khist[i] = atomicAdd(a.bin_cur + i, khist[i]);
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < PRA_ITEMS; k++) {
+    for (int k = 0; k < IT; k++) {
         u64 p = base + (u64)k * 64 + lane;
         if ((kept >> k) & 1u) {
             u32 d = dg[k];
@@ -1201,7 +1205,7 @@ __global__ void k_dedup_scatter(NewtScatterArgs a)
         if (j < 3 || j >= 8) {
             u64 v = red[0][j];
             bool is_and = pra_summary_is_and(j);
-            for (int w2 = 1; w2 < PRA_WAVES; w2++) v = is_and ? (v & red[w2][j]) : (v | red[w2][j]);
+            for (int w2 = 1; w2 < SW; w2++) v = is_and ? (v & red[w2][j]) : (v | red[w2][j]);
             unsigned long long* dst = (unsigned long long*)(a.rep_words + (u64)rep * PRA_PART_WORDS + j);
             if (is_and && v != ~0ULL) atomicAnd(dst, (unsigned long long)v);
             if (!is_and && v != 0) atomicOr(dst, (unsigned long long)v);
@@ -1578,9 +1582,9 @@ void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
 {
     if (a.dict)
-        hipLaunchKernelGGL(k_dedup_scatter<true>, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(k_dedup_scatter<true>, dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0, st, a);
     else
-        hipLaunchKernelGGL(k_dedup_scatter<false>, dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(k_dedup_scatter<false>, dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0, st, a);
 }
 
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
