@@ -25,6 +25,7 @@ typedef uint32_t u32;
 // threads of the dedup decision and of the newt scatter per PRA_TILE candidates
 #define PRA_DECIDE_BLOCK 1024
 #define PRA_SCATTER_BLOCK 1024
+#define PRA_SCATTER_WIDE_TILES 64   // tiles from which the newt scatter uses PRA_SCATTER_BLOCK
 #define PRA_BIN_ITEMS 8
 #define PRA_BIN_CAP (PRA_BIN_BLOCK * PRA_BIN_ITEMS)
 #define PRA_RADIX_BITS 8

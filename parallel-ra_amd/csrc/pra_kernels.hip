@@ -1033,11 +1033,11 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
 // Stable partition of the kept candidates by destination bucket; JOIN-type
 // rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
-template <bool PACKED>
-__global__ __launch_bounds__(PRA_SCATTER_BLOCK) void k_dedup_scatter(NewtScatterArgs a)
+template <bool PACKED, int SB>
+__global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
 {
     // PRA_TILE candidates per block: SB threads of IT items, SW waves
-    constexpr int SB = PRA_SCATTER_BLOCK, SW = SB / 64, IT = PRA_TILE / SB;
+    constexpr int SW = SB / 64, IT = PRA_TILE / SB;
     if (a.ctl[1]) return;
     a.n = a.ctl[0];
     // blocks past the candidates hold nothing; the summary reduce skips them
@@ -1083,9 +1083,9 @@ __global__ __launch_bounds__(PRA_SCATTER_BLOCK) void k_dedup_scatter(NewtScatter
     if (packed)
         for (int i = threadIdx.x; i < PRA_RADIX; i += SB) khist[i] = 0;
     __syncthreads();
-    // key columns summarised (unpacked): the first 8; later ones report the
+    // key columns summarised (unpacked): the first 16; later ones report the
     // identity values (OR 0, AND all ones), which the host reads as "varies"
-    constexpr int NC = PACKED ? 1 : 8;
+    constexpr int NC = PACKED ? 1 : 16;
     const int kc = a.cw < NC ? a.cw : NC;
     u64 kor = 0, kand = ~0ULL, kmiss = 0;
     u64 o[NC], an[NC];
@@ -1581,10 +1581,19 @@ void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)
 
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
 {
-    if (a.dict)
-        hipLaunchKernelGGL(k_dedup_scatter<true>, dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0, st, a);
+    // many tiles: wide blocks hide the per-candidate loads; few tiles: narrow
+    // blocks keep the per-block scans short
+    const bool wide = a.nblocks >= PRA_SCATTER_WIDE_TILES;
+    if (a.dict && wide)
+        hipLaunchKernelGGL((k_dedup_scatter<true, PRA_SCATTER_BLOCK>), dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0, st,
+                           a);
+    else if (a.dict)
+        hipLaunchKernelGGL((k_dedup_scatter<true, PRA_BLOCK>), dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
+    else if (wide)
+        hipLaunchKernelGGL((k_dedup_scatter<false, PRA_SCATTER_BLOCK>), dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0,
+                           st, a);
     else
-        hipLaunchKernelGGL(k_dedup_scatter<false>, dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0, st, a);
+        hipLaunchKernelGGL((k_dedup_scatter<false, PRA_BLOCK>), dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
 }
 
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
