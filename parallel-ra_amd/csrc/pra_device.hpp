@@ -22,6 +22,8 @@ typedef uint32_t u32;
 #define PRA_SMALL_SORT 2048
 #define PRA_SMALL_SORT_KW 8
 #define PRA_SMALL_SORT_BLOCK 1024
+// one-block LSD sort (all passes in one launch) of up to this many rows
+#define PRA_BLOCK_SORT 16384
 // threads of the dedup decision and of the newt scatter per PRA_TILE candidates
 #define PRA_DECIDE_BLOCK 1024
 #define PRA_SCATTER_BLOCK 1024

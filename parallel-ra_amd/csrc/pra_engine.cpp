@@ -303,12 +303,12 @@ struct KStat {
 enum {
     K_JOIN_COUNT, K_JOIN_EXPAND, K_COPY_EXPAND, K_DEDUP_CLAIM, K_DEDUP_DECIDE, K_DEDUP_VERIFY, K_DEDUP_SCATTER,
     K_RADIX_HIST, K_RADIX_SCATTER, K_TABLE_BUILD, K_INDEX_BUILD, K_SCAN, K_GATHER, K_MERGE, K_BIN_SORT, K_SMALL_SORT,
-    K_COUNT
+    K_BLOCK_SORT, K_COUNT
 };
 static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand", "dedup_claim", "dedup_decide",
                                       "dedup_verify_hist", "dedup_scatter", "radix_hist", "radix_scatter",
                                       "table_build", "index_build", "scan", "gather_rows", "merge_full",
-                                      "bin_sort", "small_sort"};
+                                      "bin_sort", "small_sort", "block_sort"};
 
 } // namespace
 
@@ -669,6 +669,23 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
             pra_launch_small_sort(c->st, r.d(), n, w, jcc, keycols, (u64)c->nb, bucketed ? 1 : 0,
                                   c->sort_tmp.as<u64>(), bout);
             check_launch("small_sort");
+        }
+        r.buf.swap(c->sort_tmp);
+        r.sorted = true;
+        return bout != nullptr;
+    }
+    // up to PRA_BLOCK_SORT rows: one block runs every pass in one launch
+    if (n <= PRA_BLOCK_SORT && passes.size() >= 2 && passes.size() <= PRA_MAXPASS) {
+        PassList pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.np = (int)passes.size();
+        for (size_t i = 0; i < passes.size(); i++) pl.p[i] = passes[i];
+        c->sort_tmp.reserve(n * w * 8, c->st);
+        u64* bout = (bnd && bucket_last) ? bnd : nullptr;
+        {
+            Prof p(c, K_BLOCK_SORT, 8.0 * n * pl.np + 16.0 * n * w);
+            pra_launch_block_sort(c->st, r.d(), n, w, pl, c->sort_tmp.as<u64>(), bout, jcc, (u64)c->nb);
+            check_launch("block_sort");
         }
         r.buf.swap(c->sort_tmp);
         r.sorted = true;

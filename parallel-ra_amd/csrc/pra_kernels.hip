@@ -277,6 +277,104 @@ __global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort(const u64* 
     }
 }
 
+// One block sorts n <= PRA_BLOCK_SORT rows with all the LSD passes of `pl`
+// (least significant first; digits from the rows, as k_sort_scatter's): the
+// row indices stay in registers and LDS, every pass ranks them stably by
+// wave-local counts and a scan over waves and digits, and the rows are
+// gathered into `out` at the end.  With bnd, the last pass is the bucket
+// digit and bnd[s] = first row of bucket s (FNV(row[0:jcc]) % nb), bnd[nb] = n.
+__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_block_sort(const u64* rows, u64 n, int w, PassList pl, u64* out,
+                                                              u64* bnd, int jcc, u64 nb)
+{
+    constexpr int IT = PRA_BLOCK_SORT / PRA_BIN_BLOCK;
+    constexpr int WAVES = PRA_BIN_BLOCK / 64;
+    constexpr u32 PER_WAVE = PRA_BLOCK_SORT / WAVES;
+    __shared__ u32 sidx[PRA_BLOCK_SORT];
+    __shared__ u32 wh[WAVES][PRA_RADIX];
+    __shared__ u32 wsum[PRA_RADIX / 64];
+    const u32 m = (u32)n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u64 lt = pra_lanemask_lt();
+    u32 id[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) id[k] = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+#pragma unroll 1
+    for (int p = 0; p < pl.np; p++) {
+        const DigitSpec ds = pl.p[p];
+        for (int i = threadIdx.x; i < WAVES * PRA_RADIX; i += PRA_BIN_BLOCK) (&wh[0][0])[i] = 0;
+        __syncthreads();
+        u32 dg[IT], lc[IT];
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            bool valid = i < m;
+            dg[k] = 0;
+            lc[k] = 0;
+            if ((u32)wave * PER_WAVE + (u32)k * 64 >= m) continue;   // item row wholly past the rows
+            u32 d = valid ? pra_digit(rows + (u64)id[k] * w, ds) : 0u;
+            u64 peers = pra_match<PRA_RADIX_BITS>(d, __ballot(valid));
+            dg[k] = d;
+            if (valid) {
+                u32 before = wh[wave][d];
+                lc[k] = before + (u32)__popcll(peers & lt);
+                if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
+            }
+        }
+        __syncthreads();
+        const int d = threadIdx.x;   // digit of this thread in the scan (threads < PRA_RADIX)
+        u32 acc = 0;
+        if (d < PRA_RADIX)
+            for (int v = 0; v < WAVES; v++) {
+                u32 t = wh[v][d];
+                wh[v][d] = acc;
+                acc += t;
+            }
+        u32 x = acc;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            u32 y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (d < PRA_RADIX && lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (d < PRA_RADIX) {
+            u32 base = x - acc;
+            for (int v = 0; v < wave; v++) base += wsum[v];
+            for (int v = 0; v < WAVES; v++) wh[v][d] += base;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            if (i < m) sidx[wh[wave][dg[k]] + lc[k]] = id[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+            if (i < m) id[k] = sidx[i];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        u32 i = (u32)wave * PER_WAVE + (u32)k * 64 + lane;
+        if (i < m) {
+            const u64* src = rows + (u64)id[k] * w;
+            u64* dst = out + (u64)i * w;
+            for (int c = 0; c < w; c++) dst[c] = src[c];
+        }
+    }
+    if (bnd) {
+        // buckets (bucket of row i-1, bucket of row i] start at row i
+        for (u32 i = threadIdx.x; i <= m; i += PRA_BIN_BLOCK) {
+            const u64 prev = i == 0 ? 0 : pra_fnv(rows + (u64)sidx[i - 1] * w, jcc) % nb + 1;
+            const u64 cur = i == m ? nb : pra_fnv(rows + (u64)sidx[i] * w, jcc) % nb;
+            for (u64 s = prev; s <= cur && s <= nb; s++) bnd[s] = i;
+        }
+    }
+}
+
 // bitwise OR and AND of column blockIdx.y over all rows, used to skip radix
 // passes whose digit is the same for every row
 __global__ void k_col_orand2(const u64* rows, u64 n, int w, u64* out /* [0..15] or, [16..31] and */)
@@ -1660,6 +1758,12 @@ void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jc
 {
     hipLaunchKernelGGL(k_small_sort, dim3(1), dim3(PRA_SMALL_SORT_BLOCK), 0, st, rows, n, w, jcc, kc, nb, bucketed, out,
                        bnd);
+}
+
+void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u64* out, u64* bnd,
+                           int jcc, u64 nb)
+{
+    hipLaunchKernelGGL(k_block_sort, dim3(1), dim3(PRA_BIN_BLOCK), 0, st, rows, n, w, pl, out, bnd, jcc, nb);
 }
 
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out)

@@ -143,6 +143,8 @@ void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int
 void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jcc, int kc, u64 nb, int bucketed,
                            u64* out, u64* bnd);
 void pra_launch_add_bounds(hipStream_t st, u64* a, const u64* b, u64 n);
+void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u64* out, u64* bnd,
+                           int jcc, u64 nb);
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
                          u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
