@@ -1044,6 +1044,9 @@ __global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host
 // stores the key; every candidate then lowers the slot's arrival mark to
 // (epoch << 40 | p+1), so the earliest arrival of this iteration wins, while a
 // mark from an earlier iteration (key already in full or delta) always wins.
+// KW > 0: keys of exactly KW words, loaded once into registers; KW == 0: any
+// key width, read from the candidate row as needed.
+template <int KW>
 __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, u64* table, u64 cap, u64* slot_of,
                                u64 epoch, u32* flags)
 {
@@ -1052,9 +1055,13 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
     u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const u64* key = cand + p * cw;
+    u64 kr[KW > 0 ? KW : 1];
+    if (KW > 0)
+#pragma unroll
+        for (int c = 0; c < KW; c++) kr[c] = key[c];
     const int stride = 2 + kw;
     const u64 mark = (epoch << 40) | (p + 1);
-    u64 h = pra_keyhash(key, kw);
+    u64 h = KW > 0 ? pra_keyhash(kr, KW) : pra_keyhash(key, kw);
     u64 m = cap - 1, s = h & m;
     for (u64 q = 0; q < cap; q++) {
         u64* slot = table + s * stride;
@@ -1064,7 +1071,12 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
             cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
             if (cur == 0) {
                 mine = true;
-                for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
+                if (KW > 0) {
+#pragma unroll
+                    for (int c = 0; c < KW; c++) slot[2 + c] = kr[c];
+                } else {
+                    for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
+                }
             }
         }
         if (mine || cur == h) {
@@ -1092,6 +1104,9 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
 // candidates per destination bucket per tile.
 // One block of PRA_DECIDE_BLOCK threads per PRA_TILE candidates: each
 // candidate is a chain of dependent loads, so many waves per tile hide them.
+// KW > 0: keys of exactly KW words (the slot's words and the candidate's are
+// loaded together); KW == 0: any key width.
+template <int KW>
 __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw,
                                                                         int kw, const u64* table, const u64* slot_of,
                                                                         u64 epoch, u32* keep, const u32* dest, u32 nb,
@@ -1113,8 +1128,19 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
                 const u64* slot = table + s * (2 + kw);
                 const u64* key = cand + p * cw;
                 bool eq = true;
-                for (int c = 0; c < kw; c++)
-                    if (slot[2 + c] != key[c]) { eq = false; break; }
+                if (KW > 0) {
+                    u64 x[KW > 0 ? KW : 1], y[KW > 0 ? KW : 1];
+#pragma unroll
+                    for (int c = 0; c < KW; c++) {
+                        x[c] = slot[2 + c];
+                        y[c] = key[c];
+                    }
+#pragma unroll
+                    for (int c = 0; c < KW; c++) eq = eq && x[c] == y[c];
+                } else {
+                    for (int c = 0; c < kw; c++)
+                        if (slot[2 + c] != key[c]) { eq = false; break; }
+                }
                 if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
                 if (slot[1] == ((epoch << 40) | (p + 1))) {
                     kp = 1;
@@ -1801,16 +1827,32 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
                              u64 cap, u64* slot_of, u64 epoch, u32* flags)
 {
     if (!capn) return;
-    hipLaunchKernelGGL(k_dedup_claim2, dim3(grid_for(capn)), dim3(PRA_BLOCK), 0, st, cand, ctl, cw, kw, table, cap,
-                       slot_of, epoch, flags);
+    const dim3 g(grid_for(capn)), b(PRA_BLOCK);
+    switch (kw) {
+    case 1: hipLaunchKernelGGL(k_dedup_claim2<1>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
+    case 2: hipLaunchKernelGGL(k_dedup_claim2<2>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
+    case 3: hipLaunchKernelGGL(k_dedup_claim2<3>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
+    case 4: hipLaunchKernelGGL(k_dedup_claim2<4>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
+    default: hipLaunchKernelGGL(k_dedup_claim2<0>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags);
+    }
 }
 
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags)
 {
-    hipLaunchKernelGGL(k_dedup_decide_hist, dim3(nblocks), dim3(PRA_DECIDE_BLOCK), 0, st, cand, ctl, cw, kw, table,
-                       slot_of, epoch, keep, dest, nb, hist, nblocks, flags);
+    const dim3 g(nblocks), b(PRA_DECIDE_BLOCK);
+#define PRA_DECIDE(K)                                                                                                  \
+    hipLaunchKernelGGL(k_dedup_decide_hist<K>, g, b, 0, st, cand, ctl, cw, kw, table, slot_of, epoch, keep, dest, nb, \
+                       hist, nblocks, flags)
+    switch (kw) {
+    case 1: PRA_DECIDE(1); break;
+    case 2: PRA_DECIDE(2); break;
+    case 3: PRA_DECIDE(3); break;
+    case 4: PRA_DECIDE(4); break;
+    default: PRA_DECIDE(0);
+    }
+#undef PRA_DECIDE
 }
 
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a)
