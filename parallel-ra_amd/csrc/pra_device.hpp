@@ -98,6 +98,27 @@ __host__ __device__ inline u64 pra_keyhash(const u64* k, int n)
     return h ? h : 1;
 }
 
+// Copy a row of w words between distinct buffers with every load of a chunk
+// of 8 words issued before its stores.
+__device__ inline void pra_copy_row(u64* __restrict__ dst, const u64* __restrict__ src, int w)
+{
+    int c = 0;
+    for (; c + 8 <= w; c += 8) {
+        u64 t[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) t[j] = src[c + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) dst[c + j] = t[j];
+    }
+    u64 t[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if (c + j < w) t[j] = src[c + j];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if (c + j < w) dst[c + j] = t[j];
+}
+
 // One radix digit of a row: either bits of a column or bits of the bucket.
 struct DigitSpec {
     int col;       // column index, or -1 for the bucket digit

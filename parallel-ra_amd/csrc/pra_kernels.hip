@@ -210,7 +210,7 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
             u64 pos = (u64)s_base[dg[k]] + wh[wave][dg[k]] + lc[k];
             const u64* src = in + idx * w;
             u64* dst = out + pos * w;
-            for (int c = 0; c < w; c++) dst[c] = src[c];
+            pra_copy_row(dst, src, w);
             if (IDX) iout[pos] = iin ? iin[idx] : (u32)idx;
         }
     }
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort(const u64* 
     for (u32 i = threadIdx.x; i < m; i += PRA_SMALL_SORT_BLOCK) {
         const u64* src = rows + (u64)idx[i] * w;
         u64* dst = out + (u64)i * w;
-        for (int c = 0; c < w; c++) dst[c] = src[c];
+        pra_copy_row(dst, src, w);
     }
     if (bnd && bucketed) {
         for (u32 i = threadIdx.x; i <= m; i += PRA_SMALL_SORT_BLOCK) {
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_block_sort(const u64* rows, u
         if (i < m) {
             const u64* src = rows + (u64)id[k] * w;
             u64* dst = out + (u64)i * w;
-            for (int c = 0; c < w; c++) dst[c] = src[c];
+            pra_copy_row(dst, src, w);
         }
     }
     if (bnd) {
@@ -457,7 +457,7 @@ __global__ void k_gather_rows(const u64* in, const u32* idx, u64 n, int w, u64* 
     if (i >= n) return;
     const u64* src = in + (u64)idx[i] * w;
     u64* dst = out + i * w;
-    for (int c = 0; c < w; c++) dst[c] = src[c];
+    pra_copy_row(dst, src, w);
 }
 
 // Packed-key sort of the newt rows.  The newt scatter has put every row's
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys,
         if (i < m) {
             const u64* src = rows + (u64)id[k] * w;
             u64* dst = out + (start + i) * w;
-            for (int c = 0; c < w; c++) dst[c] = src[c];
+            pra_copy_row(dst, src, w);
         }
     }
 }
@@ -669,7 +669,7 @@ __global__ void k_compact_rows(const u64* in, u64 n, int w, const u64* flags, co
     if (i >= n || !flags[i]) return;
     const u64* src = in + i * w;
     u64* dst = out + pos[i] * w;
-    for (int c = 0; c < w; c++) dst[c] = src[c];
+    pra_copy_row(dst, src, w);
 }
 
 // ------------------------------------------------------- join index --------
@@ -843,7 +843,7 @@ __global__ void k_merge_scatter(const u64* F, u64 nf, const u64* D, u64 nd, cons
         pos = j + rank[j];
     }
     u64* dst = out + pos * w;
-    for (int c = 0; c < w; c++) dst[c] = src[c];
+    pra_copy_row(dst, src, w);
 }
 
 // One thread per join pair: locate the outer row by binary search over the
@@ -1269,7 +1269,7 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
             u64 pos = (u64)wh[wave][d] + lc[k];
             const u64* src = a.cand + p * a.cw;
             u64* dst = a.newt + pos * a.wn;
-            for (int c = 0; c < a.cw; c++) dst[c] = src[c];
+            pra_copy_row(dst, src, a.cw);
             if (a.intern) {
                 u64 rank = pos - a.offs[(u64)d * a.nblocks];
                 u64 key = ((u64)a.tag << 46) | ((u64)d << 28);
