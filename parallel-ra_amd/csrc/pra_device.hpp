@@ -18,7 +18,8 @@ typedef uint32_t u32;
 #define PRA_TILE (PRA_BLOCK * PRA_ITEMS)
 // packed-key sort bins: rows per bin sorted in LDS by one block
 #define PRA_BIN_BLOCK 1024
-// one-block LDS sort of small inputs: rows, key words (bucket + columns), threads
+// one-block LDS sort of small inputs: rows, key words per row at that many rows
+// (n x key words must fit PRA_SMALL_SORT x PRA_SMALL_SORT_KW), threads
 #define PRA_SMALL_SORT 2048
 #define PRA_SMALL_SORT_KW 8
 #define PRA_SMALL_SORT_BLOCK 1024

@@ -661,7 +661,7 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
     // small inputs with several varying bytes: one block sorts them in LDS
     const bool bucketed = c->nb > 1 && by_bucket;
     const int kw = keycols + (bucketed ? 1 : 0);
-    if (n <= PRA_SMALL_SORT && kw <= PRA_SMALL_SORT_KW && passes.size() >= 2) {
+    if (n <= PRA_SMALL_SORT && n * kw <= (u64)PRA_SMALL_SORT * PRA_SMALL_SORT_KW && passes.size() >= 2) {
         c->sort_tmp.reserve(n * w * 8, c->st);
         u64* bout = (bnd && bucketed && bucket_last) ? bnd : nullptr;
         {

@@ -216,7 +216,8 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
     }
 }
 
-// One block sorts n <= PRA_SMALL_SORT rows by (bucket, col0 .. col[kc-1])
+// One block sorts n <= PRA_SMALL_SORT rows, whose key words (n x kw) fit the
+// PRA_SMALL_SORT x PRA_SMALL_SORT_KW words of LDS, by (bucket, col0 .. col[kc-1])
 // (the bucket FNV(row[0:jcc]) % nb leads when `bucketed`; columns compare as
 // unsigned words) with a bitonic network over row indices in LDS, then
 // gathers the rows into `out`.  Unstable: rows with equal keys are equal.
