@@ -1045,24 +1045,35 @@ __global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host
 // stores the key; every candidate then lowers the slot's arrival mark to
 // (epoch << 40 | p+1), so the earliest arrival of this iteration wins, while a
 // mark from an earlier iteration (key already in full or delta) always wins.
-// KW > 0: keys of exactly KW words, loaded once into registers; KW == 0: any
-// key width, read from the candidate row as needed.
-template <int KW>
-__global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, u64* table, u64 cap, u64* slot_of,
+// MAXKW > 0: keys of at most MAXKW words (exactly MAXKW with EXACT), loaded
+// once into registers with every word in flight together; MAXKW == 0: any key
+// width, read from the row as needed.
+template <int MAXKW, bool EXACT>
+__global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw_, u64* table, u64 cap, u64* slot_of,
                                u64 epoch, u32* flags)
 {
+    const int kw = EXACT ? MAXKW : kw_;
     if (ctl[1]) return;
     const u64 n = ctl[0];
     u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const u64* key = cand + p * cw;
-    u64 kr[KW > 0 ? KW : 1];
-    if (KW > 0)
+    u64 kr[MAXKW > 0 ? MAXKW : 1];
+    u64 h;
+    if (MAXKW > 0) {
 #pragma unroll
-        for (int c = 0; c < KW; c++) kr[c] = key[c];
+        for (int c = 0; c < MAXKW; c++) kr[c] = c < kw ? key[c] : 0;
+        // pra_keyhash over the register copy
+        h = 0x9E3779B97F4A7C15ULL ^ (u64)kw;
+#pragma unroll
+        for (int c = 0; c < MAXKW; c++)
+            if (c < kw) h = pra_mix64(h ^ (kr[c] + 0x632BE59BD9B4E019ULL * (u64)(c + 1)));
+        h = h ? h : 1;
+    } else {
+        h = pra_keyhash(key, kw);
+    }
     const int stride = 2 + kw;
     const u64 mark = (epoch << 40) | (p + 1);
-    u64 h = KW > 0 ? pra_keyhash(kr, KW) : pra_keyhash(key, kw);
     u64 m = cap - 1, s = h & m;
     for (u64 q = 0; q < cap; q++) {
         u64* slot = table + s * stride;
@@ -1072,9 +1083,10 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
             cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
             if (cur == 0) {
                 mine = true;
-                if (KW > 0) {
+                if (MAXKW > 0) {
 #pragma unroll
-                    for (int c = 0; c < KW; c++) slot[2 + c] = kr[c];
+                    for (int c = 0; c < MAXKW; c++)
+                        if (c < kw) slot[2 + c] = kr[c];
                 } else {
                     for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
                 }
@@ -1105,14 +1117,17 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw, 
 // candidates per destination bucket per tile.
 // One block of PRA_DECIDE_BLOCK threads per PRA_TILE candidates: each
 // candidate is a chain of dependent loads, so many waves per tile hide them.
-// KW > 0: keys of exactly KW words (the slot's words and the candidate's are
-// loaded together); KW == 0: any key width.
-template <int KW>
+// MAXKW > 0: keys of at most MAXKW words (exactly MAXKW with EXACT; the
+// slot's words and the candidate's are loaded together); MAXKW == 0: any key
+// width.
+template <int MAXKW, bool EXACT>
 __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw,
-                                                                        int kw, const u64* table, const u64* slot_of,
-                                                                        u64 epoch, u32* keep, const u32* dest, u32 nb,
-                                                                        u32* hist, u32 nblocks, u32* flags)
+                                                                        int kw_, const u64* table,
+                                                                        const u64* slot_of, u64 epoch, u32* keep,
+                                                                        const u32* dest, u32 nb, u32* hist,
+                                                                        u32 nblocks, u32* flags)
 {
+    const int kw = EXACT ? MAXKW : kw_;
     if (ctl[1]) return;
     const u64 n = ctl[0];
     __shared__ u32 h[PRA_RADIX];
@@ -1129,15 +1144,15 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
                 const u64* slot = table + s * (2 + kw);
                 const u64* key = cand + p * cw;
                 bool eq = true;
-                if (KW > 0) {
-                    u64 x[KW > 0 ? KW : 1], y[KW > 0 ? KW : 1];
+                if (MAXKW > 0) {
+                    u64 x[MAXKW > 0 ? MAXKW : 1], y[MAXKW > 0 ? MAXKW : 1];
 #pragma unroll
-                    for (int c = 0; c < KW; c++) {
-                        x[c] = slot[2 + c];
-                        y[c] = key[c];
+                    for (int c = 0; c < MAXKW; c++) {
+                        x[c] = c < kw ? slot[2 + c] : 0;
+                        y[c] = c < kw ? key[c] : 0;
                     }
 #pragma unroll
-                    for (int c = 0; c < KW; c++) eq = eq && x[c] == y[c];
+                    for (int c = 0; c < MAXKW; c++) eq = eq && x[c] == y[c];
                 } else {
                     for (int c = 0; c < kw; c++)
                         if (slot[2 + c] != key[c]) { eq = false; break; }
@@ -1829,13 +1844,22 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
 {
     if (!capn) return;
     const dim3 g(grid_for(capn)), b(PRA_BLOCK);
+#define PRA_CLAIM(K, E)                                                                                                \
+    hipLaunchKernelGGL((k_dedup_claim2<K, E>), g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags)
     switch (kw) {
-    case 1: hipLaunchKernelGGL(k_dedup_claim2<1>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
-    case 2: hipLaunchKernelGGL(k_dedup_claim2<2>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
-    case 3: hipLaunchKernelGGL(k_dedup_claim2<3>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
-    case 4: hipLaunchKernelGGL(k_dedup_claim2<4>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags); break;
-    default: hipLaunchKernelGGL(k_dedup_claim2<0>, g, b, 0, st, cand, ctl, cw, kw, table, cap, slot_of, epoch, flags);
+    case 1: PRA_CLAIM(1, true); break;
+    case 2: PRA_CLAIM(2, true); break;
+    case 3: PRA_CLAIM(3, true); break;
+    case 4: PRA_CLAIM(4, true); break;
+    default:
+        if (kw <= 8)
+            PRA_CLAIM(8, false);
+        else if (kw <= 16)
+            PRA_CLAIM(16, false);
+        else
+            PRA_CLAIM(0, false);
     }
+#undef PRA_CLAIM
 }
 
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
@@ -1843,15 +1867,21 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                                   u32 nblocks, u32* flags)
 {
     const dim3 g(nblocks), b(PRA_DECIDE_BLOCK);
-#define PRA_DECIDE(K)                                                                                                  \
-    hipLaunchKernelGGL(k_dedup_decide_hist<K>, g, b, 0, st, cand, ctl, cw, kw, table, slot_of, epoch, keep, dest, nb, \
-                       hist, nblocks, flags)
+#define PRA_DECIDE(K, E)                                                                                               \
+    hipLaunchKernelGGL((k_dedup_decide_hist<K, E>), g, b, 0, st, cand, ctl, cw, kw, table, slot_of, epoch, keep, dest, \
+                       nb, hist, nblocks, flags)
     switch (kw) {
-    case 1: PRA_DECIDE(1); break;
-    case 2: PRA_DECIDE(2); break;
-    case 3: PRA_DECIDE(3); break;
-    case 4: PRA_DECIDE(4); break;
-    default: PRA_DECIDE(0);
+    case 1: PRA_DECIDE(1, true); break;
+    case 2: PRA_DECIDE(2, true); break;
+    case 3: PRA_DECIDE(3, true); break;
+    case 4: PRA_DECIDE(4, true); break;
+    default:
+        if (kw <= 8)
+            PRA_DECIDE(8, false);
+        else if (kw <= 16)
+            PRA_DECIDE(16, false);
+        else
+            PRA_DECIDE(0, false);
     }
 #undef PRA_DECIDE
 }
