@@ -791,7 +791,15 @@ __global__ void k_join_count_sorted(const u64* outer, u64 n, int wo, int jcc, co
         if (pra_row_cmp(inner + m * wi, key, jcc) < 0) a = m + 1; else z = m;
     }
     u64 first = a;
-    z = hi;               // first row > key
+    // first row > key: gallop from the first match (key runs are short), then
+    // a binary search inside the last step
+    u64 step = 1;
+    z = first;
+    while (z < hi && pra_row_cmp(inner + z * wi, key, jcc) <= 0) {
+        a = z + 1;
+        z = first + step < hi ? first + step : hi;
+        step <<= 1;
+    }
     while (a < z) {
         u64 m = (a + z) >> 1;
         if (pra_row_cmp(inner + m * wi, key, jcc) <= 0) a = m + 1; else z = m;
