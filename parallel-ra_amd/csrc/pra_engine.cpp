@@ -540,7 +540,7 @@ static void check_flags(u32 f)
     if (f & (PRA_FLAG_COLLISION | PRA_FLAG_INDEX_COLLISION))
         throw PraError(PRA_E_COLLISION, "64-bit key hash collision between distinct keys");
     if (f & PRA_FLAG_TABLE_FULL) throw PraError(PRA_E_HIP, "hash table overflow");
-    if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort look-back did not complete");
+    if (f & PRA_FLAG_SORT_STALL) throw PraError(PRA_E_HIP, "sort or scan look-back did not complete");
     if (f & PRA_FLAG_BIN_OVERFLOW) throw PraError(PRA_E_HIP, "sort bin exceeds its capacity");
     if (f & PRA_FLAG_SCAN_OVERFLOW) throw PraError(PRA_E_HIP, "prefix sum exceeds 2^46");
 }

@@ -1223,8 +1223,9 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     __syncthreads();
-    // per-block summaries: the packed key's OR / AND / dictionary miss and its
-    // byte-digit histograms (packed mode), or the key columns' OR / AND
+    // per-block summaries: the packed key's OR / AND / dictionary miss and the
+    // block's share of each top-byte bin (packed mode), or the key columns'
+    // OR / AND
     __shared__ u32 khist[PRA_RADIX];
     __shared__ u64 red[SW][PRA_PART_WORDS];
     constexpr bool packed = PACKED;
