@@ -1123,8 +1123,9 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw_,
 // Keep the winners, verify every key against its slot (a 64-bit hash
 // collision between distinct keys is reported, never merged) and count kept
 // candidates per destination bucket per tile.
-// One block of PRA_DECIDE_BLOCK threads per PRA_TILE candidates: each
-// candidate is a chain of dependent loads, so many waves per tile hide them.
+// Blocks of PRA_DECIDE_BLOCK threads take tiles of PRA_TILE candidates in
+// turn: each candidate is a chain of dependent loads, so many waves per tile
+// hide them.
 // MAXKW > 0: keys of at most MAXKW words (exactly MAXKW with EXACT; the
 // slot's words and the candidate's are loaded together); MAXKW == 0: any key
 // width.
@@ -1139,43 +1140,51 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
     if (ctl[1]) return;
     const u64 n = ctl[0];
     __shared__ u32 h[PRA_RADIX];
-    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
-    __syncthreads();
-    u64 base = (u64)blockIdx.x * PRA_TILE;
-#pragma unroll
-    for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
-        u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
-        if (p < n) {
-            u64 s = slot_of[p];
-            u32 kp = 0;
-            if (s != ~0ULL) {
-                const u64* slot = table + s * (2 + kw);
-                const u64* key = cand + p * cw;
-                bool eq = true;
-                if (MAXKW > 0) {
-                    u64 x[MAXKW > 0 ? MAXKW : 1], y[MAXKW > 0 ? MAXKW : 1];
-#pragma unroll
-                    for (int c = 0; c < MAXKW; c++) {
-                        x[c] = c < kw ? slot[2 + c] : 0;
-                        y[c] = c < kw ? key[c] : 0;
-                    }
-#pragma unroll
-                    for (int c = 0; c < MAXKW; c++) eq = eq && x[c] == y[c];
-                } else {
-                    for (int c = 0; c < kw; c++)
-                        if (slot[2 + c] != key[c]) { eq = false; break; }
-                }
-                if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
-                if (slot[1] == ((epoch << 40) | (p + 1))) {
-                    kp = 1;
-                    atomicAdd(&h[dest[p]], 1u);
-                }
-            }
-            keep[p] = kp;
+    for (u32 tile = blockIdx.x; tile < nblocks; tile += gridDim.x) {
+        u64 base = (u64)tile * PRA_TILE;
+        if (base >= n) {
+            // a tile past the candidates keeps nothing
+            for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = 0;
+            continue;
         }
+        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
+            u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
+            if (p < n) {
+                u64 s = slot_of[p];
+                u32 kp = 0;
+                if (s != ~0ULL) {
+                    const u64* slot = table + s * (2 + kw);
+                    const u64* key = cand + p * cw;
+                    bool eq = true;
+                    if (MAXKW > 0) {
+                        u64 x[MAXKW > 0 ? MAXKW : 1], y[MAXKW > 0 ? MAXKW : 1];
+#pragma unroll
+                        for (int c = 0; c < MAXKW; c++) {
+                            x[c] = c < kw ? slot[2 + c] : 0;
+                            y[c] = c < kw ? key[c] : 0;
+                        }
+#pragma unroll
+                        for (int c = 0; c < MAXKW; c++) eq = eq && x[c] == y[c];
+                    } else {
+                        for (int c = 0; c < kw; c++)
+                            if (slot[2 + c] != key[c]) { eq = false; break; }
+                    }
+                    if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
+                    if (slot[1] == ((epoch << 40) | (p + 1))) {
+                        kp = 1;
+                        atomicAdd(&h[dest[p]], 1u);
+                    }
+                }
+                keep[p] = kp;
+            }
+        }
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = h[i];
+        __syncthreads();
     }
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + blockIdx.x] = h[i];
 }
 
 // Stable partition of the kept candidates by destination bucket; JOIN-type
@@ -1875,7 +1884,9 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags)
 {
-    const dim3 g(nblocks), b(PRA_DECIDE_BLOCK);
+    // tiles past the candidate count (known only on the device) cost a block
+    // each, and a block fills a compute unit: a bounded grid strides over them
+    const dim3 g(std::min<u32>(nblocks, PRA_DECIDE_GRID)), b(PRA_DECIDE_BLOCK);
 #define PRA_DECIDE(K, E)                                                                                               \
     hipLaunchKernelGGL((k_dedup_decide_hist<K, E>), g, b, 0, st, cand, ctl, cw, kw, table, slot_of, epoch, keep, dest, \
                        nb, hist, nblocks, flags)
