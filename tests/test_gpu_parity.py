@@ -292,3 +292,51 @@ def test_skewed_keys_take_the_byte_pass_sort(nb):
     assert o.count(To) == 3000 + 1 + 3000
     ks = g.kernel_stats()
     assert ks["bin_sort"]["launches"] == 0 and ks["radix_scatter"]["launches"] > 0
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+@pytest.mark.parametrize("n", [2, 3, 100, 200, 1500, 3000, 20000])
+def test_wide_rows_sort_paths(nb, n):
+    # five-column rows (one column of random 64-bit values, four of 24-bit
+    # ones): the acopy's rows (six columns, the id moved) and the interned
+    # copy's (once their dictionary ranks do not fit one packed 64-bit key)
+    # are sorted column by column, by the one-block network (n <= 2048), the
+    # one-block radix passes (n <= 16384) or the one-sweep byte passes; both
+    # must order them as the oracle does
+    print(f"[wide sort] nb={nb} n={n}", flush=True)
+    rng = np.random.default_rng(n * 10 + nb)
+    vals = rng.integers(0, 2**24, size=(n, 5), dtype=np.uint64)
+    vals[:, 2] = rng.integers(0, 2**63, size=n, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    rows = np.concatenate([vals, np.arange(7000, 7000 + n, dtype=np.uint64)[:, None]], axis=1)
+
+    def build(b):
+        A = b.rel_create(5, 1, 300)
+        B = b.rel_create(5, 2, 301)
+        C = b.rel_create(5, 1, 302)
+        b.load_rows(A, rows, FULL)
+        s = b.scc_create(True, 40)
+        b.scc_add_relation(s, A, True)
+        b.scc_add_relation(s, B, False)
+        b.scc_add_relation(s, C, False)
+        b.add_acopy(s, B, A, DELTA, [3, 0, 4, 1, 2, 5])
+        b.add_copy(s, C, A, DELTA, [2, 4, 0, 3, 1])
+        return A, B, C, s
+
+    g = gpu_ctx(nb)
+    g.set_profiling(True)
+    Ag, Bg, Cg, sg = build(g)
+    g.execute()
+    o = O.Oracle(nb)
+    Ao, Bo, Co, so = build(o)
+    o.execute(1)
+    assert_same_rel(o, Bo, g, Bg, "acopy target")
+    assert_same_rel(o, Co, g, Cg, "copy target")
+    assert o.history(so) == g.history(sg)
+    ks = g.kernel_stats()
+    if n <= 1500:
+        assert ks["small_sort"]["launches"] > 0
+    elif n <= 16384:
+        assert ks["block_sort"]["launches"] > 0
+    else:
+        assert ks["radix_scatter"]["launches"] > 0
+    g.close()
