@@ -28,6 +28,7 @@ typedef uint32_t u32;
 // threads of the dedup decision and of the newt scatter per PRA_TILE candidates
 #define PRA_DECIDE_BLOCK 1024
 #define PRA_DECIDE_GRID 512
+#define PRA_COUNT_SCAN_BLOCK 1024
 #define PRA_SCATTER_BLOCK 1024
 #define PRA_SCATTER_WIDE_TILES 64   // tiles from which the newt scatter uses PRA_SCATTER_BLOCK
 #define PRA_BIN_ITEMS 8

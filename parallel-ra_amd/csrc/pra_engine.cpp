@@ -1300,30 +1300,46 @@ static void run_iteration(pra_scc* S)
         u64 n = O.n;
         ra.n_outer = ra.active ? n : 0;
         if (!ra.active) continue;
-        ra.cnt.reserve(std::max<u64>(1, n) * 8, c->st);
         ra.off.reserve((n + 1) * 8, c->st);
         ra.istart.reserve(std::max<u64>(1, n) * 8, c->st);
-        if (ra.type == RA_JOIN) {
+        if (n == 0) {
+            pra_launch_fill_u64(c->st, ra.off.as<u64>(), 1, 0);
+        } else if (ra.type == RA_JOIN && ra.sorted_probe) {
             Rows& I = ra.inner->ver[ra.inner_v];
-            Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + 16.0 * n);
-            if (ra.sorted_probe) {
+            ra.cnt.reserve(n * 8, c->st);
+            {
+                Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + 16.0 * n);
                 pra_launch_join_count_sorted(c->st, O.d(), n, ra.outer->w, ra.in0->jcc, I.d(), ra.inner->w,
                                              ra.inner->bnd[ra.inner_v].as<u64>(), (u64)nb, ra.cnt.as<u64>(),
                                              ra.istart.as<u64>());
-            } else if (ra.inner->idx_cap[ra.inner_v] == 0) {
-                pra_launch_fill_u64(c->st, ra.cnt.as<u64>(), n, 0);
-            } else {
-                pra_launch_join_count(c->st, O.d(), n, ra.outer->w, ra.in0->jcc, ra.inner->idx[ra.inner_v].as<u64>(),
-                                      ra.inner->idx_cap[ra.inner_v], I.d(), ra.inner->w, ra.cnt.as<u64>(),
-                                      ra.istart.as<u64>());
+                check_launch("join_count");
             }
-            check_launch("join_count");
+            scan_u64(c, ra.cnt.as<u64>(), ra.off.as<u64>(), n, ra.off.as<u64>() + n);
         } else {
-            int op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
-            pra_launch_copy_count(c->st, O.d(), n, ra.outer->w, op, ra.cnt.as<u64>());
-            check_launch("copy_count");
+            // the counts and their offsets in one pass
+            CountScanArgs a;
+            memset(&a, 0, sizeof(a));
+            a.outer = O.d();
+            a.n = n;
+            a.wo = ra.outer->w;
+            a.off = ra.off.as<u64>();
+            const bool join = ra.type == RA_JOIN;
+            if (join) {
+                a.table = ra.inner->idx[ra.inner_v].as<u64>();
+                a.cap = ra.inner->idx_cap[ra.inner_v];
+                a.inner = ra.inner->ver[ra.inner_v].d();
+                a.wi = ra.inner->w;
+                a.jcc = ra.in0->jcc;
+                a.istart = ra.istart.as<u64>();
+            } else {
+                a.op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
+            }
+            prepare_lookback(c, n);
+            const u32 ep = next_sort_epoch(c);
+            Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + (join ? 24.0 : 8.0) * n);
+            pra_launch_count_scan(c->st, a, join, c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags());
+            check_launch("count_scan");
         }
-        scan_u64(c, ra.cnt.as<u64>(), ra.off.as<u64>(), n, ra.off.as<u64>() + n);
         bucket_bounds(c, ra.outer, ra.outer_v);
     }
 
@@ -1519,13 +1535,14 @@ static void run_iteration(pra_scc* S)
         c->offs.reserve((size_t)nb * nblocks * 4 + 8, c->st);
         {
             ProfPerCand p(c, K_DEDUP_VERIFY, 8.0 * kw + 16.0, oi);
+            // one tile: the decision writes the offsets itself
             pra_launch_dedup_decide_hist(c->st, c->cand.as<u64>(), ctl, cw, kw, tab, c->slot_of.as<u64>(), epoch,
                                          c->keep.as<u32>(), c->dest.as<u32>(), (u32)nb, c->hist.as<u32>(), nblocks,
-                                         c->dflags());
+                                         c->dflags(), nblocks == 1 ? c->offs.as<u32>() : nullptr);
             check_launch("dedup_decide_hist");
         }
         u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
-        scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
+        if (nblocks > 1) scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
         DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
         {
             NewtScatterArgs a;

@@ -31,7 +31,7 @@ __global__ void k_table_clear(u64* t, u64 cap, int stride)
     }
 }
 
-template <typename T>
+template <typename T, int NW = PRA_WAVES>
 __device__ inline T block_exclusive_scan(T v, T* lds, T* total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -45,16 +45,16 @@ __device__ inline T block_exclusive_scan(T v, T* lds, T* total)
     __syncthreads();
     if (threadIdx.x == 0) {
         T acc = 0;
-        for (int w = 0; w < PRA_WAVES; w++) {
+        for (int w = 0; w < NW; w++) {
             T t = lds[w];
             lds[w] = acc;
             acc += t;
         }
-        lds[PRA_WAVES] = acc;
+        lds[NW] = acc;
     }
     __syncthreads();
     T res = x - v + lds[wave];
-    *total = lds[PRA_WAVES];
+    *total = lds[NW];
     __syncthreads();
     return res;
 }
@@ -738,33 +738,6 @@ __global__ void k_index_insert(const u64* rows, int w, int jcc, const u64* gstar
 
 // --------------------------------------------------------------- join ------
 
-// Count the inner rows matching each outer row's first jcc columns.
-__global__ void k_join_count(const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
-                             const u64* inner, int wi, u64* cnt, u64* istart)
-{
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const u64* key = outer + i * wo;
-    u64 h = pra_keyhash(key, jcc);
-    u64 m = cap - 1, s = h & m, c = 0, st = 0;
-    for (u64 p = 0; p < cap; p++) {
-        const u64* slot = table + s * 3;
-        u64 sh = slot[0];
-        if (sh == 0) break;
-        if (sh == h) {
-            const u64* f = inner + slot[1] * wi;
-            bool eq = true;
-            for (int x = 0; x < jcc; x++)
-                if (f[x] != key[x]) { eq = false; break; }
-            if (eq) { c = slot[2]; st = slot[1]; }
-            break;
-        }
-        s = (s + 1) & m;
-    }
-    cnt[i] = c;
-    istart[i] = st;
-}
-
 // lexicographic comparison of the first n words of two rows
 __device__ inline int pra_row_cmp(const u64* a, const u64* b, int n)
 {
@@ -956,14 +929,6 @@ __device__ inline bool pra_filter(int op, u64 x, u64 y)
     }
 }
 
-__global__ void k_copy_count(const u64* rows, u64 n, int w, int op, u64* cnt)
-{
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const u64* r = rows + i * w;
-    cnt[i] = pra_filter(op, r[0], r[1]) ? 1 : 0;
-}
-
 __global__ void k_copy_expand(CopyExpandArgs a)
 {
     if (a.ctl[1]) return;
@@ -1134,17 +1099,20 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
                                                                         int kw_, const u64* table,
                                                                         const u64* slot_of, u64 epoch, u32* keep,
                                                                         const u32* dest, u32 nb, u32* hist,
-                                                                        u32 nblocks, u32* flags)
+                                                                        u32 nblocks, u32* flags, u32* offs1)
 {
     const int kw = EXACT ? MAXKW : kw_;
     if (ctl[1]) return;
     const u64 n = ctl[0];
     __shared__ u32 h[PRA_RADIX];
+    __shared__ u32 sl[PRA_DECIDE_BLOCK / 64 + 1];
     for (u32 tile = blockIdx.x; tile < nblocks; tile += gridDim.x) {
         u64 base = (u64)tile * PRA_TILE;
         if (base >= n) {
             // a tile past the candidates keeps nothing
             for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = 0;
+            if (offs1)
+                for (u32 i = threadIdx.x; i <= nb; i += PRA_DECIDE_BLOCK) offs1[i] = 0;
             continue;
         }
         for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
@@ -1182,7 +1150,17 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
             }
         }
         __syncthreads();
-        for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = h[i];
+        if (offs1) {
+            // a single tile: its counts are the whole histogram, whose
+            // exclusive offsets (and total, offs1[nb]) go out directly
+            u32 tot;
+            const u32 v = threadIdx.x < nb ? h[threadIdx.x] : 0u;
+            const u32 ex = block_exclusive_scan<u32, PRA_DECIDE_BLOCK / 64>(v, sl, &tot);
+            if (threadIdx.x < nb) offs1[threadIdx.x] = ex;
+            if (threadIdx.x == 0) offs1[nb] = tot;
+        } else {
+            for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = h[i];
+        }
         __syncthreads();
     }
 }
@@ -1638,6 +1616,129 @@ __global__ void k_scan_lookback(const T* in, T* out, u64 n, u64* status, u32* ti
     }
 }
 
+// One rule's per-row output counts and their exclusive prefix sums in one
+// pass.  JOIN: a row's count is the length of its key's run of inner rows,
+// found in the static inner's key index (istart = the run's first row); COPY
+// and COPY_FILTER: 1 for rows passing the filter `op`.  Tiles of PRA_TILE
+// rows (PRA_TILE / PRA_COUNT_SCAN_BLOCK consecutive rows per thread) are
+// taken in ticket order and find their offset by decoupled look-back over
+// digit 0 of the status words, as k_scan_lookback does; off[n] = total.
+template <bool JOIN>
+__global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan(CountScanArgs a, u64* status, u32* tile_ctr,
+                                                                     u32 epoch, u32* flags)
+{
+    constexpr int IT = PRA_TILE / PRA_COUNT_SCAN_BLOCK;
+    constexpr int NW = PRA_COUNT_SCAN_BLOCK / 64;
+    __shared__ u64 lds[NW + 1];
+    __shared__ u32 s_tile;
+    __shared__ u64 s_excl;
+    if (threadIdx.x == 0) {
+        u32 t = atomicAdd(tile_ctr, 1u);
+        if (t == gridDim.x - 1) atomicExch(tile_ctr, 0u);
+        s_tile = t;
+    }
+    __syncthreads();
+    const u32 tile = s_tile;
+    const u64 base = (u64)tile * PRA_TILE + (u64)threadIdx.x * IT;
+    u64 c[IT];
+    if (JOIN) {
+        // every item's key hash and first index slot are loaded before any
+        // item walks on (a probe mostly ends at its first slot)
+        const u64 m = a.cap - 1;
+        u64 h[IT], f0[IT], f1[IT], f2[IT];
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const u64 i = base + k;
+            h[k] = i < a.n && a.cap ? pra_keyhash(a.outer + i * a.wo, a.jcc) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const u64* sl = a.table + (h[k] & m) * 3;
+            f0[k] = h[k] ? sl[0] : 0;
+            f1[k] = h[k] ? sl[1] : 0;
+            f2[k] = h[k] ? sl[2] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const u64 i = base + k;
+            u64 cnt = 0, st = 0;
+            if (h[k]) {
+                const u64* key = a.outer + i * a.wo;
+                u64 s = h[k] & m, sh = f0[k], ss = f1[k], sc = f2[k];
+                for (u64 q = 0; q < a.cap && sh != 0; q++) {
+                    if (sh == h[k]) {
+                        const u64* f = a.inner + ss * a.wi;
+                        bool eq = true;
+                        for (int x = 0; x < a.jcc; x++)
+                            if (f[x] != key[x]) { eq = false; break; }
+                        if (eq) { cnt = sc; st = ss; }
+                        break;
+                    }
+                    s = (s + 1) & m;
+                    const u64* sl = a.table + s * 3;
+                    sh = sl[0];
+                    ss = sl[1];
+                    sc = sl[2];
+                }
+            }
+            c[k] = cnt;
+            if (i < a.n) a.istart[i] = st;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const u64 i = base + k;
+            const u64* r = a.outer + i * a.wo;
+            c[k] = i < a.n && pra_filter(a.op, r[0], r[1]) ? 1 : 0;
+        }
+    }
+    u64 sum = 0;
+#pragma unroll
+    for (int k = 0; k < IT; k++) sum += c[k];
+    u64 tot;
+    const u64 ex = block_exclusive_scan<u64, NW>(sum, lds, &tot);
+    if (threadIdx.x < 64) {
+        constexpr u64 LIM = 1ULL << 46;
+        u64* my = status + (u64)tile * PRA_RADIX;
+        u64 excl = 0;
+        if (threadIdx.x == 0) {
+            if (tot >= LIM) atomicOr(flags, PRA_FLAG_SCAN_OVERFLOW);
+            __hip_atomic_store(my, st_pack(epoch, tile == 0 ? PRA_ST_INC : PRA_ST_AGG, tot & (LIM - 1)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tile != 0) {
+            excl = pra_lookback_wave(status, tile, epoch, flags);
+            if (threadIdx.x == 0) {
+                if (excl + tot >= LIM) atomicOr(flags, PRA_FLAG_SCAN_OVERFLOW);
+                __hip_atomic_store(my, st_pack(epoch, PRA_ST_INC, (excl + tot) & (LIM - 1)), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (threadIdx.x == 0) {
+            s_excl = excl;
+            if (tile == gridDim.x - 1) a.off[a.n] = excl + tot;
+        }
+    }
+    __syncthreads();
+    u64 run = s_excl + ex;
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        if (base + k < a.n) a.off[base + k] = run;
+        run += c[k];
+    }
+}
+
+void pra_launch_count_scan(hipStream_t st, const CountScanArgs& a, bool join, u64* status, u32* tile_ctr, u32 epoch,
+                           u32* flags)
+{
+    if (!a.n) return;
+    const dim3 g(pra_tiles(a.n)), b(PRA_COUNT_SCAN_BLOCK);
+    if (join)
+        hipLaunchKernelGGL(k_count_scan<true>, g, b, 0, st, a, status, tile_ctr, epoch, flags);
+    else
+        hipLaunchKernelGGL(k_count_scan<false>, g, b, 0, st, a, status, tile_ctr, epoch, flags);
+}
+
 template <typename T>
 static void scan_impl(hipStream_t st, const T* in, T* out, u64 n, u64* status, u32* tile_ctr, u32 epoch, u32* flags,
                       T* total)
@@ -1693,14 +1794,6 @@ void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, co
                        cap, flags);
 }
 
-void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
-                           const u64* inner, int wi, u64* cnt, u64* istart)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_join_count, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, outer, n, wo, jcc, table, cap, inner,
-                       wi, cnt, istart);
-}
-
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap)
 {
     if (!cap) return;
@@ -1722,12 +1815,6 @@ void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp,
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host)
 {
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(PRA_BLOCK), 0, st, src, words, flags, host);
-}
-
-void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_copy_count, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, op, cnt);
 }
 
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)
@@ -1882,14 +1969,14 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
 
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
-                                  u32 nblocks, u32* flags)
+                                  u32 nblocks, u32* flags, u32* offs1)
 {
     // tiles past the candidate count (known only on the device) cost a block
     // each, and a block fills a compute unit: a bounded grid strides over them
     const dim3 g(std::min<u32>(nblocks, PRA_DECIDE_GRID)), b(PRA_DECIDE_BLOCK);
 #define PRA_DECIDE(K, E)                                                                                               \
     hipLaunchKernelGGL((k_dedup_decide_hist<K, E>), g, b, 0, st, cand, ctl, cw, kw, table, slot_of, epoch, keep, dest, \
-                       nb, hist, nblocks, flags)
+                       nb, hist, nblocks, flags, offs1)
     switch (kw) {
     case 1: PRA_DECIDE(1, true); break;
     case 2: PRA_DECIDE(2, true); break;

@@ -43,6 +43,17 @@ struct CopyExpandArgs {
     int map[PRA_MAXW];
 };
 
+// One rule's per-row output counts and their exclusive prefix sums
+struct CountScanArgs {
+    const u64* outer;    // outer rows (wo words each)
+    const u64* table;    // JOIN: the static inner's key index, cap slots of [hash, first row, run length]
+    const u64* inner;    // JOIN: inner rows (wi words each)
+    u64* off;            // n + 1: exclusive prefix of the counts, off[n] = total
+    u64* istart;         // JOIN: first inner row of each outer row's key run
+    u64 n, cap;
+    int wo, wi, jcc, op;
+};
+
 #define PRA_MAXRA 64
 
 struct PlanArgs {
@@ -117,14 +128,13 @@ void pra_launch_group_compact(hipStream_t st, const u64* flags, const u64* gpos,
 void pra_launch_compact_rows(hipStream_t st, const u64* in, u64 n, int w, const u64* flags, const u64* pos, u64* out);
 void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, const u64* gstart, u64 ng, u64* table,
                              u64 cap, u32* flags);
-void pra_launch_join_count(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* table, u64 cap,
-                           const u64* inner, int wi, u64* cnt, u64* istart);
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap);
 void pra_launch_plan(hipStream_t st, const PlanArgs& a);
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out);
 void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
-void pra_launch_copy_count(hipStream_t st, const u64* rows, u64 n, int w, int op, u64* cnt);
+void pra_launch_count_scan(hipStream_t st, const CountScanArgs& a, bool join, u64* status, u32* tile_ctr, u32 epoch,
+                           u32* flags);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
@@ -152,7 +162,7 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
                              u64 cap, u64* slot_of, u64 epoch, u32* flags);
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
-                                  u32 nblocks, u32* flags);
+                                  u32 nblocks, u32* flags, u32* offs1 = nullptr);
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
 void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
