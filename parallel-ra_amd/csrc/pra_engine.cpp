@@ -1544,6 +1544,22 @@ static void run_iteration(pra_scc* S)
         u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
         if (nblocks > 1) scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
         DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
+        R->kgofs.reserve(PRA_RADIX * 4, c->st);
+        NewtFoldArgs f;   // bin_cur: set with the scatter's bins
+        memset(&f, 0, sizeof(f));
+        f.rep_words = c->rep_words.as<u64>();
+        f.ctl = ctl;
+        f.stat = st2o;
+        f.kgofs = R->kgofs.as<u32>();
+        f.seqbase = sq ? sq->as<u64>() : nullptr;
+        f.offs = c->offs.as<u32>();
+        f.total = total_dev;
+        f.nblocks = nblocks;
+        f.nb = (u32)nb;
+        f.packed = pk ? 1 : 0;
+        f.intern = intern ? 1 : 0;
+        // one tile: the scatter's block folds the summaries itself
+        const bool fold_here = nblocks == 1;
         {
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
@@ -1578,6 +1594,7 @@ static void run_iteration(pra_scc* S)
                 a.bin_keys = R->bin_keys.as<u64>();
                 a.bin_idx = R->bin_idx.as<u32>();
                 a.bin_cur = R->bin_cur.as<u32>();
+                f.bin_cur = a.bin_cur;
                 R->nkeys.reserve(C * 8, c->st);
                 a.dict = c->dict.as<u64>();
                 a.dict_cap = c->dict_cap;
@@ -1585,26 +1602,13 @@ static void run_iteration(pra_scc* S)
                 a.rank_radix = c->dict_n;
                 a.key_lshift = pk_ls[oi];
             }
+            a.fold_here = fold_here ? 1 : 0;
+            a.fold = f;
             ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
         }
-        R->kgofs.reserve(PRA_RADIX * 4, c->st);
-        {
-            NewtFoldArgs f;
-            memset(&f, 0, sizeof(f));
-            f.bin_cur = pk ? R->bin_cur.as<u32>() : nullptr;
-            f.rep_words = c->rep_words.as<u64>();
-            f.ctl = ctl;
-            f.stat = st2o;
-            f.kgofs = R->kgofs.as<u32>();
-            f.seqbase = sq ? sq->as<u64>() : nullptr;
-            f.offs = c->offs.as<u32>();
-            f.total = total_dev;
-            f.nblocks = nblocks;
-            f.nb = (u32)nb;
-            f.packed = pk ? 1 : 0;
-            f.intern = intern ? 1 : 0;
+        if (!fold_here) {
             pra_launch_newt_fold(c->st, f);
             check_launch("newt_fold");
         }

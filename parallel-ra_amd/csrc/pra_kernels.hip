@@ -1165,6 +1165,61 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
     }
 }
 
+// Combine the PRA_NREP replicas of the scatter summaries and reset them to
+// their identity values for the next scatter.  Part 0 (packed keys): the
+// top-byte bin counters, whose exclusive offsets go to kgofs and whose
+// largest count goes to stat[6]; the counters return to zero.  Part 1:
+// summary words -> stat[1..3] (packed key OR / AND / dictionary miss) and
+// stat[8..39] (key column OR / AND).  Part 2: the kept total -> stat[0], and
+// the seq counters advance by the new tuples of each bucket (bucket d's rows
+// start at offs[d * nblocks]).  PRA_BLOCK threads; counters and replicas are
+// read by atomic exchange, so the scatter's own block may run it too.
+__device__ inline void pra_newt_fold_part(const NewtFoldArgs& a, u32 y, u32 t, u32* lds, u32* mx)
+{
+    if (y == 0) {
+        if (!a.packed) return;
+        const u32 v = atomicExch(&a.bin_cur[t], 0u);
+        u32 tot;
+        a.kgofs[t] = block_exclusive_scan<u32>(v, lds, &tot);
+        if (t == 0) *mx = 0;
+        __syncthreads();
+        atomicMax(mx, v);
+        __syncthreads();
+        if (t == 0) a.stat[6] = *mx;
+    } else if (y == 1) {
+        if (t < PRA_PART_WORDS && !(t >= 3 && t < 8)) {
+            const bool is_and = pra_summary_is_and(t);
+            const u64 id = is_and ? ~0ULL : 0;
+            u64 v = id;
+            for (int r = 0; r < PRA_NREP; r++) {
+                unsigned long long* p = (unsigned long long*)(a.rep_words + (u64)r * PRA_PART_WORDS + t);
+                const u64 x = atomicExch(p, (unsigned long long)id);
+                v = is_and ? (v & x) : (v | x);
+            }
+            a.stat[t < 3 ? t + 1 : t] = v;
+        }
+    } else {
+        if (t == 0) a.stat[0] = *a.total;
+        if (t < a.nb && a.intern) {
+            const u32 start = a.offs[(u64)t * a.nblocks];
+            const u32 end = t + 1 < a.nb ? a.offs[(u64)(t + 1) * a.nblocks] : *a.total;
+            a.seqbase[t] += (u64)(end - start);
+        }
+    }
+}
+
+// k_newt_fold's three parts in one block after the scatter of a single tile
+__device__ inline void pra_scatter_fold_tail(const NewtFoldArgs& f)
+{
+    __shared__ u32 flds[PRA_WAVES + 1];
+    __shared__ u32 fmx;
+    __syncthreads();
+    for (u32 y = 0; y < 3; y++) {
+        pra_newt_fold_part(f, y, threadIdx.x, flds, &fmx);
+        __syncthreads();
+    }
+}
+
 // Stable partition of the kept candidates by destination bucket; JOIN-type
 // rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
@@ -1176,7 +1231,10 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
     if (a.ctl[1]) return;
     a.n = a.ctl[0];
     // blocks past the candidates hold nothing; the summary reduce skips them
-    if ((u64)blockIdx.x * PRA_TILE >= a.n) return;
+    if ((u64)blockIdx.x * PRA_TILE >= a.n) {
+        if (SB == PRA_BLOCK && a.fold_here) pra_scatter_fold_tail(a.fold);
+        return;
+    }
     __shared__ u32 wh[SW][PRA_RADIX];
     for (int i = threadIdx.x; i < SW * PRA_RADIX; i += SB) (&wh[0][0])[i] = 0;
     __syncthreads();
@@ -1347,50 +1405,16 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
             if (!is_and && v != 0) atomicOr(dst, (unsigned long long)v);
         }
     }
+    if (SB == PRA_BLOCK && a.fold_here) pra_scatter_fold_tail(a.fold);
 }
 
-// Combine the PRA_NREP replicas of the scatter summaries and reset them to
-// their identity values for the next scatter.  Block 0 (packed keys): the
-// top-byte bin counters, whose exclusive offsets go to kgofs and whose
-// largest count goes to stat[6]; the counters return to zero.  Block 1: summary words -> stat[1..3]
-// (packed key OR / AND / dictionary miss) and stat[8..39] (key column OR /
-// AND).  Block 2: the kept total -> stat[0], and the seq counters advance by
-// the new tuples of each bucket (bucket d's rows start at offs[d * nblocks]).
+// One block per part of pra_newt_fold_part.
 __global__ void k_newt_fold(NewtFoldArgs a)
 {
     if (a.ctl[1]) return;
     __shared__ u32 lds[PRA_WAVES + 1];
     __shared__ u32 mx;
-    const u32 y = blockIdx.x, t = threadIdx.x;
-    if (y == 0) {
-        if (!a.packed) return;
-        u32 v = a.bin_cur[t];
-        a.bin_cur[t] = 0;
-        u32 tot;
-        a.kgofs[t] = block_exclusive_scan<u32>(v, lds, &tot);
-        if (t == 0) mx = 0;
-        __syncthreads();
-        atomicMax(&mx, v);
-        __syncthreads();
-        if (t == 0) a.stat[6] = mx;
-    } else if (y == 1) {
-        if (t < PRA_PART_WORDS && !(t >= 3 && t < 8)) {
-            const bool is_and = pra_summary_is_and(t);
-            u64 v = is_and ? ~0ULL : 0;
-            for (int r = 0; r < PRA_NREP; r++) {
-                u64* p = a.rep_words + (u64)r * PRA_PART_WORDS + t;
-                v = is_and ? (v & *p) : (v | *p);
-                *p = is_and ? ~0ULL : 0;
-            }
-            a.stat[t < 3 ? t + 1 : t] = v;
-        }
-    } else {
-        if (t == 0) a.stat[0] = *a.total;
-        if (t >= a.nb || !a.intern) return;
-        u32 start = a.offs[(u64)t * a.nblocks];
-        u32 end = t + 1 < a.nb ? a.offs[(u64)(t + 1) * a.nblocks] : *a.total;
-        a.seqbase[t] += (u64)(end - start);
-    }
+    pra_newt_fold_part(a, blockIdx.x, threadIdx.x, lds, &mx);
 }
 
 // identity values of the summary replicas (all counts 0, OR words 0, AND

@@ -110,6 +110,8 @@ struct NewtScatterArgs {
     u32 nblocks, nb;
     int cw, wn, intern;
     u32 tag;
+    int fold_here;       // one tile (PRA_BLOCK threads): the block also does k_newt_fold's work
+    NewtFoldArgs fold;
 };
 
 void pra_launch_fill_u64(hipStream_t st, u64* p, u64 n, u64 v);
