@@ -1408,7 +1408,8 @@ static void run_iteration(pra_scc* S)
     // 3+4. per output: arrival plan, emit candidates, deduplicate, partition
     //      and intern, summaries for the sort
     // rs == nullptr: the iteration's whole emission; else one threshold round
-    auto run_output = [&](size_t oi, const RoundSpec* rs) {
+    // planned: the arrival plan was made by the iteration's batched plan
+    auto run_output = [&](size_t oi, const RoundSpec* rs, bool planned = false) {
         pra_rel* R = outs[oi];
         const u64 C = rs ? rs->cap : cap[oi];
         const bool pk = packed[oi] && !rs;
@@ -1425,7 +1426,7 @@ static void run_iteration(pra_scc* S)
         c->slot_of.reserve(C * 8, c->st);
         c->keep.reserve(C * 4, c->st);
         NT.buf.reserve((nt_base + C) * R->w * 8, c->st, nt_base * R->w * 8);
-        {
+        if (!planned) {
             PlanArgs pa;
             memset(&pa, 0, sizeof(pa));
             pa.srcoff = c->stage.as<u64>();
@@ -1613,8 +1614,40 @@ static void run_iteration(pra_scc* S)
             check_launch("newt_fold");
         }
     };
+    // every output's arrival plan in one launch (rule and output indices fit
+    // its per-rule tables)
+    const bool plan_all = nra <= PRA_MAXRA && no <= PRA_MAXRA;
+    if (plan_all) {
+        PlanAllArgs pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.srcoff = c->stage.as<u64>();
+        pl.posadj = c->posadj_dev.as<u64>();
+        pl.stage2 = c->stage2.as<u64>();
+        pl.joined = c->joined_dev.as<u64>();
+        pl.thresh = c->threshold;
+        pl.nb = nb;
+        pl.nra = nra;
+        pl.sw = (int)SW;
+        int nblk = 0;
+        for (size_t oi = 0; oi < no; oi++) {
+            pl.cap[oi] = cap[oi];
+            if (cap[oi]) pl.outs[nblk++] = (unsigned char)oi;
+        }
+        for (int r = 0; r < nra; r++) {
+            RADesc& ra = S->ras[r];
+            pl.rout[r] = 0xFF;
+            if (!ra.active) continue;
+            const size_t oi = std::find(outs.begin(), outs.end(), ra.out) - outs.begin();
+            pl.rout[r] = (unsigned char)oi;
+            pl.is_join[r] = ra.type == RA_JOIN;
+            pl.off[r] = ra.off.as<u64>();
+            pl.bnd[r] = ra.outer->bnd[ra.outer_v].as<u64>();
+        }
+        pra_launch_plan_all(c->st, pl, nblk);
+        check_launch("plan_all");
+    }
     for (size_t oi = 0; oi < no; oi++)
-        if (cap[oi]) run_output(oi, nullptr);
+        if (cap[oi]) run_output(oi, nullptr, plan_all);
 
     // 5. new-tuple counts, sort-key summaries and error flags: the
     //    per-iteration host readback (repeated once for outputs whose

@@ -987,6 +987,51 @@ __global__ void k_plan(PlanArgs a)
     if (a.count_joined && (rounds || tot <= a.cap)) *a.joined += jtot;
 }
 
+// k_plan for every output of an iteration at once: block b plans output
+// outs[b] over its active rules in rule order (source-major, rule-minor
+// arrival, as k_plan).
+__global__ void k_plan_all(PlanAllArgs a)
+{
+    const int oi = a.outs[blockIdx.x];
+    const int nb = a.nb;
+    __shared__ unsigned char lr[PRA_MAXRA];
+    __shared__ int nlr;
+    if (threadIdx.x == 0) {
+        int k = 0;
+        for (int r = 0; r < a.nra; r++)
+            if (a.rout[r] == oi) lr[k++] = (unsigned char)r;
+        nlr = k;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nlr * nb; e += blockDim.x) {
+        const int r = lr[e / nb], s = e % nb;
+        const u64 lo = a.off[r][a.bnd[r][s]], hi = a.off[r][a.bnd[r][s + 1]];
+        a.srcoff[(u64)r * 2 * nb + s] = lo;
+        a.srcoff[(u64)r * 2 * nb + nb + s] = hi - lo;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    u64 tot = 0, jtot = 0;
+    bool rounds = false;
+    for (int s = 0; s < nb; s++)
+        for (int k = 0; k < nlr; k++) {
+            const int r = lr[k];
+            const u64* so = a.srcoff + (u64)r * 2 * nb;
+            a.posadj[(u64)r * nb + s] = tot - so[s];
+            const u64 cnt = so[nb + s];
+            if (a.is_join[r]) {
+                if ((long long)cnt > a.thresh) rounds = true;
+                jtot += cnt;
+            }
+            tot += cnt;
+        }
+    u64* ctl = a.stage2 + (u64)oi * a.sw + 4;
+    ctl[0] = tot;
+    ctl[1] = rounds ? 2 : tot > a.cap[oi] ? 1 : 0;
+    if ((rounds || tot <= a.cap[oi]) && jtot) atomicAdd((unsigned long long*)a.joined, (unsigned long long)jtot);
+}
+
 // Split of one threshold round: the first outer row i (of n, offsets off
 // relative to base = off[0]) such that the call's distinct outputs through
 // row i, kp[off[i + 1] - base] (exclusive prefix of first-occurrence flags),
@@ -1829,6 +1874,11 @@ void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap)
 void pra_launch_plan(hipStream_t st, const PlanArgs& a)
 {
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks)
+{
+    if (nblocks > 0) hipLaunchKernelGGL(k_plan_all, dim3(nblocks), dim3(PRA_BLOCK), 0, st, a);
 }
 
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out)

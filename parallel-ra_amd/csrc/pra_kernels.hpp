@@ -74,6 +74,24 @@ struct PlanArgs {
     int count_joined;            // add the join pairs to *joined
 };
 
+// Arrival plans of every output of one iteration (one block per output;
+// PlanArgs's fields per rule and per output, with the threshold-round check
+// and the join-pair count on)
+struct PlanAllArgs {
+    u64* srcoff;
+    u64* posadj;
+    u64* stage2;         // per output sw words; ctl = stage2 + output * sw + 4
+    u64* joined;
+    long long thresh;
+    int nb, nra, sw;
+    const u64* off[PRA_MAXRA];       // per rule: exclusive output offsets over its outer rows
+    const u64* bnd[PRA_MAXRA];       // per rule: first outer row of each source bucket (nb + 1)
+    u64 cap[PRA_MAXRA];              // per output: candidate capacity
+    unsigned char rout[PRA_MAXRA];   // per rule: its output (0xFF: inactive)
+    unsigned char is_join[PRA_MAXRA];
+    unsigned char outs[PRA_MAXRA];   // per block: the output it plans
+};
+
 // Summaries of one newt scatter (k_newt_fold)
 struct NewtFoldArgs {
     u32* bin_cur;      // top-byte bin counters of the packed keys (reset here)
@@ -132,6 +150,7 @@ void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, co
                              u64 cap, u32* flags);
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap);
 void pra_launch_plan(hipStream_t st, const PlanArgs& a);
+void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks);
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out);
 void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
