@@ -634,9 +634,13 @@ static u32 next_sort_epoch(pra_ctx* c)
 // columns: digits equal in every row are skipped.  When the bucket digit is
 // sorted and nb <= 256, the bucket bounds are written to `bnd` (nb+1 words)
 // and true is returned.
+// With `defer`, a sort that takes the one-block path is not launched: its job
+// (into `defer_out`, which the caller swaps with r's buffer once the batch
+// has been launched) is returned in *defer, with defer->n = 0 otherwise.
 static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, const u64* orm, u64* bnd,
-                             bool by_bucket = true)
+                             bool by_bucket = true, SmallSortJob* defer = nullptr, DBuf* defer_out = nullptr)
 {
+    if (defer) defer->n = 0;
     u64 n = r.n;
     if (n <= 1) {
         r.sorted = true;
@@ -662,8 +666,14 @@ static bool sort_rows_masked(pra_ctx* c, Rows& r, int w, int jcc, int keycols, c
     const bool bucketed = c->nb > 1 && by_bucket;
     const int kw = keycols + (bucketed ? 1 : 0);
     if (n <= PRA_SMALL_SORT && n * kw <= (u64)PRA_SMALL_SORT * PRA_SMALL_SORT_KW && passes.size() >= 2) {
-        c->sort_tmp.reserve(n * w * 8, c->st);
         u64* bout = (bnd && bucketed && bucket_last) ? bnd : nullptr;
+        if (defer) {
+            defer_out->reserve(n * w * 8, c->st);
+            *defer = SmallSortJob{r.d(), defer_out->as<u64>(), bout, n, w, jcc, keycols, bucketed ? 1 : 0};
+            r.sorted = true;
+            return bout != nullptr;
+        }
+        c->sort_tmp.reserve(n * w * 8, c->st);
         {
             Prof p(c, K_SMALL_SORT, 8.0 * n * kw + 16.0 * n * w);
             pra_launch_small_sort(c->st, r.d(), n, w, jcc, keycols, (u64)c->nb, bucketed ? 1 : 0,
@@ -1716,6 +1726,27 @@ static void run_iteration(pra_scc* S)
         R->touched(PRA_DELTA);
     }
     std::vector<char> bnd_ok(no, 0);
+    // one-block sorts of all outputs go out together, into the emptied delta
+    // buffers (swapped back with newt's below)
+    SmallSortBatch ssb;
+    memset(&ssb, 0, sizeof(ssb));
+    ssb.nb = (u64)nb;
+    std::vector<size_t> ss_out;
+    auto flush_small_sorts = [&]() {
+        if (ss_out.empty()) return;
+        double bytes = 0;
+        for (size_t k = 0; k < ss_out.size(); k++) {
+            const SmallSortJob& j = ssb.job[k];
+            bytes += 8.0 * j.n * (j.kc + j.bucketed) + 16.0 * j.n * j.w;
+        }
+        {
+            Prof p(c, K_SMALL_SORT, bytes);
+            pra_launch_small_sort_many(c->st, ssb, (int)ss_out.size());
+            check_launch("small_sort_many");
+        }
+        for (size_t oi2 : ss_out) outs[oi2]->ver[PRA_NEWT].buf.swap(outs[oi2]->ver[PRA_DELTA].buf);
+        ss_out.clear();
+    };
     for (size_t oi = 0; oi < no; oi++) {
         pra_rel* R = outs[oi];
         Rows& NT = R->ver[PRA_NEWT];
@@ -1731,13 +1762,21 @@ static void run_iteration(pra_scc* S)
         const u64* masks = (packed[oi] || in_rounds[oi]) ? nullptr : &st2[oi * SW + 8];
         if (packed[oi] && !in_rounds[oi] && c->dict_state == 1)
             c->dict_state = ++c->dict_rebuilds > 3 ? 2 : 0;
-        bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, masks, bnd);
+        SmallSortJob job;
+        bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, masks, bnd, true, &job,
+                                      &R->ver[PRA_DELTA].buf);
+        if (job.n) {
+            ssb.job[ss_out.size()] = job;
+            ss_out.push_back(oi);
+            if (ss_out.size() == PRA_SORT_JOBS) flush_small_sorts();
+        }
         if (!bnd_ok[oi] && nb == 1) {
             pra_launch_fill_u64(c->st, bnd, 1, 0);
             pra_launch_fill_u64(c->st, bnd + 1, 1, NT.n);
             bnd_ok[oi] = 1;
         }
     }
+    flush_small_sorts();
     for (pra_rel* R : S->rels) {
         Rows& D = R->ver[PRA_DELTA];
         D.buf.swap(R->ver[PRA_NEWT].buf);

@@ -222,8 +222,8 @@ __global__ void k_sort_scatter(const u64* in, u64* out, u64 n, int w, DigitSpec 
 // unsigned words) with a bitonic network over row indices in LDS, then
 // gathers the rows into `out`.  Unstable: rows with equal keys are equal.
 // With bnd (bucketed), bnd[s] = first row of bucket s, bnd[nb] = n.
-__global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort(const u64* rows, u64 n, int w, int jcc, int kc,
-                                                                     u64 nb, int bucketed, u64* out, u64* bnd)
+__device__ inline void pra_small_sort_block(const u64* rows, u64 n, int w, int jcc, int kc, u64 nb, int bucketed,
+                                            u64* out, u64* bnd)
 {
     __shared__ u64 key[PRA_SMALL_SORT * PRA_SMALL_SORT_KW];
     __shared__ u32 idx[PRA_SMALL_SORT];
@@ -276,6 +276,20 @@ __global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort(const u64* 
             for (u64 s = prev; s <= cur && s <= nb; s++) bnd[s] = i;
         }
     }
+}
+
+__global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort(const u64* rows, u64 n, int w, int jcc, int kc,
+                                                                     u64 nb, int bucketed, u64* out, u64* bnd)
+{
+    pra_small_sort_block(rows, n, w, jcc, kc, nb, bucketed, out, bnd);
+}
+
+// Independent small sorts (one per output relation of an iteration), one
+// block each.
+__global__ __launch_bounds__(PRA_SMALL_SORT_BLOCK) void k_small_sort_many(SmallSortBatch b)
+{
+    const SmallSortJob& j = b.job[blockIdx.x];
+    pra_small_sort_block(j.rows, j.n, j.w, j.jcc, j.kc, b.nb, j.bucketed, j.out, j.bnd);
 }
 
 // One block sorts n <= PRA_BLOCK_SORT rows with all the LSD passes of `pl`
@@ -1979,6 +1993,11 @@ void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jc
 {
     hipLaunchKernelGGL(k_small_sort, dim3(1), dim3(PRA_SMALL_SORT_BLOCK), 0, st, rows, n, w, jcc, kc, nb, bucketed, out,
                        bnd);
+}
+
+void pra_launch_small_sort_many(hipStream_t st, const SmallSortBatch& b, int njobs)
+{
+    if (njobs > 0) hipLaunchKernelGGL(k_small_sort_many, dim3(njobs), dim3(PRA_SMALL_SORT_BLOCK), 0, st, b);
 }
 
 void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u64* out, u64* bnd,

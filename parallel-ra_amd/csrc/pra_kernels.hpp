@@ -92,6 +92,20 @@ struct PlanAllArgs {
     unsigned char outs[PRA_MAXRA];   // per block: the output it plans
 };
 
+// One small sort of k_small_sort (rows -> out, bucket bounds -> bnd)
+struct SmallSortJob {
+    const u64* rows;
+    u64* out;
+    u64* bnd;
+    u64 n;
+    int w, jcc, kc, bucketed;
+};
+#define PRA_SORT_JOBS 64
+struct SmallSortBatch {
+    SmallSortJob job[PRA_SORT_JOBS];
+    u64 nb;
+};
+
 // Summaries of one newt scatter (k_newt_fold)
 struct NewtFoldArgs {
     u32* bin_cur;      // top-byte bin counters of the packed keys (reset here)
@@ -173,6 +187,7 @@ void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n
 void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int ls, u64 nb, u64* bnd);
 void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jcc, int kc, u64 nb, int bucketed,
                            u64* out, u64* bnd);
+void pra_launch_small_sort_many(hipStream_t st, const SmallSortBatch& b, int njobs);
 void pra_launch_add_bounds(hipStream_t st, u64* a, const u64* b, u64 n);
 void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u64* out, u64* bnd,
                            int jcc, u64 nb);
