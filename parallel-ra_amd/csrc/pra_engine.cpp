@@ -1304,6 +1304,21 @@ static void run_iteration(pra_scc* S)
     // 1. per-RA output counts and their exclusive offsets (the arrival plan
     //    reads them at the source-bucket bounds)
     c->stage.reserve((size_t)std::max(1, nra) * 2 * nb * 8, c->st);
+    CountScanBatch csb;
+    memset(&csb, 0, sizeof(csb));
+    double cs_bytes = 0;
+    auto flush_count_scans = [&]() {
+        if (!csb.njobs) return;
+        prepare_lookback(c, (u64)csb.tile_base[csb.njobs] * PRA_TILE);
+        const u32 ep = next_sort_epoch(c);
+        {
+            Prof p(c, K_JOIN_COUNT, cs_bytes);
+            pra_launch_count_scan_many(c->st, csb, c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags());
+            check_launch("count_scan_many");
+        }
+        memset(&csb, 0, sizeof(csb));
+        cs_bytes = 0;
+    };
     for (int r = 0; r < nra; r++) {
         RADesc& ra = S->ras[r];
         Rows& O = ra.outer->ver[ra.outer_v];
@@ -1326,32 +1341,31 @@ static void run_iteration(pra_scc* S)
             }
             scan_u64(c, ra.cnt.as<u64>(), ra.off.as<u64>(), n, ra.off.as<u64>() + n);
         } else {
-            // the counts and their offsets in one pass
-            CountScanArgs a;
-            memset(&a, 0, sizeof(a));
-            a.outer = O.d();
-            a.n = n;
-            a.wo = ra.outer->w;
-            a.off = ra.off.as<u64>();
-            const bool join = ra.type == RA_JOIN;
-            if (join) {
-                a.table = ra.inner->idx[ra.inner_v].as<u64>();
-                a.cap = ra.inner->idx_cap[ra.inner_v];
-                a.inner = ra.inner->ver[ra.inner_v].d();
-                a.wi = ra.inner->w;
-                a.jcc = ra.in0->jcc;
-                a.istart = ra.istart.as<u64>();
+            // the counts and their offsets in one pass, batched over rules
+            CountScanJob& j = csb.job[csb.njobs];
+            memset(&j, 0, sizeof(j));
+            j.outer = O.d();
+            j.n = n;
+            j.wo = ra.outer->w;
+            j.off = ra.off.as<u64>();
+            j.join = ra.type == RA_JOIN;
+            if (j.join) {
+                j.table = ra.inner->idx[ra.inner_v].as<u64>();
+                j.cap = ra.inner->idx_cap[ra.inner_v];
+                j.inner = ra.inner->ver[ra.inner_v].d();
+                j.wi = ra.inner->w;
+                j.jcc = ra.in0->jcc;
+                j.istart = ra.istart.as<u64>();
             } else {
-                a.op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
+                j.op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
             }
-            prepare_lookback(c, n);
-            const u32 ep = next_sort_epoch(c);
-            Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + (join ? 24.0 : 8.0) * n);
-            pra_launch_count_scan(c->st, a, join, c->lb_status.as<u64>(), c->lb_ctr.as<u32>(), ep, c->dflags());
-            check_launch("count_scan");
+            csb.tile_base[csb.njobs + 1] = csb.tile_base[csb.njobs] + pra_tiles(n);
+            cs_bytes += 8.0 * n * ra.outer->w + (j.join ? 24.0 : 8.0) * n;
+            if (++csb.njobs == PRA_CS_JOBS) flush_count_scans();
         }
         bucket_bounds(c, ra.outer, ra.outer_v);
     }
+    flush_count_scans();
 
     // 2. output relations, their rule kinds, and candidate capacities: the
     //    previous iteration's count with headroom, else an upper bound from the

@@ -1699,29 +1699,20 @@ __global__ void k_scan_lookback(const T* in, T* out, u64 n, u64* status, u32* ti
     }
 }
 
-// One rule's per-row output counts and their exclusive prefix sums in one
-// pass.  JOIN: a row's count is the length of its key's run of inner rows,
-// found in the static inner's key index (istart = the run's first row); COPY
-// and COPY_FILTER: 1 for rows passing the filter `op`.  Tiles of PRA_TILE
-// rows (PRA_TILE / PRA_COUNT_SCAN_BLOCK consecutive rows per thread) are
-// taken in ticket order and find their offset by decoupled look-back over
-// digit 0 of the status words, as k_scan_lookback does; off[n] = total.
+// One tile (`tile` of `ntiles`, status words from `status`) of one rule's
+// per-row output counts and their exclusive prefix sums.  JOIN: a row's count
+// is the length of its key's run of inner rows, found in the static inner's
+// key index (istart = the run's first row); COPY and COPY_FILTER: 1 for rows
+// passing the filter `op`.  A tile holds PRA_TILE rows (PRA_TILE /
+// PRA_COUNT_SCAN_BLOCK consecutive rows per thread) and finds its offset by
+// decoupled look-back over digit 0 of the status words, as k_scan_lookback
+// does; the last tile writes off[n] = total.
 template <bool JOIN>
-__global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan(CountScanArgs a, u64* status, u32* tile_ctr,
-                                                                     u32 epoch, u32* flags)
+__device__ inline void pra_count_scan_tile(const CountScanArgs& a, u64* status, u32 tile, u32 ntiles, u32 epoch,
+                                           u32* flags, u64* lds, u64* s_excl)
 {
     constexpr int IT = PRA_TILE / PRA_COUNT_SCAN_BLOCK;
     constexpr int NW = PRA_COUNT_SCAN_BLOCK / 64;
-    __shared__ u64 lds[NW + 1];
-    __shared__ u32 s_tile;
-    __shared__ u64 s_excl;
-    if (threadIdx.x == 0) {
-        u32 t = atomicAdd(tile_ctr, 1u);
-        if (t == gridDim.x - 1) atomicExch(tile_ctr, 0u);
-        s_tile = t;
-    }
-    __syncthreads();
-    const u32 tile = s_tile;
     const u64 base = (u64)tile * PRA_TILE + (u64)threadIdx.x * IT;
     u64 c[IT];
     if (JOIN) {
@@ -1798,12 +1789,12 @@ __global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan(CountScanAr
             }
         }
         if (threadIdx.x == 0) {
-            s_excl = excl;
-            if (tile == gridDim.x - 1) a.off[a.n] = excl + tot;
+            *s_excl = excl;
+            if (tile == ntiles - 1) a.off[a.n] = excl + tot;
         }
     }
     __syncthreads();
-    u64 run = s_excl + ex;
+    u64 run = *s_excl + ex;
 #pragma unroll
     for (int k = 0; k < IT; k++) {
         if (base + k < a.n) a.off[base + k] = run;
@@ -1811,15 +1802,53 @@ __global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan(CountScanAr
     }
 }
 
-void pra_launch_count_scan(hipStream_t st, const CountScanArgs& a, bool join, u64* status, u32* tile_ctr, u32 epoch,
-                           u32* flags)
+// Counts and scans of several rules in one launch: tiles are taken in ticket
+// order; a block's ticket names its rule (the job whose tile range holds it)
+// and its tile within that rule, whose look-back covers that rule's tiles.
+__global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan_many(CountScanBatch b, u64* status,
+                                                                          u32* tile_ctr, u32 epoch, u32* flags)
 {
-    if (!a.n) return;
-    const dim3 g(pra_tiles(a.n)), b(PRA_COUNT_SCAN_BLOCK);
-    if (join)
-        hipLaunchKernelGGL(k_count_scan<true>, g, b, 0, st, a, status, tile_ctr, epoch, flags);
+    __shared__ u64 lds[PRA_COUNT_SCAN_BLOCK / 64 + 1];
+    __shared__ u32 s_tile;
+    __shared__ int s_job;
+    __shared__ u64 s_excl;
+    if (threadIdx.x == 0) {
+        u32 t = atomicAdd(tile_ctr, 1u);
+        if (t == gridDim.x - 1) atomicExch(tile_ctr, 0u);
+        int k = 0;
+        while (k + 1 < b.njobs && b.tile_base[k + 1] <= t) k++;
+        s_job = k;
+        s_tile = t;
+    }
+    __syncthreads();
+    const int k = s_job;
+    const CountScanJob& j = b.job[k];
+    CountScanArgs a;
+    a.outer = j.outer;
+    a.table = j.table;
+    a.inner = j.inner;
+    a.off = j.off;
+    a.istart = j.istart;
+    a.n = j.n;
+    a.cap = j.cap;
+    a.wo = j.wo;
+    a.wi = j.wi;
+    a.jcc = j.jcc;
+    a.op = j.op;
+    const u32 t0 = b.tile_base[k], nt = b.tile_base[k + 1] - t0;
+    u64* st = status + (u64)t0 * PRA_RADIX;
+    if (j.join)
+        pra_count_scan_tile<true>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
     else
-        hipLaunchKernelGGL(k_count_scan<false>, g, b, 0, st, a, status, tile_ctr, epoch, flags);
+        pra_count_scan_tile<false>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
+}
+
+void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* status, u32* tile_ctr, u32 epoch,
+                                u32* flags)
+{
+    if (b.njobs <= 0 || !b.tile_base[b.njobs]) return;
+    hipLaunchKernelGGL(k_count_scan_many, dim3(b.tile_base[b.njobs]), dim3(PRA_COUNT_SCAN_BLOCK), 0, st, b, status,
+                       tile_ctr, epoch, flags);
 }
 
 template <typename T>

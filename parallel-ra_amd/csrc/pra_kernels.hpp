@@ -54,6 +54,25 @@ struct CountScanArgs {
     int wo, wi, jcc, op;
 };
 
+// Several rules' counts and scans in one launch: tiles of all jobs in ticket
+// order, job k owning tiles [tile_base[k], tile_base[k + 1]) of the launch
+// and the status words of those tiles
+struct CountScanJob {
+    const u64* outer;
+    const u64* table;
+    const u64* inner;
+    u64* off;
+    u64* istart;
+    u64 n, cap;
+    int wo, wi, jcc, op, join;
+};
+#define PRA_CS_JOBS 40
+struct CountScanBatch {
+    CountScanJob job[PRA_CS_JOBS];
+    u32 tile_base[PRA_CS_JOBS + 1];
+    int njobs;
+};
+
 #define PRA_MAXRA 64
 
 struct PlanArgs {
@@ -168,8 +187,8 @@ void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks);
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out);
 void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
-void pra_launch_count_scan(hipStream_t st, const CountScanArgs& a, bool join, u64* status, u32* tile_ctr, u32 epoch,
-                           u32* flags);
+void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* status, u32* tile_ctr, u32 epoch,
+                                u32* flags);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
