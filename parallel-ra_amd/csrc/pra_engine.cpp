@@ -283,7 +283,7 @@ struct RADesc {
     int v0 = 0, v1 = 0, op = PRA_F_TRUE;
     std::vector<int> map;
     // per-iteration state
-    DBuf cnt, off, istart, posadj;
+    DBuf off, istart, posadj;
     u64 n_outer = 0;
     bool active = false;   // outer (and inner) non-empty this iteration
     bool sorted_probe = false;   // inner changes inside the SCC: binary search, no hash index
@@ -1329,17 +1329,6 @@ static void run_iteration(pra_scc* S)
         ra.istart.reserve(std::max<u64>(1, n) * 8, c->st);
         if (n == 0) {
             pra_launch_fill_u64(c->st, ra.off.as<u64>(), 1, 0);
-        } else if (ra.type == RA_JOIN && ra.sorted_probe) {
-            Rows& I = ra.inner->ver[ra.inner_v];
-            ra.cnt.reserve(n * 8, c->st);
-            {
-                Prof p(c, K_JOIN_COUNT, 8.0 * n * ra.outer->w + 16.0 * n);
-                pra_launch_join_count_sorted(c->st, O.d(), n, ra.outer->w, ra.in0->jcc, I.d(), ra.inner->w,
-                                             ra.inner->bnd[ra.inner_v].as<u64>(), (u64)nb, ra.cnt.as<u64>(),
-                                             ra.istart.as<u64>());
-                check_launch("join_count");
-            }
-            scan_u64(c, ra.cnt.as<u64>(), ra.off.as<u64>(), n, ra.off.as<u64>() + n);
         } else {
             // the counts and their offsets in one pass, batched over rules
             CountScanJob& j = csb.job[csb.njobs];
@@ -1348,19 +1337,25 @@ static void run_iteration(pra_scc* S)
             j.n = n;
             j.wo = ra.outer->w;
             j.off = ra.off.as<u64>();
-            j.join = ra.type == RA_JOIN;
-            if (j.join) {
-                j.table = ra.inner->idx[ra.inner_v].as<u64>();
-                j.cap = ra.inner->idx_cap[ra.inner_v];
+            const bool join = ra.type == RA_JOIN;
+            j.kind = !join ? PRA_CS_FILTER : ra.sorted_probe ? PRA_CS_SORTED : PRA_CS_INDEX;
+            if (join) {
                 j.inner = ra.inner->ver[ra.inner_v].d();
                 j.wi = ra.inner->w;
                 j.jcc = ra.in0->jcc;
                 j.istart = ra.istart.as<u64>();
+                if (ra.sorted_probe) {
+                    j.ibnd = ra.inner->bnd[ra.inner_v].as<u64>();
+                    j.nb = (u64)nb;
+                } else {
+                    j.table = ra.inner->idx[ra.inner_v].as<u64>();
+                    j.cap = ra.inner->idx_cap[ra.inner_v];
+                }
             } else {
                 j.op = ra.type == RA_COPY_FILTER ? ra.op : PRA_F_TRUE;
             }
             csb.tile_base[csb.njobs + 1] = csb.tile_base[csb.njobs] + pra_tiles(n);
-            cs_bytes += 8.0 * n * ra.outer->w + (j.join ? 24.0 : 8.0) * n;
+            cs_bytes += 8.0 * n * ra.outer->w + (join ? 24.0 : 8.0) * n;
             if (++csb.njobs == PRA_CS_JOBS) flush_count_scans();
         }
         bucket_bounds(c, ra.outer, ra.outer_v);

@@ -760,16 +760,13 @@ __device__ inline int pra_row_cmp(const u64* a, const u64* b, int n)
     return 0;
 }
 
-// Join count against an inner version kept sorted by (bucket, columns): the
-// rows matching an outer key form one run inside the key's bucket, found by
-// two binary searches (used for inner relations that change every iteration,
-// where a hash index would be rebuilt each time).
-__global__ void k_join_count_sorted(const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
-                                    const u64* ibnd, u64 nb, u64* cnt, u64* istart)
+// Run of inner rows matching an outer key, in an inner version kept sorted by
+// (bucket, columns): one run inside the key's bucket, found by a binary
+// search and a gallop (used for inner relations that change every iteration,
+// where a hash index would be rebuilt each time).  Returns its length.
+__device__ inline u64 pra_sorted_run(const u64* key, int jcc, const u64* inner, int wi, const u64* ibnd, u64 nb,
+                                     u64* first_out)
 {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const u64* key = outer + i * wo;
     u64 b = nb == 1 ? 0 : pra_fnv(key, jcc) % nb;
     u64 lo = ibnd[b], hi = ibnd[b + 1];
     u64 a = lo, z = hi;   // first row >= key
@@ -791,8 +788,8 @@ __global__ void k_join_count_sorted(const u64* outer, u64 n, int wo, int jcc, co
         u64 m = (a + z) >> 1;
         if (pra_row_cmp(inner + m * wi, key, jcc) <= 0) a = m + 1; else z = m;
     }
-    cnt[i] = a - first;
-    istart[i] = first;
+    *first_out = first;
+    return a - first;
 }
 
 // Merge of a sorted delta into a sorted full (both ordered by (bucket, all w
@@ -1700,14 +1697,15 @@ __global__ void k_scan_lookback(const T* in, T* out, u64 n, u64* status, u32* ti
 }
 
 // One tile (`tile` of `ntiles`, status words from `status`) of one rule's
-// per-row output counts and their exclusive prefix sums.  JOIN: a row's count
-// is the length of its key's run of inner rows, found in the static inner's
-// key index (istart = the run's first row); COPY and COPY_FILTER: 1 for rows
-// passing the filter `op`.  A tile holds PRA_TILE rows (PRA_TILE /
+// per-row output counts and their exclusive prefix sums.  PRA_CS_INDEX
+// (join): a row's count is the length of its key's run of inner rows, found
+// in the static inner's key index (istart = the run's first row);
+// PRA_CS_SORTED (join): the same run searched in the inner's sorted rows;
+// PRA_CS_FILTER (copy, copy_filter): 1 for rows passing the filter `op`.  A tile holds PRA_TILE rows (PRA_TILE /
 // PRA_COUNT_SCAN_BLOCK consecutive rows per thread) and finds its offset by
 // decoupled look-back over digit 0 of the status words, as k_scan_lookback
 // does; the last tile writes off[n] = total.
-template <bool JOIN>
+template <int KIND>
 __device__ inline void pra_count_scan_tile(const CountScanArgs& a, u64* status, u32 tile, u32 ntiles, u32 epoch,
                                            u32* flags, u64* lds, u64* s_excl)
 {
@@ -1715,7 +1713,18 @@ __device__ inline void pra_count_scan_tile(const CountScanArgs& a, u64* status, 
     constexpr int NW = PRA_COUNT_SCAN_BLOCK / 64;
     const u64 base = (u64)tile * PRA_TILE + (u64)threadIdx.x * IT;
     u64 c[IT];
-    if (JOIN) {
+    if (KIND == PRA_CS_SORTED) {
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const u64 i = base + k;
+            c[k] = 0;
+            if (i < a.n) {
+                u64 first;
+                c[k] = pra_sorted_run(a.outer + i * a.wo, a.jcc, a.inner, a.wi, a.ibnd, a.nb, &first);
+                a.istart[i] = first;
+            }
+        }
+    } else if (KIND == PRA_CS_INDEX) {
         // every item's key hash and first index slot are loaded before any
         // item walks on (a probe mostly ends at its first slot)
         const u64 m = a.cap - 1;
@@ -1835,12 +1844,16 @@ __global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan_many(CountS
     a.wi = j.wi;
     a.jcc = j.jcc;
     a.op = j.op;
+    a.ibnd = j.ibnd;
+    a.nb = j.nb;
     const u32 t0 = b.tile_base[k], nt = b.tile_base[k + 1] - t0;
     u64* st = status + (u64)t0 * PRA_RADIX;
-    if (j.join)
-        pra_count_scan_tile<true>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
+    if (j.kind == PRA_CS_INDEX)
+        pra_count_scan_tile<PRA_CS_INDEX>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
+    else if (j.kind == PRA_CS_SORTED)
+        pra_count_scan_tile<PRA_CS_SORTED>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
     else
-        pra_count_scan_tile<false>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
+        pra_count_scan_tile<PRA_CS_FILTER>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
 }
 
 void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* status, u32* tile_ctr, u32 epoch,
@@ -2125,14 +2138,6 @@ void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words)
 {
     hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * PRA_PART_WORDS + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK),
                        0, st, rep_words);
-}
-
-void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
-                                  const u64* ibnd, u64 nb, u64* cnt, u64* istart)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_join_count_sorted, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, outer, n, wo, jcc, inner, wi,
-                       ibnd, nb, cnt, istart);
 }
 
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,

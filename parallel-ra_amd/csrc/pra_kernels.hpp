@@ -50,23 +50,28 @@ struct CountScanArgs {
     const u64* inner;    // JOIN: inner rows (wi words each)
     u64* off;            // n + 1: exclusive prefix of the counts, off[n] = total
     u64* istart;         // JOIN: first inner row of each outer row's key run
-    u64 n, cap;
+    const u64* ibnd;     // sorted inner: its bucket bounds
+    u64 n, cap, nb;
     int wo, wi, jcc, op;
 };
 
 // Several rules' counts and scans in one launch: tiles of all jobs in ticket
 // order, job k owning tiles [tile_base[k], tile_base[k + 1]) of the launch
 // and the status words of those tiles
+#define PRA_CS_FILTER 0   // copy / copy_filter: 1 per row passing the filter
+#define PRA_CS_INDEX 1    // join over a static inner's key index
+#define PRA_CS_SORTED 2   // join over an inner kept sorted by (bucket, columns)
 struct CountScanJob {
     const u64* outer;
     const u64* table;
     const u64* inner;
+    const u64* ibnd;     // PRA_CS_SORTED: the inner's bucket bounds (nb + 1)
     u64* off;
     u64* istart;
-    u64 n, cap;
-    int wo, wi, jcc, op, join;
+    u64 n, cap, nb;
+    int wo, wi, jcc, op, kind;
 };
-#define PRA_CS_JOBS 40
+#define PRA_CS_JOBS 36
 struct CountScanBatch {
     CountScanJob job[PRA_CS_JOBS];
     u32 tile_base[PRA_CS_JOBS + 1];
@@ -221,7 +226,5 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
 void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
-void pra_launch_join_count_sorted(hipStream_t st, const u64* outer, u64 n, int wo, int jcc, const u64* inner, int wi,
-                                  const u64* ibnd, u64 nb, u64* cnt, u64* istart);
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
                       u64 nb, int w, u64* rank, u64* out);
