@@ -1856,6 +1856,11 @@ __global__ __launch_bounds__(PRA_COUNT_SCAN_BLOCK) void k_count_scan_many(CountS
         pra_count_scan_tile<PRA_CS_FILTER>(a, st, s_tile - t0, nt, epoch, flags, lds, &s_excl);
 }
 
+// kernel arguments of one launch are limited to 4 KiB
+static_assert(sizeof(CountScanBatch) + 4 * sizeof(void*) <= 4096, "count-scan batch exceeds the kernel arguments");
+static_assert(sizeof(SmallSortBatch) <= 4096, "small-sort batch exceeds the kernel arguments");
+static_assert(sizeof(PlanAllArgs) <= 4096, "plan batch exceeds the kernel arguments");
+
 void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* status, u32* tile_ctr, u32 epoch,
                                 u32* flags)
 {

@@ -340,3 +340,47 @@ def test_wide_rows_sort_paths(nb, n):
     else:
         assert ks["radix_scatter"]["launches"] > 0
     g.close()
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+def test_scc_with_more_rules_than_one_launch_holds(nb):
+    # 95 rules over 64 relations in one fixed-point SCC (the most an SCC may
+    # hold): more rules than one batched count-and-scan launch (36) or one
+    # arrival-plan launch (64) takes, so the first is split and the second
+    # falls back to one plan per output; 31 acopy targets sort in one batch
+    print(f"[many rules] nb={nb}", flush=True)
+    rng = np.random.default_rng(70 + nb)
+    pairs = sorted({(int(a), int(b)) for a, b in rng.integers(0, 40, size=(300, 2))})
+    rows = np.array([[a, b, 9000 + k] for k, (a, b) in enumerate(pairs)], dtype=np.uint64)
+
+    def build(b):
+        A = b.rel_create(2, 1, 400)
+        b.load_rows(A, rows, FULL)
+        s = b.scc_create(True, 41)
+        b.scc_add_relation(s, A, True)
+        outs = []
+        for i in range(31):
+            B = b.rel_create(2, 1, 401 + i)
+            b.scc_add_relation(s, B, False)
+            b.add_acopy(s, B, A, DELTA, [1, 0, 2] if i % 2 else [0, 1, 2])
+            outs.append(B)
+        for i in range(32):
+            C = b.rel_create(2, 1, 501 + i)
+            b.scc_add_relation(s, C, False)
+            b.add_copy(s, C, A, DELTA, [1, 0])
+            b.add_copy(s, C, A, DELTA, [0, 1] if i % 3 else [1, 0])
+            outs.append(C)
+        return outs, s
+
+    g = gpu_ctx(nb)
+    g.set_profiling(True)
+    og, sg = build(g)
+    g.execute()
+    o = O.Oracle(nb)
+    oo, so = build(o)
+    o.execute(1)
+    for i, (ro, rg) in enumerate(zip(oo, og)):
+        assert_same_rel(o, ro, g, rg, f"output {i}")
+    assert o.history(so) == g.history(sg)
+    assert g.kernel_stats()["small_sort"]["launches"] >= 1
+    g.close()
