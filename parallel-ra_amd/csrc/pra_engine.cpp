@@ -566,10 +566,14 @@ static void sync_check(pra_ctx* c) { readback(c, nullptr, 0); }
 // synchronization.  Completion of the event (the end of the kernel, with its
 // system-scope release) is what makes the words valid to read; after a while
 // the host falls back to a blocking wait.
-static u64* publish_begin(pra_ctx* c, const u64* src, size_t words)
+// fold: a newt fold still to run, done by the publishing kernel itself
+static u64* publish_begin(pra_ctx* c, const u64* src, size_t words, const NewtFoldArgs* fold = nullptr)
 {
     u64* h = c->mailbox_buf(words + 1);
-    pra_launch_publish(c->st, src, words, c->flags_buf.as<u32>(), h);
+    if (fold)
+        pra_launch_newt_fold_publish(c->st, *fold, src, words, c->flags_buf.as<u32>(), h);
+    else
+        pra_launch_publish(c->st, src, words, c->flags_buf.as<u32>(), h);
     check_launch("publish");
     if (!c->pub_ev) HIPCHK(hipEventCreateWithFlags(&c->pub_ev, hipEventDisableTiming));
     HIPCHK(hipEventRecord(c->pub_ev, c->st));
@@ -1427,6 +1431,12 @@ static void run_iteration(pra_scc* S)
     // 3+4. per output: arrival plan, emit candidates, deduplicate, partition
     //      and intern, summaries for the sort
     // rs == nullptr: the iteration's whole emission; else one threshold round
+    // the last output of the main pass leaves its fold to the publication
+    size_t last_oi = no;
+    for (size_t oi = 0; oi < no; oi++)
+        if (cap[oi]) last_oi = oi;
+    NewtFoldArgs last_fold;
+    bool have_last_fold = false;
     // planned: the arrival plan was made by the iteration's batched plan
     auto run_output = [&](size_t oi, const RoundSpec* rs, bool planned = false) {
         pra_rel* R = outs[oi];
@@ -1628,7 +1638,11 @@ static void run_iteration(pra_scc* S)
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");
         }
-        if (!fold_here) {
+        if (!fold_here && planned && oi == last_oi) {
+            // the iteration's last output: folded by the summary's publication
+            last_fold = f;
+            have_last_fold = true;
+        } else if (!fold_here) {
             pra_launch_newt_fold(c->st, f);
             check_launch("newt_fold");
         }
@@ -1676,7 +1690,7 @@ static void run_iteration(pra_scc* S)
     // their full logs while the host waits for the first readback
     std::vector<char> early(S->rels.size(), 0);
     for (int pass = 0; pass < 2; pass++) {
-        u64* h = publish_begin(c, c->stage2.as<u64>(), no * SW);
+        u64* h = publish_begin(c, c->stage2.as<u64>(), no * SW, pass == 0 && have_last_fold ? &last_fold : nullptr);
         if (pass == 0)
             for (size_t i = 0; i < S->rels.size(); i++) {
                 pra_rel* R = S->rels[i];

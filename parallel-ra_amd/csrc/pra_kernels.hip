@@ -1473,6 +1473,23 @@ __global__ void k_newt_fold(NewtFoldArgs a)
     pra_newt_fold_part(a, blockIdx.x, threadIdx.x, lds, &mx);
 }
 
+// The last output's fold and then the iteration's summary publication
+// (k_publish) in one block: the fold's parts run in turn (skipped, as in
+// k_newt_fold, when the output overflowed its candidate capacity), then
+// `words` summary words and the error flags go to pinned host memory.
+__global__ void k_newt_fold_publish(NewtFoldArgs a, const u64* src, u64 words, const u32* flags, u64* host)
+{
+    __shared__ u32 lds[PRA_WAVES + 1];
+    __shared__ u32 mx;
+    if (!a.ctl[1])
+        for (u32 y = 0; y < 3; y++) {
+            pra_newt_fold_part(a, y, threadIdx.x, lds, &mx);
+            __syncthreads();
+        }
+    for (u64 i = threadIdx.x; i < words; i += blockDim.x) host[i] = src[i];
+    if (threadIdx.x == 0) host[words] = *flags;
+}
+
 // identity values of the summary replicas (all counts 0, OR words 0, AND
 // words all ones)
 __global__ void k_newt_rep_init(u64* rep_words)
@@ -1945,6 +1962,12 @@ void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks)
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out)
 {
     hipLaunchKernelGGL(k_find_split, dim3(1), dim3(1), 0, st, off, n, kp, thresh, out);
+}
+
+void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,
+                                  const u32* flags, u64* host)
+{
+    hipLaunchKernelGGL(k_newt_fold_publish, dim3(1), dim3(PRA_BLOCK), 0, st, a, src, words, flags, host);
 }
 
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host)

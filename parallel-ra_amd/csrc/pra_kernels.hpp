@@ -224,6 +224,8 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags, u32* offs1 = nullptr);
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
+void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,
+                                  const u32* flags, u64* host);
 void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
