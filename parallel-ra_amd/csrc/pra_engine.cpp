@@ -1634,6 +1634,9 @@ static void run_iteration(pra_scc* S)
             }
             a.fold_here = fold_here ? 1 : 0;
             a.fold = f;
+            // column masks only pick the sort passes of outputs too large for
+            // the one-block sort, which compares whole key words
+            a.summarize = C > PRA_SMALL_SORT || C * (u64)(R->w + 1) > (u64)PRA_SMALL_SORT * PRA_SMALL_SORT_KW;
             ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
             pra_launch_dedup_scatter(c->st, a);
             check_launch("dedup_scatter");

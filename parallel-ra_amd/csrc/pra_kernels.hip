@@ -1333,10 +1333,11 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
     if (packed)
         for (int i = threadIdx.x; i < PRA_RADIX; i += SB) khist[i] = 0;
     __syncthreads();
-    // key columns summarised (unpacked): the first 16; later ones report the
-    // identity values (OR 0, AND all ones), which the host reads as "varies"
+    // key columns summarised (unpacked, when asked): the first 16; later ones
+    // report the identity values (OR 0, AND all ones), which the host reads as
+    // "varies"
     constexpr int NC = PACKED ? 1 : 16;
-    const int kc = a.cw < NC ? a.cw : NC;
+    const int kc = !a.summarize ? 0 : a.cw < NC ? a.cw : NC;
     u64 kor = 0, kand = ~0ULL, kmiss = 0;
     u64 o[NC], an[NC];
 #pragma unroll

@@ -167,6 +167,7 @@ struct NewtScatterArgs {
     int cw, wn, intern;
     u32 tag;
     int fold_here;       // one tile (PRA_BLOCK threads): the block also does k_newt_fold's work
+    int summarize;       // unpacked: collect the key columns' OR / AND (else reported as "varies")
     NewtFoldArgs fold;
 };
 
