@@ -1438,7 +1438,20 @@ static void run_iteration(pra_scc* S)
     NewtFoldArgs last_fold;
     bool have_last_fold = false;
     // planned: the arrival plan was made by the iteration's batched plan
-    auto run_output = [&](size_t oi, const RoundSpec* rs, bool planned = false) {
+    // per output: its rows (dest, slot_of, keep), candidate words and
+    // histogram words in the shared buffers when its claim and decision are
+    // batched with other outputs
+    struct OutRegion {
+        u64 rb = 0, cb = 0, hb = 0;
+    };
+    std::vector<OutRegion> reg(no);
+    std::vector<ClaimJob> claim_jobs(no);
+    std::vector<DecideJob> decide_jobs(no);
+    // phase 0: one output start to end; 1: up to its candidates, 2: from its
+    // kept counts on (the claims and decisions in between batched over the
+    // outputs, each in its own region of the candidate buffers)
+    auto run_output = [&](size_t oi, const RoundSpec* rs, bool planned = false, int phase = 0) {
+        const OutRegion rg = phase ? reg[oi] : OutRegion{};
         pra_rel* R = outs[oi];
         const u64 C = rs ? rs->cap : cap[oi];
         const bool pk = packed[oi] && !rs;
@@ -1455,6 +1468,16 @@ static void run_iteration(pra_scc* S)
         c->slot_of.reserve(C * 8, c->st);
         c->keep.reserve(C * 4, c->st);
         NT.buf.reserve((nt_base + C) * R->w * 8, c->st, nt_base * R->w * 8);
+        const u32 nblocks = pra_tiles(C);
+        c->hist.reserve((size_t)nb * nblocks * 4, c->st);
+        c->offs.reserve((size_t)nb * nblocks * 4 + 8, c->st);
+        u64* cand0 = c->cand.as<u64>() + rg.cb;
+        u32* dest0 = c->dest.as<u32>() + rg.rb;
+        u64* slot0 = c->slot_of.as<u64>() + rg.rb;
+        u32* keep0 = c->keep.as<u32>() + rg.rb;
+        u32* hist0 = c->hist.as<u32>() + rg.hb;
+        u32* offs0 = c->offs.as<u32>() + rg.hb;
+        if (phase != 2) {
         if (!planned) {
             PlanArgs pa;
             memset(&pa, 0, sizeof(pa));
@@ -1516,8 +1539,8 @@ static void run_iteration(pra_scc* S)
                 a.off = ra.off.as<u64>() + lo;
                 a.istart = ra.istart.as<u64>() + lo;
                 a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
-                a.cand = c->cand.as<u64>();
-                a.dest = c->dest.as<u32>();
+                a.cand = cand0;
+                a.dest = dest0;
                 a.n_outer = hi - lo;
                 a.nb = (u64)nb;
                 a.wo = ra.outer->w;
@@ -1538,8 +1561,8 @@ static void run_iteration(pra_scc* S)
                 a.rows = O.d();
                 a.off = ra.off.as<u64>();
                 a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
-                a.cand = c->cand.as<u64>();
-                a.dest = c->dest.as<u32>();
+                a.cand = cand0;
+                a.dest = dest0;
                 a.n = ra.n_outer;
                 a.nb = (u64)nb;
                 a.w = ra.outer->w;
@@ -1554,25 +1577,30 @@ static void run_iteration(pra_scc* S)
         }
         u64* tab = R->mt.as<u64>();
         const u64 epoch = ++c->dedup_epoch;
+        if (phase == 1) {
+            // the batched claim and decision take it from here
+            ClaimJob& cj = claim_jobs[oi];
+            cj = ClaimJob{cand0, ctl, tab, slot0, R->mt_cap, epoch, cw, kw};
+            DecideJob& dj = decide_jobs[oi];
+            dj = DecideJob{cand0, ctl, tab, slot0, keep0, dest0, hist0, nblocks == 1 ? offs0 : nullptr, epoch,
+                           nblocks, cw, kw};
+            return;
+        }
         {
             ProfPerCand p(c, K_DEDUP_CLAIM, 8.0 * (kw + 1), oi);
-            pra_launch_dedup_claim2(c->st, c->cand.as<u64>(), ctl, C, cw, kw, tab, R->mt_cap, c->slot_of.as<u64>(),
-                                    epoch, c->dflags());
+            pra_launch_dedup_claim2(c->st, cand0, ctl, C, cw, kw, tab, R->mt_cap, slot0, epoch, c->dflags());
             check_launch("dedup_claim");
         }
-        u32 nblocks = pra_tiles(C);
-        c->hist.reserve((size_t)nb * nblocks * 4, c->st);
-        c->offs.reserve((size_t)nb * nblocks * 4 + 8, c->st);
         {
             ProfPerCand p(c, K_DEDUP_VERIFY, 8.0 * kw + 16.0, oi);
             // one tile: the decision writes the offsets itself
-            pra_launch_dedup_decide_hist(c->st, c->cand.as<u64>(), ctl, cw, kw, tab, c->slot_of.as<u64>(), epoch,
-                                         c->keep.as<u32>(), c->dest.as<u32>(), (u32)nb, c->hist.as<u32>(), nblocks,
-                                         c->dflags(), nblocks == 1 ? c->offs.as<u32>() : nullptr);
+            pra_launch_dedup_decide_hist(c->st, cand0, ctl, cw, kw, tab, slot0, epoch, keep0, dest0, (u32)nb, hist0,
+                                         nblocks, c->dflags(), nblocks == 1 ? offs0 : nullptr);
             check_launch("dedup_decide_hist");
         }
-        u32* total_dev = c->offs.as<u32>() + (size_t)nb * nblocks;
-        if (nblocks > 1) scan_u32(c, c->hist.as<u32>(), c->offs.as<u32>(), (u64)nb * nblocks, total_dev);
+        }
+        u32* total_dev = offs0 + (size_t)nb * nblocks;
+        if (nblocks > 1) scan_u32(c, hist0, offs0, (u64)nb * nblocks, total_dev);
         DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
         R->kgofs.reserve(PRA_RADIX * 4, c->st);
         NewtFoldArgs f;   // bin_cur: set with the scatter's bins
@@ -1582,7 +1610,7 @@ static void run_iteration(pra_scc* S)
         f.stat = st2o;
         f.kgofs = R->kgofs.as<u32>();
         f.seqbase = sq ? sq->as<u64>() : nullptr;
-        f.offs = c->offs.as<u32>();
+        f.offs = offs0;
         f.total = total_dev;
         f.nblocks = nblocks;
         f.nb = (u32)nb;
@@ -1594,10 +1622,10 @@ static void run_iteration(pra_scc* S)
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
             a.ctl = ctl;
-            a.cand = c->cand.as<u64>();
-            a.keep = c->keep.as<u32>();
-            a.dest = c->dest.as<u32>();
-            a.offs = c->offs.as<u32>();
+            a.cand = cand0;
+            a.keep = keep0;
+            a.dest = dest0;
+            a.offs = offs0;
             a.seqbase = sq ? sq->as<u64>() : nullptr;
             a.newt = NT.d() + nt_base * R->w;
             a.nblocks = nblocks;
@@ -1682,8 +1710,79 @@ static void run_iteration(pra_scc* S)
         pra_launch_plan_all(c->st, pl, nblk);
         check_launch("plan_all");
     }
-    for (size_t oi = 0; oi < no; oi++)
-        if (cap[oi]) run_output(oi, nullptr, plan_all);
+    size_t nactive = 0;
+    for (size_t oi = 0; oi < no; oi++) nactive += cap[oi] ? 1 : 0;
+    if (plan_all && nactive >= 2) {
+        // every output's candidates in its own region; their claims and
+        // decisions batched by key width
+        u64 rt = 0, ct = 0, ht = 0;
+        for (size_t oi = 0; oi < no; oi++) {
+            if (!cap[oi]) continue;
+            const int cwo = out_intern[oi] == 1 ? outs[oi]->arity : outs[oi]->arity + 1;
+            reg[oi] = OutRegion{rt, ct, ht};
+            rt += cap[oi];
+            ct += cap[oi] * (u64)cwo;
+            ht += (u64)nb * pra_tiles(cap[oi]) + 2;
+        }
+        c->cand.reserve(ct * 8, c->st);
+        c->dest.reserve(rt * 4, c->st);
+        c->slot_of.reserve(rt * 8, c->st);
+        c->keep.reserve(rt * 4, c->st);
+        c->hist.reserve(ht * 4, c->st);
+        c->offs.reserve(ht * 4 + 8, c->st);
+        for (size_t oi = 0; oi < no; oi++)
+            if (cap[oi]) run_output(oi, nullptr, true, 1);
+        for (int cls = 1; cls <= 7; cls++) {
+            ClaimBatch cb;
+            DecideBatch db;
+            memset(&cb, 0, sizeof(cb));
+            memset(&db, 0, sizeof(db));
+            db.nb = (u32)nb;
+            std::vector<size_t> members;
+            auto flush = [&]() {
+                if (!cb.njobs) return;
+                // timed launches: bytes per candidate of every member output
+                size_t before = c->pending.size();
+                {
+                    Prof p(c, K_DEDUP_CLAIM, 0.0);
+                    pra_launch_dedup_claim_many(c->st, cb, cls, c->dflags());
+                    check_launch("dedup_claim_many");
+                }
+                if (c->pending.size() == before + 1)
+                    for (size_t oi : members) c->per_candidate.emplace_back(before, 8.0 * (claim_jobs[oi].kw + 1), oi);
+                before = c->pending.size();
+                {
+                    Prof p(c, K_DEDUP_VERIFY, 0.0);
+                    pra_launch_dedup_decide_many(c->st, db, cls, c->dflags());
+                    check_launch("dedup_decide_many");
+                }
+                if (c->pending.size() == before + 1)
+                    for (size_t oi : members) c->per_candidate.emplace_back(before, 8.0 * decide_jobs[oi].kw + 16.0, oi);
+                memset(&cb, 0, sizeof(cb));
+                memset(&db, 0, sizeof(db));
+                db.nb = (u32)nb;
+                members.clear();
+            };
+            for (size_t oi = 0; oi < no; oi++) {
+                if (!cap[oi] || pra_dedup_class(claim_jobs[oi].kw) != cls) continue;
+                const int k = cb.njobs;
+                cb.job[k] = claim_jobs[oi];
+                cb.block_base[k + 1] = cb.block_base[k] + (u32)((cap[oi] + PRA_BLOCK - 1) / PRA_BLOCK);
+                cb.njobs++;
+                db.job[k] = decide_jobs[oi];
+                db.block_base[k + 1] = db.block_base[k] + decide_jobs[oi].nblocks;
+                db.njobs++;
+                members.push_back(oi);
+                if (cb.njobs == PRA_DEDUP_JOBS) flush();
+            }
+            flush();
+        }
+        for (size_t oi = 0; oi < no; oi++)
+            if (cap[oi]) run_output(oi, nullptr, true, 2);
+    } else {
+        for (size_t oi = 0; oi < no; oi++)
+            if (cap[oi]) run_output(oi, nullptr, plan_all);
+    }
 
     // 5. new-tuple counts, sort-key summaries and error flags: the
     //    per-iteration host readback (repeated once for outputs whose
@@ -1727,7 +1826,7 @@ static void run_iteration(pra_scc* S)
         }
     for (auto& pc : c->per_candidate)
         if (std::get<0>(pc) < c->pending.size())
-            c->pending[std::get<0>(pc)].bytes = std::get<1>(pc) * (double)st2[std::get<2>(pc) * SW + 4];
+            c->pending[std::get<0>(pc)].bytes += std::get<1>(pc) * (double)st2[std::get<2>(pc) * SW + 4];
     c->per_candidate.clear();
     for (size_t oi = 0; oi < no; oi++) {
         pra_rel* R = outs[oi];

@@ -1141,6 +1141,82 @@ __global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw_,
     atomicOr(flags, PRA_FLAG_TABLE_FULL);
 }
 
+// Candidate p of k_dedup_claim2 (p < the candidate count), for the batched
+// claim.
+template <int MAXKW, bool EXACT>
+__device__ __forceinline__ void pra_claim_one(const u64* cand, int cw, int kw_, u64* table, u64 cap, u64* slot_of,
+                                              u64 epoch, u32* flags, u64 p)
+{
+    const int kw = EXACT ? MAXKW : kw_;
+    const u64* key = cand + p * cw;
+    u64 kr[MAXKW > 0 ? MAXKW : 1];
+    u64 h;
+    if (MAXKW > 0) {
+#pragma unroll
+        for (int c = 0; c < MAXKW; c++) kr[c] = c < kw ? key[c] : 0;
+        // pra_keyhash over the register copy
+        h = 0x9E3779B97F4A7C15ULL ^ (u64)kw;
+#pragma unroll
+        for (int c = 0; c < MAXKW; c++)
+            if (c < kw) h = pra_mix64(h ^ (kr[c] + 0x632BE59BD9B4E019ULL * (u64)(c + 1)));
+        h = h ? h : 1;
+    } else {
+        h = pra_keyhash(key, kw);
+    }
+    const int stride = 2 + kw;
+    const u64 mark = (epoch << 40) | (p + 1);
+    u64 m = cap - 1, s = h & m;
+    for (u64 q = 0; q < cap; q++) {
+        u64* slot = table + s * stride;
+        u64 cur = slot[0];
+        bool mine = false;
+        if (cur == 0) {
+            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
+            if (cur == 0) {
+                mine = true;
+                if (MAXKW > 0) {
+#pragma unroll
+                    for (int c = 0; c < MAXKW; c++)
+                        if (c < kw) slot[2 + c] = kr[c];
+                } else {
+                    for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
+                }
+            }
+        }
+        if (mine || cur == h) {
+            // the claimant lowers the mark at once; a later candidate first
+            // reads it: a mark from an earlier iteration (key already present)
+            // or an earlier arrival of this one already beats ours
+            if (mine) {
+                atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
+            } else {
+                u64 old = slot[1];
+                if (old > mark && old >= (epoch << 40))
+                    atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
+            }
+            slot_of[p] = s;
+            return;
+        }
+        s = (s + 1) & m;
+    }
+    slot_of[p] = ~0ULL;
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+// Claims of several outputs in one launch: a block serves the job whose
+// block range holds it, as k_dedup_claim2 serves its one output.
+template <int MAXKW, bool EXACT>
+__global__ void k_dedup_claim_many(ClaimBatch b, u32* flags)
+{
+    int k = 0;
+    while (k + 1 < b.njobs && b.block_base[k + 1] <= blockIdx.x) k++;
+    const ClaimJob& j = b.job[k];
+    if (j.ctl[1]) return;
+    const u64 p = (u64)(blockIdx.x - b.block_base[k]) * blockDim.x + threadIdx.x;
+    if (p >= j.ctl[0]) return;
+    pra_claim_one<MAXKW, EXACT>(j.cand, j.cw, j.kw, j.table, j.cap, j.slot_of, j.epoch, flags, p);
+}
+
 // Keep the winners, verify every key against its slot (a 64-bit hash
 // collision between distinct keys is reported, never merged) and count kept
 // candidates per destination bucket per tile.
@@ -1219,6 +1295,87 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u6
         }
         __syncthreads();
     }
+}
+
+// Tile `tile` of k_dedup_decide_hist (h, sl: the block's LDS counters and scan
+// words), for the batched decision.
+template <int MAXKW, bool EXACT>
+__device__ __forceinline__ void pra_decide_tile(const u64* cand, u64 n, int cw, int kw_, const u64* table,
+                                                const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb,
+                                                u32* hist, u32 nblocks, u32* flags, u32* offs1, u32 tile, u32* h,
+                                                u32* sl)
+{
+    const int kw = EXACT ? MAXKW : kw_;
+    u64 base = (u64)tile * PRA_TILE;
+    if (base >= n) {
+        // a tile past the candidates keeps nothing
+        for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = 0;
+        if (offs1)
+            for (u32 i = threadIdx.x; i <= nb; i += PRA_DECIDE_BLOCK) offs1[i] = 0;
+        return;
+    }
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
+        u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
+        if (p < n) {
+            u64 s = slot_of[p];
+            u32 kp = 0;
+            if (s != ~0ULL) {
+                const u64* slot = table + s * (2 + kw);
+                const u64* key = cand + p * cw;
+                bool eq = true;
+                if (MAXKW > 0) {
+                    u64 x[MAXKW > 0 ? MAXKW : 1], y[MAXKW > 0 ? MAXKW : 1];
+#pragma unroll
+                    for (int c = 0; c < MAXKW; c++) {
+                        x[c] = c < kw ? slot[2 + c] : 0;
+                        y[c] = c < kw ? key[c] : 0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < MAXKW; c++) eq = eq && x[c] == y[c];
+                } else {
+                    for (int c = 0; c < kw; c++)
+                        if (slot[2 + c] != key[c]) { eq = false; break; }
+                }
+                if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
+                if (slot[1] == ((epoch << 40) | (p + 1))) {
+                    kp = 1;
+                    atomicAdd(&h[dest[p]], 1u);
+                }
+            }
+            keep[p] = kp;
+        }
+    }
+    __syncthreads();
+    if (offs1) {
+        // a single tile: its counts are the whole histogram, whose
+        // exclusive offsets (and total, offs1[nb]) go out directly
+        u32 tot;
+        const u32 v = threadIdx.x < nb ? h[threadIdx.x] : 0u;
+        const u32 ex = block_exclusive_scan<u32, PRA_DECIDE_BLOCK / 64>(v, sl, &tot);
+        if (threadIdx.x < nb) offs1[threadIdx.x] = ex;
+        if (threadIdx.x == 0) offs1[nb] = tot;
+    } else {
+        for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = h[i];
+    }
+    __syncthreads();
+}
+
+// Decisions of several outputs in one launch: block -> (job, tile) by the
+// jobs' block ranges; no stride, every tile has its block.
+template <int MAXKW, bool EXACT>
+__global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_many(DecideBatch b, u32* flags)
+{
+    __shared__ u32 h[PRA_RADIX];
+    __shared__ u32 sl[PRA_DECIDE_BLOCK / 64 + 1];
+    int k = 0;
+    while (k + 1 < b.njobs && b.block_base[k + 1] <= blockIdx.x) k++;
+    const DecideJob& j = b.job[k];
+    if (j.ctl[1]) return;
+    pra_decide_tile<MAXKW, EXACT>(j.cand, j.ctl[0], j.cw, j.kw, j.table, j.slot_of, j.epoch, j.keep, j.dest, b.nb,
+                                  j.hist, j.nblocks, flags, j.offs1, blockIdx.x - b.block_base[k], h, sl);
 }
 
 // Combine the PRA_NREP replicas of the scatter summaries and reset them to
@@ -2130,6 +2287,44 @@ void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u6
     }
 #undef PRA_CLAIM
 }
+
+int pra_dedup_class(int kw)
+{
+    return kw >= 1 && kw <= 4 ? kw : kw <= 8 ? 5 : kw <= 16 ? 6 : 7;
+}
+
+#define PRA_DEDUP_DISPATCH(CLS, LAUNCH)                                                                                \
+    switch (CLS) {                                                                                                     \
+    case 1: LAUNCH(1, true); break;                                                                                    \
+    case 2: LAUNCH(2, true); break;                                                                                    \
+    case 3: LAUNCH(3, true); break;                                                                                    \
+    case 4: LAUNCH(4, true); break;                                                                                    \
+    case 5: LAUNCH(8, false); break;                                                                                   \
+    case 6: LAUNCH(16, false); break;                                                                                  \
+    default: LAUNCH(0, false);                                                                                         \
+    }
+
+void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags)
+{
+    if (b.njobs <= 0 || !b.block_base[b.njobs]) return;
+    const dim3 g(b.block_base[b.njobs]), bl(PRA_BLOCK);
+#define PRA_CLAIM_MANY(K, E) hipLaunchKernelGGL((k_dedup_claim_many<K, E>), g, bl, 0, st, b, flags)
+    PRA_DEDUP_DISPATCH(cls, PRA_CLAIM_MANY)
+#undef PRA_CLAIM_MANY
+}
+
+void pra_launch_dedup_decide_many(hipStream_t st, const DecideBatch& b, int cls, u32* flags)
+{
+    if (b.njobs <= 0 || !b.block_base[b.njobs]) return;
+    const dim3 g(b.block_base[b.njobs]), bl(PRA_DECIDE_BLOCK);
+#define PRA_DECIDE_MANY(K, E) hipLaunchKernelGGL((k_dedup_decide_many<K, E>), g, bl, 0, st, b, flags)
+    PRA_DEDUP_DISPATCH(cls, PRA_DECIDE_MANY)
+#undef PRA_DECIDE_MANY
+}
+#undef PRA_DEDUP_DISPATCH
+
+static_assert(sizeof(ClaimBatch) + sizeof(void*) <= 4096, "claim batch exceeds the kernel arguments");
+static_assert(sizeof(DecideBatch) + sizeof(void*) <= 4096, "decision batch exceeds the kernel arguments");
 
 void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ctl, int cw, int kw, const u64* table,
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,

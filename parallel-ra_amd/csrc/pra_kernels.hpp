@@ -116,6 +116,43 @@ struct PlanAllArgs {
     unsigned char outs[PRA_MAXRA];   // per block: the output it plans
 };
 
+// Membership claims and decisions of several output relations in one launch
+// each (one job per output; blocks [block_base[k], block_base[k + 1]) serve
+// job k)
+struct ClaimJob {
+    const u64* cand;
+    const u64* ctl;
+    u64* table;
+    u64* slot_of;
+    u64 cap, epoch;
+    int cw, kw;
+};
+struct DecideJob {
+    const u64* cand;
+    const u64* ctl;
+    const u64* table;
+    const u64* slot_of;
+    u32* keep;
+    const u32* dest;
+    u32* hist;
+    u32* offs1;          // one tile: bucket offsets and total written directly
+    u64 epoch;
+    u32 nblocks;
+    int cw, kw;
+};
+#define PRA_DEDUP_JOBS 32
+struct ClaimBatch {
+    ClaimJob job[PRA_DEDUP_JOBS];
+    u32 block_base[PRA_DEDUP_JOBS + 1];
+    int njobs;
+};
+struct DecideBatch {
+    DecideJob job[PRA_DEDUP_JOBS];
+    u32 block_base[PRA_DEDUP_JOBS + 1];
+    int njobs;
+    u32 nb;
+};
+
 // One small sort of k_small_sort (rows -> out, bucket bounds -> bnd)
 struct SmallSortJob {
     const u64* rows;
@@ -225,6 +262,10 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
                                   const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb, u32* hist,
                                   u32 nblocks, u32* flags, u32* offs1 = nullptr);
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
+// key-width class of a claim / decision launch (jobs of one batch share it)
+int pra_dedup_class(int kw);
+void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags);
+void pra_launch_dedup_decide_many(hipStream_t st, const DecideBatch& b, int cls, u32* flags);
 void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,
                                   const u32* flags, u64* host);
 void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words);
