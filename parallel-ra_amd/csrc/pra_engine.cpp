@@ -1478,126 +1478,126 @@ static void run_iteration(pra_scc* S)
         u32* hist0 = c->hist.as<u32>() + rg.hb;
         u32* offs0 = c->offs.as<u32>() + rg.hb;
         if (phase != 2) {
-        if (!planned) {
-            PlanArgs pa;
-            memset(&pa, 0, sizeof(pa));
-            pa.srcoff = c->stage.as<u64>();
-            pa.posadj = c->posadj_dev.as<u64>();
-            pa.ctl = ctl;
-            pa.joined = c->joined_dev.as<u64>();
-            pa.flags = c->dflags();
-            pa.cap = C;
-            pa.thresh = c->threshold;
-            pa.nb = nb;
-            pa.rounds_check = rs ? 0 : 1;
-            pa.count_joined = rs ? 0 : 1;
-            if (rs) {
-                for (size_t k = 0; k < rs->ras.size(); k++) {
-                    RADesc& ra = S->ras[rs->ras[k]];
-                    pa.ras[pa.nr] = rs->ras[k];
-                    pa.is_join[pa.nr] = ra.type == RA_JOIN;
-                    pa.off[pa.nr] = ra.off.as<u64>();
-                    pa.lo[pa.nr] = rs->bounds->as<u64>() + k * 2 * nb;
-                    pa.hi[pa.nr] = rs->bounds->as<u64>() + k * 2 * nb + nb;
-                    pa.nr++;
+            if (!planned) {
+                PlanArgs pa;
+                memset(&pa, 0, sizeof(pa));
+                pa.srcoff = c->stage.as<u64>();
+                pa.posadj = c->posadj_dev.as<u64>();
+                pa.ctl = ctl;
+                pa.joined = c->joined_dev.as<u64>();
+                pa.flags = c->dflags();
+                pa.cap = C;
+                pa.thresh = c->threshold;
+                pa.nb = nb;
+                pa.rounds_check = rs ? 0 : 1;
+                pa.count_joined = rs ? 0 : 1;
+                if (rs) {
+                    for (size_t k = 0; k < rs->ras.size(); k++) {
+                        RADesc& ra = S->ras[rs->ras[k]];
+                        pa.ras[pa.nr] = rs->ras[k];
+                        pa.is_join[pa.nr] = ra.type == RA_JOIN;
+                        pa.off[pa.nr] = ra.off.as<u64>();
+                        pa.lo[pa.nr] = rs->bounds->as<u64>() + k * 2 * nb;
+                        pa.hi[pa.nr] = rs->bounds->as<u64>() + k * 2 * nb + nb;
+                        pa.nr++;
+                    }
+                } else {
+                    for (int r = 0; r < nra; r++) {
+                        RADesc& ra = S->ras[r];
+                        if (ra.out != R || !ra.active) continue;
+                        pa.ras[pa.nr] = r;
+                        pa.is_join[pa.nr] = ra.type == RA_JOIN;
+                        pa.off[pa.nr] = ra.off.as<u64>();
+                        pa.lo[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>();
+                        pa.hi[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>() + 1;
+                        pa.nr++;
+                    }
                 }
-            } else {
-                for (int r = 0; r < nra; r++) {
-                    RADesc& ra = S->ras[r];
-                    if (ra.out != R || !ra.active) continue;
-                    pa.ras[pa.nr] = r;
-                    pa.is_join[pa.nr] = ra.type == RA_JOIN;
-                    pa.off[pa.nr] = ra.off.as<u64>();
-                    pa.lo[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>();
-                    pa.hi[pa.nr] = ra.outer->bnd[ra.outer_v].as<u64>() + 1;
-                    pa.nr++;
+                pra_launch_plan(c->st, pa);
+                check_launch("plan");
+            }
+            for (int r = 0; r < nra; r++) {
+                RADesc& ra = S->ras[r];
+                if (ra.out != R || !ra.active) continue;
+                size_t k = 0;   // position in the round's rule list
+                if (rs) {
+                    while (k < rs->ras.size() && rs->ras[k] != r) k++;
+                    if (k == rs->ras.size()) continue;
+                }
+                Rows& O = ra.outer->ver[ra.outer_v];
+                if (ra.type == RA_JOIN) {
+                    // a round emits one window of rows per source; otherwise all rows at once
+                    const int nwin = rs ? nb : 1;
+                    for (int s = 0; s < nwin; s++) {
+                    u64 lo = rs ? rs->lo[k * nb + s] : 0, hi = rs ? rs->hi[k * nb + s] : ra.n_outer;
+                    if (hi <= lo) continue;
+                    JoinExpandArgs a;
+                    memset(&a, 0, sizeof(a));
+                    a.ctl = ctl;
+                    a.outer = O.d() + lo * ra.outer->w;
+                    a.inner = ra.inner->ver[ra.inner_v].d();
+                    a.off = ra.off.as<u64>() + lo;
+                    a.istart = ra.istart.as<u64>() + lo;
+                    a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
+                    a.cand = cand0;
+                    a.dest = dest0;
+                    a.n_outer = hi - lo;
+                    a.nb = (u64)nb;
+                    a.wo = ra.outer->w;
+                    a.wi = ra.inner->w;
+                    a.jcc = ra.in0->jcc;
+                    a.left = ra.left;
+                    a.cw = cw;
+                    a.out_jcc = R->jcc;
+                    for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+                    ProfPerCand p(c, K_JOIN_EXPAND, 8.0 * ra.inner->w + 8.0 * cw, oi);
+                    pra_launch_join_expand(c->st, a, std::min<u64>(C, ub[oi]));
+                    check_launch("join_expand");
+                    }
+                } else {
+                    CopyExpandArgs a;
+                    memset(&a, 0, sizeof(a));
+                    a.ctl = ctl;
+                    a.rows = O.d();
+                    a.off = ra.off.as<u64>();
+                    a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
+                    a.cand = cand0;
+                    a.dest = dest0;
+                    a.n = ra.n_outer;
+                    a.nb = (u64)nb;
+                    a.w = ra.outer->w;
+                    a.in_jcc = ra.outer->jcc;
+                    a.cw = cw;
+                    a.out_jcc = R->jcc;
+                    for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+                    Prof p(c, K_COPY_EXPAND, (double)ra.n_outer * 8.0 * (ra.outer->w + cw));
+                    pra_launch_copy_expand(c->st, a);
+                    check_launch("copy_expand");
                 }
             }
-            pra_launch_plan(c->st, pa);
-            check_launch("plan");
-        }
-        for (int r = 0; r < nra; r++) {
-            RADesc& ra = S->ras[r];
-            if (ra.out != R || !ra.active) continue;
-            size_t k = 0;   // position in the round's rule list
-            if (rs) {
-                while (k < rs->ras.size() && rs->ras[k] != r) k++;
-                if (k == rs->ras.size()) continue;
+            u64* tab = R->mt.as<u64>();
+            const u64 epoch = ++c->dedup_epoch;
+            if (phase == 1) {
+                // the batched claim and decision take it from here
+                ClaimJob& cj = claim_jobs[oi];
+                cj = ClaimJob{cand0, ctl, tab, slot0, R->mt_cap, epoch, cw, kw};
+                DecideJob& dj = decide_jobs[oi];
+                dj = DecideJob{cand0, ctl, tab, slot0, keep0, dest0, hist0, nblocks == 1 ? offs0 : nullptr, epoch,
+                               nblocks, cw, kw};
+                return;
             }
-            Rows& O = ra.outer->ver[ra.outer_v];
-            if (ra.type == RA_JOIN) {
-                // a round emits one window of rows per source; otherwise all rows at once
-                const int nwin = rs ? nb : 1;
-                for (int s = 0; s < nwin; s++) {
-                u64 lo = rs ? rs->lo[k * nb + s] : 0, hi = rs ? rs->hi[k * nb + s] : ra.n_outer;
-                if (hi <= lo) continue;
-                JoinExpandArgs a;
-                memset(&a, 0, sizeof(a));
-                a.ctl = ctl;
-                a.outer = O.d() + lo * ra.outer->w;
-                a.inner = ra.inner->ver[ra.inner_v].d();
-                a.off = ra.off.as<u64>() + lo;
-                a.istart = ra.istart.as<u64>() + lo;
-                a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
-                a.cand = cand0;
-                a.dest = dest0;
-                a.n_outer = hi - lo;
-                a.nb = (u64)nb;
-                a.wo = ra.outer->w;
-                a.wi = ra.inner->w;
-                a.jcc = ra.in0->jcc;
-                a.left = ra.left;
-                a.cw = cw;
-                a.out_jcc = R->jcc;
-                for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
-                ProfPerCand p(c, K_JOIN_EXPAND, 8.0 * ra.inner->w + 8.0 * cw, oi);
-                pra_launch_join_expand(c->st, a, std::min<u64>(C, ub[oi]));
-                check_launch("join_expand");
-                }
-            } else {
-                CopyExpandArgs a;
-                memset(&a, 0, sizeof(a));
-                a.ctl = ctl;
-                a.rows = O.d();
-                a.off = ra.off.as<u64>();
-                a.posadj = c->posadj_dev.as<u64>() + (size_t)r * nb;
-                a.cand = cand0;
-                a.dest = dest0;
-                a.n = ra.n_outer;
-                a.nb = (u64)nb;
-                a.w = ra.outer->w;
-                a.in_jcc = ra.outer->jcc;
-                a.cw = cw;
-                a.out_jcc = R->jcc;
-                for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
-                Prof p(c, K_COPY_EXPAND, (double)ra.n_outer * 8.0 * (ra.outer->w + cw));
-                pra_launch_copy_expand(c->st, a);
-                check_launch("copy_expand");
+            {
+                ProfPerCand p(c, K_DEDUP_CLAIM, 8.0 * (kw + 1), oi);
+                pra_launch_dedup_claim2(c->st, cand0, ctl, C, cw, kw, tab, R->mt_cap, slot0, epoch, c->dflags());
+                check_launch("dedup_claim");
             }
-        }
-        u64* tab = R->mt.as<u64>();
-        const u64 epoch = ++c->dedup_epoch;
-        if (phase == 1) {
-            // the batched claim and decision take it from here
-            ClaimJob& cj = claim_jobs[oi];
-            cj = ClaimJob{cand0, ctl, tab, slot0, R->mt_cap, epoch, cw, kw};
-            DecideJob& dj = decide_jobs[oi];
-            dj = DecideJob{cand0, ctl, tab, slot0, keep0, dest0, hist0, nblocks == 1 ? offs0 : nullptr, epoch,
-                           nblocks, cw, kw};
-            return;
-        }
-        {
-            ProfPerCand p(c, K_DEDUP_CLAIM, 8.0 * (kw + 1), oi);
-            pra_launch_dedup_claim2(c->st, cand0, ctl, C, cw, kw, tab, R->mt_cap, slot0, epoch, c->dflags());
-            check_launch("dedup_claim");
-        }
-        {
-            ProfPerCand p(c, K_DEDUP_VERIFY, 8.0 * kw + 16.0, oi);
-            // one tile: the decision writes the offsets itself
-            pra_launch_dedup_decide_hist(c->st, cand0, ctl, cw, kw, tab, slot0, epoch, keep0, dest0, (u32)nb, hist0,
-                                         nblocks, c->dflags(), nblocks == 1 ? offs0 : nullptr);
-            check_launch("dedup_decide_hist");
-        }
+            {
+                ProfPerCand p(c, K_DEDUP_VERIFY, 8.0 * kw + 16.0, oi);
+                // one tile: the decision writes the offsets itself
+                pra_launch_dedup_decide_hist(c->st, cand0, ctl, cw, kw, tab, slot0, epoch, keep0, dest0, (u32)nb, hist0,
+                                             nblocks, c->dflags(), nblocks == 1 ? offs0 : nullptr);
+                check_launch("dedup_decide_hist");
+            }
         }
         u32* total_dev = offs0 + (size_t)nb * nblocks;
         if (nblocks > 1) scan_u32(c, hist0, offs0, (u64)nb * nblocks, total_dev);
@@ -1757,7 +1757,8 @@ static void run_iteration(pra_scc* S)
                     check_launch("dedup_decide_many");
                 }
                 if (c->pending.size() == before + 1)
-                    for (size_t oi : members) c->per_candidate.emplace_back(before, 8.0 * decide_jobs[oi].kw + 16.0, oi);
+                    for (size_t oi : members)
+                        c->per_candidate.emplace_back(before, 8.0 * decide_jobs[oi].kw + 16.0, oi);
                 memset(&cb, 0, sizeof(cb));
                 memset(&db, 0, sizeof(db));
                 db.nb = (u32)nb;
