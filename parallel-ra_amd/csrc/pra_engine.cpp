@@ -337,7 +337,10 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_words, rctl, ttab, kp;
+    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_words, rctl, ttab, kp, jobtab;
+    int rep_sets = 0;            // summary replica sets in rep_words (one per output of a batched partition)
+    char* job_stage = nullptr;   // pinned staging of batched kernels' job tables
+    size_t job_stage_cap = 0;
     u64 dict_cap = 0, dict_n = 0;
     int dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -373,7 +376,7 @@ struct pra_ctx {
         seq.clear();
         for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_words, &rctl, &ttab, &kp})
+                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_words, &rctl, &ttab, &kp, &jobtab})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -381,6 +384,7 @@ struct pra_ctx {
         if (host_stage) (void)hipHostFree(host_stage);
         if (host_pos) (void)hipHostFree(host_pos);
         if (mailbox) (void)hipHostFree(mailbox);
+        if (job_stage) (void)hipHostFree(job_stage);
         if (pub_ev) (void)hipEventDestroy(pub_ev);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -443,6 +447,23 @@ struct pra_ctx {
             host_stage_cap = c;
         }
         return host_stage;
+    }
+    // pinned staging for job tables copied to the device; the copies of one
+    // iteration have completed before the next iteration writes it again (its
+    // readback waits for the stream), and a larger buffer waits for them too
+    char* job_stage_buf(size_t bytes)
+    {
+        if (bytes > job_stage_cap) {
+            if (job_stage) {
+                HIPCHK(hipStreamSynchronize(st));
+                HIPCHK(hipHostFree(job_stage));
+            }
+            job_stage = nullptr;
+            const size_t c = std::max<size_t>(bytes, 64 << 10);
+            HIPCHK(hipHostMalloc((void**)&job_stage, c, hipHostMallocDefault));
+            job_stage_cap = c;
+        }
+        return job_stage;
     }
     u64* mailbox_buf(size_t words)
     {
@@ -1437,6 +1458,10 @@ static void run_iteration(pra_scc* S)
         if (cap[oi]) last_oi = oi;
     NewtFoldArgs last_fold;
     bool have_last_fold = false;
+    // batched partitions (unpacked, packed) and the folds that wait for them
+    std::vector<NewtScatterArgs> scatter_jobs[2];
+    std::vector<size_t> scatter_outs[2];
+    std::vector<NewtFoldArgs> later_folds;
     // planned: the arrival plan was made by the iteration's batched plan
     // per output: its rows (dest, slot_of, keep), candidate words and
     // histogram words in the shared buffers when its claim and decision are
@@ -1605,7 +1630,9 @@ static void run_iteration(pra_scc* S)
         R->kgofs.reserve(PRA_RADIX * 4, c->st);
         NewtFoldArgs f;   // bin_cur: set with the scatter's bins
         memset(&f, 0, sizeof(f));
-        f.rep_words = c->rep_words.as<u64>();
+        // batched partitions: one summary replica set per output
+        u64* rep0 = c->rep_words.as<u64>() + (phase ? oi : 0) * (size_t)PRA_NREP * PRA_PART_WORDS;
+        f.rep_words = rep0;
         f.ctl = ctl;
         f.stat = st2o;
         f.kgofs = R->kgofs.as<u32>();
@@ -1618,6 +1645,7 @@ static void run_iteration(pra_scc* S)
         f.intern = intern ? 1 : 0;
         // one tile: the scatter's block folds the summaries itself
         const bool fold_here = nblocks == 1;
+        bool batched = false;
         {
             NewtScatterArgs a;
             memset(&a, 0, sizeof(a));
@@ -1634,7 +1662,7 @@ static void run_iteration(pra_scc* S)
             a.wn = R->w;
             a.intern = intern ? 1 : 0;
             a.tag = R->tag;
-            a.rep_words = c->rep_words.as<u64>();
+            a.rep_words = rep0;
             if (pk) {
                 // bins of twice the average top-byte share (a fuller bin sends
                 // the sort to its byte-pass fallback)
@@ -1665,14 +1693,23 @@ static void run_iteration(pra_scc* S)
             // column masks only pick the sort passes of outputs too large for
             // the one-block sort, which compares whole key words
             a.summarize = C > PRA_SMALL_SORT || C * (u64)(R->w + 1) > (u64)PRA_SMALL_SORT * PRA_SMALL_SORT_KW;
-            ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
-            pra_launch_dedup_scatter(c->st, a);
-            check_launch("dedup_scatter");
+            if (phase == 2 && nblocks < PRA_SCATTER_WIDE_TILES) {
+                // narrow partitions of a batched iteration go out together
+                scatter_jobs[pk ? 1 : 0].push_back(a);
+                scatter_outs[pk ? 1 : 0].push_back(oi);
+                batched = true;
+            } else {
+                ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
+                pra_launch_dedup_scatter(c->st, a);
+                check_launch("dedup_scatter");
+            }
         }
         if (!fold_here && planned && oi == last_oi) {
             // the iteration's last output: folded by the summary's publication
             last_fold = f;
             have_last_fold = true;
+        } else if (!fold_here && batched) {
+            later_folds.push_back(f);
         } else if (!fold_here) {
             pra_launch_newt_fold(c->st, f);
             check_launch("newt_fold");
@@ -1778,8 +1815,52 @@ static void run_iteration(pra_scc* S)
             }
             flush();
         }
+        if (c->rep_sets < (int)no) {
+            // every set at its identity values (sets at rest always are)
+            c->rep_words.reserve(no * (size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
+            pra_launch_newt_rep_init(c->st, c->rep_words.as<u64>(), (int)no);
+            check_launch("newt_rep_init");
+            c->rep_sets = (int)no;
+        }
         for (size_t oi = 0; oi < no; oi++)
             if (cap[oi]) run_output(oi, nullptr, true, 2);
+        // the batched partitions: job tables through pinned staging into
+        // device memory, one launch per key kind
+        size_t stage_bytes = 0;
+        for (int pk = 0; pk < 2; pk++)
+            if (!scatter_jobs[pk].empty())
+                stage_bytes += scatter_jobs[pk].size() * sizeof(NewtScatterArgs) +
+                               ((scatter_jobs[pk].size() + 1) * 4 + 255) / 256 * 256;
+        if (stage_bytes) {
+            char* hs = c->job_stage_buf(stage_bytes);
+            c->jobtab.reserve(stage_bytes, c->st);
+            size_t at = 0;
+            for (int pk = 0; pk < 2; pk++) {
+                const auto& J = scatter_jobs[pk];
+                if (J.empty()) continue;
+                const size_t bytes = J.size() * sizeof(NewtScatterArgs) + ((J.size() + 1) * 4 + 255) / 256 * 256;
+                memcpy(hs + at, J.data(), J.size() * sizeof(NewtScatterArgs));
+                u32* bb = (u32*)(hs + at + J.size() * sizeof(NewtScatterArgs));
+                bb[0] = 0;
+                for (size_t k = 0; k < J.size(); k++) bb[k + 1] = bb[k] + J[k].nblocks;
+                HIPCHK(hipMemcpyAsync(c->jobtab.as<char>() + at, hs + at, bytes, hipMemcpyHostToDevice, c->st));
+                const size_t before = c->pending.size();
+                {
+                    Prof p(c, K_DEDUP_SCATTER, 0.0);
+                    pra_launch_dedup_scatter_many(c->st, (const NewtScatterArgs*)(c->jobtab.as<char>() + at),
+                                                  (int)J.size(), bb[J.size()], pk == 1);
+                    check_launch("dedup_scatter_many");
+                }
+                if (c->pending.size() == before + 1)
+                    for (size_t k = 0; k < J.size(); k++)
+                        c->per_candidate.emplace_back(before, 8.0 * J[k].cw + 8.0, scatter_outs[pk][k]);
+                at += bytes;
+            }
+        }
+        for (const NewtFoldArgs& f : later_folds) {
+            pra_launch_newt_fold(c->st, f);
+            check_launch("newt_fold");
+        }
     } else {
         for (size_t oi = 0; oi < no; oi++)
             if (cap[oi]) run_output(oi, nullptr, plan_all);
@@ -2088,6 +2169,7 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
         pra_launch_newt_rep_init(c->st, c->rep_words.as<u64>());
+        c->rep_sets = 1;
         check_launch("init");
         HIPCHK(hipStreamSynchronize(c->st));
     } catch (const PraError& e) {

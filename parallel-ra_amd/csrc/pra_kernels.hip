@@ -1437,14 +1437,14 @@ __device__ inline void pra_scatter_fold_tail(const NewtFoldArgs& f)
 // rows get their intern id tag<<46 | bucket<<28 | seq, seq counted per
 // (tag, bucket) in arrival order (src/RAM/RA_tasks.cpp:858-875).
 template <bool PACKED, int SB>
-__global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
+__device__ __forceinline__ void pra_scatter_block(NewtScatterArgs& a, const u32 bid)
 {
     // PRA_TILE candidates per block: SB threads of IT items, SW waves
     constexpr int SW = SB / 64, IT = PRA_TILE / SB;
     if (a.ctl[1]) return;
     a.n = a.ctl[0];
     // blocks past the candidates hold nothing; the summary reduce skips them
-    if ((u64)blockIdx.x * PRA_TILE >= a.n) {
+    if ((u64)bid * PRA_TILE >= a.n) {
         if (SB == PRA_BLOCK && a.fold_here) pra_scatter_fold_tail(a.fold);
         return;
     }
@@ -1453,7 +1453,7 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u64 lt = pra_lanemask_lt();
-    u64 base = (u64)blockIdx.x * PRA_TILE + (u64)wave * (PRA_TILE / SW);
+    u64 base = (u64)bid * PRA_TILE + (u64)wave * (PRA_TILE / SW);
     u32 dg[IT], lc[IT], kept = 0;
 #pragma unroll
     for (int k = 0; k < IT; k++) {
@@ -1473,7 +1473,7 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
     }
     __syncthreads();
     for (u32 d = threadIdx.x; d < a.nb; d += SB) {
-        u32 acc = a.offs[(u64)d * a.nblocks + blockIdx.x];
+        u32 acc = a.offs[(u64)d * a.nblocks + bid];
         for (int v = 0; v < SW; v++) {
             u32 t = wh[v][d];
             wh[v][d] = acc;
@@ -1605,9 +1605,9 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     __syncthreads();
-    // fold into replica blockIdx.x % PRA_NREP of the summaries (identity
+    // fold into replica bid % PRA_NREP of the summaries (identity
     // values are not sent); k_newt_fold combines and resets the replicas
-    const u32 rep = blockIdx.x % PRA_NREP;
+    const u32 rep = bid % PRA_NREP;
     if (threadIdx.x < PRA_PART_WORDS) {
         const int j = threadIdx.x;
         if (j < 3 || j >= 8) {
@@ -1620,6 +1620,26 @@ __global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
         }
     }
     if (SB == PRA_BLOCK && a.fold_here) pra_scatter_fold_tail(a.fold);
+}
+
+template <bool PACKED, int SB>
+__global__ __launch_bounds__(SB) void k_dedup_scatter(NewtScatterArgs a)
+{
+    pra_scatter_block<PACKED, SB>(a, blockIdx.x);
+}
+
+// Partitions of several outputs' candidates in one launch (PRA_BLOCK-thread
+// blocks): the jobs' arguments and block ranges are read from device memory
+// (njobs arguments, then njobs + 1 block bases), a block serving the job whose
+// range holds it as k_dedup_scatter serves its one output.
+template <bool PACKED>
+__global__ __launch_bounds__(PRA_BLOCK) void k_dedup_scatter_many(const NewtScatterArgs* jobs, int njobs)
+{
+    const u32* block_base = (const u32*)(jobs + njobs);
+    int k = 0;
+    while (k + 1 < njobs && block_base[k + 1] <= blockIdx.x) k++;
+    NewtScatterArgs a = jobs[k];
+    pra_scatter_block<PACKED, PRA_BLOCK>(a, blockIdx.x - block_base[k]);
 }
 
 // One block per part of pra_newt_fold_part.
@@ -2157,6 +2177,15 @@ void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
         hipLaunchKernelGGL((k_dedup_scatter<false, PRA_BLOCK>), dim3(a.nblocks), dim3(PRA_BLOCK), 0, st, a);
 }
 
+void pra_launch_dedup_scatter_many(hipStream_t st, const NewtScatterArgs* jobs, int njobs, u32 nblocks, bool packed)
+{
+    if (njobs <= 0 || !nblocks) return;
+    if (packed)
+        hipLaunchKernelGGL(k_dedup_scatter_many<true>, dim3(nblocks), dim3(PRA_BLOCK), 0, st, jobs, njobs);
+    else
+        hipLaunchKernelGGL(k_dedup_scatter_many<false>, dim3(nblocks), dim3(PRA_BLOCK), 0, st, jobs, njobs);
+}
+
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
                                   u32* flags)
 {
@@ -2358,10 +2387,11 @@ void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a)
     hipLaunchKernelGGL(k_newt_fold, dim3(3), dim3(PRA_BLOCK), 0, st, a);
 }
 
-void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words)
+void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words, int sets)
 {
-    hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * PRA_PART_WORDS + PRA_BLOCK - 1) / PRA_BLOCK), dim3(PRA_BLOCK),
-                       0, st, rep_words);
+    for (int k = 0; k < sets; k++)
+        hipLaunchKernelGGL(k_newt_rep_init, dim3((PRA_NREP * PRA_PART_WORDS + PRA_BLOCK - 1) / PRA_BLOCK),
+                           dim3(PRA_BLOCK), 0, st, rep_words + (size_t)k * PRA_NREP * PRA_PART_WORDS);
 }
 
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,

@@ -234,6 +234,8 @@ void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* st
                                 u32* flags);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
 void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
+// jobs: device memory holding njobs arguments, then njobs + 1 u32 block bases
+void pra_launch_dedup_scatter_many(hipStream_t st, const NewtScatterArgs* jobs, int njobs, u32 nblocks, bool packed);
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,
                                   u32* flags);
 void pra_launch_table_verify_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* table, u64 cap,
@@ -268,7 +270,8 @@ void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u
 void pra_launch_dedup_decide_many(hipStream_t st, const DecideBatch& b, int cls, u32* flags);
 void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,
                                   const u32* flags, u64* host);
-void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words);
+// `sets` consecutive replica sets (one per output of a batched partition)
+void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words, int sets = 1);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
                       u64 nb, int w, u64* rank, u64* out);
