@@ -1092,13 +1092,12 @@ static void merge_into_full(pra_ctx* c, pra_rel* R)
     c->sort_tmp.reserve(n * R->w * 8, c->st);
     {
         Prof p(c, K_MERGE, 16.0 * R->w * n);
+        // the merged bucket s holds full's and delta's rows of bucket s: the
+        // merge adds delta's bounds to full's
         pra_launch_merge(c->st, F.d(), F.n, fb, D.d(), D.n, db, (u64)c->nb, R->w, c->tmp64a.as<u64>(),
-                         c->sort_tmp.as<u64>());
+                         c->sort_tmp.as<u64>(), R->bnd[PRA_FULL].as<u64>());
         check_launch("merge");
     }
-    // the merged bucket s holds full's and delta's rows of bucket s
-    pra_launch_add_bounds(c->st, R->bnd[PRA_FULL].as<u64>(), db, (u64)c->nb + 1);
-    check_launch("add_bounds");
     F.buf.swap(c->sort_tmp);
     F.n = n;
     F.sorted = true;

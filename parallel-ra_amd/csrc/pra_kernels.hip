@@ -816,8 +816,13 @@ __global__ void k_merge_rank(const u64* D, u64 nd, const u64* dbnd, const u64* F
 
 // Place every row of the merge: delta row j at j + rank[j], full row i after
 // the delta rows ranked at or before it (rank is non-decreasing).
-__global__ void k_merge_scatter(const u64* F, u64 nf, const u64* D, u64 nd, const u64* rank, int w, u64* out)
+// (with add_to: block 0 also adds delta's bucket bounds to full's, the
+// bounds of the merged version; k_merge_rank has read full's before)
+__global__ void k_merge_scatter(const u64* F, u64 nf, const u64* D, u64 nd, const u64* rank, int w, u64* out,
+                                u64* add_to, const u64* dbnd, u64 nbb)
 {
+    if (add_to && blockIdx.x == 0)
+        for (u64 i = threadIdx.x; i < nbb; i += blockDim.x) add_to[i] += dbnd[i];
     u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nf + nd) return;
     const u64* src;
@@ -2395,23 +2400,12 @@ void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words, int sets)
 }
 
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
-                      u64 nb, int w, u64* rank, u64* out)
+                      u64 nb, int w, u64* rank, u64* out, u64* add_to)
 {
     if (!nd) return;
     hipLaunchKernelGGL(k_merge_rank, dim3(grid_for(nd)), dim3(PRA_BLOCK), 0, st, D, nd, dbnd, F, fbnd, nb, w, rank);
-    hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(nf + nd)), dim3(PRA_BLOCK), 0, st, F, nf, D, nd, rank, w, out);
-}
-
-// bucket bounds of a merge of two bucket-partitioned versions: a[s] += b[s]
-__global__ void k_add_bounds(u64* a, const u64* b, u64 n)
-{
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] += b[i];
-}
-
-void pra_launch_add_bounds(hipStream_t st, u64* a, const u64* b, u64 n)
-{
-    hipLaunchKernelGGL(k_add_bounds, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, a, b, n);
+    hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(nf + nd)), dim3(PRA_BLOCK), 0, st, F, nf, D, nd, rank, w, out,
+                       add_to, dbnd, nb + 1);
 }
 
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out)

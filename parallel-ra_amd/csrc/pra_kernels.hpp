@@ -252,7 +252,6 @@ void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int
 void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jcc, int kc, u64 nb, int bucketed,
                            u64* out, u64* bnd);
 void pra_launch_small_sort_many(hipStream_t st, const SmallSortBatch& b, int njobs);
-void pra_launch_add_bounds(hipStream_t st, u64* a, const u64* b, u64 n);
 void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u64* out, u64* bnd,
                            int jcc, u64 nb);
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
@@ -273,5 +272,7 @@ void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u
 // `sets` consecutive replica sets (one per output of a batched partition)
 void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words, int sets = 1);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
+// merge of sorted delta into sorted full (out); add_to (nullable): full's
+// bucket bounds, which become the merged version's
 void pra_launch_merge(hipStream_t st, const u64* F, u64 nf, const u64* fbnd, const u64* D, u64 nd, const u64* dbnd,
-                      u64 nb, int w, u64* rank, u64* out);
+                      u64 nb, int w, u64* rank, u64* out, u64* add_to = nullptr);
