@@ -1641,8 +1641,13 @@ template <bool PACKED>
 __global__ __launch_bounds__(PRA_BLOCK) void k_dedup_scatter_many(const NewtScatterArgs* jobs, int njobs)
 {
     const u32* block_base = (const u32*)(jobs + njobs);
+    // the job: the number of later jobs' block bases at or below this block,
+    // 64 bases per load round
     int k = 0;
-    while (k + 1 < njobs && block_base[k + 1] <= blockIdx.x) k++;
+    for (int j0 = 1; j0 < njobs; j0 += 64) {
+        const int j = j0 + (int)(threadIdx.x & 63);
+        k += __popcll(__ballot(j < njobs && block_base[j] <= blockIdx.x));
+    }
     NewtScatterArgs a = jobs[k];
     pra_scatter_block<PACKED, PRA_BLOCK>(a, blockIdx.x - block_base[k]);
 }
