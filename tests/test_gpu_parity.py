@@ -384,3 +384,52 @@ def test_scc_with_more_rules_than_one_launch_holds(nb):
     assert o.history(so) == g.history(sg)
     assert g.kernel_stats()["small_sort"]["launches"] >= 1
     g.close()
+
+
+@pytest.mark.parametrize("nb", [1, 3])
+def test_candidate_overflow_rerun_in_a_batched_iteration(nb):
+    # two outputs per iteration (the batched claim / decision / partition
+    # path): A = reachability over 100 sources feeding a 5-node chain into Z,
+    # B = A's new pairs keyed on Z joined with Z's 2,000 out-edges.  B's
+    # candidates jump from 2,000 to 200,000 in the iteration where the pairs
+    # (Z, source) appear, past their capacity (twice the last count, at least
+    # 65,536): the output is re-run at its exact size, with the same result as
+    # the oracle
+    print(f"[overflow rerun] nb={nb}", flush=True)
+    chain = [1000, 1001, 1002, 1003, 1004]        # c1 .. c4, Z
+    g1 = [(x, chain[0]) for x in range(100)] + list(zip(chain[:-1], chain[1:]))
+    g2 = [(chain[-1], 5000 + j) for j in range(2000)]
+
+    def build(b):
+        G1 = b.rel_create(2, 1, 700)
+        G2 = b.rel_create(2, 1, 701)
+        A = b.rel_create(2, 1, 702)
+        B = b.rel_create(2, 1, 703)
+        b.load_rows(G1, with_rowid(g1), FULL)
+        b.load_rows(G2, with_rowid(g2), FULL)
+        fix = b.scc_create(True, 50)
+        for r, mv in ((G1, False), (G2, False), (A, True), (B, False)):
+            b.scc_add_relation(fix, r, mv)
+        b.add_join(fix, A, A, DELTA, G1, FULL, [3, 1])
+        b.add_join(fix, B, A, DELTA, G2, FULL, [3, 1])
+        seed = b.scc_create(False, 51)
+        b.scc_add_relation(seed, G1, False)
+        b.scc_add_relation(seed, A, True)
+        b.add_copy(seed, A, G1, FULL, [1, 0])
+        b.add_dependency(seed, fix)
+        return A, B, fix
+
+    g = gpu_ctx(nb)
+    g.set_profiling(True)
+    Ag, Bg, fg = build(g)
+    g.execute()
+    # one batched claim per iteration, and one more for the re-run
+    assert g.kernel_stats()["dedup_claim"]["launches"] == len(g.history(fg)) + 1
+    o = O.Oracle(nb)
+    Ao, Bo, fo = build(o)
+    o.execute(1)
+    assert_same_rel(o, Ao, g, Ag, "A")
+    assert_same_rel(o, Bo, g, Bg, "B")
+    assert o.history(fo) == g.history(fg)
+    assert o.count(Bo) > 100 * 2000
+    g.close()
