@@ -337,10 +337,10 @@ struct pra_ctx {
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_words, rctl, ttab, kp, jobtab;
-    int rep_sets = 0;            // summary replica sets in rep_words (one per output of a batched partition)
-    char* job_stage = nullptr;   // pinned staging of batched kernels' job tables
-    size_t job_stage_cap = 0;
+    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_words, rctl, ttab, kp, jobtab[2];
+    int rep_sets = 0;              // summary replica sets in rep_words (one per output of a batched partition)
+    char* job_stage[2] = {};       // pinned staging of batched kernels' job tables (expansions, partitions)
+    size_t job_stage_cap[2] = {};
     u64 dict_cap = 0, dict_n = 0;
     int dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
@@ -376,7 +376,7 @@ struct pra_ctx {
         seq.clear();
         for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_words, &rctl, &ttab, &kp, &jobtab})
+                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_words, &rctl, &ttab, &kp, &jobtab[0], &jobtab[1]})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -384,7 +384,8 @@ struct pra_ctx {
         if (host_stage) (void)hipHostFree(host_stage);
         if (host_pos) (void)hipHostFree(host_pos);
         if (mailbox) (void)hipHostFree(mailbox);
-        if (job_stage) (void)hipHostFree(job_stage);
+        for (char* js : job_stage)
+            if (js) (void)hipHostFree(js);
         if (pub_ev) (void)hipEventDestroy(pub_ev);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -451,19 +452,19 @@ struct pra_ctx {
     // pinned staging for job tables copied to the device; the copies of one
     // iteration have completed before the next iteration writes it again (its
     // readback waits for the stream), and a larger buffer waits for them too
-    char* job_stage_buf(size_t bytes)
+    char* job_stage_buf(int slot, size_t bytes)
     {
-        if (bytes > job_stage_cap) {
-            if (job_stage) {
+        if (bytes > job_stage_cap[slot]) {
+            if (job_stage[slot]) {
                 HIPCHK(hipStreamSynchronize(st));
-                HIPCHK(hipHostFree(job_stage));
+                HIPCHK(hipHostFree(job_stage[slot]));
             }
-            job_stage = nullptr;
+            job_stage[slot] = nullptr;
             const size_t c = std::max<size_t>(bytes, 64 << 10);
-            HIPCHK(hipHostMalloc((void**)&job_stage, c, hipHostMallocDefault));
-            job_stage_cap = c;
+            HIPCHK(hipHostMalloc((void**)&job_stage[slot], c, hipHostMallocDefault));
+            job_stage_cap[slot] = c;
         }
-        return job_stage;
+        return job_stage[slot];
     }
     u64* mailbox_buf(size_t words)
     {
@@ -1461,6 +1462,14 @@ static void run_iteration(pra_scc* S)
     std::vector<NewtScatterArgs> scatter_jobs[2];
     std::vector<size_t> scatter_outs[2];
     std::vector<NewtFoldArgs> later_folds;
+    // batched expansions: join (arguments, blocks, per-candidate bytes and
+    // output) and copy (arguments, blocks, bytes)
+    std::vector<JoinExpandArgs> jexp_jobs;
+    std::vector<u32> jexp_blocks;
+    std::vector<std::pair<double, size_t>> jexp_per;
+    std::vector<CopyExpandArgs> cexp_jobs;
+    std::vector<u32> cexp_blocks;
+    double cexp_bytes = 0;
     // planned: the arrival plan was made by the iteration's batched plan
     // per output: its rows (dest, slot_of, keep), candidate words and
     // histogram words in the shared buffers when its claim and decision are
@@ -1574,6 +1583,12 @@ static void run_iteration(pra_scc* S)
                     a.cw = cw;
                     a.out_jcc = R->jcc;
                     for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+                    if (phase == 1) {
+                        jexp_jobs.push_back(a);
+                        jexp_blocks.push_back(pra_join_expand_blocks(std::min<u64>(C, ub[oi])));
+                        jexp_per.emplace_back(8.0 * ra.inner->w + 8.0 * cw, oi);
+                        continue;
+                    }
                     ProfPerCand p(c, K_JOIN_EXPAND, 8.0 * ra.inner->w + 8.0 * cw, oi);
                     pra_launch_join_expand(c->st, a, std::min<u64>(C, ub[oi]));
                     check_launch("join_expand");
@@ -1594,6 +1609,14 @@ static void run_iteration(pra_scc* S)
                     a.cw = cw;
                     a.out_jcc = R->jcc;
                     for (int i = 0; i < cw; i++) a.map[i] = ra.map[i];
+                    if (phase == 1) {
+                        if (a.n) {
+                            cexp_jobs.push_back(a);
+                            cexp_blocks.push_back((u32)((a.n + PRA_BLOCK - 1) / PRA_BLOCK));
+                            cexp_bytes += (double)ra.n_outer * 8.0 * (ra.outer->w + cw);
+                        }
+                        continue;
+                    }
                     Prof p(c, K_COPY_EXPAND, (double)ra.n_outer * 8.0 * (ra.outer->w + cw));
                     pra_launch_copy_expand(c->st, a);
                     check_launch("copy_expand");
@@ -1768,6 +1791,43 @@ static void run_iteration(pra_scc* S)
         c->offs.reserve(ht * 4 + 8, c->st);
         for (size_t oi = 0; oi < no; oi++)
             if (cap[oi]) run_output(oi, nullptr, true, 1);
+        // the expansions of all rules: job tables through pinned staging into
+        // device memory, one launch per kind
+        {
+            auto table_bytes = [](size_t n, size_t sz) { return n * sz + ((n + 1) * 4 + 255) / 256 * 256; };
+            const size_t jb = jexp_jobs.empty() ? 0 : table_bytes(jexp_jobs.size(), sizeof(JoinExpandArgs));
+            const size_t cb = cexp_jobs.empty() ? 0 : table_bytes(cexp_jobs.size(), sizeof(CopyExpandArgs));
+            if (jb + cb) {
+                char* hs = c->job_stage_buf(0, jb + cb);
+                c->jobtab[0].reserve(jb + cb, c->st);
+                char* dv = c->jobtab[0].as<char>();
+                auto put = [&](size_t at, const void* jobs, size_t n, size_t sz, const std::vector<u32>& blocks) {
+                    memcpy(hs + at, jobs, n * sz);
+                    u32* bb = (u32*)(hs + at + n * sz);
+                    bb[0] = 0;
+                    for (size_t k = 0; k < n; k++) bb[k + 1] = bb[k] + blocks[k];
+                    HIPCHK(hipMemcpyAsync(dv + at, hs + at, table_bytes(n, sz), hipMemcpyHostToDevice, c->st));
+                    return bb[n];
+                };
+                if (jb) {
+                    const u32 nbk = put(0, jexp_jobs.data(), jexp_jobs.size(), sizeof(JoinExpandArgs), jexp_blocks);
+                    const size_t before = c->pending.size();
+                    {
+                        Prof p(c, K_JOIN_EXPAND, 0.0);
+                        pra_launch_expand_many(c->st, dv, (int)jexp_jobs.size(), nbk, true);
+                        check_launch("join_expand_many");
+                    }
+                    if (c->pending.size() == before + 1)
+                        for (auto& pe : jexp_per) c->per_candidate.emplace_back(before, pe.first, pe.second);
+                }
+                if (cb) {
+                    const u32 nbk = put(jb, cexp_jobs.data(), cexp_jobs.size(), sizeof(CopyExpandArgs), cexp_blocks);
+                    Prof p(c, K_COPY_EXPAND, cexp_bytes);
+                    pra_launch_expand_many(c->st, dv + jb, (int)cexp_jobs.size(), nbk, false);
+                    check_launch("copy_expand_many");
+                }
+            }
+        }
         for (int cls = 1; cls <= 7; cls++) {
             ClaimBatch cb;
             DecideBatch db;
@@ -1831,8 +1891,8 @@ static void run_iteration(pra_scc* S)
                 stage_bytes += scatter_jobs[pk].size() * sizeof(NewtScatterArgs) +
                                ((scatter_jobs[pk].size() + 1) * 4 + 255) / 256 * 256;
         if (stage_bytes) {
-            char* hs = c->job_stage_buf(stage_bytes);
-            c->jobtab.reserve(stage_bytes, c->st);
+            char* hs = c->job_stage_buf(1, stage_bytes);
+            c->jobtab[1].reserve(stage_bytes, c->st);
             size_t at = 0;
             for (int pk = 0; pk < 2; pk++) {
                 const auto& J = scatter_jobs[pk];
@@ -1842,11 +1902,11 @@ static void run_iteration(pra_scc* S)
                 u32* bb = (u32*)(hs + at + J.size() * sizeof(NewtScatterArgs));
                 bb[0] = 0;
                 for (size_t k = 0; k < J.size(); k++) bb[k + 1] = bb[k] + J[k].nblocks;
-                HIPCHK(hipMemcpyAsync(c->jobtab.as<char>() + at, hs + at, bytes, hipMemcpyHostToDevice, c->st));
+                HIPCHK(hipMemcpyAsync(c->jobtab[1].as<char>() + at, hs + at, bytes, hipMemcpyHostToDevice, c->st));
                 const size_t before = c->pending.size();
                 {
                     Prof p(c, K_DEDUP_SCATTER, 0.0);
-                    pra_launch_dedup_scatter_many(c->st, (const NewtScatterArgs*)(c->jobtab.as<char>() + at),
+                    pra_launch_dedup_scatter_many(c->st, (const NewtScatterArgs*)(c->jobtab[1].as<char>() + at),
                                                   (int)J.size(), bb[J.size()], pk == 1);
                     check_launch("dedup_scatter_many");
                 }

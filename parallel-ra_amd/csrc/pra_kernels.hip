@@ -881,14 +881,15 @@ __device__ inline u64 pra_block_row_of(const u64* off, u64 n, u64 o0, u32* s_cnt
     return lo;
 }
 
-__global__ void k_join_expand(JoinExpandArgs a)
+// block `bid` of `gdim` of one rule's join expansion
+__device__ __forceinline__ void pra_join_expand_block(const JoinExpandArgs& a, u32 bid, u32 gdim)
 {
     if (a.ctl[1]) return;
     const u64 first = a.off[0], total = a.off[a.n_outer];   // first > 0 for a window of rows
     __shared__ u64 s_off[PRA_BLOCK + 1];
     __shared__ u32 s_cnt;
-    const u64 step = (u64)gridDim.x * blockDim.x;
-    for (u64 o0 = first + (u64)blockIdx.x * blockDim.x; o0 < total; o0 += step) {
+    const u64 step = (u64)gdim * blockDim.x;
+    for (u64 o0 = first + (u64)bid * blockDim.x; o0 < total; o0 += step) {
         __syncthreads();
         // the block's first row, then the offsets of the next PRA_BLOCK rows
         // in LDS, where each output finds its row
@@ -931,6 +932,38 @@ __global__ void k_join_expand(JoinExpandArgs a)
     }
 }
 
+__global__ void k_join_expand(JoinExpandArgs a)
+{
+    pra_join_expand_block(a, blockIdx.x, gridDim.x);
+}
+
+// job of block `bid` in a device job table (block bases after the njobs
+// arguments): the number of later jobs' bases at or below it, 64 bases per
+// load round
+__device__ inline int pra_job_of(const u32* block_base, int njobs, u32 bid)
+{
+    int k = 0;
+    for (int j0 = 1; j0 < njobs; j0 += 64) {
+        const int j = j0 + (int)(threadIdx.x & 63);
+        k += __popcll(__ballot(j < njobs && block_base[j] <= bid));
+    }
+    return k;
+}
+
+// Join expansions of several rules in one launch (job table in device
+// memory); a block whose job has no outputs left for it leaves before it
+// reads the job's arguments.
+__global__ void k_join_expand_many(const JoinExpandArgs* jobs, int njobs)
+{
+    const u32* block_base = (const u32*)(jobs + njobs);
+    const int k = pra_job_of(block_base, njobs, blockIdx.x);
+    const u32 bid = blockIdx.x - block_base[k];
+    const JoinExpandArgs* jp = jobs + k;
+    if (jp->ctl[1] || jp->off[0] + (u64)bid * PRA_BLOCK >= jp->off[jp->n_outer]) return;
+    const JoinExpandArgs a = *jp;
+    pra_join_expand_block(a, bid, block_base[k + 1] - block_base[k]);
+}
+
 __device__ inline bool pra_filter(int op, u64 x, u64 y)
 {
     switch (op) {
@@ -945,10 +978,10 @@ __device__ inline bool pra_filter(int op, u64 x, u64 y)
     }
 }
 
-__global__ void k_copy_expand(CopyExpandArgs a)
+// outer row i of one rule's copy expansion
+__device__ __forceinline__ void pra_copy_expand_row(const CopyExpandArgs& a, u64 i)
 {
     if (a.ctl[1]) return;
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     if (a.off[i + 1] == a.off[i]) return;
     const u64* r = a.rows + i * a.w;
@@ -962,6 +995,21 @@ __global__ void k_copy_expand(CopyExpandArgs a)
         if (c < a.out_jcc) h = pra_fnv_word(h, v);
     }
     a.dest[pos] = a.nb == 1 ? 0u : (u32)(h % a.nb);
+}
+
+__global__ void k_copy_expand(CopyExpandArgs a)
+{
+    pra_copy_expand_row(a, (u64)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Copy expansions of several rules in one launch (job table as
+// k_join_expand_many's; every block has rows)
+__global__ void k_copy_expand_many(const CopyExpandArgs* jobs, int njobs)
+{
+    const u32* block_base = (const u32*)(jobs + njobs);
+    const int k = pra_job_of(block_base, njobs, blockIdx.x);
+    const CopyExpandArgs a = jobs[k];
+    pra_copy_expand_row(a, (u64)(blockIdx.x - block_base[k]) * blockDim.x + threadIdx.x);
 }
 
 // Arrival plan of one output relation, computed on the device: for source
@@ -2129,12 +2177,27 @@ void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, co
                        cap, flags);
 }
 
+u32 pra_join_expand_blocks(u64 cap)
+{
+    u64 g = (cap + PRA_BLOCK - 1) / PRA_BLOCK;
+    return (u32)(g > 16384 ? 16384 : g);
+}
+
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap)
 {
     if (!cap) return;
-    u64 g = (cap + PRA_BLOCK - 1) / PRA_BLOCK;
-    if (g > 16384) g = 16384;
-    hipLaunchKernelGGL(k_join_expand, dim3((unsigned)g), dim3(PRA_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_join_expand, dim3(pra_join_expand_blocks(cap)), dim3(PRA_BLOCK), 0, st, a);
+}
+
+void pra_launch_expand_many(hipStream_t st, const void* jobs, int njobs, u32 nblocks, bool join)
+{
+    if (njobs <= 0 || !nblocks) return;
+    if (join)
+        hipLaunchKernelGGL(k_join_expand_many, dim3(nblocks), dim3(PRA_BLOCK), 0, st, (const JoinExpandArgs*)jobs,
+                           njobs);
+    else
+        hipLaunchKernelGGL(k_copy_expand_many, dim3(nblocks), dim3(PRA_BLOCK), 0, st, (const CopyExpandArgs*)jobs,
+                           njobs);
 }
 
 void pra_launch_plan(hipStream_t st, const PlanArgs& a)

@@ -225,6 +225,10 @@ void pra_launch_compact_rows(hipStream_t st, const u64* in, u64 n, int w, const 
 void pra_launch_index_insert(hipStream_t st, const u64* rows, int w, int jcc, const u64* gstart, u64 ng, u64* table,
                              u64 cap, u32* flags);
 void pra_launch_join_expand(hipStream_t st, const JoinExpandArgs& a, u64 cap);
+u32 pra_join_expand_blocks(u64 cap);   // grid of a join expansion of up to cap outputs
+// jobs: device memory holding njobs JoinExpandArgs (join) or CopyExpandArgs,
+// then njobs + 1 u32 block bases
+void pra_launch_expand_many(hipStream_t st, const void* jobs, int njobs, u32 nblocks, bool join);
 void pra_launch_plan(hipStream_t st, const PlanArgs& a);
 void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks);
 void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
