@@ -48,11 +48,18 @@ typedef struct pra_scc pra_scc;
 #define PRA_E_UNSUPPORTED -4
 #define PRA_E_COLLISION -5  /* 64-bit key hash collision detected */
 #define PRA_E_STATE -6      /* call out of order */
+#define PRA_E_BUDGET -7     /* the context's device memory budget would be exceeded */
 
 /* Context = one LIE (src/lie/lie.h:13-153) on one device with `nbuckets`
  * virtual buckets, i.e. the layout of the reference run on `nbuckets` MPI
- * ranks (mpi_comm::create, src/comm/comm.cpp:22-28).  hbm_budget 0 = no cap. */
+ * ranks (mpi_comm::create, src/comm/comm.cpp:22-28).  hbm_budget caps the
+ * device bytes the context holds at any time (relation versions, tables and
+ * scratch; 0 = no cap): a call that would need more fails with PRA_E_BUDGET
+ * and leaves the context unusable except for pra_ctx_destroy. */
 int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out);
+/* device bytes the context holds now and at most so far (no counterpart in
+ * the reference; what budget admission measures) */
+int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak);
 int pra_ctx_destroy(pra_ctx* ctx);
 const char* pra_last_error(pra_ctx* ctx);
 /* LIE::set_batch_size (src/lie/lie.h:116) */

@@ -1,0 +1,159 @@
+// Process-wide cache of large device blocks, with its ownership rule kept free
+// of any HIP type so that the rule can be exercised on the CPU with fake
+// stream handles and a fake allocator.
+//
+// Returning tens of GB to the driver, and first use of freshly mapped device
+// memory, cost far more than the work large blocks hold, so released blocks of
+// at least `min_block` bytes are kept and handed to later requests of the same
+// power-of-two size class (or the next one up).
+//
+// Ownership rule (no events, no cross-stream ordering):
+//   * put(stream, block): the block was released while work enqueued on
+//     `stream` may still read or write it.  It is OWNED by that stream: only a
+//     request on the same stream may take it, and that reuse is ordered behind
+//     the earlier work by the stream itself.
+//   * idle(stream): the host has observed that every piece of work enqueued on
+//     `stream` so far has completed (a stream synchronization returned).  The
+//     stream's owned blocks become IDLE: any stream may take them, and flush()
+//     may free them.
+//   * flush(device) frees idle blocks only; an owned block is never freed or
+//     handed to another stream before its owner is declared idle.
+// The caller must call idle() only after a synchronization that happened after
+// the put() calls it covers (the cache records no time, so it relies on the
+// host order of put, synchronize, idle).
+#ifndef PRA_BLOCKCACHE_HPP
+#define PRA_BLOCKCACHE_HPP
+
+#include <cstddef>
+#include <mutex>
+#include <vector>
+
+namespace pra {
+
+struct BlockAllocator {
+    // returns nullptr when the device has no room
+    void* (*alloc)(size_t bytes, void* user);
+    void (*release)(void* p, void* user);
+    void* user;
+};
+
+class BlockCache {
+  public:
+    explicit BlockCache(size_t min_block) : min_(min_block) {}
+
+    size_t min_block() const { return min_; }
+
+    // power-of-two size class of a request
+    size_t size_class(size_t bytes) const
+    {
+        size_t c = min_;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+
+    // A cached block for `bytes` on `stream`: idle or owned by `stream`, of
+    // the request's class or the next one, smallest first.  nullptr on a miss.
+    void* take(int device, const void* stream, size_t bytes, size_t* got)
+    {
+        const size_t cls = size_class(bytes);
+        std::lock_guard<std::mutex> lk(mu_);
+        long best = -1;
+        for (size_t i = 0; i < free_.size(); i++) {
+            const Entry& b = free_[i];
+            if (b.device == device && (!b.owner || b.owner == stream) && b.size >= cls && b.size <= 2 * cls &&
+                (best < 0 || b.size < free_[(size_t)best].size))
+                best = (long)i;
+        }
+        if (best < 0) return nullptr;
+        Entry e = free_[(size_t)best];
+        free_.erase(free_.begin() + best);
+        *got = e.size;
+        return e.p;
+    }
+
+    // take(), else a fresh block of the request's class from the allocator;
+    // when the allocator fails, the idle blocks of the device are freed and
+    // the allocation retried once.  nullptr when that fails too.
+    void* get(int device, const void* stream, size_t bytes, size_t* got, const BlockAllocator& a)
+    {
+        if (void* p = take(device, stream, bytes, got)) return p;
+        const size_t cls = size_class(bytes);
+        void* p = a.alloc(cls, a.user);
+        if (!p) {
+            flush(device, a);
+            p = a.alloc(cls, a.user);
+        }
+        if (p) *got = cls;
+        return p;
+    }
+
+    void put(int device, const void* stream, void* p, size_t size)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.push_back(Entry{p, size, device, stream});
+    }
+
+    // every piece of work enqueued on `stream` so far has completed
+    void idle(const void* stream)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& b : free_)
+            if (b.owner == stream) b.owner = nullptr;
+    }
+
+    // free the idle blocks of a device; returns the bytes freed
+    size_t flush(int device, const BlockAllocator& a)
+    {
+        std::vector<Entry> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t i = 0; i < free_.size();)
+                if (free_[i].device == device && !free_[i].owner) {
+                    drop.push_back(free_[i]);
+                    free_.erase(free_.begin() + i);
+                } else {
+                    i++;
+                }
+        }
+        size_t bytes = 0;
+        for (auto& b : drop) {
+            a.release(b.p, a.user);
+            bytes += b.size;
+        }
+        return bytes;
+    }
+
+    // bytes cached for a device: owned by `stream` (any owner when stream is
+    // null), and idle
+    size_t owned_bytes(int device, const void* stream) const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        size_t s = 0;
+        for (auto& b : free_)
+            if (b.device == device && b.owner && (!stream || b.owner == stream)) s += b.size;
+        return s;
+    }
+    size_t idle_bytes(int device) const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        size_t s = 0;
+        for (auto& b : free_)
+            if (b.device == device && !b.owner) s += b.size;
+        return s;
+    }
+
+  private:
+    struct Entry {
+        void* p;
+        size_t size;
+        int device;
+        const void* owner;   // stream that may still use the block, or null when idle
+    };
+    size_t min_;
+    mutable std::mutex mu_;
+    std::vector<Entry> free_;
+};
+
+} // namespace pra
+
+#endif

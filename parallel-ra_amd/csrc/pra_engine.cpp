@@ -10,6 +10,7 @@
 // Membership of full u delta is an open-addressing table keyed by the dedup
 // columns; join probe sides get a (key -> first row, row count) index.
 #include "../../include/pra.h"
+#include "pra_blockcache.hpp"
 #include "pra_kernels.hpp"
 
 #include <algorithm>
@@ -49,77 +50,74 @@ static void check_launch(const char* what)
     if (e != hipSuccess) throw PraError(PRA_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Blocks of at least 64 MiB are kept in a process-wide cache when released
-// instead of going back to the driver, and handed to later requests of the
-// same power-of-two size class (or the next one): returning tens of GB to the
-// driver, and first use of fresh device memory, cost far more than the work
-// such blocks hold.  A block released while its stream still runs stays with
-// that stream (its reuse there is ordered by the stream itself); when the
-// owning context retires its stream (after synchronizing it) the block
-// becomes free for every stream.  Out of memory flushes the free blocks.
+// Large blocks (>= 64 MiB) go through the process-wide block cache of
+// pra_blockcache.hpp (its ownership rule: a block released while its stream
+// runs is reused only on that stream until a host synchronization of the
+// stream declares it idle).  Out of memory: idle blocks are freed, then the
+// requesting stream is synchronized, its own blocks freed and the
+// stream-ordered pool trimmed before one more attempt.
 namespace {
 constexpr size_t kBigBlock = 64ULL << 20;
-struct BigFree {
-    void* p;
-    size_t size;
-    int device;
-    hipStream_t owner;   // stream that released it while running, or null when idle
-};
-std::mutex g_big_mu;
-std::vector<BigFree> g_big_free;
+pra::BlockCache g_big(kBigBlock);
 
-size_t big_class(size_t bytes)
+void* hip_block_alloc(size_t bytes, void*)
 {
-    size_t c = kBigBlock;
-    while (c < bytes) c <<= 1;
-    return c;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+void hip_block_free(void* p, void*) { (void)hipFree(p); }
+const pra::BlockAllocator kHipBlocks{hip_block_alloc, hip_block_free, nullptr};
+
+// Device bytes held by each context, keyed by its stream (every buffer of a
+// context is ordered on the context's stream): live bytes, their peak, and the
+// context's cap (0 = none).
+struct MemAcct {
+    long long live = 0, peak = 0;
+    u64 budget = 0;
+};
+std::mutex g_acct_mu;
+std::map<hipStream_t, MemAcct> g_acct;
+
+// charge `bytes` more to the context of stream s (fails past its budget while
+// `check`), or credit them back when negative
+void acct_charge(hipStream_t s, long long bytes, bool check)
+{
+    std::lock_guard<std::mutex> lk(g_acct_mu);
+    auto it = g_acct.find(s);
+    if (it == g_acct.end()) return;
+    MemAcct& a = it->second;
+    if (check && bytes > 0 && a.budget && (u64)(a.live + bytes) > a.budget)
+        throw PraError(PRA_E_BUDGET, "device memory budget exceeded: " + std::to_string(a.live) + " bytes held + " +
+                                         std::to_string(bytes) + " requested > budget " + std::to_string(a.budget));
+    a.live += bytes;
+    if (a.live > a.peak) a.peak = a.live;
 }
 
-// give the idle blocks of a device back to the driver
-void big_flush(int dev)
+// the stream-ordered pool keeps freed memory for reuse; give it back
+void trim_pool(int dev)
 {
-    std::vector<BigFree> drop;
-    {
-        std::lock_guard<std::mutex> lk(g_big_mu);
-        for (size_t i = 0; i < g_big_free.size();)
-            if (g_big_free[i].device == dev && !g_big_free[i].owner) {
-                drop.push_back(g_big_free[i]);
-                g_big_free.erase(g_big_free.begin() + i);
-            } else {
-                i++;
-            }
-    }
-    for (auto& b : drop) (void)hipFree(b.p);
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) (void)hipMemPoolTrimTo(pool, 0);
 }
 
 void* big_get(size_t bytes, hipStream_t st, size_t* got)
 {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
-    const size_t cls = big_class(bytes);
-    {
-        std::lock_guard<std::mutex> lk(g_big_mu);
-        long best = -1;
-        for (size_t i = 0; i < g_big_free.size(); i++) {
-            const BigFree& b = g_big_free[i];
-            if (b.device == dev && (!b.owner || b.owner == st) && b.size >= cls && b.size <= 2 * cls &&
-                (best < 0 || b.size < g_big_free[(size_t)best].size))
-                best = (long)i;
-        }
-        if (best >= 0) {
-            BigFree b = g_big_free[(size_t)best];
-            g_big_free.erase(g_big_free.begin() + best);
-            *got = b.size;
-            return b.p;
-        }
+    void* p = g_big.get(dev, st, bytes, got, kHipBlocks);
+    if (!p) {
+        HIPCHK(hipStreamSynchronize(st));
+        g_big.idle(st);
+        g_big.flush(dev, kHipBlocks);
+        trim_pool(dev);
+        p = g_big.get(dev, st, bytes, got, kHipBlocks);
     }
-    void* p = nullptr;
-    if (hipMalloc(&p, cls) != hipSuccess) {
-        (void)hipGetLastError();
-        big_flush(dev);
-        HIPCHK(hipMalloc(&p, cls));
-    }
-    *got = cls;
+    if (!p)
+        throw PraError(PRA_E_HIP, "hipMalloc: out of memory (" + std::to_string(g_big.size_class(bytes)) + " bytes)");
     return p;
 }
 
@@ -127,18 +125,11 @@ void big_put(void* p, size_t size, hipStream_t st)
 {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    std::lock_guard<std::mutex> lk(g_big_mu);
-    g_big_free.push_back(BigFree{p, size, dev, st});
+    g_big.put(dev, st, p, size);
 }
 
-// the stream's work is complete and it is about to be destroyed: its blocks
-// are idle from now on
-void big_retire(hipStream_t st)
-{
-    std::lock_guard<std::mutex> lk(g_big_mu);
-    for (auto& b : g_big_free)
-        if (b.owner == st) b.owner = nullptr;
-}
+// every piece of work enqueued on st so far has completed
+void stream_idle(hipStream_t st) { g_big.idle(st); }
 } // namespace
 
 // Device buffer owned by one context and ordered on its stream: small blocks
@@ -175,6 +166,7 @@ struct DBuf {
                 big_put(p, cap, st);
             else
                 (void)hipFreeAsync(p, st);
+            acct_charge(st, -(long long)cap, false);
         }
         p = nullptr;
         cap = 0;
@@ -188,14 +180,30 @@ struct DBuf {
         nc = std::max<size_t>(nc, 256);
         void* np = nullptr;
         bool nbig = nc >= kBigBlock;
-        if (nbig) {
-            np = big_get(nc, s, &nc);
-        } else if (hipMallocAsync(&np, nc, s) != hipSuccess) {
-            int dev = 0;
-            (void)hipGetLastError();
-            HIPCHK(hipGetDevice(&dev));
-            big_flush(dev);
-            HIPCHK(hipMallocAsync(&np, nc, s));
+        if (nbig) nc = g_big.size_class(nc);
+        acct_charge(s, (long long)nc, true);
+        try {
+            if (nbig) {
+                const size_t want = nc;
+                np = big_get(want, s, &nc);
+                acct_charge(s, (long long)nc - (long long)want, false);   // a cached block of the next class
+            } else if (hipMallocAsync(&np, nc, s) != hipSuccess) {
+                int dev = 0;
+                (void)hipGetLastError();
+                HIPCHK(hipGetDevice(&dev));
+                g_big.flush(dev, kHipBlocks);
+                if (hipMallocAsync(&np, nc, s) != hipSuccess) {
+                    (void)hipGetLastError();
+                    HIPCHK(hipStreamSynchronize(s));
+                    g_big.idle(s);
+                    g_big.flush(dev, kHipBlocks);
+                    trim_pool(dev);
+                    HIPCHK(hipMallocAsync(&np, nc, s));
+                }
+            }
+        } catch (...) {
+            acct_charge(s, -(long long)nc, false);
+            throw;
         }
         if (p && used) HIPCHK(hipMemcpyAsync(np, p, used, hipMemcpyDeviceToDevice, s));
         if (p) {
@@ -203,6 +211,7 @@ struct DBuf {
                 big_put(p, cap, s);
             else
                 HIPCHK(hipFreeAsync(p, s));
+            acct_charge(st ? st : s, -(long long)cap, false);
         }
         p = np;
         cap = nc;
@@ -380,7 +389,11 @@ struct pra_ctx {
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
-        if (st) big_retire(st);
+        if (st) {
+            stream_idle(st);
+            std::lock_guard<std::mutex> lk(g_acct_mu);
+            g_acct.erase(st);
+        }
         if (host_stage) (void)hipHostFree(host_stage);
         if (host_pos) (void)hipHostFree(host_pos);
         if (mailbox) (void)hipHostFree(mailbox);
@@ -576,6 +589,7 @@ static u64* readback(pra_ctx* c, const void* src, size_t words)
     if (words) HIPCHK(hipMemcpyAsync(h, src, words * 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(h + words, c->flags_buf.p, 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    stream_idle(c->st);
     check_flags((u32)h[words]);
     return h;
 }
@@ -2221,6 +2235,10 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
             HIPCHK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
         }
         HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        {
+            std::lock_guard<std::mutex> lk(g_acct_mu);
+            g_acct[c->st].budget = hbm_budget;
+        }
         c->device = device;
         c->nb = nbuckets;
         c->budget = hbm_budget;
@@ -2304,11 +2322,22 @@ int pra_ctx_kernel_stat(pra_ctx* ctx, int idx, const char** name, uint64_t* laun
     return PRA_OK;
 }
 
+int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak)
+{
+    if (!ctx) return PRA_E_ARG;
+    std::lock_guard<std::mutex> lk(g_acct_mu);
+    auto it = g_acct.find(ctx->st);
+    if (live) *live = it == g_acct.end() ? 0 : (uint64_t)it->second.live;
+    if (peak) *peak = it == g_acct.end() ? 0 : (uint64_t)it->second.peak;
+    return PRA_OK;
+}
+
 int pra_ctx_synchronize(pra_ctx* ctx)
 {
     if (!ctx) return PRA_E_ARG;
     API_BEGIN(ctx)
     HIPCHK(hipStreamSynchronize(ctx->st));
+    stream_idle(ctx->st);
     API_END
 }
 

@@ -23,7 +23,7 @@ F_EQ, F_NEQ, F_LT, F_LE, F_GT, F_GE, F_TRUE, F_FALSE = range(8)
 EXPORTS = (
     "pra_ctx_create", "pra_ctx_destroy", "pra_last_error", "pra_ctx_set_batch_size",
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
-    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize",
+    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
     "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
@@ -72,6 +72,7 @@ def lib():
             "pra_ctx_kernel_stat": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _u64p,
                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
             "pra_ctx_synchronize": [_vp],
+            "pra_ctx_device_bytes": [_vp, _u64p, _u64p],
             "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                ctypes.POINTER(_vp)],
             "pra_rel_destroy": [_vp],
@@ -184,6 +185,12 @@ class Context:
 
     def synchronize(self):
         self._chk(self.L.pra_ctx_synchronize(self.c), "synchronize")
+
+    def device_bytes(self):
+        """(bytes held now, peak bytes held) on the device by this context"""
+        a, b = _u64(), _u64()
+        self._chk(self.L.pra_ctx_device_bytes(self.c, ctypes.byref(a), ctypes.byref(b)), "device_bytes")
+        return int(a.value), int(b.value)
 
     def joined(self):
         v = _u64()

@@ -1,0 +1,153 @@
+// CPU check of the large-block cache's ownership rule
+// (parallel-ra_amd/csrc/pra_blockcache.hpp) with fake stream handles and a
+// fake allocator that records every allocation and release.  Prints "ok" and
+// exits 0 when every check holds; otherwise names the first failed check.
+#include "../parallel-ra_amd/csrc/pra_blockcache.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <set>
+
+namespace {
+
+struct FakeDevice {
+    size_t capacity = 0, used = 0;
+    std::set<void*> live;
+    std::map<void*, size_t> sizes;
+    int allocs = 0, frees = 0;
+    char* next = reinterpret_cast<char*>(0x100000);
+};
+
+void* fake_alloc(size_t bytes, void* user)
+{
+    auto* d = static_cast<FakeDevice*>(user);
+    if (d->used + bytes > d->capacity) return nullptr;
+    void* p = d->next;
+    d->next += bytes + 4096;
+    d->used += bytes;
+    d->live.insert(p);
+    d->sizes[p] = bytes;
+    d->allocs++;
+    return p;
+}
+
+void fake_release(void* p, void* user)
+{
+    auto* d = static_cast<FakeDevice*>(user);
+    if (!d->live.count(p)) {
+        std::printf("FAIL: release of a block that is not live\n");
+        std::exit(1);
+    }
+    d->live.erase(p);
+    d->frees++;
+    d->used -= d->sizes[p];
+}
+
+int failures = 0;
+void expect(bool ok, const char* what)
+{
+    if (!ok) {
+        std::printf("FAIL: %s\n", what);
+        failures++;
+    }
+}
+
+} // namespace
+
+int main()
+{
+    const size_t MB = 1 << 20;
+    FakeDevice dev;
+    dev.capacity = 1024 * MB;
+    pra::BlockAllocator a{fake_alloc, fake_release, &dev};
+    pra::BlockCache cache(64 * MB);
+    int sa_tag = 0, sb_tag = 0;
+    const void* sA = &sa_tag;   // two fake streams
+    const void* sB = &sb_tag;
+    size_t got = 0;
+
+    // size classes are powers of two from the minimum block up
+    expect(cache.size_class(1) == 64 * MB, "class of 1 byte is the minimum block");
+    expect(cache.size_class(64 * MB + 1) == 128 * MB, "class rounds up to the next power of two");
+
+    // a block released while stream A may still use it is owned by A
+    void* b = cache.get(0, sA, 100 * MB, &got, a);
+    expect(b != nullptr && got == 128 * MB, "fresh block of the request's class");
+    cache.put(0, sA, b, got);
+    expect(cache.owned_bytes(0, sA) == 128 * MB, "released block is owned by its stream");
+
+    // another stream never gets it while A owns it
+    void* c = cache.get(0, sB, 100 * MB, &got, a);
+    expect(c != nullptr && c != b, "an owned block is not handed to another stream");
+    cache.put(0, sB, c, got);
+
+    // a flush never frees an owned block
+    size_t freed = cache.flush(0, a);
+    expect(freed == 0 && dev.live.count(b) && dev.live.count(c), "flush frees no owned block");
+
+    // the owning stream reuses it without any synchronization
+    void* b2 = cache.get(0, sA, 90 * MB, &got, a);
+    expect(b2 == b, "the owning stream reuses its own block");
+    cache.put(0, sA, b2, got);
+
+    // after A is declared idle its blocks are free for every stream
+    int sc_tag = 0;
+    const void* sC = &sc_tag;   // a stream that owns nothing
+    cache.idle(sA);
+    expect(cache.owned_bytes(0, sA) == 0 && cache.idle_bytes(0) == 128 * MB, "idle moves A's blocks to the pool");
+    expect(cache.owned_bytes(0, sB) == 128 * MB, "idle of A leaves B's blocks owned");
+    void* b3 = cache.get(0, sC, 70 * MB, &got, a);
+    expect(b3 == b, "an idle block goes to another stream");
+    cache.put(0, sC, b3, got);
+
+    // classes: a request takes its class or the next one up, never larger
+    cache.idle(sB);
+    cache.idle(sC);   // pool: b, c (128 MiB each), all idle
+    void* small = cache.get(0, sA, 10 * MB, &got, a);
+    expect((small == b || small == c) && got == 128 * MB, "a 64 MiB request may take a block of the next class");
+    cache.put(0, sA, small, got);
+    cache.idle(sA);
+    void* big = cache.get(0, sA, 300 * MB, &got, a);
+    expect(big != b && big != c && got == 512 * MB, "a 512 MiB request gets a fresh block");
+    cache.put(0, sA, big, got);
+    void* tiny = cache.get(0, sB, 1 * MB, &got, a);
+    expect(tiny != big && got <= 128 * MB, "a 64 MiB request never takes a 512 MiB block");
+    cache.put(0, sB, tiny, got);
+    cache.idle(sB);
+
+    // out of room: 768 MiB are held (b and c idle, `big` owned by A) and C
+    // needs 512 MiB more; the idle blocks are freed to make space, the owned
+    // one is not
+    expect(dev.used == 768 * MB, "bookkeeping before the pressure case");
+    void* d0 = cache.get(0, sC, 400 * MB, &got, a);
+    expect(d0 != nullptr && got == 512 * MB, "allocation succeeds after freeing idle blocks");
+    expect(!dev.live.count(b) && !dev.live.count(c), "idle blocks were freed");
+    expect(dev.live.count(big) == 1, "a block owned by A survives C's out-of-memory flush");
+    if (d0) cache.put(0, sC, d0, got);
+    // still no room and nothing idle: B's request fails instead of taking the
+    // blocks A and C own
+    void* d1 = cache.get(0, sB, 400 * MB, &got, a);
+    expect(d1 == nullptr && dev.live.count(big) == 1 && dev.live.count(d0) == 1,
+           "a request that cannot fit fails; owned blocks stay");
+
+    // flush after everything is idle returns every block
+    cache.idle(sA);
+    cache.idle(sB);
+    cache.idle(sC);
+    cache.flush(0, a);
+    expect(dev.live.empty(), "flush of an all-idle cache frees every block");
+    expect(dev.allocs == dev.frees, "allocations and releases balance");
+
+    // devices are separate
+    void* e = cache.get(1, sA, 64 * MB, &got, a);
+    cache.put(1, sA, e, got);
+    cache.idle(sA);
+    size_t took = 0;
+    expect(cache.take(0, sA, 64 * MB, &took) == nullptr, "a block of device 1 is not handed out on device 0");
+    cache.flush(1, a);
+
+    if (failures) return 1;
+    std::printf("ok\n");
+    return 0;
+}
