@@ -221,6 +221,35 @@ def small_dag():
     return P.with_rowid([(1, 2), (1, 3), (1, 4), (2, 5), (3, 5), (3, 6), (4, 7), (4, 8), (2, 8)])
 
 
+def threshold_split(b):
+    """Two join rules into one output where rule 0's call emits 20,500,000
+    distinct tuples in one iteration -- past the reference's 20,000,000-output
+    local-join threshold (src/RAM/RA_tasks.cpp:312, src/RA/parallel_join.cpp:
+    157-230) -- so the call stops after the outer row that passes it and
+    execute_in_batches runs a second round: rule 0's first 4,001 rows, then
+    all of rule 1, then rule 0's last 99 rows reach the interning.
+         O(z, x) <- A(y, x) DELTA JOIN B(y, z) FULL   (rule 0: 4,100 x 5,000)
+         O(z, x) <- A(y, x) DELTA JOIN C(y, z) FULL   (rule 1: 4,100 x 10)"""
+    A = b.rel_create(2, 1, 21)
+    B = b.rel_create(2, 1, 22)
+    C = b.rel_create(2, 1, 23)
+    O = b.rel_create(2, 1, 25)
+    x = np.arange(4100, dtype=np.uint64)
+    b.load_rows(A, np.stack([np.ones_like(x), x, 1000 + x], axis=1), P.FULL)
+    z = np.arange(5000, dtype=np.uint64)
+    b.load_rows(B, np.stack([np.ones_like(z), z, 20000 + z], axis=1), P.FULL)
+    zc = np.arange(9000000, 9000010, dtype=np.uint64)
+    b.load_rows(C, np.stack([np.ones_like(zc), zc, 30000 + np.arange(10, dtype=np.uint64)], axis=1), P.FULL)
+    s = b.scc_create(False, 1)
+    b.scc_add_relation(s, A, True)
+    b.scc_add_relation(s, B, False)
+    b.scc_add_relation(s, C, False)
+    b.scc_add_relation(s, O, False)
+    b.add_join(s, O, A, P.DELTA, B, P.FULL, [3, 1])
+    b.add_join(s, O, A, P.DELTA, C, P.FULL, [3, 1])
+    return O
+
+
 def cases():
     """(name, builder(rec) -> None, tc edges or None, rank counts, batch sizes)"""
     out = []
@@ -243,6 +272,7 @@ def cases():
                 [1]))
     for seed in range(6):
         out.append((f"random_program_s{seed}", lambda b, s=seed: P.random_program(b, s), None, [1, 3], [1, 3]))
+    out.append(("threshold_split", threshold_split, None, [1], [1]))
     return out
 
 
