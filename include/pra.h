@@ -57,6 +57,20 @@ typedef struct pra_scc pra_scc;
  * scratch; 0 = no cap): a call that would need more fails with PRA_E_BUDGET
  * and leaves the context unusable except for pra_ctx_destroy. */
 int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out);
+/* Several program instances in one context (no counterpart in the reference,
+ * which runs one program per MPI job): pra_ctx_set_group(ctx, g) makes the
+ * following pra_rel_create and pra_scc_create calls build instance g
+ * (0 <= g < PRA_MAX_GROUPS; instance 0 before any call).  Each instance is a
+ * separate LIE -- its own relations, SCCs, intern counters, dictionary and
+ * join-pair count; rules and dependencies connect only its own relations and
+ * SCCs.  With more than one instance, pra_execute runs every instance's LIE
+ * in lock step: the current SCCs of all instances iterate together, each
+ * phase of an iteration one batched launch over all of them, and each
+ * instance's results equal those of its own run alone. */
+#define PRA_MAX_GROUPS 65536
+int pra_ctx_set_group(pra_ctx* ctx, int group);
+/* join pairs of instance `group` so far */
+int pra_ctx_group_joined(pra_ctx* ctx, int group, uint64_t* out);
 /* device bytes the context holds now and at most so far (no counterpart in
  * the reference; what budget admission measures) */
 int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak);
@@ -142,6 +156,17 @@ int pra_scc_history(pra_scc* scc, uint64_t* out, uint64_t cap, uint64_t* n);
 /* LIE::execute (src/lie/lie.cpp:197-448): run every SCC created on the
  * context in dependency order, first runnable in creation order. */
 int pra_execute(pra_ctx* ctx);
+
+/* Stable partition of device rows by bucket = tuple_hash(row[0:jcc]) % k
+ * (k = the context's bucket count): the routing of input rows to their owner
+ * rank (buffer_data_to_hash_buffer_col, src/IO/parallel_io.cpp:239-297) and of
+ * the all-to-all tuple exchange (src/comm/all_to_allv_comm.cpp:64-142) as one
+ * device step.  rows: n rows of w u64 words; out_rows: n rows, bucket 0's
+ * first, each bucket's rows in input order; bucket_bounds: k + 1 u64 words,
+ * bucket b = rows [bounds[b], bounds[b+1]).  All three are device pointers; the
+ * work is ordered on the context's stream (pra_ctx_synchronize waits for it). */
+int pra_partition_rows(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t w, uint32_t jcc, uint64_t* out_rows,
+                       uint64_t* bucket_bounds);
 
 /* Batched execution (no counterpart in the reference, which runs one program
  * per MPI job): pra_execute on n independent contexts of the same device,

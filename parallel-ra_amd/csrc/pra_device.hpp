@@ -57,12 +57,12 @@ typedef uint32_t u32;
 #define PRA_ST_AGG 1ULL
 #define PRA_ST_INC 2ULL
 
+// summary word j of the newt scatter combines by AND (packed key AND, key
+// column ANDs) or by OR (everything else)
+__host__ __device__ inline bool pra_summary_is_and(int j) { return j == 1 || (j >= 24 && j < 40); }
+
 // FNV-1a-64 over the 8 little-endian bytes of each word, exactly the
 // reference's tuple_hash (src/hash/hash.h:11-30).
-__host__ // summary word j of the newt scatter combines by AND (packed key AND, key
-// column ANDs) or by OR (everything else)
-__device__ inline bool pra_summary_is_and(int j) { return j == 1 || (j >= 24 && j < 40); }
-
 #define PRA_FNV_OFFSET 14695981039346656037ULL
 
 // one u64 word (8 little-endian bytes) of FNV-1a-64

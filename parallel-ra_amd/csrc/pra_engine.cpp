@@ -12,6 +12,7 @@
 #include "../../include/pra.h"
 #include "pra_blockcache.hpp"
 #include "pra_kernels.hpp"
+#include "pra_partition.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -248,6 +249,7 @@ struct pra_scc;
 
 struct pra_rel {
     pra_ctx* ctx = nullptr;
+    int group = 0;
     int arity = 0, jcc = 0, canonical = 1, w = 0;
     u32 tag = 0;
     Rows ver[3];  // PRA_DELTA, PRA_FULL, PRA_NEWT
@@ -256,6 +258,14 @@ struct pra_rel {
     u64 mt_cap = 0, mt_entries = 0;
     int mt_kw = -1;
     bool mt_valid = false;
+    // dense membership (interned outputs with dictionary-ranked keys): bitmap
+    // over the key's rank index, built for dictionary generation bm_gen and
+    // kept current by the iterations that take the dense path; the
+    // per-iteration claim table (empty between iterations)
+    DBuf bm, dtab;
+    u64 bm_gen = ~0ULL, dtab_cap = 0;
+    int bm_kw = 0;
+    bool bm_valid = false;
     // join index (key -> first row, count) per version
     DBuf idx[2];
     u64 idx_cap[2] = {0, 0};
@@ -321,8 +331,24 @@ static const char* kNames[K_COUNT] = {"join_count", "join_expand", "copy_expand"
 
 } // namespace
 
+// One program instance (group) of a context.  A context holds one or more
+// independent programs whose SCCs run in lock step (pra_ctx_set_group); each
+// has its own intern counters (the reference keeps them per LIE,
+// src/lie/lie.h:36), value dictionary, join-pair count and iteration count.
+struct Group {
+    std::map<u32, std::unique_ptr<DBuf>> seq;  // per intern tag: nb counters
+    // order-preserving value dictionary for packed sort keys
+    DBuf dict;
+    u64 dict_cap = 0, dict_n = 0;
+    int dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
+    u64 dict_gen = 0;                        // dictionary builds so far
+    DBuf joined;                             // join pairs so far (one device word)
+    u64 iterations = 0;
+};
+
 struct pra_scc {
     pra_ctx* ctx = nullptr;
+    int group = 0;
     int fixpoint = 1, id = 0;
     std::vector<pra_rel*> rels;
     std::vector<int> status;
@@ -340,18 +366,19 @@ struct pra_ctx {
     std::vector<pra_rel*> rels;
     std::vector<pra_scc*> sccs;
     std::vector<std::pair<pra_scc*, pra_scc*>> deps;
-    std::map<u32, std::unique_ptr<DBuf>> seq;  // per intern tag: nb counters
+    std::vector<std::unique_ptr<Group>> groups;   // program instances; relations and SCCs name theirs
+    int cur_group = 0;                            // instance of the next pra_rel_create / pra_scc_create
     // scratch
     DBuf hist, offs, sort_tmp, flags_buf, stage, stage2, orand, tmp64a, tmp64b, tmp64c;
     DBuf cand, dest, slot_of, keep;
     DBuf ghist, gofs, lb_status, lb_ctr;
+    DBuf part_scratch;   // pra_partition_rows
     // order-preserving value dictionary for packed sort keys
-    DBuf dict, ktmp, idxa, idxb, posadj_dev, joined_dev, rep_words, rctl, ttab, kp, jobtab[2];
+    DBuf ktmp, idxa, idxb, posadj_dev, rep_words, rctl, ttab, kp, jobtab[4];
     int rep_sets = 0;              // summary replica sets in rep_words (one per output of a batched partition)
-    char* job_stage[2] = {};       // pinned staging of batched kernels' job tables (expansions, partitions)
-    size_t job_stage_cap[2] = {};
-    u64 dict_cap = 0, dict_n = 0;
-    int dict_state = 0, dict_rebuilds = 0;   // state: 0 stale, 1 valid, 2 disabled
+    char* job_stage[4] = {};       // pinned staging of batched kernels' job tables (expansions, partitions,
+    size_t job_stage_cap[4] = {};  // dense claims, plans)
+    bool dense_on = true;        // dense membership for outputs that allow it
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
     u32 sort_epoch = 0;
     u64 dedup_epoch = 0;
@@ -362,7 +389,6 @@ struct pra_ctx {
     size_t mailbox_cap = 0;
     hipEvent_t pub_ev = nullptr;
     size_t host_pos_cap = 0;
-    u64 iterations = 0;
     std::string err;
     // profiling
     u32 prof_mask = 0;           // bit k: time kernel class k
@@ -382,10 +408,11 @@ struct pra_ctx {
         resolve_prof();
         // stream-ordered buffers are returned before their stream goes away
         if (st) (void)hipStreamSynchronize(st);
-        seq.clear();
+        groups.clear();
         for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
-                        &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &dict, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &joined_dev, &rep_words, &rctl, &ttab, &kp, &jobtab[0], &jobtab[1]})
+                        &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &ktmp,
+                        &idxa, &idxb, &posadj_dev, &rep_words, &rctl, &ttab, &kp, &jobtab[0], &jobtab[1], &jobtab[2], &jobtab[3],
+                        &part_scratch})
             b->release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -509,6 +536,13 @@ struct pra_ctx {
 };
 
 namespace {
+
+// instance g of a context (created on first use)
+static Group& grp(pra_ctx* c, int g)
+{
+    while ((int)c->groups.size() <= g) c->groups.emplace_back(new Group());
+    return *c->groups[(size_t)g];
+}
 
 struct Prof {
     pra_ctx* c;
@@ -919,18 +953,22 @@ static void unique_rows(pra_ctx* c, Rows& r, int w)
     r.n = h[0];
 }
 
-// (Re)build the value dictionary from the key columns of every relation.
-static void build_dictionary(pra_ctx* c)
+// (Re)build instance gi's value dictionary from the key columns of every
+// relation of that instance.
+static void build_dictionary(pra_ctx* c, int gi)
 {
+    Group& G = grp(c, gi);
     u64 total = 0;
     for (pra_rel* R : c->rels)
-        for (int v : {PRA_FULL, PRA_DELTA}) total += R->ver[v].n * (u64)R->arity;
+        if (R->group == gi)
+            for (int v : {PRA_FULL, PRA_DELTA}) total += R->ver[v].n * (u64)R->arity;
     if (total == 0) return;
     Rows V;
     V.buf.reserve(total * 8, c->st);
     u64 off = 0;
     for (pra_rel* R : c->rels)
         for (int v : {PRA_FULL, PRA_DELTA}) {
+            if (R->group != gi) continue;
             Rows& r = R->ver[v];
             pra_launch_collect_cols(c->st, r.d(), r.n, R->w, R->arity, V.d() + off);
             off += r.n * (u64)R->arity;
@@ -940,14 +978,15 @@ static void build_dictionary(pra_ctx* c)
     sort_rows(c, V, 1, 0, 1, false);
     unique_rows(c, V, 1);
     u64 cap = pow2_at_least(std::max<u64>(64, 2 * V.n));
-    c->dict.reserve(cap * 16, c->st);
-    pra_launch_fill_u64(c->st, c->dict.as<u64>(), cap * 2, 0);
-    pra_launch_dict_insert(c->st, V.d(), V.n, c->dict.as<u64>(), cap, c->dflags());
+    G.dict.reserve(cap * 16, c->st);
+    pra_launch_fill_u64(c->st, G.dict.as<u64>(), cap * 2, 0);
+    pra_launch_dict_insert(c->st, V.d(), V.n, G.dict.as<u64>(), cap, c->dflags());
     check_launch("dict_insert");
     sync_check(c);
-    c->dict_cap = cap;
-    c->dict_n = V.n;
-    c->dict_state = 1;
+    G.dict_cap = cap;
+    G.dict_n = V.n;
+    G.dict_state = 1;
+    G.dict_gen++;
 }
 
 static void ensure_sorted(pra_ctx* c, pra_rel* R, int v)
@@ -1060,15 +1099,61 @@ static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
     R->mt_cap = cap;
 }
 
-static DBuf& seq_counters(pra_ctx* c, u32 tag)
+// Dense membership of interned output R keyed on kw columns: the bitmap of
+// full u delta for the current dictionary (rebuilt when the dictionary or the
+// relation changed since it was built) and a claim table of at least 2C
+// slots.  False when the key does not allow it: more than 4 columns, a rank
+// space over 2^35 bits, or a value of full u delta without a rank.
+static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C)
 {
-    auto it = c->seq.find(tag);
-    if (it != c->seq.end()) return *it->second;
+    Group& G = grp(c, R->group);
+    if (!c->dense_on || G.dict_state != 1 || kw < 1 || kw > 4 || G.dict_n == 0) return false;
+    unsigned __int128 bits = 1;
+    for (int i = 0; i < kw; i++) {
+        bits *= G.dict_n;
+        if (bits > ((unsigned __int128)1 << 35)) return false;
+    }
+    if (!R->bm_valid || R->bm_gen != G.dict_gen || R->bm_kw != kw) {
+        const u64 words = ((u64)bits + 63) / 64;
+        Prof p(c, K_TABLE_BUILD, 8.0 * words + 8.0 * kw * (R->ver[PRA_FULL].n + R->ver[PRA_DELTA].n));
+        R->bm.reserve(words * 8, c->st);
+        pra_launch_fill_u64(c->st, R->bm.as<u64>(), words, 0);
+        c->orand.reserve(32 * 8, c->st);
+        pra_launch_fill_u64(c->st, c->orand.as<u64>(), 1, 0);
+        for (int v : {PRA_FULL, PRA_DELTA}) {
+            Rows& r = R->ver[v];
+            pra_launch_dense_build(c->st, r.d(), r.n, R->w, kw, G.dict.as<u64>(), G.dict_cap, G.dict_n,
+                                   R->bm.as<u32>(), c->orand.as<u64>());
+            check_launch("dense_build");
+        }
+        if (readback(c, c->orand.p, 1)[0]) {
+            R->bm_valid = false;
+            return false;
+        }
+        R->bm_valid = true;
+        R->bm_gen = G.dict_gen;
+        R->bm_kw = kw;
+    }
+    const u64 cap = pow2_at_least(std::max<u64>(1024, 2 * C));
+    if (cap > R->dtab_cap) {
+        R->dtab.reserve(cap * 16, c->st);
+        pra_launch_dense_table_clear(c->st, R->dtab.as<u64>(), cap);
+        check_launch("dense_table_clear");
+        R->dtab_cap = cap;
+    }
+    return true;
+}
+
+static DBuf& seq_counters(pra_ctx* c, int gi, u32 tag)
+{
+    Group& G = grp(c, gi);
+    auto it = G.seq.find(tag);
+    if (it != G.seq.end()) return *it->second;
     std::unique_ptr<DBuf> b(new DBuf());
     b->reserve((size_t)c->nb * 8, c->st);
     pra_launch_fill_u64(c->st, b->as<u64>(), (u64)c->nb, 0);
     DBuf& ref = *b;
-    c->seq[tag] = std::move(b);
+    G.seq[tag] = std::move(b);
     return ref;
 }
 
@@ -1120,6 +1205,39 @@ static void merge_into_full(pra_ctx* c, pra_rel* R)
     R->bnd_valid[PRA_FULL] = true;
 }
 
+// The SCCs of one lock-step iteration seen as one: the rules of every member
+// in member order, then rule order, and every member's relations.  Members
+// come from different instances of the context and share no relation.
+struct RAView {
+    std::vector<RADesc*> v;
+    RADesc& operator[](size_t i) const { return *v[i]; }
+    size_t size() const { return v.size(); }
+    struct It {
+        RADesc* const* p;
+        RADesc& operator*() const { return **p; }
+        It& operator++()
+        {
+            ++p;
+            return *this;
+        }
+        bool operator!=(const It& o) const { return p != o.p; }
+    };
+    It begin() const { return It{v.data()}; }
+    It end() const { return It{v.data() + v.size()}; }
+};
+struct Merged {
+    pra_ctx* ctx = nullptr;
+    RAView ras;
+    std::vector<pra_rel*> rels;
+    explicit Merged(pra_ctx* c, const std::vector<pra_scc*>& members) : ctx(c)
+    {
+        for (pra_scc* m : members) {
+            for (auto& ra : m->ras) ras.v.push_back(&ra);
+            rels.insert(rels.end(), m->rels.begin(), m->rels.end());
+        }
+    }
+};
+
 // One threshold round of an output relation: the rules emitting (RA index
 // order) and, per listed rule, the outer rows [lo, hi) of every source.
 struct RoundSpec {
@@ -1136,7 +1254,7 @@ static u64 read_word(pra_ctx* c, const u64* p) { return readback(c, p, 1)[0]; }
 // stops (src/RA/parallel_join.cpp:157-230): its outputs are deduplicated per
 // call, and the call returns after the first row through which it emitted
 // more than the threshold.  Returns that row relative to start, or end-start.
-static u64 find_split(pra_ctx* c, pra_scc* S, int r, u64 start, u64 end, int cw)
+static u64 find_split(pra_ctx* c, Merged* S, int r, u64 start, u64 end, int cw)
 {
     RADesc& ra = S->ras[r];
     const int nb = c->nb;
@@ -1204,7 +1322,7 @@ static u64 find_split(pra_ctx* c, pra_scc* S, int r, u64 start, u64 end, int cw)
 // Interning and first-arrival dedup run per round, rounds in order.  Returns
 // the new tuples of all rounds.
 template <class F>
-static u64 run_rounds(pra_scc* S, pra_rel* R, F&& run_round, u64* st2o_dev)
+static u64 run_rounds(Merged* S, pra_rel* R, F&& run_round, u64* st2o_dev)
 {
     pra_ctx* c = S->ctx;
     const int nb = c->nb;
@@ -1307,10 +1425,14 @@ static u64 run_rounds(pra_scc* S, pra_rel* R, F&& run_round, u64* st2o_dev)
 
 static bool is_intern_type(int type) { return type != RA_ACOPY; }
 
-// One semi-naive iteration of an SCC (src/RAM/RA_tasks.cpp:1121-1269).
-static void run_iteration(pra_scc* S)
+// One semi-naive iteration (src/RAM/RA_tasks.cpp:1121-1269) of one SCC, or
+// of the current SCCs of several instances in lock step: their rules, outputs
+// and per-phase launches are taken together, while dedup, intern ids and
+// dictionaries stay per instance.
+static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
 {
-    pra_ctx* c = S->ctx;
+    Merged merged(c, members);
+    Merged* S = &merged;
     const int nb = c->nb;
     const int nra = (int)S->ras.size();
 
@@ -1438,7 +1560,7 @@ static void run_iteration(pra_scc* S)
     }
     for (size_t i = 0; i < no; i++)
         for (size_t j = i + 1; j < no; j++)
-            if (out_intern[i] && out_intern[j] && outs[i]->tag == outs[j]->tag)
+            if (out_intern[i] && out_intern[j] && outs[i]->tag == outs[j]->tag && outs[i]->group == outs[j]->group)
                 throw PraError(PRA_E_UNSUPPORTED, "two interned relations of one SCC share an intern tag");
 
     // per output relation: [0] new tuples, [1] packed-key OR, [2] AND, [3] dictionary miss,
@@ -1447,21 +1569,26 @@ static void run_iteration(pra_scc* S)
     const size_t SW = 40;
     c->stage2.reserve(std::max<size_t>(1, no) * SW * 8, c->st);
     c->posadj_dev.reserve((size_t)std::max(1, nra) * nb * 8, c->st);
-    if (!c->joined_dev.p) {
-        c->joined_dev.reserve(8, c->st);
-        pra_launch_fill_u64(c->st, c->joined_dev.as<u64>(), 1, 0);
+    for (size_t oi = 0; oi < no; oi++) {
+        Group& G = grp(c, outs[oi]->group);
+        if (!G.joined.p) {
+            G.joined.reserve(8, c->st);
+            pra_launch_fill_u64(c->st, G.joined.as<u64>(), 1, 0);
+        }
     }
-    // packed sort keys need the value dictionary (rebuilt after a miss, given up
-    // after repeated misses)
-    bool any_intern = false;
-    for (size_t oi = 0; oi < no; oi++) any_intern = any_intern || (out_intern[oi] && ub[oi]);
-    if (any_intern && c->dict_state == 0) build_dictionary(c);
+    // packed sort keys need the instance's value dictionary (rebuilt after a
+    // miss, given up after repeated misses)
+    for (size_t oi = 0; oi < no; oi++)
+        if (out_intern[oi] && ub[oi] && grp(c, outs[oi]->group).dict_state == 0)
+            build_dictionary(c, outs[oi]->group);
     std::vector<char> packed(no, 0);
     std::vector<u64> pk_mul(no, 0);
     std::vector<int> pk_ls(no, 0);
-    for (size_t oi = 0; oi < no; oi++)
-        packed[oi] = c->dict_state == 1 && out_intern[oi] && ub[oi] &&
-                     packed_layout((u64)nb, c->dict_n, outs[oi]->arity, &pk_mul[oi], &pk_ls[oi]);
+    for (size_t oi = 0; oi < no; oi++) {
+        const Group& G = grp(c, outs[oi]->group);
+        packed[oi] = G.dict_state == 1 && out_intern[oi] && ub[oi] &&
+                     packed_layout((u64)nb, G.dict_n, outs[oi]->arity, &pk_mul[oi], &pk_ls[oi]);
+    }
 
     // 3+4. per output: arrival plan, emit candidates, deduplicate, partition
     //      and intern, summaries for the sort
@@ -1494,6 +1621,10 @@ static void run_iteration(pra_scc* S)
     std::vector<OutRegion> reg(no);
     std::vector<ClaimJob> claim_jobs(no);
     std::vector<DecideJob> decide_jobs(no);
+    // dense membership: taken per output, switched off for the rest of the
+    // iteration by a dictionary miss; the dense claims of a batched iteration
+    std::vector<char> use_dense(no, 0), dense_off(no, 0);
+    std::vector<std::pair<DenseArgs, size_t>> dense_jobs;
     // phase 0: one output start to end; 1: up to its candidates, 2: from its
     // kept counts on (the claims and decisions in between batched over the
     // outputs, each in its own region of the candidate buffers)
@@ -1509,7 +1640,15 @@ static void run_iteration(pra_scc* S)
         Rows& NT = R->ver[PRA_NEWT];
         u64* st2o = c->stage2.as<u64>() + oi * SW;
         u64* ctl = st2o + 4;
-        ensure_member_table(c, R, kw, C);
+        // dense membership unless the output was redone the general way in
+        // this iteration, or this is a threshold round
+        const bool dn = phase != 2 ? (!rs && !dense_off[oi] && packed[oi] && ensure_dense(c, R, kw, C))
+                                   : use_dense[oi] != 0;
+        use_dense[oi] = dn;
+        if (dn)
+            R->mt_valid = false;   // the general table is not kept current meanwhile
+        else
+            R->bm_valid = false, ensure_member_table(c, R, kw, C);
         c->cand.reserve(C * cw * 8, c->st);
         c->dest.reserve(C * 4, c->st);
         c->slot_of.reserve(C * 8, c->st);
@@ -1531,7 +1670,7 @@ static void run_iteration(pra_scc* S)
                 pa.srcoff = c->stage.as<u64>();
                 pa.posadj = c->posadj_dev.as<u64>();
                 pa.ctl = ctl;
-                pa.joined = c->joined_dev.as<u64>();
+                pa.joined = grp(c, R->group).joined.as<u64>();
                 pa.flags = c->dflags();
                 pa.cap = C;
                 pa.thresh = c->threshold;
@@ -1636,6 +1775,45 @@ static void run_iteration(pra_scc* S)
                     check_launch("copy_expand");
                 }
             }
+            if (dn) {
+                DenseArgs da;
+                memset(&da, 0, sizeof(da));
+                da.cand = cand0;
+                da.ctl = ctl;
+                da.slot_of = slot0;
+                da.keep = keep0;
+                da.dest = dest0;
+                da.hist = hist0;
+                da.offs1 = nblocks == 1 ? offs0 : nullptr;
+                Group& G = grp(c, R->group);
+                da.dict = G.dict.as<u64>();
+                da.dict_cap = G.dict_cap;
+                da.D = G.dict_n;
+                da.bitmap = R->bm.as<u32>();
+                da.table = R->dtab.as<u64>();
+                da.cap = R->dtab_cap;
+                da.nblocks = nblocks;
+                da.nb = (u32)nb;
+                da.cw = cw;
+                da.kw = kw;
+                if (phase == 1) {
+                    // launched after the batched expansions
+                    dense_jobs.emplace_back(da, oi);
+                    return;
+                }
+                {
+                    ProfPerCand p(c, K_DEDUP_CLAIM, 8.0 * (kw + 1), oi);
+                    pra_launch_dense_claim(c->st, da, C, c->dflags());
+                    check_launch("dense_claim");
+                }
+                {
+                    ProfPerCand p(c, K_DEDUP_VERIFY, 16.0, oi);
+                    pra_launch_dense_decide(c->st, da, c->dflags());
+                    check_launch("dense_decide");
+                }
+            }
+        }
+        if (phase != 2 && !dn) {
             u64* tab = R->mt.as<u64>();
             const u64 epoch = ++c->dedup_epoch;
             if (phase == 1) {
@@ -1662,7 +1840,7 @@ static void run_iteration(pra_scc* S)
         }
         u32* total_dev = offs0 + (size_t)nb * nblocks;
         if (nblocks > 1) scan_u32(c, hist0, offs0, (u64)nb * nblocks, total_dev);
-        DBuf* sq = intern ? &seq_counters(c, R->tag) : nullptr;
+        DBuf* sq = intern ? &seq_counters(c, R->group, R->tag) : nullptr;
         R->kgofs.reserve(PRA_RADIX * 4, c->st);
         NewtFoldArgs f;   // bin_cur: set with the scatter's bins
         memset(&f, 0, sizeof(f));
@@ -1718,10 +1896,11 @@ static void run_iteration(pra_scc* S)
                 a.bin_cur = R->bin_cur.as<u32>();
                 f.bin_cur = a.bin_cur;
                 R->nkeys.reserve(C * 8, c->st);
-                a.dict = c->dict.as<u64>();
-                a.dict_cap = c->dict_cap;
+                Group& G = grp(c, R->group);
+                a.dict = G.dict.as<u64>();
+                a.dict_cap = G.dict_cap;
                 a.keys = R->nkeys.as<u64>();
-                a.rank_radix = c->dict_n;
+                a.rank_radix = G.dict_n;
                 a.key_lshift = pk_ls[oi];
             }
             a.fold_here = fold_here ? 1 : 0;
@@ -1760,7 +1939,7 @@ static void run_iteration(pra_scc* S)
         pl.srcoff = c->stage.as<u64>();
         pl.posadj = c->posadj_dev.as<u64>();
         pl.stage2 = c->stage2.as<u64>();
-        pl.joined = c->joined_dev.as<u64>();
+        for (size_t oi = 0; oi < no; oi++) pl.joined[oi] = grp(c, outs[oi]->group).joined.as<u64>();
         pl.thresh = c->threshold;
         pl.nb = nb;
         pl.nra = nra;
@@ -1842,6 +2021,59 @@ static void run_iteration(pra_scc* S)
                 }
             }
         }
+        // the dense claims and decisions: one table per key width through
+        // pinned staging, one launch each
+        if (!dense_jobs.empty()) {
+            auto tbytes = [](size_t n) { return (n * sizeof(DenseArgs) + 2 * (n + 1) * 4 + 255) / 256 * 256; };
+            size_t total = 0;
+            for (int kw = 1; kw <= 4; kw++) {
+                size_t n = 0;
+                for (auto& dj : dense_jobs) n += dj.first.kw == kw;
+                if (n) total += tbytes(n);
+            }
+            char* hs = c->job_stage_buf(2, total);
+            c->jobtab[2].reserve(total, c->st);
+            size_t at = 0;
+            for (int kw = 1; kw <= 4; kw++) {
+                std::vector<size_t> members;
+                for (size_t k = 0; k < dense_jobs.size(); k++)
+                    if (dense_jobs[k].first.kw == kw) members.push_back(k);
+                if (members.empty()) continue;
+                const size_t n = members.size();
+                DenseArgs* J = (DenseArgs*)(hs + at);
+                u32* cb = (u32*)(J + n);
+                u32* db = cb + n + 1;
+                cb[0] = db[0] = 0;
+                for (size_t k = 0; k < n; k++) {
+                    const DenseArgs& a = dense_jobs[members[k]].first;
+                    const size_t oi = dense_jobs[members[k]].second;
+                    J[k] = a;
+                    cb[k + 1] = cb[k] + (u32)((cap[oi] + PRA_BLOCK - 1) / PRA_BLOCK);
+                    db[k + 1] = db[k] + a.nblocks;
+                }
+                HIPCHK(hipMemcpyAsync(c->jobtab[2].as<char>() + at, hs + at, tbytes(n), hipMemcpyHostToDevice, c->st));
+                const size_t before = c->pending.size();
+                {
+                    Prof p(c, K_DEDUP_CLAIM, 0.0);
+                    pra_launch_dense_many(c->st, (const DenseArgs*)(c->jobtab[2].as<char>() + at), (int)n, kw, cb[n], 0,
+                                          c->dflags());
+                    check_launch("dense_claim_many");
+                }
+                if (c->pending.size() == before + 1)
+                    for (size_t k : members)
+                        c->per_candidate.emplace_back(before, 8.0 * (kw + 1), dense_jobs[k].second);
+                const size_t before2 = c->pending.size();
+                {
+                    Prof p(c, K_DEDUP_VERIFY, 0.0);
+                    pra_launch_dense_many(c->st, (const DenseArgs*)(c->jobtab[2].as<char>() + at), (int)n, kw, 0, db[n],
+                                          c->dflags());
+                    check_launch("dense_decide_many");
+                }
+                if (c->pending.size() == before2 + 1)
+                    for (size_t k : members) c->per_candidate.emplace_back(before2, 16.0, dense_jobs[k].second);
+                at += tbytes(n);
+            }
+        }
         for (int cls = 1; cls <= 7; cls++) {
             ClaimBatch cb;
             DecideBatch db;
@@ -1875,7 +2107,7 @@ static void run_iteration(pra_scc* S)
                 members.clear();
             };
             for (size_t oi = 0; oi < no; oi++) {
-                if (!cap[oi] || pra_dedup_class(claim_jobs[oi].kw) != cls) continue;
+                if (!cap[oi] || use_dense[oi] || pra_dedup_class(claim_jobs[oi].kw) != cls) continue;
                 const int k = cb.njobs;
                 cb.job[k] = claim_jobs[oi];
                 cb.block_base[k + 1] = cb.block_base[k] + (u32)((cap[oi] + PRA_BLOCK - 1) / PRA_BLOCK);
@@ -1946,7 +2178,10 @@ static void run_iteration(pra_scc* S)
     // deltas of relations whose full version no rule of the SCC reads go to
     // their full logs while the host waits for the first readback
     std::vector<char> early(S->rels.size(), 0);
-    for (int pass = 0; pass < 2; pass++) {
+    // pass 1 redoes outputs whose candidates overflowed their capacity (now
+    // sized exactly) or whose dense claims met a value without a dictionary
+    // rank (now the general way); pass 2 the few that needed both
+    for (int pass = 0; pass < 3; pass++) {
         u64* h = publish_begin(c, c->stage2.as<u64>(), no * SW, pass == 0 && have_last_fold ? &last_fold : nullptr);
         if (pass == 0)
             for (size_t i = 0; i < S->rels.size(); i++) {
@@ -1960,11 +2195,18 @@ static void run_iteration(pra_scc* S)
         for (size_t oi = 0; oi < no; oi++) {
             if (!cap[oi]) continue;
             std::copy(h + oi * SW, h + (oi + 1) * SW, st2.begin() + oi * SW);
-            if (st2[oi * SW + 5] == 1) redo.push_back(oi);
+            if (st2[oi * SW + 5] == 1 || st2[oi * SW + 5] == 3) redo.push_back(oi);
         }
         if (redo.empty()) break;
-        if (pass == 1) throw PraError(PRA_E_HIP, "candidate capacity overflow after resizing");
+        if (pass == 2) throw PraError(PRA_E_HIP, "candidate capacity overflow after resizing");
         for (size_t oi : redo) {
+            if (st2[oi * SW + 5] == 3) {
+                // claims that stopped at the miss left slots behind
+                pra_rel* R = outs[oi];
+                pra_launch_dense_table_clear(c->st, R->dtab.as<u64>(), R->dtab_cap);
+                check_launch("dense_table_clear");
+                dense_off[oi] = 1;
+            }
             cap[oi] = st2[oi * SW + 4];
             run_output(oi, nullptr);
         }
@@ -2040,8 +2282,8 @@ static void run_iteration(pra_scc* S)
         // packed mode collects no column masks (a dictionary miss sorts every
         // digit), and a round's masks cover that round only
         const u64* masks = (packed[oi] || in_rounds[oi]) ? nullptr : &st2[oi * SW + 8];
-        if (packed[oi] && !in_rounds[oi] && c->dict_state == 1)
-            c->dict_state = ++c->dict_rebuilds > 3 ? 2 : 0;
+        Group& G = grp(c, R->group);
+        if (packed[oi] && !in_rounds[oi] && G.dict_state == 1) G.dict_state = ++G.dict_rebuilds > 3 ? 2 : 0;
         SmallSortJob job;
         bnd_ok[oi] = sort_rows_masked(c, NT, R->w, R->jcc, out_intern[oi] ? R->arity : R->w, masks, bnd, true, &job,
                                       &R->ver[PRA_DELTA].buf);
@@ -2067,7 +2309,7 @@ static void run_iteration(pra_scc* S)
         auto it = std::find(outs.begin(), outs.end(), R);
         if (it != outs.end() && bnd_ok[it - outs.begin()]) R->bnd_valid[PRA_DELTA] = true;
     }
-    c->iterations++;
+    for (pra_scc* m : members) grp(c, m->group).iterations++;
 }
 
 static u64 delta_sum(pra_scc* S)
@@ -2144,14 +2386,15 @@ static u64 scc_step(pra_scc* S)
     pra_ctx* c = S->ctx;
     if (!S->running) throw PraError(PRA_E_STATE, "SCC not started");
     u32 iters = S->fixpoint ? c->batch_size : 1;
-    for (u32 b = 0; b < iters; b++) run_iteration(S);
+    for (u32 b = 0; b < iters; b++) run_iteration(c, {S});
     u64 d = delta_sum(S);
     S->hist.push_back(d);
     return d;
 }
 
-// SCC epilogue: delta folded into full for every relation of the SCC
-static void scc_end(pra_scc* S)
+// SCC epilogue: delta folded into full for every relation of the SCC (then
+// the error flags checked, unless the caller checks them for several SCCs)
+static void scc_end(pra_scc* S, bool check = true)
 {
     pra_ctx* c = S->ctx;
     if (!S->running) throw PraError(PRA_E_STATE, "SCC not started");
@@ -2159,7 +2402,7 @@ static void scc_end(pra_scc* S)
         delta_to_full(c, R);
         R->scc_changing = R->merge_full = R->full_read = false;
     }
-    sync_check(c);
+    if (check) sync_check(c);
     S->running = false;
     S->done = true;
 }
@@ -2175,16 +2418,56 @@ static void scc_run(pra_scc* S)
     scc_end(S);
 }
 
-static pra_scc* one_runnable(pra_ctx* c)
+// first runnable SCC (of instance g, or of any instance when g < 0) in
+// creation order (src/lie/lie.cpp:27-60)
+static pra_scc* one_runnable(pra_ctx* c, int g = -1)
 {
     for (pra_scc* s : c->sccs) {
-        if (s->done) continue;
+        if (s->done || (g >= 0 && s->group != g)) continue;
         bool blocked = false;
         for (auto& d : c->deps)
             if (d.second == s && !d.first->done) blocked = true;
         if (!blocked) return s;
     }
     return nullptr;
+}
+
+// Every instance's LIE::execute (src/lie/lie.cpp:197-448) in lock step: at
+// each step every instance's first runnable SCC is taken, and those of the
+// same kind (fixed point, or once) as the first one run together -- one
+// merged iteration after another, each instance's SCC leaving at the end of
+// the batch that brings its delta to zero, exactly where its own LIE would
+// stop.  The others wait for the next step.
+static void execute_lockstep(pra_ctx* c)
+{
+    for (;;) {
+        std::vector<pra_scc*> members;
+        int kind = -1;
+        for (size_t g = 0; g < c->groups.size(); g++) {
+            pra_scc* s = one_runnable(c, (int)g);
+            if (!s) continue;
+            if (kind < 0) kind = s->fixpoint;
+            if (s->fixpoint == kind) members.push_back(s);
+        }
+        if (members.empty()) break;
+        for (pra_scc* s : members) scc_begin(s, true);
+        std::vector<pra_scc*> active = members;
+        while (!active.empty()) {
+            const u32 iters = kind ? c->batch_size : 1;
+            for (u32 b = 0; b < iters; b++) run_iteration(c, active);
+            std::vector<pra_scc*> next;
+            for (pra_scc* s : active) {
+                const u64 d = delta_sum(s);
+                s->hist.push_back(d);
+                if (!kind || d == 0)
+                    scc_end(s, false);
+                else
+                    next.push_back(s);
+            }
+            active.swap(next);
+        }
+        sync_check(c);
+    }
 }
 
 static int fail(pra_ctx* c, int code, const std::string& m)
@@ -2242,6 +2525,9 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->device = device;
         c->nb = nbuckets;
         c->budget = hbm_budget;
+        grp(c.get(), 0);
+        const char* dense_env = getenv("PRA_DENSE_MEMBERSHIP");   // "0": the general hash table for every output
+        c->dense_on = !(dense_env && dense_env[0] == '0');
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
@@ -2289,14 +2575,19 @@ int pra_ctx_joined_tuples(pra_ctx* ctx, uint64_t* out)
 {
     if (!ctx || !out) return PRA_E_ARG;
     API_BEGIN(ctx)
-    *out = ctx->joined_dev.p ? readback(ctx, ctx->joined_dev.p, 1)[0] : 0;
+    u64 t = 0;
+    for (auto& g : ctx->groups)
+        if (g->joined.p) t += readback(ctx, g->joined.p, 1)[0];
+    *out = t;
     API_END
 }
 
 int pra_ctx_iterations(pra_ctx* ctx, uint64_t* out)
 {
     if (!ctx || !out) return PRA_E_ARG;
-    *out = ctx->iterations;
+    u64 t = 0;
+    for (auto& g : ctx->groups) t += g->iterations;
+    *out = t;
     return PRA_OK;
 }
 
@@ -2320,6 +2611,48 @@ int pra_ctx_kernel_stat(pra_ctx* ctx, int idx, const char** name, uint64_t* laun
     if (ms) *ms = ctx->kst[idx].ms;
     if (bytes) *bytes = ctx->kst[idx].bytes;
     return PRA_OK;
+}
+
+static int partition_args(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t w, uint32_t jcc,
+                          uint64_t* out_rows, uint64_t* bucket_bounds)
+{
+    if (!ctx) return PRA_E_ARG;
+    if (w < 1 || w > PRA_MAXW || jcc > w) return fail(ctx, PRA_E_ARG, "row width or join-column count out of range");
+    if (n >> 32) return fail(ctx, PRA_E_ARG, "more than 2^32 - 1 rows");
+    if (!bucket_bounds || (n && (!rows || !out_rows))) return fail(ctx, PRA_E_ARG, "null buffer");
+    return PRA_OK;
+}
+
+int pra_partition_rows(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t w, uint32_t jcc, uint64_t* out_rows,
+                       uint64_t* bucket_bounds)
+{
+    const int rc = partition_args(ctx, rows, n, w, jcc, out_rows, bucket_bounds);
+    if (rc != PRA_OK) return rc;
+    API_BEGIN(ctx)
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->part_scratch.reserve(pra_partition_scratch_words(n, (u32)ctx->nb) * 4 + 256, ctx->st);
+    pra_launch_partition(ctx->st, rows, n, (int)w, (int)jcc, (u32)ctx->nb, out_rows, bucket_bounds,
+                         ctx->part_scratch.as<u32>());
+    check_launch("partition");
+    API_END
+}
+
+int pra_ctx_set_group(pra_ctx* ctx, int group)
+{
+    if (!ctx || group < 0 || group >= PRA_MAX_GROUPS) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    grp(ctx, group);
+    ctx->cur_group = group;
+    API_END
+}
+
+int pra_ctx_group_joined(pra_ctx* ctx, int group, uint64_t* out)
+{
+    if (!ctx || !out || group < 0 || group >= (int)ctx->groups.size()) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    Group& G = *ctx->groups[(size_t)group];
+    *out = G.joined.p ? readback(ctx, G.joined.p, 1)[0] : 0;
+    API_END
 }
 
 int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak)
@@ -2353,6 +2686,7 @@ int pra_rel_create(pra_ctx* ctx, uint32_t arity, uint32_t jcc, uint32_t tag, int
     r->tag = tag;
     r->canonical = canonical;
     r->w = (int)arity + 1;
+    r->group = ctx->cur_group;
     ctx->rels.push_back(r);
     *out = r;
     return PRA_OK;
@@ -2384,6 +2718,7 @@ int pra_rel_load_rows(pra_rel* rel, const uint64_t* rows, uint64_t nrows, int ve
         HIPCHK(hipStreamSynchronize(c->st));
         rel->touched(version);
         rel->mt_valid = false;
+        rel->bm_valid = false;
     }
     API_END
 }
@@ -2489,6 +2824,7 @@ int pra_scc_create(pra_ctx* ctx, int fixpoint, int id, pra_scc** out)
     s->ctx = ctx;
     s->fixpoint = fixpoint != 0;
     s->id = id;
+    s->group = ctx->cur_group;
     ctx->sccs.push_back(s);
     *out = s;
     return PRA_OK;
@@ -2510,16 +2846,18 @@ int pra_scc_destroy(pra_scc* scc)
 
 int pra_scc_add_relation(pra_scc* scc, pra_rel* rel, int move_full_to_delta)
 {
-    if (!scc || !rel || rel->ctx != scc->ctx) return PRA_E_ARG;
+    if (!scc || !rel || rel->ctx != scc->ctx || rel->group != scc->group) return PRA_E_ARG;
     if (scc->rels.size() >= 64) return fail(scc->ctx, PRA_E_ARG, "at most 64 relations per SCC");
     scc->rels.push_back(rel);
     scc->status.push_back(move_full_to_delta != 0);
     return PRA_OK;
 }
 
+// the relations belong to the SCC's context and instance
 static bool same_ctx(pra_scc* s, pra_rel* a, pra_rel* b = nullptr, pra_rel* d = nullptr)
 {
-    return a && a->ctx == s->ctx && (!b || b->ctx == s->ctx) && (!d || d->ctx == s->ctx);
+    auto ok = [&](pra_rel* r) { return r->ctx == s->ctx && r->group == s->group; };
+    return a && ok(a) && (!b || ok(b)) && (!d || ok(d));
 }
 
 int pra_scc_add_join(pra_scc* scc, pra_rel* out, pra_rel* in0, int ver0, pra_rel* in1, int ver1, const int* proj,
@@ -2586,7 +2924,7 @@ int pra_scc_add_copy_filter(pra_scc* scc, pra_rel* out, pra_rel* in, int ver, co
 
 int pra_scc_add_dependency(pra_scc* before, pra_scc* after)
 {
-    if (!before || !after || before->ctx != after->ctx) return PRA_E_ARG;
+    if (!before || !after || before->ctx != after->ctx || before->group != after->group) return PRA_E_ARG;
     before->ctx->deps.push_back({before, after});
     return PRA_OK;
 }
@@ -2645,8 +2983,12 @@ int pra_execute(pra_ctx* ctx)
     if (!ctx) return PRA_E_ARG;
     API_BEGIN(ctx)
     HIPCHK(hipSetDevice(ctx->device));
-    pra_scc* s;
-    while ((s = one_runnable(ctx)) != nullptr) scc_run(s);
+    if (ctx->groups.size() > 1) {
+        execute_lockstep(ctx);
+    } else {
+        pra_scc* s;
+        while ((s = one_runnable(ctx)) != nullptr) scc_run(s);
+    }
     API_END
 }
 

@@ -1093,7 +1093,7 @@ __global__ void k_plan_all(PlanAllArgs a)
     u64* ctl = a.stage2 + (u64)oi * a.sw + 4;
     ctl[0] = tot;
     ctl[1] = rounds ? 2 : tot > a.cap[oi] ? 1 : 0;
-    if ((rounds || tot <= a.cap[oi]) && jtot) atomicAdd((unsigned long long*)a.joined, (unsigned long long)jtot);
+    if ((rounds || tot <= a.cap[oi]) && jtot) atomicAdd((unsigned long long*)a.joined[oi], (unsigned long long)jtot);
 }
 
 // Split of one threshold round: the first outer row i (of n, offsets off
@@ -1123,79 +1123,13 @@ __global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host
 // ---------------------------------------------------- dedup / newt ---------
 
 
-// Claim (or find) the slot of each candidate's key.  The first claimant
-// stores the key; every candidate then lowers the slot's arrival mark to
-// (epoch << 40 | p+1), so the earliest arrival of this iteration wins, while a
-// mark from an earlier iteration (key already in full or delta) always wins.
-// MAXKW > 0: keys of at most MAXKW words (exactly MAXKW with EXACT), loaded
-// once into registers with every word in flight together; MAXKW == 0: any key
-// width, read from the row as needed.
-template <int MAXKW, bool EXACT>
-__global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw_, u64* table, u64 cap, u64* slot_of,
-                               u64 epoch, u32* flags)
-{
-    const int kw = EXACT ? MAXKW : kw_;
-    if (ctl[1]) return;
-    const u64 n = ctl[0];
-    u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const u64* key = cand + p * cw;
-    u64 kr[MAXKW > 0 ? MAXKW : 1];
-    u64 h;
-    if (MAXKW > 0) {
-#pragma unroll
-        for (int c = 0; c < MAXKW; c++) kr[c] = c < kw ? key[c] : 0;
-        // pra_keyhash over the register copy
-        h = 0x9E3779B97F4A7C15ULL ^ (u64)kw;
-#pragma unroll
-        for (int c = 0; c < MAXKW; c++)
-            if (c < kw) h = pra_mix64(h ^ (kr[c] + 0x632BE59BD9B4E019ULL * (u64)(c + 1)));
-        h = h ? h : 1;
-    } else {
-        h = pra_keyhash(key, kw);
-    }
-    const int stride = 2 + kw;
-    const u64 mark = (epoch << 40) | (p + 1);
-    u64 m = cap - 1, s = h & m;
-    for (u64 q = 0; q < cap; q++) {
-        u64* slot = table + s * stride;
-        u64 cur = slot[0];
-        bool mine = false;
-        if (cur == 0) {
-            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)h);
-            if (cur == 0) {
-                mine = true;
-                if (MAXKW > 0) {
-#pragma unroll
-                    for (int c = 0; c < MAXKW; c++)
-                        if (c < kw) slot[2 + c] = kr[c];
-                } else {
-                    for (int c = 0; c < kw; c++) slot[2 + c] = key[c];
-                }
-            }
-        }
-        if (mine || cur == h) {
-            // the claimant lowers the mark at once; a later candidate first
-            // reads it: a mark from an earlier iteration (key already present)
-            // or an earlier arrival of this one already beats ours
-            if (mine) {
-                atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
-            } else {
-                u64 old = slot[1];
-                if (old > mark && old >= (epoch << 40))
-                    atomicMin((unsigned long long*)(slot + 1), (unsigned long long)mark);
-            }
-            slot_of[p] = s;
-            return;
-        }
-        s = (s + 1) & m;
-    }
-    slot_of[p] = ~0ULL;
-    atomicOr(flags, PRA_FLAG_TABLE_FULL);
-}
-
-// Candidate p of k_dedup_claim2 (p < the candidate count), for the batched
-// claim.
+// Claim (or find) the slot of candidate p's key (p < the candidate count).
+// The first claimant stores the key; every candidate then lowers the slot's
+// arrival mark to (epoch << 40 | p+1), so the earliest arrival of this
+// iteration wins, while a mark from an earlier iteration (key already in full
+// or delta) always wins.  MAXKW > 0: keys of at most MAXKW words (exactly
+// MAXKW with EXACT), loaded once into registers with every word in flight
+// together; MAXKW == 0: any key width, read from the row as needed.
 template <int MAXKW, bool EXACT>
 __device__ __forceinline__ void pra_claim_one(const u64* cand, int cw, int kw_, u64* table, u64 cap, u64* slot_of,
                                               u64 epoch, u32* flags, u64 p)
@@ -1256,6 +1190,17 @@ __device__ __forceinline__ void pra_claim_one(const u64* cand, int cw, int kw_, 
     atomicOr(flags, PRA_FLAG_TABLE_FULL);
 }
 
+// One output's claims: one thread per candidate.
+template <int MAXKW, bool EXACT>
+__global__ void k_dedup_claim2(const u64* cand, const u64* ctl, int cw, int kw_, u64* table, u64 cap, u64* slot_of,
+                               u64 epoch, u32* flags)
+{
+    if (ctl[1]) return;
+    const u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ctl[0]) return;
+    pra_claim_one<MAXKW, EXACT>(cand, cw, kw_, table, cap, slot_of, epoch, flags, p);
+}
+
 // Claims of several outputs in one launch: a block serves the job whose
 // block range holds it, as k_dedup_claim2 serves its one output.
 template <int MAXKW, bool EXACT>
@@ -1273,85 +1218,11 @@ __global__ void k_dedup_claim_many(ClaimBatch b, u32* flags)
 // Keep the winners, verify every key against its slot (a 64-bit hash
 // collision between distinct keys is reported, never merged) and count kept
 // candidates per destination bucket per tile.
-// Blocks of PRA_DECIDE_BLOCK threads take tiles of PRA_TILE candidates in
-// turn: each candidate is a chain of dependent loads, so many waves per tile
-// hide them.
 // MAXKW > 0: keys of at most MAXKW words (exactly MAXKW with EXACT; the
 // slot's words and the candidate's are loaded together); MAXKW == 0: any key
 // width.
-template <int MAXKW, bool EXACT>
-__global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw,
-                                                                        int kw_, const u64* table,
-                                                                        const u64* slot_of, u64 epoch, u32* keep,
-                                                                        const u32* dest, u32 nb, u32* hist,
-                                                                        u32 nblocks, u32* flags, u32* offs1)
-{
-    const int kw = EXACT ? MAXKW : kw_;
-    if (ctl[1]) return;
-    const u64 n = ctl[0];
-    __shared__ u32 h[PRA_RADIX];
-    __shared__ u32 sl[PRA_DECIDE_BLOCK / 64 + 1];
-    for (u32 tile = blockIdx.x; tile < nblocks; tile += gridDim.x) {
-        u64 base = (u64)tile * PRA_TILE;
-        if (base >= n) {
-            // a tile past the candidates keeps nothing
-            for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = 0;
-            if (offs1)
-                for (u32 i = threadIdx.x; i <= nb; i += PRA_DECIDE_BLOCK) offs1[i] = 0;
-            continue;
-        }
-        for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
-            u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
-            if (p < n) {
-                u64 s = slot_of[p];
-                u32 kp = 0;
-                if (s != ~0ULL) {
-                    const u64* slot = table + s * (2 + kw);
-                    const u64* key = cand + p * cw;
-                    bool eq = true;
-                    if (MAXKW > 0) {
-                        u64 x[MAXKW > 0 ? MAXKW : 1], y[MAXKW > 0 ? MAXKW : 1];
-#pragma unroll
-                        for (int c = 0; c < MAXKW; c++) {
-                            x[c] = c < kw ? slot[2 + c] : 0;
-                            y[c] = c < kw ? key[c] : 0;
-                        }
-#pragma unroll
-                        for (int c = 0; c < MAXKW; c++) eq = eq && x[c] == y[c];
-                    } else {
-                        for (int c = 0; c < kw; c++)
-                            if (slot[2 + c] != key[c]) { eq = false; break; }
-                    }
-                    if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
-                    if (slot[1] == ((epoch << 40) | (p + 1))) {
-                        kp = 1;
-                        atomicAdd(&h[dest[p]], 1u);
-                    }
-                }
-                keep[p] = kp;
-            }
-        }
-        __syncthreads();
-        if (offs1) {
-            // a single tile: its counts are the whole histogram, whose
-            // exclusive offsets (and total, offs1[nb]) go out directly
-            u32 tot;
-            const u32 v = threadIdx.x < nb ? h[threadIdx.x] : 0u;
-            const u32 ex = block_exclusive_scan<u32, PRA_DECIDE_BLOCK / 64>(v, sl, &tot);
-            if (threadIdx.x < nb) offs1[threadIdx.x] = ex;
-            if (threadIdx.x == 0) offs1[nb] = tot;
-        } else {
-            for (u32 i = threadIdx.x; i < nb; i += PRA_DECIDE_BLOCK) hist[(u64)i * nblocks + tile] = h[i];
-        }
-        __syncthreads();
-    }
-}
-
-// Tile `tile` of k_dedup_decide_hist (h, sl: the block's LDS counters and scan
-// words), for the batched decision.
+// Tile `tile` of PRA_TILE candidates (h, sl: the block's LDS counters and
+// scan words).
 template <int MAXKW, bool EXACT>
 __device__ __forceinline__ void pra_decide_tile(const u64* cand, u64 n, int cw, int kw_, const u64* table,
                                                 const u64* slot_of, u64 epoch, u32* keep, const u32* dest, u32 nb,
@@ -1416,6 +1287,25 @@ __device__ __forceinline__ void pra_decide_tile(const u64* cand, u64 n, int cw, 
     __syncthreads();
 }
 
+// One output's decisions: blocks of PRA_DECIDE_BLOCK threads take tiles of
+// PRA_TILE candidates in turn (each candidate is a chain of dependent loads,
+// so many waves per tile hide them).
+template <int MAXKW, bool EXACT>
+__global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_hist(const u64* cand, const u64* ctl, int cw,
+                                                                        int kw_, const u64* table,
+                                                                        const u64* slot_of, u64 epoch, u32* keep,
+                                                                        const u32* dest, u32 nb, u32* hist,
+                                                                        u32 nblocks, u32* flags, u32* offs1)
+{
+    if (ctl[1]) return;
+    const u64 n = ctl[0];
+    __shared__ u32 h[PRA_RADIX];
+    __shared__ u32 sl[PRA_DECIDE_BLOCK / 64 + 1];
+    for (u32 tile = blockIdx.x; tile < nblocks; tile += gridDim.x)
+        pra_decide_tile<MAXKW, EXACT>(cand, n, cw, kw_, table, slot_of, epoch, keep, dest, nb, hist, nblocks, flags,
+                                      offs1, tile, h, sl);
+}
+
 // Decisions of several outputs in one launch: block -> (job, tile) by the
 // jobs' block ranges; no stride, every tile has its block.
 template <int MAXKW, bool EXACT>
@@ -1429,6 +1319,173 @@ __global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dedup_decide_many(DecideBa
     if (j.ctl[1]) return;
     pra_decide_tile<MAXKW, EXACT>(j.cand, j.ctl[0], j.cw, j.kw, j.table, j.slot_of, j.epoch, j.keep, j.dest, b.nb,
                                   j.hist, j.nblocks, flags, j.offs1, blockIdx.x - b.block_base[k], h, sl);
+}
+
+// ----------------------------------------------------- dense membership ---
+
+// rank index of a key of KW words, false when a value has no dictionary rank
+template <int KW>
+__device__ __forceinline__ bool pra_dense_index(const u64* key, const u64* dict, u64 dcap, u64 D, u64* idx)
+{
+    u64 v[KW];
+#pragma unroll
+    for (int c = 0; c < KW; c++) v[c] = key[c];
+    u64 x = 0;
+#pragma unroll
+    for (int c = 0; c < KW; c++) {
+        u64 r;
+        if (!pra_dict_rank(dict, dcap, v[c], &r)) return false;
+        x = x * D + r;
+    }
+    *idx = x;
+    return true;
+}
+
+// A candidate whose key is a member (bit set) is dropped at once
+// (slot_of = ~0); any other claims or finds the slot of x + 1 and lowers its
+// arrival mark to its position, so the earliest arrival wins.
+template <int KW>
+__device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u32* flags)
+{
+    u64 x;
+    if (!pra_dense_index<KW>(a.cand + p * a.cw, a.dict, a.dict_cap, a.D, &x)) {
+        atomicCAS((unsigned long long*)(a.ctl + 1), 0ULL, 3ULL);
+        return;
+    }
+    if ((a.bitmap[x >> 5] >> (x & 31)) & 1u) {
+        a.slot_of[p] = ~0ULL;
+        return;
+    }
+    const u64 key = x + 1, m = a.cap - 1;
+    u64 s = pra_mix64(key) & m;
+    for (u64 q = 0; q < a.cap; q++) {
+        u64* slot = a.table + 2 * s;
+        u64 cur = slot[0];
+        bool mine = false;
+        if (cur == 0) {
+            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)key);
+            mine = cur == 0;
+        }
+        if (mine || cur == key) {
+            // a later candidate reads the mark first: an earlier arrival may
+            // already hold it
+            if (mine || slot[1] > p) atomicMin((unsigned long long*)(slot + 1), (unsigned long long)p);
+            a.slot_of[p] = s;
+            return;
+        }
+        s = (s + 1) & m;
+    }
+    a.slot_of[p] = ~0ULL;
+    atomicOr(flags, PRA_FLAG_TABLE_FULL);
+}
+
+template <int KW>
+__global__ void k_dense_claim(DenseArgs a, u32* flags)
+{
+    if (a.ctl[1]) return;
+    const u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.ctl[0]) return;
+    pra_dense_claim_one<KW>(a, p, flags);
+}
+
+// Dense claims of several outputs (of one key width) in one launch; the job
+// table in device memory is followed by the claim blocks' and the decision
+// blocks' first block per job (njobs + 1 words each).
+template <int KW>
+__global__ void k_dense_claim_many(const DenseArgs* jobs, int njobs, u32* flags)
+{
+    const u32* cbase = (const u32*)(jobs + njobs);
+    const int k = pra_job_of(cbase, njobs, blockIdx.x);
+    const DenseArgs* jp = jobs + k;
+    if (jp->ctl[1]) return;
+    const u64 p = (u64)(blockIdx.x - cbase[k]) * blockDim.x + threadIdx.x;
+    if (p >= jp->ctl[0]) return;
+    pra_dense_claim_one<KW>(*jp, p, flags);
+}
+
+// Tile `tile` of the dense decision: the earliest arrival of each claimed key
+// keeps it, sets its membership bit and returns its slot to empty; kept
+// candidates are counted per destination bucket as in pra_decide_tile.
+__device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n, u32 tile, u32* h, u32* sl)
+{
+    const u64 base = (u64)tile * PRA_TILE;
+    if (base >= n) {
+        for (u32 i = threadIdx.x; i < a.nb; i += PRA_DECIDE_BLOCK) a.hist[(u64)i * a.nblocks + tile] = 0;
+        if (a.offs1)
+            for (u32 i = threadIdx.x; i <= a.nb; i += PRA_DECIDE_BLOCK) a.offs1[i] = 0;
+        return;
+    }
+    for (int i = threadIdx.x; i < PRA_RADIX; i += PRA_DECIDE_BLOCK) h[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
+        const u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
+        if (p < n) {
+            const u64 s = a.slot_of[p];
+            u32 kp = 0;
+            if (s != ~0ULL) {
+                u64* slot = a.table + 2 * s;
+                if (slot[1] == p) {
+                    kp = 1;
+                    atomicAdd(&h[a.dest[p]], 1u);
+                    const u64 x = slot[0] - 1;
+                    atomicOr(&a.bitmap[x >> 5], 1u << (x & 31));
+                    // only the winner writes the slot; the others compare the
+                    // mark with their own position, which ~0 never equals
+                    slot[0] = 0;
+                    slot[1] = ~0ULL;
+                }
+            }
+            a.keep[p] = kp;
+        }
+    }
+    __syncthreads();
+    if (a.offs1) {
+        u32 tot;
+        const u32 v = threadIdx.x < a.nb ? h[threadIdx.x] : 0u;
+        const u32 ex = block_exclusive_scan<u32, PRA_DECIDE_BLOCK / 64>(v, sl, &tot);
+        if (threadIdx.x < a.nb) a.offs1[threadIdx.x] = ex;
+        if (threadIdx.x == 0) a.offs1[a.nb] = tot;
+    } else {
+        for (u32 i = threadIdx.x; i < a.nb; i += PRA_DECIDE_BLOCK) a.hist[(u64)i * a.nblocks + tile] = h[i];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dense_decide(DenseArgs a)
+{
+    if (a.ctl[1]) return;
+    const u64 n = a.ctl[0];
+    __shared__ u32 h[PRA_RADIX];
+    __shared__ u32 sl[PRA_DECIDE_BLOCK / 64 + 1];
+    for (u32 tile = blockIdx.x; tile < a.nblocks; tile += gridDim.x) pra_dense_decide_tile(a, n, tile, h, sl);
+}
+
+// Dense decisions of several outputs in one launch: one block per tile of
+// each job (table as k_dense_claim_many).
+__global__ __launch_bounds__(PRA_DECIDE_BLOCK) void k_dense_decide_many(const DenseArgs* jobs, int njobs)
+{
+    __shared__ u32 h[PRA_RADIX];
+    __shared__ u32 sl[PRA_DECIDE_BLOCK / 64 + 1];
+    const u32* dbase = (const u32*)(jobs + njobs) + njobs + 1;
+    const int k = pra_job_of(dbase, njobs, blockIdx.x);
+    const DenseArgs a = jobs[k];
+    if (a.ctl[1]) return;
+    pra_dense_decide_tile(a, a.ctl[0], blockIdx.x - dbase[k], h, sl);
+}
+
+// the membership bits of n rows (key = their first kw columns)
+template <int KW>
+__global__ void k_dense_build(const u64* rows, u64 n, int w, const u64* dict, u64 dcap, u64 D, u32* bitmap, u64* miss)
+{
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 x;
+    if (!pra_dense_index<KW>(rows + i * w, dict, dcap, D, &x)) {
+        atomicAdd((unsigned long long*)miss, 1ULL);
+        return;
+    }
+    atomicOr(&bitmap[x >> 5], 1u << (x & 31));
 }
 
 // Combine the PRA_NREP replicas of the scatter summaries and reset them to
@@ -2405,6 +2462,56 @@ int pra_dedup_class(int kw)
     case 6: LAUNCH(16, false); break;                                                                                  \
     default: LAUNCH(0, false);                                                                                         \
     }
+
+void pra_launch_dense_claim(hipStream_t st, const DenseArgs& a, u64 capn, u32* flags)
+{
+    if (!capn) return;
+    const dim3 g(grid_for(capn)), b(PRA_BLOCK);
+    switch (a.kw) {
+    case 1: hipLaunchKernelGGL(k_dense_claim<1>, g, b, 0, st, a, flags); break;
+    case 2: hipLaunchKernelGGL(k_dense_claim<2>, g, b, 0, st, a, flags); break;
+    case 3: hipLaunchKernelGGL(k_dense_claim<3>, g, b, 0, st, a, flags); break;
+    default: hipLaunchKernelGGL(k_dense_claim<4>, g, b, 0, st, a, flags); break;
+    }
+}
+
+void pra_launch_dense_many(hipStream_t st, const DenseArgs* jobs, int njobs, int kw, u32 claim_blocks,
+                           u32 decide_blocks, u32* flags)
+{
+    if (njobs <= 0) return;
+    if (claim_blocks) {
+        const dim3 g(claim_blocks), b(PRA_BLOCK);
+        switch (kw) {
+        case 1: hipLaunchKernelGGL(k_dense_claim_many<1>, g, b, 0, st, jobs, njobs, flags); break;
+        case 2: hipLaunchKernelGGL(k_dense_claim_many<2>, g, b, 0, st, jobs, njobs, flags); break;
+        case 3: hipLaunchKernelGGL(k_dense_claim_many<3>, g, b, 0, st, jobs, njobs, flags); break;
+        default: hipLaunchKernelGGL(k_dense_claim_many<4>, g, b, 0, st, jobs, njobs, flags); break;
+        }
+    }
+    if (decide_blocks)
+        hipLaunchKernelGGL(k_dense_decide_many, dim3(decide_blocks), dim3(PRA_DECIDE_BLOCK), 0, st, jobs, njobs);
+}
+
+void pra_launch_dense_decide(hipStream_t st, const DenseArgs& a, u32*)
+{
+    const dim3 g(std::min<u32>(a.nblocks, PRA_DECIDE_GRID)), b(PRA_DECIDE_BLOCK);
+    hipLaunchKernelGGL(k_dense_decide, g, b, 0, st, a);
+}
+
+void pra_launch_dense_build(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* dict, u64 dict_cap, u64 D,
+                            u32* bitmap, u64* miss)
+{
+    if (!n) return;
+    const dim3 g(grid_for(n)), b(PRA_BLOCK);
+    switch (kw) {
+    case 1: hipLaunchKernelGGL(k_dense_build<1>, g, b, 0, st, rows, n, w, dict, dict_cap, D, bitmap, miss); break;
+    case 2: hipLaunchKernelGGL(k_dense_build<2>, g, b, 0, st, rows, n, w, dict, dict_cap, D, bitmap, miss); break;
+    case 3: hipLaunchKernelGGL(k_dense_build<3>, g, b, 0, st, rows, n, w, dict, dict_cap, D, bitmap, miss); break;
+    default: hipLaunchKernelGGL(k_dense_build<4>, g, b, 0, st, rows, n, w, dict, dict_cap, D, bitmap, miss); break;
+    }
+}
+
+void pra_launch_dense_table_clear(hipStream_t st, u64* table, u64 cap) { pra_launch_table_clear(st, table, cap, 2); }
 
 void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags)
 {

@@ -105,12 +105,12 @@ struct PlanAllArgs {
     u64* srcoff;
     u64* posadj;
     u64* stage2;         // per output sw words; ctl = stage2 + output * sw + 4
-    u64* joined;
     long long thresh;
     int nb, nra, sw;
     const u64* off[PRA_MAXRA];       // per rule: exclusive output offsets over its outer rows
     const u64* bnd[PRA_MAXRA];       // per rule: first outer row of each source bucket (nb + 1)
     u64 cap[PRA_MAXRA];              // per output: candidate capacity
+    u64* joined[PRA_MAXRA];          // per output: its instance's join-pair count
     unsigned char rout[PRA_MAXRA];   // per rule: its output (0xFF: inactive)
     unsigned char is_join[PRA_MAXRA];
     unsigned char outs[PRA_MAXRA];   // per block: the output it plans
@@ -140,6 +140,32 @@ struct DecideJob {
     u32 nblocks;
     int cw, kw;
 };
+// Dense membership of an interned output whose key columns all have
+// dictionary ranks: the key's rank index x = ((r0 * D) + r1) * D + ..
+// (D = dictionary size) selects one bit of a bitmap over D^kw bits that holds
+// the keys of full u delta (and of newt, set as tuples are kept).  Within an
+// iteration, candidates that are not members claim a slot of a small table
+// keyed exactly by x + 1, slots [x + 1, earliest arrival] (empty: [0, ~0]);
+// the earliest arrival keeps the tuple, sets its bit and resets the slot, so
+// the table is empty again after every iteration.  A value missing from the
+// dictionary sets ctl[1] = 3 and the output is redone the general way.
+struct DenseArgs {
+    const u64* cand;
+    u64* ctl;
+    u64* slot_of;
+    u32* keep;
+    const u32* dest;
+    u32* hist;
+    u32* offs1;          // one tile: bucket offsets and total written directly
+    const u64* dict;
+    u64 dict_cap, D;
+    u32* bitmap;
+    u64* table;
+    u64 cap;
+    u32 nblocks, nb;
+    int cw, kw;
+};
+
 #define PRA_DEDUP_JOBS 32
 struct ClaimBatch {
     ClaimJob job[PRA_DEDUP_JOBS];
@@ -269,6 +295,18 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a);
 // key-width class of a claim / decision launch (jobs of one batch share it)
 int pra_dedup_class(int kw);
+// dense membership: per-iteration claims and decisions of one output, and the
+// bitmap of a version's rows (misses counted into *miss)
+void pra_launch_dense_claim(hipStream_t st, const DenseArgs& a, u64 capn, u32* flags);
+void pra_launch_dense_decide(hipStream_t st, const DenseArgs& a, u32* flags);
+// dense claims, then decisions, of several outputs of key width kw: `jobs`
+// (device) holds njobs DenseArgs, then njobs + 1 first claim blocks, then
+// njobs + 1 first decision blocks (one per tile)
+void pra_launch_dense_many(hipStream_t st, const DenseArgs* jobs, int njobs, int kw, u32 claim_blocks,
+                           u32 decide_blocks, u32* flags);
+void pra_launch_dense_build(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* dict, u64 dict_cap, u64 D,
+                            u32* bitmap, u64* miss);
+void pra_launch_dense_table_clear(hipStream_t st, u64* table, u64 cap);
 void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags);
 void pra_launch_dedup_decide_many(hipStream_t st, const DecideBatch& b, int cls, u32* flags);
 void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,

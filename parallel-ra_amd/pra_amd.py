@@ -23,13 +23,13 @@ F_EQ, F_NEQ, F_LT, F_LE, F_GT, F_GE, F_TRUE, F_FALSE = range(8)
 EXPORTS = (
     "pra_ctx_create", "pra_ctx_destroy", "pra_last_error", "pra_ctx_set_batch_size",
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
-    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes",
+    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes", "pra_ctx_set_group", "pra_ctx_group_joined",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
     "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
     "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_begin", "pra_scc_step",
     "pra_scc_end", "pra_scc_history",
-    "pra_execute", "pra_execute_many",
+    "pra_partition_rows", "pra_execute", "pra_execute_many",
     "pra_server_create", "pra_server_submit_tc", "pra_server_wait", "pra_server_rows", "pra_server_destroy",
 )
 
@@ -73,6 +73,8 @@ def lib():
                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
             "pra_ctx_synchronize": [_vp],
             "pra_ctx_device_bytes": [_vp, _u64p, _u64p],
+            "pra_ctx_set_group": [_vp, ctypes.c_int],
+            "pra_ctx_group_joined": [_vp, ctypes.c_int, _u64p],
             "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                ctypes.POINTER(_vp)],
             "pra_rel_destroy": [_vp],
@@ -96,6 +98,7 @@ def lib():
             "pra_scc_step": [_vp, _u64p],
             "pra_scc_end": [_vp],
             "pra_scc_history": [_vp, _u64p, _u64, _u64p],
+            "pra_partition_rows": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
             "pra_execute": [_vp],
             "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],
             "pra_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.POINTER(_vp)],
@@ -186,11 +189,26 @@ class Context:
     def synchronize(self):
         self._chk(self.L.pra_ctx_synchronize(self.c), "synchronize")
 
+    def set_group(self, g):
+        """build program instance g with the following rel_create / scc_create calls"""
+        self._chk(self.L.pra_ctx_set_group(self.c, int(g)), "set_group")
+
+    def group_joined(self, g):
+        v = _u64()
+        self._chk(self.L.pra_ctx_group_joined(self.c, int(g), ctypes.byref(v)), "group_joined")
+        return int(v.value)
+
     def device_bytes(self):
         """(bytes held now, peak bytes held) on the device by this context"""
         a, b = _u64(), _u64()
         self._chk(self.L.pra_ctx_device_bytes(self.c, ctypes.byref(a), ctypes.byref(b)), "device_bytes")
         return int(a.value), int(b.value)
+
+    def partition_rows(self, rows, n, w, jcc, out_rows, bucket_bounds):
+        """stable partition of n device rows (w words) by tuple_hash(row[0:jcc]) % nbuckets;
+        rows / out_rows / bucket_bounds are device addresses (ints); ordered on the context's stream"""
+        self._chk(self.L.pra_partition_rows(self.c, rows, int(n), int(w), int(jcc), out_rows, bucket_bounds),
+                  "partition_rows")
 
     def joined(self):
         v = _u64()

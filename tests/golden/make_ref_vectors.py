@@ -272,6 +272,7 @@ def cases():
                 [1]))
     for seed in range(6):
         out.append((f"random_program_s{seed}", lambda b, s=seed: P.random_program(b, s), None, [1, 3], [1, 3]))
+    out.append(("id_valued", lambda b: P.id_valued_program(b), None, [1, 3], [1]))
     out.append(("threshold_split", threshold_split, None, [1], [1]))
     return out
 

@@ -434,6 +434,36 @@ def random_program(b, seed, nbase=2, nderived=4, dom=9):
     return {"rels": rels, "sccs": [s1, s2], "rules": rules}
 
 
+def id_valued_program(b, n=40, seed=3):
+    """Intern ids flowing into data columns after the first SCC has already
+    ranked the values it saw:
+         SCC 1 (once):      X(y, x)      <- copy A(x, y) FULL
+         SCC 2 (to a fixed point, A status true):
+                            B(id, x | y) <- acopy A(x, y | id) DELTA   [2, 0, 1]
+                            O(id, x)     <- copy B DELTA               [0, 1]
+    O's candidates carry A's row ids, values no relation held when the first
+    SCC ran.  Returns (O, B, X, scc1, scc2)."""
+    rng = random.Random(seed)
+    pairs = sorted({(rng.randrange(20), rng.randrange(20)) for _ in range(n)})
+    A = b.rel_create(2, 1, 31)
+    X = b.rel_create(2, 1, 32)
+    B = b.rel_create(2, 1, 33)
+    O = b.rel_create(2, 1, 34)
+    b.load_rows(A, np.array([[x, y, 1000003 + 7 * i] for i, (x, y) in enumerate(pairs)], dtype=np.uint64), FULL)
+    s1 = b.scc_create(False, 1)
+    b.scc_add_relation(s1, A, False)
+    b.scc_add_relation(s1, X, False)
+    b.add_copy(s1, X, A, FULL, [1, 0])
+    s2 = b.scc_create(True, 2)
+    b.scc_add_relation(s2, A, True)
+    b.scc_add_relation(s2, B, False)
+    b.scc_add_relation(s2, O, False)
+    b.add_acopy(s2, B, A, DELTA, [2, 0, 1])
+    b.add_copy(s2, O, B, DELTA, [0, 1])
+    b.add_dependency(s1, s2)
+    return O, B, X, s1, s2
+
+
 TWO_RULE_TAG = 5
 
 
