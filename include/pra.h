@@ -167,6 +167,16 @@ int pra_execute(pra_ctx* ctx);
  * work is ordered on the context's stream (pra_ctx_synchronize waits for it). */
 int pra_partition_rows(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t w, uint32_t jcc, uint64_t* out_rows,
                        uint64_t* bucket_bounds);
+/* pra_partition_rows without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its scratch is a
+ * static device array of 4,195,328 bytes that every GPU the library runs on
+ * holds until the process ends; it takes at most 8,388,608 rows (PRA_E_ARG
+ * beyond).  Same results as pra_partition_rows.  All calls on one GPU share
+ * that GPU's scratch: calls that overlap on one GPU (from different contexts
+ * or streams) race. */
+int pra_partition_rows_noalloc(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t w, uint32_t jcc,
+                               uint64_t* out_rows, uint64_t* bucket_bounds);
 
 /* Batched execution (no counterpart in the reference, which runs one program
  * per MPI job): pra_execute on n independent contexts of the same device,

@@ -2504,6 +2504,7 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         HIPCHK(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) return PRA_E_ARG;
         HIPCHK(hipSetDevice(device));
+        HIPCHK(pra_partition_scratch_init(device));
         // freed stream-ordered allocations stay in the device's pool for reuse
         // (instead of going back to the driver at every synchronization) up to
         // a quarter of the device memory, so that many contexts in a row do
@@ -2634,6 +2635,23 @@ int pra_partition_rows(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t 
     pra_launch_partition(ctx->st, rows, n, (int)w, (int)jcc, (u32)ctx->nb, out_rows, bucket_bounds,
                          ctx->part_scratch.as<u32>());
     check_launch("partition");
+    API_END
+}
+
+int pra_partition_rows_noalloc(pra_ctx* ctx, const uint64_t* rows, uint64_t n, uint32_t w, uint32_t jcc,
+                               uint64_t* out_rows, uint64_t* bucket_bounds)
+{
+    if (ctx && n > PRA_NOALLOC_MAX_ROWS)
+        return fail(ctx, PRA_E_ARG, "allocation-free partition takes at most " + std::to_string(PRA_NOALLOC_MAX_ROWS) +
+                                        " rows, got " + std::to_string(n));
+    const int rc = partition_args(ctx, rows, n, w, jcc, out_rows, bucket_bounds);
+    if (rc != PRA_OK) return rc;
+    API_BEGIN(ctx)
+    HIPCHK(hipSetDevice(ctx->device));
+    u32* scratch = pra_partition_scratch(ctx->device);
+    if (!scratch) throw PraError(PRA_E_STATE, "allocation-free scratch of this device was not set up");
+    pra_launch_partition(ctx->st, rows, n, (int)w, (int)jcc, (u32)ctx->nb, out_rows, bucket_bounds, scratch);
+    check_launch("partition_noalloc");
     API_END
 }
 

@@ -21,6 +21,8 @@
 #include "pra_device.hpp"
 #include "pra_partition.hpp"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kPartBlock = 256;
@@ -140,6 +142,14 @@ __global__ void k_part_empty(u64* bounds, u32 nb)
     if (t <= nb) bounds[t] = 0;
 }
 
+// The allocation-free entry point's scratch: one array per device, for the
+// whole process, holding H and T of a partition of at most
+// PRA_NOALLOC_MAX_ROWS rows into at most 256 buckets.
+static __device__ __align__(256) u32 scratch[PRA_NOALLOC_SCRATCH_WORDS];
+
+std::mutex g_scratch_mu;
+u32* g_scratch_dev[PRA_MAX_DEVICES] = {};
+
 } // namespace
 
 u32 pra_partition_tiles(u64 n) { return (u32)((n + PRA_PART_TILE - 1) / PRA_PART_TILE); }
@@ -160,4 +170,26 @@ void pra_launch_partition(hipStream_t st, const u64* rows, u64 n, int w, int jcc
     hipLaunchKernelGGL(k_part_scan, dim3(nb), dim3(kScanBlock), 0, st, H, ntiles, T);
     hipLaunchKernelGGL(k_part_scatter, dim3(ntiles), dim3(kPartBlock), 0, st, rows, n, w, jcc, nb, H, T, ntiles, out,
                        bounds);
+}
+
+// Once per device, with that device current: the device address of this
+// device's copy of `scratch`.  Host code uses only this address, never the
+// array's name (which names a host-side stand-in).
+hipError_t pra_partition_scratch_init(int device)
+{
+    if (device < 0 || device >= PRA_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return hipSuccess;
+    u32* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = ptr;
+    return hipSuccess;
+}
+
+u32* pra_partition_scratch(int device)
+{
+    if (device < 0 || device >= PRA_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[device];
 }

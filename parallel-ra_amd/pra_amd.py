@@ -29,7 +29,7 @@ EXPORTS = (
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
     "pra_scc_add_copy_filter", "pra_scc_add_dependency", "pra_scc_run", "pra_scc_begin", "pra_scc_step",
     "pra_scc_end", "pra_scc_history",
-    "pra_partition_rows", "pra_execute", "pra_execute_many",
+    "pra_partition_rows", "pra_partition_rows_noalloc", "pra_execute", "pra_execute_many",
     "pra_server_create", "pra_server_submit_tc", "pra_server_wait", "pra_server_rows", "pra_server_destroy",
 )
 
@@ -99,6 +99,7 @@ def lib():
             "pra_scc_end": [_vp],
             "pra_scc_history": [_vp, _u64p, _u64, _u64p],
             "pra_partition_rows": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
+            "pra_partition_rows_noalloc": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
             "pra_execute": [_vp],
             "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],
             "pra_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.POINTER(_vp)],
@@ -209,6 +210,11 @@ class Context:
         rows / out_rows / bucket_bounds are device addresses (ints); ordered on the context's stream"""
         self._chk(self.L.pra_partition_rows(self.c, rows, int(n), int(w), int(jcc), out_rows, bucket_bounds),
                   "partition_rows")
+
+    def partition_rows_noalloc(self, rows, n, w, jcc, out_rows, bucket_bounds):
+        """partition_rows without device allocation (static scratch; at most 8,388,608 rows)"""
+        self._chk(self.L.pra_partition_rows_noalloc(self.c, rows, int(n), int(w), int(jcc), out_rows, bucket_bounds),
+                  "partition_rows_noalloc")
 
     def joined(self):
         v = _u64()
