@@ -64,14 +64,15 @@ def main():
     ap.add_argument("--count", type=int, default=256, help="batch/serve: number of instances / requests")
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=50000)
-    ap.add_argument("--slots", type=int, default=4, help="batch/serve: concurrent instances")
+    ap.add_argument("--slots", type=int, default=64, help="serve: requests in flight at most")
     ap.add_argument("--budget-gb", type=float, default=160.0, help="batch/serve: HBM admission budget")
-    ap.add_argument("--rates", default="50,200,1000", help="serve: Poisson arrival rates (req/s)")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--grid", type=int, default=150)
     ap.add_argument("--buckets", type=int, default=8)
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-closed-form", action="store_true", help="batch/serve: skip the CPU |TC| check")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -202,35 +203,34 @@ def main():
 
 def est_bytes(n):
     """device bytes a TC instance on n vertices with m = 1.3 n edges needs at
-    its peak: ~0.2 n^2 tuples x (up to 4 x 32 B of membership table slots
-    while it grows, 2 x 24 B of full log and sort space rounded up to
-    power-of-two blocks, scratch)"""
-    return int(0.2 * n * n * 200) + (64 << 20)
+    its peak: ~0.2 n^2 tuples of 24 B held in power-of-two blocks (up to 48 B
+    each), the dense membership bitmap (n^2 bits) and a scratch allowance"""
+    return int(0.2 * n * n * 48) + n * n // 8 + (64 << 20)
 
 
 def bench_batch(args):
-    """Config 4: independent TC instances, plain (one after another) vs batched
-    (pra_execute_many: up to --slots instances overlapping, admitted in waves
-    under the HBM budget); results compared per instance by device checksum."""
+    """Config 4: independent TC instances, plain (one context per instance,
+    one after another) against batched (waves of instances admitted by
+    estimated bytes under the HBM budget, each wave one context whose
+    instances run in lock step: every phase of an iteration one batched
+    launch over all of them).  Every instance is compared bit for bit
+    (device checksum of its rows in order, count, join pairs) and its |TC|
+    against the SCC-condensation count computed on the CPU."""
     import pra_amd
-    from programs import config4_instances, random_graph, tc_program
+    from programs import config4_instances, random_graph, tc_program, tc_size_condensation
     inst = config4_instances(count=args.count, nmin=args.nmin, nmax=args.nmax)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     nb = args.buckets
-    log(f"batch: {sum(m for _, m, _ in inst)} edges generated;")
-    log(f"batch: {len(graphs)} instances, n in [{min(n for n, _, _ in inst)}, {max(n for n, _, _ in inst)}]")
+    log(f"batch: {len(graphs)} instances, n in [{min(n for n, _, _ in inst)}, {max(n for n, _, _ in inst)}], "
+        f"{sum(m for _, m, _ in inst)} edges")
 
-    def make(e):
-        c = pra_amd.Context(nbuckets=nb, batch_size=1)
-        h = tc_program(c, e)
-        c.synchronize()
-        return c, h
-
-    # plain: one instance at a time
+    # plain: one instance at a time, each on its own context
     plain = []
     t_plain = 0.0
     for k, e in enumerate(graphs):
-        c, (T, G, sfix, _) = make(e)
+        c = pra_amd.Context(nbuckets=nb, batch_size=1)
+        T, G, sfix, _ = tc_program(c, e)
+        c.synchronize()
         t0 = time.perf_counter()
         c.execute()
         c.synchronize()
@@ -239,42 +239,56 @@ def bench_batch(args):
         c.close()
         if (k + 1) % 32 == 0:
             log(f"  plain: {k + 1}/{len(graphs)} instances, {t_plain:.2f} s")
+
     # batched: waves admitted by estimated bytes, largest first
     order = sorted(range(len(graphs)), key=lambda i: -inst[i][0])
     budget = int(args.budget_gb * (1 << 30))
     batched = [None] * len(graphs)
     t_batch = 0.0
     pos = 0
+    waves = 0
     while pos < len(order):
         wave, used = [], 0
         while pos < len(order) and (not wave or used + est_bytes(inst[order[pos]][0]) <= budget):
             wave.append(order[pos])
             used += est_bytes(inst[order[pos]][0])
             pos += 1
-        ctxs = [make(graphs[i]) for i in wave]
+        c = pra_amd.Context(nbuckets=nb, batch_size=1)
+        hs = []
+        for g, i in enumerate(wave):
+            c.set_group(g)
+            hs.append(tc_program(c, graphs[i]))
+        c.synchronize()
         t0 = time.perf_counter()
-        rc = pra_amd.execute_many([c for c, _ in ctxs], concurrency=args.slots)
-        for c, _ in ctxs:
+        try:
+            c.execute()
             c.synchronize()
+        except pra_amd.PraError as ex:
+            raise RuntimeError(f"wave of instances {wave} failed: {ex}")
         t_batch += time.perf_counter() - t0
-        if rc != [0] * len(ctxs):
-            bad = [(wave[k], inst[wave[k]][0], rc[k], c.last_error()) for k, (c, _) in enumerate(ctxs) if rc[k]]
-            raise RuntimeError(f"batched instances failed: {bad}")
-        for i, (c, (T, G, sfix, _)) in zip(wave, ctxs):
-            batched[i] = (c.count(T), c.joined(), c.checksum(T))
-            c.close()
-        log(f"  batched: wave of {len(wave)} done, {pos}/{len(order)} instances, {t_batch:.2f} s")
+        waves += 1
+        for g, (i, (T, G, sfix, _)) in enumerate(zip(wave, hs)):
+            batched[i] = (c.count(T), c.group_joined(g), c.checksum(T))
+        peak = c.device_bytes()[1]
+        c.close()
+        log(f"  batched: wave of {len(wave)} (estimated {used / 2**30:.1f} GiB, peak {peak / 2**30:.1f} GiB), "
+            f"{pos}/{len(order)} instances, {t_batch:.2f} s")
     exact = all(a == b for a, b in zip(plain, batched))
+    closed = None
+    if not args.no_closed_form:
+        closed = all(plain[i][0] == tc_size_condensation(graphs[i][:, :2])[0] for i in range(len(graphs)))
     joined = sum(j for _, j, _ in plain)
-    log(f"plain {t_plain:.3f} s, batched {t_batch:.3f} s, bit-exact per instance: {exact}")
+    log(f"plain {t_plain:.3f} s, batched {t_batch:.3f} s ({waves} waves), bit-exact per instance: {exact}, "
+        f"|TC| = closed form: {closed}")
     out = {
         "metric": METRIC, "value": joined / t_batch, "unit": "joined tuples/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"tc_batch{len(graphs)}_n{args.nmin}-{args.nmax}", "buckets": nb,
-                   "slots": args.slots, "tuples": sum(t for t, _, _ in plain), "joined_tuples": joined},
+                   "budget_gb": args.budget_gb, "waves": waves, "tuples": sum(t for t, _, _ in plain),
+                   "joined_tuples": joined},
         "plain_value": joined / t_plain, "plain_s": t_plain, "batched_s": t_batch,
-        "batched_over_plain": t_plain / t_batch, "bit_exact_per_instance": exact,
+        "batched_over_plain": t_plain / t_batch, "bit_exact_per_instance": exact, "tc_equals_closed_form": closed,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -282,13 +296,17 @@ def bench_batch(args):
 
 def bench_serve(args):
     """Config 5: Poisson request streams of TC solves (sizes from the batched
-    distribution) into the serving loop; p50/p99 latency and aggregate joined
-    tuples/s per arrival rate; every result checked against a one-slot run."""
+    distribution) into the serving loop (continuous batching: up to --slots
+    requests in flight advance together, admitted by measured device bytes
+    under the HBM budget); p50/p99 latency and aggregate joined tuples/s per
+    arrival rate; every result equal across rates and to the SCC-condensation
+    |TC| computed on the CPU."""
     import numpy as np
     import pra_amd
-    from programs import config4_instances, random_graph
+    from programs import config4_instances, random_graph, tc_size_condensation
     inst = config4_instances(count=args.count, seed=5, nmin=args.nmin, nmax=args.nmax)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
+    want = None if args.no_closed_form else [tc_size_condensation(e[:, :2])[0] for e in graphs]
     budget = int(args.budget_gb * (1 << 30))
     ref = None
     rows = []
@@ -316,7 +334,8 @@ def bench_serve(args):
             ref = got
         rows.append({"rate": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                      "p99_ms": float(np.percentile(lat, 99) * 1e3), "joined_tuples_per_s": joined / span,
-                     "results_equal_across_rates": got == ref})
+                     "results_equal_across_rates": got == ref,
+                     "tc_equals_closed_form": None if want is None else [t for t, _ in got] == want})
         log(f"rate {rate:7.1f}/s: p50 {rows[-1]['p50_ms']:.1f} ms p99 {rows[-1]['p99_ms']:.1f} ms "
             f"{joined / span / 1e9:.3f} G joined/s")
     out = {

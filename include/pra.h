@@ -71,6 +71,19 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
 int pra_ctx_set_group(pra_ctx* ctx, int group);
 /* join pairs of instance `group` so far */
 int pra_ctx_group_joined(pra_ctx* ctx, int group, uint64_t* out);
+/* One lock-step iteration instead of pra_execute's run to the end: every
+ * instance without an SCC in progress starts its next runnable one, then one
+ * iteration runs over all SCCs in progress.  *active = instances with work
+ * left.  Instances built between steps join at the next step (the serving
+ * loop admits requests this way). */
+int pra_ctx_step(pra_ctx* ctx, uint64_t* active);
+/* instance `group`: finished (no SCC in progress or runnable), its iterations
+ * so far, and the device bytes its relations hold (any of the three may be
+ * null) */
+int pra_ctx_group_status(pra_ctx* ctx, int group, int* done, uint64_t* iterations, uint64_t* bytes);
+/* free instance `group` (its relations and SCCs; their handles become
+ * invalid) and reset its counters, so that the id can build a new program */
+int pra_ctx_release_group(pra_ctx* ctx, int group);
 /* device bytes the context holds now and at most so far (no counterpart in
  * the reference; what budget admission measures) */
 int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak);
@@ -187,12 +200,17 @@ int pra_execute_many(pra_ctx** ctxs, int n, int concurrency, int* rc);
 
 /* Serving (no counterpart in the reference): a request loop that solves
  * transitive-closure requests -- the program of tests/transitive_closure.cpp
- * on the given edge rows (m x 3 words: src, dst, row id) -- on one GPU.
- * Requests are admitted in arrival order when one of `slots` workers is free
- * and their estimated device bytes fit into `hbm_budget` beside the running
- * ones (0 = no budget); each runs on its own context and stream, so admitted
- * requests overlap on the device.  keep_rows != 0 keeps each result's rows on
- * the host for pra_server_rows.  Times are seconds since server creation. */
+ * on the given edge rows (m x 3 words: src, dst, row id) -- on one GPU with
+ * continuous batching.  Up to `slots` requests are in flight as instances of
+ * one context and advance together, one lock-step iteration at a time (each
+ * phase one batched launch over all of them); between iterations finished
+ * requests are retired and queued ones admitted, in arrival order, when the
+ * device bytes the context holds, plus the growth still expected of the
+ * requests in flight, plus the new request's estimate fit into `hbm_budget`
+ * (0 = no budget; an idle server always admits).  est_bytes 0 = the server's
+ * own estimate (random graphs with 1.3 edges per vertex).  keep_rows != 0
+ * keeps each result's rows on the host for pra_server_rows.  Times are
+ * seconds since server creation. */
 typedef struct pra_server pra_server;
 typedef struct {
     int status;

@@ -344,6 +344,10 @@ struct Group {
     u64 dict_gen = 0;                        // dictionary builds so far
     DBuf joined;                             // join pairs so far (one device word)
     u64 iterations = 0;
+    // lock-step execution: the instance's SCC in progress and the iterations
+    // of its current batch
+    pra_scc* running = nullptr;
+    u32 batch_it = 0;
 };
 
 struct pra_scc {
@@ -2432,42 +2436,51 @@ static pra_scc* one_runnable(pra_ctx* c, int g = -1)
     return nullptr;
 }
 
-// Every instance's LIE::execute (src/lie/lie.cpp:197-448) in lock step: at
-// each step every instance's first runnable SCC is taken, and those of the
-// same kind (fixed point, or once) as the first one run together -- one
-// merged iteration after another, each instance's SCC leaving at the end of
-// the batch that brings its delta to zero, exactly where its own LIE would
-// stop.  The others wait for the next step.
-static void execute_lockstep(pra_ctx* c)
+// One lock-step iteration over every instance's LIE::execute
+// (src/lie/lie.cpp:197-448): an instance without an SCC in progress starts
+// its next runnable one; then one merged iteration runs over the SCCs in
+// progress, and each SCC records its batch and ends where its own LIE would:
+// after its one iteration, or after a batch that leaves its delta at zero.
+// Returns the instances with work left.
+static u64 lockstep_step(pra_ctx* c)
 {
-    for (;;) {
-        std::vector<pra_scc*> members;
-        int kind = -1;
-        for (size_t g = 0; g < c->groups.size(); g++) {
+    std::vector<pra_scc*> members;
+    for (size_t g = 0; g < c->groups.size(); g++) {
+        Group& G = *c->groups[g];
+        if (!G.running) {
             pra_scc* s = one_runnable(c, (int)g);
             if (!s) continue;
-            if (kind < 0) kind = s->fixpoint;
-            if (s->fixpoint == kind) members.push_back(s);
+            scc_begin(s, true);
+            G.running = s;
+            G.batch_it = 0;
         }
-        if (members.empty()) break;
-        for (pra_scc* s : members) scc_begin(s, true);
-        std::vector<pra_scc*> active = members;
-        while (!active.empty()) {
-            const u32 iters = kind ? c->batch_size : 1;
-            for (u32 b = 0; b < iters; b++) run_iteration(c, active);
-            std::vector<pra_scc*> next;
-            for (pra_scc* s : active) {
-                const u64 d = delta_sum(s);
-                s->hist.push_back(d);
-                if (!kind || d == 0)
-                    scc_end(s, false);
-                else
-                    next.push_back(s);
-            }
-            active.swap(next);
-        }
-        sync_check(c);
+        members.push_back(G.running);
     }
+    if (members.empty()) return 0;
+    run_iteration(c, members);
+    u64 left = 0;
+    for (pra_scc* s : members) {
+        Group& G = grp(c, s->group);
+        if (!s->fixpoint || ++G.batch_it == c->batch_size) {
+            G.batch_it = 0;
+            const u64 d = delta_sum(s);
+            s->hist.push_back(d);
+            if (!s->fixpoint || d == 0) {
+                scc_end(s, false);
+                G.running = nullptr;
+            }
+        }
+    }
+    for (size_t g = 0; g < c->groups.size(); g++)
+        if (c->groups[g]->running || one_runnable(c, (int)g)) left++;
+    return left;
+}
+
+static void execute_lockstep(pra_ctx* c)
+{
+    while (lockstep_step(c)) {
+    }
+    sync_check(c);
 }
 
 static int fail(pra_ctx* c, int code, const std::string& m)
@@ -2670,6 +2683,64 @@ int pra_ctx_group_joined(pra_ctx* ctx, int group, uint64_t* out)
     API_BEGIN(ctx)
     Group& G = *ctx->groups[(size_t)group];
     *out = G.joined.p ? readback(ctx, G.joined.p, 1)[0] : 0;
+    API_END
+}
+
+int pra_ctx_step(pra_ctx* ctx, uint64_t* active)
+{
+    if (!ctx || !active) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    HIPCHK(hipSetDevice(ctx->device));
+    *active = lockstep_step(ctx);
+    if (!*active) sync_check(ctx);
+    API_END
+}
+
+int pra_ctx_group_status(pra_ctx* ctx, int group, int* done, uint64_t* iterations, uint64_t* bytes)
+{
+    if (!ctx || group < 0 || group >= (int)ctx->groups.size()) return PRA_E_ARG;
+    Group& G = *ctx->groups[(size_t)group];
+    if (done) *done = !G.running && !one_runnable(ctx, group);
+    if (iterations) *iterations = G.iterations;
+    if (bytes) {
+        u64 b = 0;
+        for (pra_rel* R : ctx->rels) {
+            if (R->group != group) continue;
+            for (auto& v : R->ver) b += v.buf.cap;
+            for (const DBuf* d : {&R->mt, &R->bm, &R->dtab, &R->idx[0], &R->idx[1], &R->nkeys, &R->kgofs,
+                                  &R->bin_keys, &R->bin_idx, &R->bin_cur, &R->bnd[0], &R->bnd[1]})
+                b += d->cap;
+        }
+        *bytes = b;
+    }
+    return PRA_OK;
+}
+
+int pra_ctx_release_group(pra_ctx* ctx, int group)
+{
+    if (!ctx || group < 0 || group >= (int)ctx->groups.size()) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    Group& G = *ctx->groups[(size_t)group];
+    if (G.running) throw PraError(PRA_E_STATE, "instance has an SCC in progress");
+    auto& deps = ctx->deps;
+    deps.erase(std::remove_if(deps.begin(), deps.end(),
+                              [&](const std::pair<pra_scc*, pra_scc*>& d) { return d.first->group == group; }),
+               deps.end());
+    std::vector<pra_scc*> keep_s;
+    for (pra_scc* s : ctx->sccs)
+        if (s->group == group)
+            delete s;
+        else
+            keep_s.push_back(s);
+    ctx->sccs.swap(keep_s);
+    std::vector<pra_rel*> keep_r;
+    for (pra_rel* r : ctx->rels)
+        if (r->group == group)
+            delete r;
+        else
+            keep_r.push_back(r);
+    ctx->rels.swap(keep_r);
+    ctx->groups[(size_t)group].reset(new Group());
     API_END
 }
 

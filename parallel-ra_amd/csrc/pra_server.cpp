@@ -1,14 +1,21 @@
-// Serving loop for transitive-closure / reachability requests on one GPU.
+// Serving loop for transitive-closure / reachability requests on one GPU,
+// with continuous batching.
 //
 // Requests (an edge list in the tsv_to_bin row layout) are queued in arrival
-// order; `slots` worker threads admit the head of the queue as soon as a slot
-// is free and its estimated device memory fits into the HBM budget next to
-// the requests already running (a request larger than the whole budget runs
-// alone).  Each admitted request is solved on its own context and stream, its
-// relations resident on the device for all iterations, so requests that
-// arrive while others run overlap with them on the GPU.  The program solved is
-// the reference's tests/transitive_closure.cpp:56-77 (copy seed, then the
-// T(DELTA) x G(FULL) join to the fixed point).
+// order.  One serving thread owns one context whose program instances
+// (pra_ctx_set_group) are the requests in flight, up to `slots` of them: each
+// lock-step iteration (pra_ctx_step) advances every request in flight by one
+// iteration, all of them in the same batched launches, and between iterations
+// the thread retires the requests that reached their fixed point (results
+// read, instance released) and admits queued ones into the freed slots when
+// their memory fits.  Admission counts the device bytes the context holds now
+// (pra_ctx_device_bytes), the growth still expected of the requests in flight
+// (estimate minus the bytes their relations hold already) and the new
+// request's estimate, against the HBM budget; the first request of an idle
+// server is always admitted.  Relations stay resident on the device from
+// admission to completion.  The program solved is the reference's
+// tests/transitive_closure.cpp:56-77 (copy seed, then the T(DELTA) x G(FULL)
+// join to the fixed point).
 #include "../../include/pra.h"
 
 #include <algorithm>
@@ -34,85 +41,184 @@ struct Request {
     bool done = false;
 };
 
+struct Slot {
+    Request* r = nullptr;
+    pra_rel* T = nullptr;
+};
+
 double seconds_since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+// device bytes a TC request on m edges is expected to need: random graphs with
+// 1.3 edges per vertex close to about 0.2 n^2 paths, held as 24-byte rows in
+// blocks up to twice that size, plus a 64 MiB allowance for scratch
+uint64_t default_estimate(uint64_t m)
+{
+    const double n = (double)m / 1.3;
+    return (uint64_t)(0.2 * n * n * 48.0) + (64ULL << 20);
+}
 
 } // namespace
 
 struct pra_server {
     int device = 0, nb = 1, slots = 1, keep_rows = 0;
-    uint64_t budget = 0, inflight = 0, next_id = 1;
+    uint64_t budget = 0, next_id = 1;
     bool stopping = false;
     Clock::time_point t0 = Clock::now();
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
     std::map<uint64_t, std::unique_ptr<Request>> all;
-    std::vector<std::thread> workers;
+    std::thread loop;
+    // owned by the serving thread
+    pra_ctx* ctx = nullptr;
+    std::vector<Slot> slot;
 
-    bool admissible()
+    int in_flight() const
     {
-        if (queue.empty()) return false;
-        return inflight == 0 || budget == 0 || inflight + queue.front()->est <= budget;
+        int k = 0;
+        for (const Slot& s : slot) k += s.r != nullptr;
+        return k;
     }
 
-    void solve(Request* r)
+    // may the head of the queue join the requests in flight?
+    bool fits(const Request* r)
     {
-        pra_ctx* c = nullptr;
+        if (!in_flight() || !budget) return true;
+        uint64_t live = 0;
+        if (pra_ctx_device_bytes(ctx, &live, nullptr) != PRA_OK) return false;
+        uint64_t reserved = 0;
+        for (size_t g = 0; g < slot.size(); g++) {
+            if (!slot[g].r) continue;
+            uint64_t held = 0;
+            pra_ctx_group_status(ctx, (int)g, nullptr, nullptr, &held);
+            if (slot[g].r->est > held) reserved += slot[g].r->est - held;
+        }
+        return live + reserved + r->est <= budget;
+    }
+
+    // the program of tests/transitive_closure.cpp as instance g
+    int build(int g, Request* r)
+    {
         pra_rel *T = nullptr, *G = nullptr;
         pra_scc *fix = nullptr, *seed = nullptr;
-        int rc = pra_ctx_create(device, nb, 0, &c);
         const int proj[2] = {3, 1}, reorder[2] = {1, 0};
-        if (rc == PRA_OK) rc = pra_ctx_set_batch_size(c, 1);
-        if (rc == PRA_OK) rc = pra_rel_create(c, 2, 1, 257, 1, &T);
-        if (rc == PRA_OK) rc = pra_rel_create(c, 2, 1, 256, 1, &G);
+        int rc = pra_ctx_set_group(ctx, g);
+        if (rc == PRA_OK) rc = pra_rel_create(ctx, 2, 1, 257, 1, &T);
+        if (rc == PRA_OK) rc = pra_rel_create(ctx, 2, 1, 256, 1, &G);
         if (rc == PRA_OK) rc = pra_rel_load_rows(G, r->edges.data(), r->m, PRA_FULL);
-        if (rc == PRA_OK) rc = pra_scc_create(c, 1, 13237, &fix);
+        if (rc == PRA_OK) rc = pra_scc_create(ctx, 1, 13237, &fix);
         if (rc == PRA_OK) rc = pra_scc_add_relation(fix, G, 0);
         if (rc == PRA_OK) rc = pra_scc_add_relation(fix, T, 1);
         if (rc == PRA_OK) rc = pra_scc_add_join(fix, T, T, PRA_DELTA, G, PRA_FULL, proj, 2);
-        if (rc == PRA_OK) rc = pra_scc_create(c, 0, 13239, &seed);
+        if (rc == PRA_OK) rc = pra_scc_create(ctx, 0, 13239, &seed);
         if (rc == PRA_OK) rc = pra_scc_add_relation(seed, G, 0);
         if (rc == PRA_OK) rc = pra_scc_add_relation(seed, T, 1);
         if (rc == PRA_OK) rc = pra_scc_add_copy(seed, T, G, PRA_FULL, reorder, 2);
         if (rc == PRA_OK) rc = pra_scc_add_dependency(seed, fix);
-        if (rc == PRA_OK) rc = pra_execute(c);
-        if (rc == PRA_OK) rc = pra_rel_count(T, PRA_FULL, &r->res.tuples);
-        if (rc == PRA_OK) rc = pra_ctx_joined_tuples(c, &r->res.joined);
-        if (rc == PRA_OK) rc = pra_ctx_iterations(c, &r->res.iterations);
+        slot[(size_t)g].T = T;
+        return rc;
+    }
+
+    void finish(Request* r, int status)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            r->res.status = status;
+            r->res.t_done = seconds_since(t0);
+            r->edges.clear();
+            r->edges.shrink_to_fit();
+            r->done = true;
+        }
+        cv_done.notify_all();
+    }
+
+    // results of a finished instance, then its release
+    void retire(int g)
+    {
+        Slot& s = slot[(size_t)g];
+        Request* r = s.r;
+        int rc = pra_rel_count(s.T, PRA_FULL, &r->res.tuples);
+        if (rc == PRA_OK) rc = pra_ctx_group_joined(ctx, g, &r->res.joined);
+        if (rc == PRA_OK) rc = pra_ctx_group_status(ctx, g, nullptr, &r->res.iterations, nullptr);
         if (rc == PRA_OK && keep_rows) {
             r->rows.resize(r->res.tuples * 3);
             uint64_t got = 0;
-            rc = pra_rel_download(T, PRA_FULL, r->rows.data(), r->res.tuples, &got);
+            rc = pra_rel_download(s.T, PRA_FULL, r->rows.data(), r->res.tuples, &got);
         }
-        if (c) pra_ctx_destroy(c);
-        r->res.status = rc;
+        const int rel = pra_ctx_release_group(ctx, g);
+        s = Slot{};
+        finish(r, rc != PRA_OK ? rc : rel);
     }
 
-    void worker()
+    // a failed step leaves the context unusable: every request in flight fails
+    // and the next ones start on a fresh context
+    void reset_context(int rc)
     {
+        for (Slot& s : slot)
+            if (s.r) finish(s.r, rc);
+        slot.assign((size_t)slots, Slot{});
+        pra_ctx_destroy(ctx);
+        ctx = nullptr;
+        if (pra_ctx_create(device, nb, 0, &ctx) == PRA_OK) pra_ctx_set_batch_size(ctx, 1);
+    }
+
+    void run()
+    {
+        slot.assign((size_t)slots, Slot{});
+        const int created = pra_ctx_create(device, nb, 0, &ctx);
+        if (created == PRA_OK) pra_ctx_set_batch_size(ctx, 1);
         for (;;) {
-            Request* r = nullptr;
+            std::vector<Request*> admit;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return (stopping && queue.empty()) || admissible(); });
-                if (!admissible()) return;
-                r = queue.front();
-                queue.pop_front();
-                inflight += r->est;
-                r->res.t_start = seconds_since(t0);
+                cv_work.wait(lk, [&] { return stopping || !queue.empty() || in_flight(); });
+                if (stopping && queue.empty() && !in_flight()) break;
+                int free_slots = slots - in_flight();
+                while (ctx && free_slots > 0 && !queue.empty() && fits(queue.front())) {
+                    admit.push_back(queue.front());
+                    queue.pop_front();
+                    free_slots--;
+                }
             }
-            solve(r);
-            {
+            if (!ctx) {   // no context could be created: fail what is queued
+                for (Request* r : admit) finish(r, created != PRA_OK ? created : PRA_E_HIP);
                 std::lock_guard<std::mutex> lk(mu);
-                inflight -= r->est;
-                r->res.t_done = seconds_since(t0);
-                r->edges.clear();
-                r->edges.shrink_to_fit();
-                r->done = true;
+                for (Request* r : queue) {
+                    r->res.status = PRA_E_HIP;
+                    r->done = true;
+                }
+                queue.clear();
+                cv_done.notify_all();
+                continue;
             }
-            cv_done.notify_all();
-            cv_work.notify_all();
+            for (Request* r : admit) {
+                int g = 0;
+                while (slot[(size_t)g].r) g++;
+                slot[(size_t)g].r = r;
+                r->res.t_start = seconds_since(t0);
+                const int rc = build(g, r);
+                if (rc != PRA_OK) {
+                    pra_ctx_release_group(ctx, g);
+                    slot[(size_t)g] = Slot{};
+                    finish(r, rc);
+                }
+            }
+            if (!in_flight()) continue;
+            uint64_t active = 0;
+            const int rc = pra_ctx_step(ctx, &active);
+            if (rc != PRA_OK) {
+                reset_context(rc);
+                continue;
+            }
+            for (int g = 0; g < slots; g++) {
+                if (!slot[(size_t)g].r) continue;
+                int done = 0;
+                pra_ctx_group_status(ctx, g, &done, nullptr, nullptr);
+                if (done) retire(g);
+            }
         }
+        if (ctx) pra_ctx_destroy(ctx);
+        ctx = nullptr;
     }
 };
 
@@ -120,14 +226,14 @@ extern "C" {
 
 int pra_server_create(int device, int nbuckets, int slots, uint64_t hbm_budget, int keep_rows, pra_server** out)
 {
-    if (!out || slots < 1 || slots > 64 || nbuckets < 1 || nbuckets > 256) return PRA_E_ARG;
+    if (!out || slots < 1 || slots > 4096 || nbuckets < 1 || nbuckets > 256) return PRA_E_ARG;
     pra_server* s = new pra_server();
     s->device = device;
     s->nb = nbuckets;
     s->slots = slots;
     s->budget = hbm_budget;
     s->keep_rows = keep_rows;
-    for (int i = 0; i < slots; i++) s->workers.emplace_back([s] { s->worker(); });
+    s->loop = std::thread([s] { s->run(); });
     *out = s;
     return PRA_OK;
 }
@@ -138,7 +244,7 @@ int pra_server_submit_tc(pra_server* s, const uint64_t* edge_rows, uint64_t m, u
     std::unique_ptr<Request> r(new Request());
     r->edges.assign(edge_rows, edge_rows + m * 3);
     r->m = m;
-    r->est = est_bytes;
+    r->est = est_bytes ? est_bytes : default_estimate(m);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stopping) return PRA_E_STATE;
@@ -185,7 +291,7 @@ int pra_server_destroy(pra_server* s)
         s->stopping = true;
     }
     s->cv_work.notify_all();
-    for (auto& t : s->workers) t.join();
+    if (s->loop.joinable()) s->loop.join();
     delete s;
     return PRA_OK;
 }

@@ -24,6 +24,7 @@ EXPORTS = (
     "pra_ctx_create", "pra_ctx_destroy", "pra_last_error", "pra_ctx_set_batch_size",
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
     "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes", "pra_ctx_set_group", "pra_ctx_group_joined",
+    "pra_ctx_step", "pra_ctx_group_status", "pra_ctx_release_group",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
     "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
     "pra_scc_add_relation", "pra_scc_add_join", "pra_scc_add_copy", "pra_scc_add_acopy",
@@ -75,6 +76,9 @@ def lib():
             "pra_ctx_device_bytes": [_vp, _u64p, _u64p],
             "pra_ctx_set_group": [_vp, ctypes.c_int],
             "pra_ctx_group_joined": [_vp, ctypes.c_int, _u64p],
+            "pra_ctx_step": [_vp, _u64p],
+            "pra_ctx_group_status": [_vp, ctypes.c_int, _intp, _u64p, _u64p],
+            "pra_ctx_release_group": [_vp, ctypes.c_int],
             "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                ctypes.POINTER(_vp)],
             "pra_rel_destroy": [_vp],
@@ -137,6 +141,9 @@ class Context:
         self.rels = []
         self.sccs = []
         self.scc_rels = {}
+        self.cur_group = 0
+        self._group_of = {}       # relation handle -> instance
+        self._scc_group = {}      # scc handle -> instance
         self.set_batch_size(batch_size)
 
     def _chk(self, rc, what):
@@ -193,11 +200,31 @@ class Context:
     def set_group(self, g):
         """build program instance g with the following rel_create / scc_create calls"""
         self._chk(self.L.pra_ctx_set_group(self.c, int(g)), "set_group")
+        self.cur_group = int(g)
 
     def group_joined(self, g):
         v = _u64()
         self._chk(self.L.pra_ctx_group_joined(self.c, int(g), ctypes.byref(v)), "group_joined")
         return int(v.value)
+
+    def step(self):
+        """one lock-step iteration over the instances; returns the instances with work left"""
+        v = _u64()
+        self._chk(self.L.pra_ctx_step(self.c, ctypes.byref(v)), "step")
+        return int(v.value)
+
+    def group_status(self, g):
+        """(finished, iterations, device bytes of its relations) of instance g"""
+        d, it, by = ctypes.c_int(), _u64(), _u64()
+        self._chk(self.L.pra_ctx_group_status(self.c, int(g), ctypes.byref(d), ctypes.byref(it), ctypes.byref(by)),
+                  "group_status")
+        return bool(d.value), int(it.value), int(by.value)
+
+    def release_group(self, g):
+        """free instance g's relations and SCCs (their handles become invalid) and reset it"""
+        self._chk(self.L.pra_ctx_release_group(self.c, int(g)), "release_group")
+        self.rels = [None if self._group_of.get(i) == g else r for i, r in enumerate(self.rels)]
+        self.sccs = [None if self._scc_group.get(i) == g else s for i, s in enumerate(self.sccs)]
 
     def device_bytes(self):
         """(bytes held now, peak bytes held) on the device by this context"""
@@ -234,6 +261,7 @@ class Context:
         self.rels.append(h)
         r = len(self.rels) - 1
         self.arity[r] = arity
+        self._group_of[r] = self.cur_group
         return r
 
     def load_rows(self, rel, rows, version=FULL):
@@ -274,6 +302,7 @@ class Context:
         self.sccs.append(h)
         s = len(self.sccs) - 1
         self.scc_rels[s] = []
+        self._scc_group[s] = self.cur_group
         return s
 
     def scc_add_relation(self, scc, rel, move_full_to_delta):

@@ -240,6 +240,72 @@ def bfs_distance_histogram(pairs):
     return total, [hist.get(d, 0) for d in range(2, maxd + 1)] + [0]
 
 
+def tc_size_condensation(pairs):
+    """|TC| of a directed graph without enumerating paths: strongly connected
+    components (scipy), then reachability sets as bitsets over the vertices,
+    propagated over the condensation DAG in reverse topological order; every
+    vertex of a component reaches everything its component reaches, and a
+    component reaches its own members only through a cycle (size > 1 or a
+    self-loop).  Returns (|TC|, number of vertices)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    p = np.asarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    verts, idx = np.unique(p, return_inverse=True)
+    n = verts.size
+    idx = idx.reshape(-1, 2)
+    a = csr_matrix((np.ones(len(idx), dtype=np.int8), (idx[:, 0], idx[:, 1])), shape=(n, n))
+    ncomp, comp = connected_components(a, directed=True, connection="strong")
+    size = np.bincount(comp, minlength=ncomp)
+    self_loop = np.zeros(ncomp, dtype=bool)
+    self_loop[comp[idx[idx[:, 0] == idx[:, 1], 0]]] = True
+    cu, cv = comp[idx[:, 0]], comp[idx[:, 1]]
+    cross = cu != cv
+    succ = [[] for _ in range(ncomp)]
+    for u, v in set(zip(cu[cross].tolist(), cv[cross].tolist())):
+        succ[u].append(v)
+    # topological order of the condensation (Kahn), then reverse
+    indeg = np.zeros(ncomp, dtype=np.int64)
+    for u in range(ncomp):
+        for v in succ[u]:
+            indeg[v] += 1
+    order = [u for u in range(ncomp) if indeg[u] == 0]
+    for u in order:
+        for v in succ[u]:
+            indeg[v] -= 1
+            if indeg[v] == 0:
+                order.append(v)
+    words = (n + 63) // 64
+    members = [[] for _ in range(ncomp)]
+    for v, c in enumerate(comp):
+        members[c].append(v)
+    own = {}
+
+    def own_bits(c):
+        if c not in own:
+            b = np.zeros(words, dtype=np.uint64)
+            m = np.asarray(members[c], dtype=np.int64)
+            np.bitwise_or.at(b, m >> 6, np.uint64(1) << (m & 63).astype(np.uint64))
+            own[c] = b
+        return own[c]
+
+    reach = {}
+    total = 0
+    for u in reversed(order):
+        r = np.zeros(words, dtype=np.uint64)
+        for v in succ[u]:
+            r |= reach[v]
+            if size[v] == 1:
+                x = members[v][0]
+                r[x >> 6] |= np.uint64(1) << np.uint64(x & 63)
+            else:
+                r |= own_bits(v)
+        if size[u] > 1 or self_loop[u]:
+            r |= own_bits(u)
+        reach[u] = r
+        total += int(size[u]) * int(np.unpackbits(r.view(np.uint8)).sum())
+    return total, n
+
+
 def tc_pairs_bruteforce(pairs):
     """set of (x, z) with a path x -> z"""
     adj = {}

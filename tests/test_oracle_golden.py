@@ -312,3 +312,14 @@ def test_random_programs_bucket_and_batch_invariance():
         h = random_program(o, seed)
         o.execute(4)
         assert all(np.array_equal(o.rows(r), x) for r, x in zip(h["rels"], ids1)), seed
+
+
+def test_tc_size_condensation_matches_bruteforce():
+    """the SCC-condensation |TC| used for config-4-sized checks equals brute
+    force on small graphs with cycles, self-loops and isolated parts"""
+    from programs import tc_size_condensation
+    graphs = [config1_graph(s)[:, :2] for s in range(4)]
+    graphs += [random_graph(n, int(1.3 * n), 31 + n)[:, :2] for n in (50, 400, 1500)]
+    graphs.append(np.array([[1, 1], [1, 2], [2, 3], [3, 2], [7, 8]], dtype=np.uint64))
+    for p in graphs:
+        assert tc_size_condensation(p)[0] == len(tc_pairs_bruteforce(p.tolist()))
