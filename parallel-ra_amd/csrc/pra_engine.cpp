@@ -361,6 +361,15 @@ struct pra_scc {
     std::vector<u64> hist;
 };
 
+// Job-table staging slots.  A slot's pinned buffer is rewritten only after the
+// host has waited for a kernel enqueued behind the slot's previous copy: every
+// table but the early appends' is staged before an iteration's summary
+// publication, or after its wait and before the next one's; the early appends
+// are staged between a publication and its wait, so they alternate between
+// two slots, and the wait of the iteration in between covers each copy.
+enum { JOB_EXPAND, JOB_SCATTER, JOB_DENSE, JOB_SCAN, JOB_FOLD, JOB_SORT, JOB_EARLY_APPEND, JOB_EARLY_APPEND2,
+       JOB_APPEND, JOB_PLAN, PRA_JOB_SLOTS };
+
 struct pra_ctx {
     int device = 0, nb = 1;
     hipStream_t st = nullptr;
@@ -378,21 +387,23 @@ struct pra_ctx {
     DBuf ghist, gofs, lb_status, lb_ctr;
     DBuf part_scratch;   // pra_partition_rows
     // order-preserving value dictionary for packed sort keys
-    DBuf ktmp, idxa, idxb, posadj_dev, rep_words, rctl, ttab, kp, jobtab[4];
+    DBuf ktmp, idxa, idxb, posadj_dev, rep_words, rctl, ttab, kp, jobtab[PRA_JOB_SLOTS];
     int rep_sets = 0;              // summary replica sets in rep_words (one per output of a batched partition)
-    char* job_stage[4] = {};       // pinned staging of batched kernels' job tables (expansions, partitions,
-    size_t job_stage_cap[4] = {};  // dense claims, plans)
+    // pinned staging of batched kernels' job tables, one slot per table of an
+    // iteration (expansions, partitions, dense claims, histogram scans, folds,
+    // bin sorts, early appends, appends)
+    char* job_stage[PRA_JOB_SLOTS] = {};
+    size_t job_stage_cap[PRA_JOB_SLOTS] = {};
+    int early_parity = 0;          // early appends' slot of the last iteration
     bool dense_on = true;        // dense membership for outputs that allow it
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
     u32 sort_epoch = 0;
     u64 dedup_epoch = 0;
     u64* host_stage = nullptr;  // pinned
     size_t host_stage_cap = 0;
-    u64* host_pos = nullptr;    // pinned
     u64* mailbox = nullptr;     // pinned, written by the publish kernel
     size_t mailbox_cap = 0;
     hipEvent_t pub_ev = nullptr;
-    size_t host_pos_cap = 0;
     std::string err;
     // profiling
     u32 prof_mask = 0;           // bit k: time kernel class k
@@ -415,9 +426,9 @@ struct pra_ctx {
         groups.clear();
         for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &rep_words, &rctl, &ttab, &kp, &jobtab[0], &jobtab[1], &jobtab[2], &jobtab[3],
-                        &part_scratch})
+                        &idxa, &idxb, &posadj_dev, &rep_words, &rctl, &ttab, &kp, &part_scratch})
             b->release();
+        for (DBuf& b : jobtab) b.release();
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (st) {
@@ -426,7 +437,6 @@ struct pra_ctx {
             g_acct.erase(st);
         }
         if (host_stage) (void)hipHostFree(host_stage);
-        if (host_pos) (void)hipHostFree(host_pos);
         if (mailbox) (void)hipHostFree(mailbox);
         for (char* js : job_stage)
             if (js) (void)hipHostFree(js);
@@ -493,9 +503,9 @@ struct pra_ctx {
         }
         return host_stage;
     }
-    // pinned staging for job tables copied to the device; the copies of one
-    // iteration have completed before the next iteration writes it again (its
-    // readback waits for the stream), and a larger buffer waits for them too
+    // pinned staging for job tables copied to the device: a slot is rewritten
+    // only after the host has waited for a kernel enqueued behind its previous
+    // copy (see the slot list), and a larger buffer waits for the stream too
     char* job_stage_buf(int slot, size_t bytes)
     {
         if (bytes > job_stage_cap[slot]) {
@@ -521,20 +531,6 @@ struct pra_ctx {
             mailbox_cap = c;
         }
         return mailbox;
-    }
-    u64* host_posadj(size_t words)
-    {
-        if (words > host_pos_cap) {
-            if (host_pos) {
-                HIPCHK(hipStreamSynchronize(st));
-                HIPCHK(hipHostFree(host_pos));
-            }
-            host_pos = nullptr;
-            size_t c = std::max<size_t>(words, 256);
-            HIPCHK(hipHostMalloc((void**)&host_pos, c * sizeof(u64)));
-            host_pos_cap = c;
-        }
-        return host_pos;
     }
     u32* dflags() { return flags_buf.as<u32>(); }
 };
@@ -576,6 +572,35 @@ struct ProfPerCand {
         c->per_candidate.emplace_back(c->pending.size() - 1, per, oi);
     }
 };
+
+// A batched launch's job table: through its pinned staging slot into device
+// memory, ordered on the context stream; returns the device copy.
+static char* stage_table(pra_ctx* c, int slot, const void* data, size_t bytes)
+{
+    char* hs = c->job_stage_buf(slot, bytes);
+    memcpy(hs, data, bytes);
+    c->jobtab[slot].reserve(bytes, c->st);
+    HIPCHK(hipMemcpyAsync(c->jobtab[slot].p, hs, bytes, hipMemcpyHostToDevice, c->st));
+    return c->jobtab[slot].as<char>();
+}
+
+// D2D copies of several ranges: one copy when there is one, else one launch
+static void copy_ranges(pra_ctx* c, int slot, const std::vector<CopyJob>& jobs)
+{
+    if (jobs.empty()) return;
+    if (jobs.size() == 1) {
+        HIPCHK(hipMemcpyAsync(jobs[0].dst, jobs[0].src, jobs[0].words * 8, hipMemcpyDeviceToDevice, c->st));
+        return;
+    }
+    std::vector<char> t(jobs.size() * sizeof(CopyJob) + (jobs.size() + 1) * 4);
+    memcpy(t.data(), jobs.data(), jobs.size() * sizeof(CopyJob));
+    u32* bb = (u32*)(t.data() + jobs.size() * sizeof(CopyJob));
+    bb[0] = 0;
+    for (size_t k = 0; k < jobs.size(); k++) bb[k + 1] = bb[k] + (u32)((jobs[k].words + 2047) / 2048);
+    const char* d = stage_table(c, slot, t.data(), t.size());
+    pra_launch_copy_many(c->st, (const CopyJob*)d, (int)jobs.size(), bb[jobs.size()]);
+    check_launch("copy_many");
+}
 
 // ----------------------------------------------------------------- scans ---
 
@@ -860,7 +885,11 @@ static bool packed_layout(u64 nb, u64 D, int arity, u64* bmul, int* ls)
 // LDS, gathers the rows and writes the bucket bounds (nb + 1 words);
 // otherwise every varying byte gets a one-sweep pass over R->nkeys, followed
 // by a row gather and a bounds search.
-static void sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 maxbin, u64 bmul, int ls, u64* bnd)
+// With `defer`, the bin sort is not launched: its job (writing into
+// `defer_out`, which the caller swaps with newt's buffer once the batch is
+// launched) goes to *defer and true is returned.
+static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 maxbin, u64 bmul, int ls, u64* bnd,
+                               BinSortJob* defer = nullptr, DBuf* defer_out = nullptr)
 {
     Rows& r = R->ver[PRA_NEWT];
     const int w = R->w;
@@ -872,6 +901,13 @@ static void sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         u32 bytes = 0;
         for (int b = 0; b < 7; b++)
             if ((vary >> (8 * b)) & (PRA_RADIX - 1)) bytes |= 1u << b;
+        if (defer) {
+            defer_out->reserve(n * w * 8, c->st);
+            *defer = BinSortJob{R->bin_keys.as<u64>(), R->bin_idx.as<u32>(), top_ofs, r.d(), defer_out->as<u64>(), bnd,
+                                n, bmul, (u64)c->nb, R->bin_cap, bytes, w, ls};
+            r.sorted = true;
+            return true;
+        }
         c->sort_tmp.reserve(n * w * 8, c->st);
         {
             Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
@@ -881,7 +917,7 @@ static void sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         }
         r.buf.swap(c->sort_tmp);
         r.sorted = true;
-        return;
+        return false;
     }
     std::vector<DigitSpec> passes;
     for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
@@ -931,6 +967,7 @@ static void sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         pra_launch_fill_u64(c->st, bnd, 1, 0);
         pra_launch_fill_u64(c->st, bnd + 1, 1, n);
     }
+    return false;
 }
 
 static int ceil_log2(u64 x)
@@ -1161,14 +1198,18 @@ static DBuf& seq_counters(pra_ctx* c, int gi, u32 tag)
     return ref;
 }
 
-// append src rows (already on device) to a version
-static void append_rows(pra_ctx* c, pra_rel* R, int v, const u64* src, u64 n)
+// append src rows (already on device) to a version; with `jobs`, the copy is
+// left to the caller's batched copy (the version grows at once)
+static void append_rows(pra_ctx* c, pra_rel* R, int v, const u64* src, u64 n, std::vector<CopyJob>* jobs = nullptr)
 {
     if (!n) return;
     Rows& r = R->ver[v];
     size_t used = (size_t)r.n * R->w * 8;
     r.buf.reserve(used + (size_t)n * R->w * 8, c->st, used);
-    HIPCHK(hipMemcpyAsync(r.d() + r.n * R->w, src, (size_t)n * R->w * 8, hipMemcpyDeviceToDevice, c->st));
+    if (jobs)
+        jobs->push_back(CopyJob{src, r.d() + r.n * R->w, n * (u64)R->w});
+    else
+        HIPCHK(hipMemcpyAsync(r.d() + r.n * R->w, src, (size_t)n * R->w * 8, hipMemcpyDeviceToDevice, c->st));
     r.n += n;
     r.sorted = false;
     R->touched(v);
@@ -1628,8 +1669,10 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
     // dense membership: taken per output, switched off for the rest of the
     // iteration by a dictionary miss; the dense claims of a batched iteration
     std::vector<char> use_dense(no, 0), dense_off(no, 0);
+    std::vector<ScanJob> scan_jobs;
     std::vector<std::pair<DenseArgs, size_t>> dense_jobs;
-    // phase 0: one output start to end; 1: up to its candidates, 2: from its
+    // phase 0: one output start to end; 1: up to its candidates, 2: its
+    // histogram scan, 3: from its partition on
     // kept counts on (the claims and decisions in between batched over the
     // outputs, each in its own region of the candidate buffers)
     auto run_output = [&](size_t oi, const RoundSpec* rs, bool planned = false, int phase = 0) {
@@ -1646,7 +1689,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         u64* ctl = st2o + 4;
         // dense membership unless the output was redone the general way in
         // this iteration, or this is a threshold round
-        const bool dn = phase != 2 ? (!rs && !dense_off[oi] && packed[oi] && ensure_dense(c, R, kw, C))
+        const bool dn = phase < 2 ? (!rs && !dense_off[oi] && packed[oi] && ensure_dense(c, R, kw, C))
                                    : use_dense[oi] != 0;
         use_dense[oi] = dn;
         if (dn)
@@ -1667,7 +1710,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         u32* keep0 = c->keep.as<u32>() + rg.rb;
         u32* hist0 = c->hist.as<u32>() + rg.hb;
         u32* offs0 = c->offs.as<u32>() + rg.hb;
-        if (phase != 2) {
+        if (phase < 2) {
             if (!planned) {
                 PlanArgs pa;
                 memset(&pa, 0, sizeof(pa));
@@ -1817,7 +1860,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                 }
             }
         }
-        if (phase != 2 && !dn) {
+        if (phase < 2 && !dn) {
             u64* tab = R->mt.as<u64>();
             const u64 epoch = ++c->dedup_epoch;
             if (phase == 1) {
@@ -1843,7 +1886,17 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
             }
         }
         u32* total_dev = offs0 + (size_t)nb * nblocks;
-        if (nblocks > 1) scan_u32(c, hist0, offs0, (u64)nb * nblocks, total_dev);
+        if (phase == 2) {
+            // the small histogram scans of a batched iteration go out together
+            if (nblocks > 1) {
+                if ((u64)nb * nblocks <= PRA_SMALL_SCAN)
+                    scan_jobs.push_back(ScanJob{hist0, offs0, total_dev, (u64)nb * nblocks});
+                else
+                    scan_u32(c, hist0, offs0, (u64)nb * nblocks, total_dev);
+            }
+            return;
+        }
+        if (phase == 0 && nblocks > 1) scan_u32(c, hist0, offs0, (u64)nb * nblocks, total_dev);
         DBuf* sq = intern ? &seq_counters(c, R->group, R->tag) : nullptr;
         R->kgofs.reserve(PRA_RADIX * 4, c->st);
         NewtFoldArgs f;   // bin_cur: set with the scatter's bins
@@ -1912,7 +1965,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
             // column masks only pick the sort passes of outputs too large for
             // the one-block sort, which compares whole key words
             a.summarize = C > PRA_SMALL_SORT || C * (u64)(R->w + 1) > (u64)PRA_SMALL_SORT * PRA_SMALL_SORT_KW;
-            if (phase == 2 && nblocks < PRA_SCATTER_WIDE_TILES) {
+            if (phase == 3 && nblocks < PRA_SCATTER_WIDE_TILES) {
                 // narrow partitions of a batched iteration go out together
                 scatter_jobs[pk ? 1 : 0].push_back(a);
                 scatter_outs[pk ? 1 : 0].push_back(oi);
@@ -1934,10 +1987,39 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
             check_launch("newt_fold");
         }
     };
-    // every output's arrival plan in one launch (rule and output indices fit
-    // its per-rule tables)
-    const bool plan_all = nra <= PRA_MAXRA && no <= PRA_MAXRA;
-    if (plan_all) {
+    // every output's arrival plan in one launch: its per-rule tables as kernel
+    // arguments, or, past their size, from tables in device memory
+    const bool plan_all = true;
+    if (nra > PRA_MAXRA || no > PRA_MAXRA) {
+        std::vector<PlanRule> rules((size_t)nra);
+        std::vector<u32> ids;
+        std::vector<PlanOut> pouts;
+        for (int r = 0; r < nra; r++) {
+            RADesc& ra = S->ras[r];
+            if (ra.active)
+                rules[(size_t)r] = PlanRule{ra.off.as<u64>(), ra.outer->bnd[ra.outer_v].as<u64>(),
+                                            ra.type == RA_JOIN ? 1u : 0u, 0u};
+        }
+        for (size_t oi = 0; oi < no; oi++) {
+            if (!cap[oi]) continue;
+            PlanOut po{cap[oi], grp(c, outs[oi]->group).joined.as<u64>(), (u32)ids.size(), 0, (u32)oi, 0};
+            for (int r = 0; r < nra; r++)
+                if (S->ras[r].active && S->ras[r].out == outs[oi]) ids.push_back((u32)r);
+            po.count = (u32)ids.size() - po.first;
+            pouts.push_back(po);
+        }
+        const size_t rb = (rules.size() * sizeof(PlanRule) + 255) / 256 * 256;
+        const size_t ib = (ids.size() * 4 + 255) / 256 * 256;
+        std::vector<char> t(rb + ib + pouts.size() * sizeof(PlanOut));
+        memcpy(t.data(), rules.data(), rules.size() * sizeof(PlanRule));
+        memcpy(t.data() + rb, ids.data(), ids.size() * 4);
+        memcpy(t.data() + rb + ib, pouts.data(), pouts.size() * sizeof(PlanOut));
+        const char* d = stage_table(c, JOB_PLAN, t.data(), t.size());
+        PlanTabArgs pa{c->stage.as<u64>(), c->posadj_dev.as<u64>(), c->stage2.as<u64>(), c->threshold, nb, (int)SW,
+                       (const PlanRule*)d, (const u32*)(d + rb), (const PlanOut*)(d + rb + ib)};
+        pra_launch_plan_tab(c->st, pa, (int)pouts.size());
+        check_launch("plan_tab");
+    } else {
         PlanAllArgs pl;
         memset(&pl, 0, sizeof(pl));
         pl.srcoff = c->stage.as<u64>();
@@ -2133,6 +2215,14 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         }
         for (size_t oi = 0; oi < no; oi++)
             if (cap[oi]) run_output(oi, nullptr, true, 2);
+        if (!scan_jobs.empty()) {
+            Prof p(c, K_SCAN, 0.0);
+            const char* d = stage_table(c, JOB_SCAN, scan_jobs.data(), scan_jobs.size() * sizeof(ScanJob));
+            pra_launch_scan_small_many(c->st, (const ScanJob*)d, (int)scan_jobs.size());
+            check_launch("scan_small_many");
+        }
+        for (size_t oi = 0; oi < no; oi++)
+            if (cap[oi]) run_output(oi, nullptr, true, 3);
         // the batched partitions: job tables through pinned staging into
         // device memory, one launch per key kind
         size_t stage_bytes = 0;
@@ -2166,9 +2256,13 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                 at += bytes;
             }
         }
-        for (const NewtFoldArgs& f : later_folds) {
-            pra_launch_newt_fold(c->st, f);
+        if (later_folds.size() == 1) {
+            pra_launch_newt_fold(c->st, later_folds[0]);
             check_launch("newt_fold");
+        } else if (!later_folds.empty()) {
+            const char* d = stage_table(c, JOB_FOLD, later_folds.data(), later_folds.size() * sizeof(NewtFoldArgs));
+            pra_launch_newt_fold_many(c->st, (const NewtFoldArgs*)d, (int)later_folds.size());
+            check_launch("newt_fold_many");
         }
     } else {
         for (size_t oi = 0; oi < no; oi++)
@@ -2187,13 +2281,16 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
     // rank (now the general way); pass 2 the few that needed both
     for (int pass = 0; pass < 3; pass++) {
         u64* h = publish_begin(c, c->stage2.as<u64>(), no * SW, pass == 0 && have_last_fold ? &last_fold : nullptr);
-        if (pass == 0)
+        if (pass == 0) {
+            std::vector<CopyJob> appends;
             for (size_t i = 0; i < S->rels.size(); i++) {
                 pra_rel* R = S->rels[i];
                 if (R->full_read || R->merge_full || !R->ver[PRA_DELTA].n) continue;
-                append_rows(c, R, PRA_FULL, R->ver[PRA_DELTA].d(), R->ver[PRA_DELTA].n);
+                append_rows(c, R, PRA_FULL, R->ver[PRA_DELTA].d(), R->ver[PRA_DELTA].n, &appends);
                 early[i] = 1;
             }
+            copy_ranges(c, (c->early_parity ^= 1) ? JOB_EARLY_APPEND2 : JOB_EARLY_APPEND, appends);
+        }
         h = publish_wait(c, h, no * SW);
         std::vector<size_t> redo;
         for (size_t oi = 0; oi < no; oi++) {
@@ -2241,15 +2338,21 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
     // 6. delta -> full, sort newt into trie order, newt -> delta
     //    (src/RAM/RA_tasks.cpp:1044-1058).  The old delta is folded into full
     //    first: the newt sort writes the next delta's bucket bounds.
-    for (size_t i = 0; i < S->rels.size(); i++) {
-        pra_rel* R = S->rels[i];
-        Rows& D = R->ver[PRA_DELTA];
-        if (R->merge_full)
-            merge_into_full(c, R);
-        else if (!early[i])
-            append_rows(c, R, PRA_FULL, D.d(), D.n);
-        D.n = 0;
-        R->touched(PRA_DELTA);
+    {
+        std::vector<CopyJob> appends;
+        for (size_t i = 0; i < S->rels.size(); i++) {
+            pra_rel* R = S->rels[i];
+            Rows& D = R->ver[PRA_DELTA];
+            if (R->merge_full)
+                merge_into_full(c, R);
+            else if (!early[i])
+                append_rows(c, R, PRA_FULL, D.d(), D.n, &appends);
+        }
+        copy_ranges(c, JOB_APPEND, appends);
+        for (pra_rel* R : S->rels) {
+            R->ver[PRA_DELTA].n = 0;
+            R->touched(PRA_DELTA);
+        }
     }
     std::vector<char> bnd_ok(no, 0);
     // one-block sorts of all outputs go out together, into the emptied delta
@@ -2258,6 +2361,12 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
     memset(&ssb, 0, sizeof(ssb));
     ssb.nb = (u64)nb;
     std::vector<size_t> ss_out;
+    // several outputs sorting by packed key: their bin sorts go out as one
+    // launch, each into its emptied delta buffer; one sorts at once
+    std::vector<BinSortJob> bin_jobs;
+    std::vector<size_t> bin_out;
+    size_t n_packed_sorts = 0;
+    for (size_t oi = 0; oi < no; oi++) n_packed_sorts += packed[oi] && !in_rounds[oi] && st2[oi * SW + 3] == 0;
     auto flush_small_sorts = [&]() {
         if (ss_out.empty()) return;
         double bytes = 0;
@@ -2279,7 +2388,12 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         R->bnd[PRA_DELTA].reserve((size_t)(nb + 1) * 8, c->st);
         u64* bnd = R->bnd[PRA_DELTA].as<u64>();
         if (packed[oi] && !in_rounds[oi] && st2[oi * SW + 3] == 0) {
-            sort_by_packed_key(c, R, st2[oi * SW + 1], st2[oi * SW + 2], st2[oi * SW + 6], pk_mul[oi], pk_ls[oi], bnd);
+            BinSortJob job;
+            if (sort_by_packed_key(c, R, st2[oi * SW + 1], st2[oi * SW + 2], st2[oi * SW + 6], pk_mul[oi], pk_ls[oi],
+                                   bnd, n_packed_sorts > 1 ? &job : nullptr, &R->ver[PRA_DELTA].buf)) {
+                bin_jobs.push_back(job);
+                bin_out.push_back(oi);
+            }
             bnd_ok[oi] = 1;
             continue;
         }
@@ -2303,6 +2417,22 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         }
     }
     flush_small_sorts();
+    // the bin sorts of all packed outputs, into the emptied delta buffers
+    if (!bin_jobs.empty()) {
+        double bytes = 0;
+        for (const BinSortJob& j : bin_jobs) bytes += 12.0 * j.n + 16.0 * j.n * j.w;
+        Prof p(c, K_BIN_SORT, bytes);
+        if (bin_jobs.size() == 1) {
+            const BinSortJob& j = bin_jobs[0];
+            pra_launch_bin_sort(c->st, j.bin_keys, j.bin_idx, j.bin_cap, j.n, j.bofs, j.bytes, j.rows, j.w, j.out,
+                                j.bmul, j.ls, j.nb, j.bnd, c->dflags());
+        } else {
+            const char* d = stage_table(c, JOB_SORT, bin_jobs.data(), bin_jobs.size() * sizeof(BinSortJob));
+            pra_launch_bin_sort_many(c->st, (const BinSortJob*)d, (int)bin_jobs.size(), c->dflags());
+        }
+        check_launch("bin_sort");
+        for (size_t oi2 : bin_out) outs[oi2]->ver[PRA_NEWT].buf.swap(outs[oi2]->ver[PRA_DELTA].buf);
+    }
     for (pra_rel* R : S->rels) {
         Rows& D = R->ver[PRA_DELTA];
         D.buf.swap(R->ver[PRA_NEWT].buf);

@@ -487,9 +487,9 @@ __global__ void k_gather_rows(const u64* in, const u32* idx, u64 n, int w, u64* 
 // and is left unsorted).  Bucket s starts at the first key >= (s * bmul) <<
 // ls (the key is bucket * bmul + column ranks, shifted left by ls); the block
 // whose bin holds that value counts its keys below it.
-__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys, const u32* bin_idx, u32 bin_cap,
-                                                            u64 n, const u32* bofs, u32 bytes, const u64* rows, int w,
-                                                            u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
+__device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n,
+                                                   const u32* bofs, u32 bytes, const u64* rows, int w, u64* out,
+                                                   u64 bmul, int ls, u64 nb, u64* bnd, u32* flags, const u32 b)
 {
     constexpr int IT = PRA_BIN_ITEMS;
     constexpr int WAVES = PRA_BIN_BLOCK / 64;
@@ -499,7 +499,6 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys,
     __shared__ u32 wh[WAVES][PRA_RADIX];
     __shared__ u32 wsum[PRA_RADIX / 64];
     __shared__ u32 below;
-    const u32 b = blockIdx.x;
     if (b == 0 && threadIdx.x == 0) {
         bnd[0] = 0;
         bnd[nb] = n;
@@ -624,6 +623,23 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys,
             pra_copy_row(dst, src, w);
         }
     }
+}
+
+__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys, const u32* bin_idx, u32 bin_cap,
+                                                            u64 n, const u32* bofs, u32 bytes, const u64* rows, int w,
+                                                            u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
+{
+    pra_bin_sort_block(bin_keys, bin_idx, bin_cap, n, bofs, bytes, rows, w, out, bmul, ls, nb, bnd, flags,
+                       blockIdx.x);
+}
+
+// The bin sorts of several outputs in one launch: block b sorts bin b % 256
+// of job b / 256 (job table in device memory).
+__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort_many(const BinSortJob* jobs, u32* flags)
+{
+    const BinSortJob& j = jobs[blockIdx.x / PRA_RADIX];
+    pra_bin_sort_block(j.bin_keys, j.bin_idx, j.bin_cap, j.n, j.bofs, j.bytes, j.rows, j.w, j.out, j.bmul, j.ls, j.nb,
+                       j.bnd, flags, blockIdx.x % PRA_RADIX);
 }
 
 // bnd[s] = first position whose key is >= (s * bmul) << ls (the key is
@@ -1094,6 +1110,41 @@ __global__ void k_plan_all(PlanAllArgs a)
     ctl[0] = tot;
     ctl[1] = rounds ? 2 : tot > a.cap[oi] ? 1 : 0;
     if ((rounds || tot <= a.cap[oi]) && jtot) atomicAdd((unsigned long long*)a.joined[oi], (unsigned long long)jtot);
+}
+
+__global__ void k_plan_tab(PlanTabArgs a)
+{
+    const PlanOut o = a.outs[blockIdx.x];
+    const int nb = a.nb;
+    for (u32 e = threadIdx.x; e < o.count * (u32)nb; e += blockDim.x) {
+        const u32 r = a.rule_ids[o.first + e / nb];
+        const int s = (int)(e % nb);
+        const PlanRule& pr = a.rules[r];
+        const u64 lo = pr.off[pr.bnd[s]], hi = pr.off[pr.bnd[s + 1]];
+        a.srcoff[(u64)r * 2 * nb + s] = lo;
+        a.srcoff[(u64)r * 2 * nb + nb + s] = hi - lo;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    u64 tot = 0, jtot = 0;
+    bool rounds = false;
+    for (int s = 0; s < nb; s++)
+        for (u32 k = 0; k < o.count; k++) {
+            const u32 r = a.rule_ids[o.first + k];
+            const u64* so = a.srcoff + (u64)r * 2 * nb;
+            a.posadj[(u64)r * nb + s] = tot - so[s];
+            const u64 cnt = so[nb + s];
+            if (a.rules[r].is_join) {
+                if ((long long)cnt > a.thresh) rounds = true;
+                jtot += cnt;
+            }
+            tot += cnt;
+        }
+    u64* ctl = a.stage2 + (u64)o.oi * a.sw + 4;
+    ctl[0] = tot;
+    ctl[1] = rounds ? 2 : tot > o.cap ? 1 : 0;
+    if ((rounds || tot <= o.cap) && jtot) atomicAdd((unsigned long long*)o.joined, (unsigned long long)jtot);
 }
 
 // Split of one threshold round: the first outer row i (of n, offsets off
@@ -1757,6 +1808,17 @@ __global__ __launch_bounds__(PRA_BLOCK) void k_dedup_scatter_many(const NewtScat
     pra_scatter_block<PACKED, PRA_BLOCK>(a, blockIdx.x - block_base[k]);
 }
 
+// The folds of several outputs in one launch: block b runs part b % 3 of job
+// b / 3 (job table in device memory).
+__global__ void k_newt_fold_many(const NewtFoldArgs* jobs)
+{
+    const NewtFoldArgs& a = jobs[blockIdx.x / 3];
+    if (a.ctl[1]) return;
+    __shared__ u32 lds[PRA_WAVES + 1];
+    __shared__ u32 mx;
+    pra_newt_fold_part(a, blockIdx.x % 3, threadIdx.x, lds, &mx);
+}
+
 // One block per part of pra_newt_fold_part.
 __global__ void k_newt_fold(NewtFoldArgs a)
 {
@@ -1906,6 +1968,49 @@ __global__ void k_scan_small(const T* in, T* out, u64 n, T* total)
         ex += v[k];
     }
     if (threadIdx.x == 0 && total) *total = tot;
+}
+
+// Exclusive scans of several u32 arrays of at most PRA_SMALL_SCAN elements in
+// one launch, one block each (job table in device memory).
+__global__ void k_scan_small_many(const ScanJob* jobs)
+{
+    const ScanJob& j = jobs[blockIdx.x];
+    constexpr int PER = PRA_SMALL_SCAN / PRA_BLOCK;
+    __shared__ u32 lds[PRA_WAVES + 1];
+    const u64 base = (u64)threadIdx.x * PER;
+    u32 v[PER];
+    u32 sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        v[k] = base + k < j.n ? j.in[base + k] : 0;
+        sum += v[k];
+    }
+    u32 tot;
+    u32 ex = block_exclusive_scan<u32>(sum, lds, &tot);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        if (base + k < j.n) j.out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0 && j.total) *j.total = tot;
+}
+
+// Word copies of several ranges in one launch: job k takes blocks
+// [block_base[k], block_base[k + 1]), 8 words per thread (block_base follows
+// the job table in device memory).
+__global__ void k_copy_many(const CopyJob* jobs, int njobs)
+{
+    const u32* block_base = (const u32*)(jobs + njobs);
+    const int k = pra_job_of(block_base, njobs, blockIdx.x);
+    const CopyJob j = jobs[k];
+    const u64 i0 = ((u64)(blockIdx.x - block_base[k]) * PRA_BLOCK + threadIdx.x) * 8;
+    u64 t[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (i0 + q < j.words) t[q] = j.src[i0 + q];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (i0 + q < j.words) j.dst[i0 + q] = t[q];
 }
 
 // Exclusive prefix of `tile` over its predecessors' published values (digit
@@ -2267,6 +2372,11 @@ void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks)
     if (nblocks > 0) hipLaunchKernelGGL(k_plan_all, dim3(nblocks), dim3(PRA_BLOCK), 0, st, a);
 }
 
+void pra_launch_plan_tab(hipStream_t st, const PlanTabArgs& a, int nouts)
+{
+    if (nouts > 0) hipLaunchKernelGGL(k_plan_tab, dim3(nouts), dim3(PRA_BLOCK), 0, st, a);
+}
+
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out)
 {
     hipLaunchKernelGGL(k_find_split, dim3(1), dim3(1), 0, st, off, n, kp, thresh, out);
@@ -2559,6 +2669,27 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
             PRA_DECIDE(0, false);
     }
 #undef PRA_DECIDE
+}
+
+void pra_launch_bin_sort_many(hipStream_t st, const BinSortJob* jobs, int njobs, u32* flags)
+{
+    if (njobs > 0)
+        hipLaunchKernelGGL(k_bin_sort_many, dim3(PRA_RADIX * njobs), dim3(PRA_BIN_BLOCK), 0, st, jobs, flags);
+}
+
+void pra_launch_scan_small_many(hipStream_t st, const ScanJob* jobs, int njobs)
+{
+    if (njobs > 0) hipLaunchKernelGGL(k_scan_small_many, dim3(njobs), dim3(PRA_BLOCK), 0, st, jobs);
+}
+
+void pra_launch_copy_many(hipStream_t st, const CopyJob* jobs, int njobs, u32 nblocks)
+{
+    if (njobs > 0 && nblocks) hipLaunchKernelGGL(k_copy_many, dim3(nblocks), dim3(PRA_BLOCK), 0, st, jobs, njobs);
+}
+
+void pra_launch_newt_fold_many(hipStream_t st, const NewtFoldArgs* jobs, int njobs)
+{
+    if (njobs > 0) hipLaunchKernelGGL(k_newt_fold_many, dim3(3 * njobs), dim3(PRA_BLOCK), 0, st, jobs);
 }
 
 void pra_launch_newt_fold(hipStream_t st, const NewtFoldArgs& a)

@@ -116,6 +116,31 @@ struct PlanAllArgs {
     unsigned char outs[PRA_MAXRA];   // per block: the output it plans
 };
 
+// k_plan_all for any number of rules and outputs, from tables in device
+// memory: block b plans outs[b] over rules rule_ids[first .. first + count)
+// (ascending), with srcoff / posadj indexed by the rule ids.
+struct PlanRule {
+    const u64* off;
+    const u64* bnd;
+    u32 is_join, pad;
+};
+struct PlanOut {
+    u64 cap;
+    u64* joined;
+    u32 first, count, oi, pad;
+};
+struct PlanTabArgs {
+    u64* srcoff;
+    u64* posadj;
+    u64* stage2;
+    long long thresh;
+    int nb, sw;
+    const PlanRule* rules;
+    const u32* rule_ids;
+    const PlanOut* outs;
+};
+void pra_launch_plan_tab(hipStream_t st, const PlanTabArgs& a, int nouts);
+
 // Membership claims and decisions of several output relations in one launch
 // each (one job per output; blocks [block_base[k], block_base[k + 1]) serve
 // job k)
@@ -164,6 +189,30 @@ struct DenseArgs {
     u64 cap;
     u32 nblocks, nb;
     int cw, kw;
+};
+
+// job tables (device memory) of the batched launches of a lock-step iteration
+struct BinSortJob {
+    const u64* bin_keys;
+    const u32* bin_idx;
+    const u32* bofs;
+    const u64* rows;
+    u64* out;
+    u64* bnd;
+    u64 n, bmul, nb;
+    u32 bin_cap, bytes;
+    int w, ls;
+};
+struct ScanJob {
+    const u32* in;
+    u32* out;
+    u32* total;
+    u64 n;
+};
+struct CopyJob {
+    const u64* src;
+    u64* dst;
+    u64 words;
 };
 
 #define PRA_DEDUP_JOBS 32
@@ -307,6 +356,13 @@ void pra_launch_dense_many(hipStream_t st, const DenseArgs* jobs, int njobs, int
 void pra_launch_dense_build(hipStream_t st, const u64* rows, u64 n, int w, int kw, const u64* dict, u64 dict_cap, u64 D,
                             u32* bitmap, u64* miss);
 void pra_launch_dense_table_clear(hipStream_t st, u64* table, u64 cap);
+// batched forms of k_bin_sort (256 blocks per job), k_scan_small (u32, one
+// block per job of at most PRA_SMALL_SCAN elements), D2D copies (job table
+// followed by njobs + 1 first blocks; 2,048 words per block) and k_newt_fold
+void pra_launch_bin_sort_many(hipStream_t st, const BinSortJob* jobs, int njobs, u32* flags);
+void pra_launch_scan_small_many(hipStream_t st, const ScanJob* jobs, int njobs);
+void pra_launch_copy_many(hipStream_t st, const CopyJob* jobs, int njobs, u32 nblocks);
+void pra_launch_newt_fold_many(hipStream_t st, const NewtFoldArgs* jobs, int njobs);
 void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags);
 void pra_launch_dedup_decide_many(hipStream_t st, const DecideBatch& b, int cls, u32* flags);
 void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,

@@ -97,3 +97,25 @@ def test_lockstep_mixed_programs_equal_oracle(k):
             assert g.group_joined(i) == o.joined()
     finally:
         g.close()
+
+
+def test_lockstep_many_instances_equal_oracle():
+    """100 instances (past the 64 rules and outputs the batched arrival plan
+    takes as kernel arguments, so it plans from tables in device memory)"""
+    import pra_amd
+    graphs = [random_graph(20 + 7 * i, int(1.3 * (20 + 7 * i)), 4000 + i) for i in range(100)]
+    g = pra_amd.Context(nbuckets=3)
+    try:
+        handles = []
+        for i, e in enumerate(graphs):
+            g.set_group(i)
+            handles.append(tc_program(g, e))
+        g.execute()
+        for i in list(range(0, 100, 9)) + [99]:
+            T, G, sfix, scopy = handles[i]
+            o, (To, Go, sfo, sco) = oracle_run(3, lambda b: tc_program(b, graphs[i]))
+            np.testing.assert_array_equal(g.rows(T), o.rows(To))
+            assert g.history(sfix) == o.history(sfo)
+            assert g.group_joined(i) == o.joined()
+    finally:
+        g.close()
