@@ -36,6 +36,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <array>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -120,18 +121,45 @@ inline int env_int(const char* name, int dflt)
 }
 
 /* Map a copy_filter predicate over columns 0 and 1 to a backend predicate code
- * by evaluating it on probe rows; predicates that are not one of the
- * comparisons below are rejected (a host function cannot run on the GPU). */
+ * by evaluating it on probe rows: every pair of a set of edge values (0, 1,
+ * 2, 5, 2^63 - 1, 2^63, 2^64 - 1), then 400 pseudo-random rows (equal, adjacent
+ * and unrelated column-0/1 values), every row with pseudo-random values in
+ * columns 2..7 (every other row below 16).  A predicate that disagrees with all of the comparisons below
+ * on any probe -- one that is not such a comparison, or reads other columns --
+ * is rejected (a host function cannot run on the GPU). */
+inline u64 splitmix64(u64& x)
+{
+    u64 z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
 inline int classify_filter(bool (*f)(const u64* const))
 {
-    static const u64 probes[][2] = {{5, 5}, {4, 5}, {5, 4}, {0, 0}, {1, 2}, {100, 3}, {0x8000000000000001ULL, 7},
-                                    {7, 0xFFFFFFFFFFFFFFFFULL}, {0xFFFFFFFFFFFFFFFFULL, 0xFFFFFFFFFFFFFFFFULL}};
+    static const u64 edge[] = {0, 1, 2, 5, 0x7FFFFFFFFFFFFFFFULL, 0x8000000000000000ULL, 0xFFFFFFFFFFFFFFFFULL};
+    std::vector<std::array<u64, 8>> probes;
+    u64 seed = 0x5EED5EED5EEDULL;
+    auto row = [&](u64 a, u64 b) {
+        std::array<u64, 8> r{};
+        r[0] = a;
+        r[1] = b;
+        const bool small = probes.size() & 1;
+        for (int k = 2; k < 8; k++) r[k] = small ? splitmix64(seed) % 16 : splitmix64(seed);
+        probes.push_back(r);
+    };
+    for (u64 a : edge)
+        for (u64 b : edge) row(a, b);
+    for (int i = 0; i < 400; i++) {
+        const u64 a = (i & 3) == 0 ? splitmix64(seed) % 16 : splitmix64(seed);
+        const u64 kind = splitmix64(seed) % 4;
+        row(a, kind == 0 ? a : kind == 1 ? a + 1 : kind == 2 ? a - 1 : splitmix64(seed));
+    }
     const int codes[] = {PRA_FILTER_EQ, PRA_FILTER_NEQ, PRA_FILTER_LT, PRA_FILTER_LE,
                          PRA_FILTER_GT, PRA_FILTER_GE, PRA_FILTER_TRUE, PRA_FILTER_FALSE};
     for (int code : codes) {
         bool ok = true;
         for (const auto& p : probes) {
-            u64 row[8] = {p[0], p[1], 0, 0, 0, 0, 0, 0};
             bool want;
             switch (code) {
             case PRA_FILTER_EQ: want = p[0] == p[1]; break;
@@ -143,11 +171,27 @@ inline int classify_filter(bool (*f)(const u64* const))
             case PRA_FILTER_TRUE: want = true; break;
             default: want = false; break;
             }
-            if (f(row) != want) { ok = false; break; }
+            if (f(p.data()) != want) {
+                ok = false;
+                break;
+            }
         }
         if (ok) return code;
     }
     return -1;
+}
+
+/* tuple_hash (src/hash/hash.h:11-30): FNV-1a-64 over the little-endian bytes
+ * of the first n words */
+inline u64 tuple_hash_host(const u64* t, int n)
+{
+    u64 h = 14695981039346656037ULL;
+    for (int i = 0; i < n; i++)
+        for (int b = 0; b < 8; b++) {
+            h ^= (t[i] >> (8 * b)) & 255ULL;
+            h *= 1099511628211ULL;
+        }
+    return h;
 }
 
 } // namespace pra_facade
@@ -243,6 +287,38 @@ public:
         uint64_t got = 0;
         pra_rel_download(handle, version, out.data(), n, &got);
         return out;
+    }
+
+    /* relation::print (src/relation/balanced_hash_relation.cpp:433-522): per
+     * bucket with rows, "<name> <rank> FULL Rows <n> columns <w>" and one
+     * "F [<j>] <words>" line per row in trie order, then the same for delta
+     * ("D <words>"); bucket b lives on rank b of the reference's layout */
+    void print(int nbuckets)
+    {
+        const int w = (int)arity + 1;
+        for (int version : {FULL, DELTA}) {
+            std::vector<u64> r = rows(version);
+            const size_t n = r.size() / (size_t)w;
+            size_t i = 0;
+            for (int b = 0; b < nbuckets; b++) {
+                size_t j = i;
+                while (j < n && (int)(pra_facade::tuple_hash_host(&r[j * w], (int)join_column_count) %
+                                      (u64)nbuckets) == b)
+                    j++;
+                if (j > i)
+                    std::cout << debug_id << " " << b << (version == FULL ? " FULL" : " DELTA") << " Rows " << j - i
+                              << " columns " << w << std::endl;
+                for (size_t k = i; k < j; k++) {
+                    if (version == FULL)
+                        std::cout << "F [" << k - i << "] ";
+                    else
+                        std::cout << "D ";
+                    for (int c = 0; c < w; c++) std::cout << r[k * w + c] << " ";
+                    std::cout << std::endl;
+                }
+                i = j;
+            }
+        }
     }
 
     /* backend side: created by LIE::execute */
@@ -497,6 +573,12 @@ public:
     void add_rule(parallel_RA* pj) { RA_list.push_back(pj); }
     void set_comm(mpi_comm& m) { mcomm = m; }
     void enable_logging() { logging = true; }
+    bool logging_enabled() const { return logging; }
+    /* RAM::print_all_relation (src/RAM/RA_tasks.cpp:56-60) */
+    void print_all_relation(int nbuckets)
+    {
+        for (u32 i = 0; i < ram_relation_count; i++) ram_relations[i]->print(nbuckets);
+    }
     int get_id() { return ram_id; }
     int get_iteration_count() { return fixpoint ? -1 : 1; }
     u32 get_ram_relation_count() { return ram_relation_count; }
@@ -628,6 +710,50 @@ public:
 
     pra_ctx* backend() { return ctx; }
 
+    /* cumulative device milliseconds per kernel class (zero unless the
+     * backend times its kernels, PRA_PHASE_TIMES=1) */
+    std::vector<double> kernel_ms()
+    {
+        std::vector<double> v;
+        for (int i = 0;; i++) {
+            double ms = 0;
+            if (pra_ctx_kernel_stat(ctx, i, nullptr, nullptr, &ms, nullptr) != PRA_OK) break;
+            v.push_back(ms);
+        }
+        return v;
+    }
+
+    /* one row of the reference's per-iteration timing table (rows
+     * src/RAM/RA_tasks.cpp:1198-1235, header src/lie/lie.cpp:387-388), per
+     * batch, in seconds: Total_time is the batch's wall time; with
+     * PRA_PHASE_TIMES=1 the device time of the kernels doing each part --
+     * computation (join counts and expansions, copies), all-to-all (the
+     * stable bucket partition that replaces the exchange), insert in newt
+     * (membership claims and decisions, tables, scans, sorts) and insert in
+     * full (merges) -- fills the other columns; buffer creation and freeing
+     * and the intra-bucket exchange have no per-iteration counterpart here */
+    void timing_row(int loop, const std::vector<double>& k0, double wall)
+    {
+        const std::vector<double> k1 = kernel_ms();
+        double comp = 0, a2a = 0, newt = 0, full = 0;
+        for (size_t i = 0; i < k1.size() && i < k0.size(); i++) {
+            const char* name = nullptr;
+            pra_ctx_kernel_stat(ctx, (int)i, &name, nullptr, nullptr, nullptr);
+            const std::string n = name ? name : "";
+            const double d = (k1[i] - k0[i]) * 1e-3;
+            if (n == "join_count" || n == "join_expand" || n == "copy_expand" || n == "index_build")
+                comp += d;
+            else if (n == "dedup_scatter")
+                a2a += d;
+            else if (n == "merge_full")
+                full += d;
+            else
+                newt += d;
+        }
+        std::cout << loop << "\t" << 0.0 << "\t" << comp << "\t" << a2a << "\t" << 0.0 << "\t" << newt << "\t" << 0.0
+                  << "\t" << full << "\t" << wall << std::endl;
+    }
+
     /* LIE::execute (src/lie/lie.cpp:197-448), one SCC at a time in steps of
      * one batch so that checkpoints are written where the reference writes
      * them (every cp_iteration loops of an SCC). */
@@ -638,6 +764,7 @@ public:
         int rc = pra_ctx_create(dev, nbuckets, 0, &ctx);
         if (rc != PRA_OK) pra_facade::die(nullptr, rc, "GPU context");
         pra_facade::chk(ctx, pra_ctx_set_batch_size(ctx, batch_size), "set_batch_size");
+        if (pra_facade::env_int("PRA_PHASE_TIMES", 0)) pra_facade::chk(ctx, pra_ctx_set_profiling(ctx, -1), "profiling");
         for (u32 i = 0; i < lie_relation_count; i++) {
             relation* r = lie_relations[i];
             r->set_restart_flag(restart_flag);
@@ -670,23 +797,26 @@ public:
                     rels[i]->load_restart_delta(ctx, restart_dir_name, separate_io, nbuckets);
             }
             pra_facade::chk(ctx, pra_scc_begin(s, restart_flag ? 0 : 1), "SCC start");
-            double t0 = MPI_Wtime();
+            if (task->get_iteration_count() != 1)
+                std::cout << "Iteration#\tBuffer_creation_time\tComputation_time\tAll_to_all_time\tBuffer_free_time"
+                             "\tInsert_in_newt_time\tIntra_comm_time\tInsert_in_full_time\tTotal_time"
+                          << std::endl;
             int loop = 0;
             uint64_t delta = 0;
             do {
                 if (io && loop % cp_iteration == 0) create_checkpoint_dump(loop, id);
+                const std::vector<double> k0 = kernel_ms();
+                const double t0 = MPI_Wtime();
                 pra_facade::chk(ctx, pra_scc_step(s, &delta), "SCC execute");
+                const double t1 = MPI_Wtime();
+                timing_row(loop, k0, t1 - t0);
+                if (task->logging_enabled()) task->print_all_relation(nbuckets);
                 if (task->get_iteration_count() == 1 || delta == 0) executed_scc_id.push_back(id);
                 if (io && loop % cp_iteration == 0) write_checkpoint_dump(loop, executed_scc_id, id);
                 loop++;
             } while (task->get_iteration_count() != 1 && delta != 0);
             pra_facade::chk(ctx, pra_scc_end(s), "SCC finish");
-            double t1 = MPI_Wtime();
-            if (task->get_iteration_count() == 1)
-                print_all_relation_size();
-            else
-                std::cout << "SCC " << id << ": " << loop << " batches to the fixed point in " << (t1 - t0) << " s"
-                          << std::endl;
+            if (task->get_iteration_count() == 1) print_all_relation_size();
             loop_counter = loop;
             pra_scc_destroy(s);
             update_task_graph(task);

@@ -85,6 +85,19 @@ int pra_ctx_destroy(pra_ctx *c)
 
 const char *pra_last_error(pra_ctx *c) { return c ? c->err : "no context"; }
 
+/* the CPU backend times no kernels: no kernel classes to report */
+int pra_ctx_set_profiling(pra_ctx *c, int mask)
+{
+    (void)mask;
+    return c ? PRA_OK : PRA_E_ARG;
+}
+
+int pra_ctx_kernel_stat(pra_ctx *c, int idx, const char **name, uint64_t *launches, double *ms, double *bytes)
+{
+    (void)c; (void)idx; (void)name; (void)launches; (void)ms; (void)bytes;
+    return PRA_E_ARG;
+}
+
 int pra_ctx_set_batch_size(pra_ctx *c, uint32_t batch)
 {
     if (!c || batch < 1) return PRA_E_ARG;

@@ -18,6 +18,7 @@
 //   filter <scc#> <out#> <in#> <ver> <op> <reorder...>   op: 0 EQ 1 NEQ 2 LT 3 LE 4 GT 5 GE on columns 0, 1
 //   dep <scc#> <scc#>
 //   batch <batch size>
+//   log <scc#>                 RAM::enable_logging: every relation of the SCC printed after each batch
 // Versions are 0 = DELTA, 1 = FULL (src/relation/balanced_hash_relation.h:13).
 //
 // Output: the reference's own checkpoint dumps, one per batch (LIE::execute
@@ -109,6 +110,10 @@ int main(int argc, char** argv)
             deps.emplace_back(a, b);
         } else if (op == "batch") {
             in >> batch;
+        } else if (op == "log") {
+            int a;
+            in >> a;
+            sccs[a]->enable_logging();
         } else {
             std::cerr << "unknown directive " << op << "\n";
             MPI_Abort(mcomm.get_comm(), 2);

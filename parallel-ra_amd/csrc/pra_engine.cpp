@@ -1143,12 +1143,14 @@ static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
 // Dense membership of interned output R keyed on kw columns: the bitmap of
 // full u delta for the current dictionary (rebuilt when the dictionary or the
 // relation changed since it was built) and a claim table of at least 2C
-// slots.  False when the key does not allow it: more than 4 columns, a rank
-// space over 2^35 bits, or a value of full u delta without a rank.
+// slots.  False when the output does not allow it: more than 4 key columns, a
+// rank space over 2^35 bits, 2^28 or more candidates, or a value of full u
+// delta without a rank.
 static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C)
 {
     Group& G = grp(c, R->group);
-    if (!c->dense_on || G.dict_state != 1 || kw < 1 || kw > 4 || G.dict_n == 0) return false;
+    if (!c->dense_on || G.dict_state != 1 || kw < 1 || kw > 4 || G.dict_n == 0 || C >= (1ULL << PRA_DENSE_PBITS))
+        return false;
     unsigned __int128 bits = 1;
     for (int i = 0; i < kw; i++) {
         bits *= G.dict_n;
@@ -1177,7 +1179,7 @@ static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C)
     }
     const u64 cap = pow2_at_least(std::max<u64>(1024, 2 * C));
     if (cap > R->dtab_cap) {
-        R->dtab.reserve(cap * 16, c->st);
+        R->dtab.reserve(cap * 8, c->st);
         pra_launch_dense_table_clear(c->st, R->dtab.as<u64>(), cap);
         check_launch("dense_table_clear");
         R->dtab_cap = cap;

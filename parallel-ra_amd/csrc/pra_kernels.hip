@@ -1393,8 +1393,11 @@ __device__ __forceinline__ bool pra_dense_index(const u64* key, const u64* dict,
 }
 
 // A candidate whose key is a member (bit set) is dropped at once
-// (slot_of = ~0); any other claims or finds the slot of x + 1 and lowers its
-// arrival mark to its position, so the earliest arrival wins.
+// (slot_of = ~0); any other claims or finds the slot holding x + 1 and lowers
+// the slot's arrival to its own position, so the earliest arrival wins.  A
+// slot is one word, (x + 1) << PRA_DENSE_PBITS | arrival (0 = empty): the
+// first candidate of a key claims it with one compare-and-swap, a later one
+// reads it and swaps only while its own arrival is earlier.
 template <int KW>
 __device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u32* flags)
 {
@@ -1407,20 +1410,26 @@ __device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u
         a.slot_of[p] = ~0ULL;
         return;
     }
-    const u64 key = x + 1, m = a.cap - 1;
+    const u64 key = x + 1, m = a.cap - 1, mine = (key << PRA_DENSE_PBITS) | p;
     u64 s = pra_mix64(key) & m;
     for (u64 q = 0; q < a.cap; q++) {
-        u64* slot = a.table + 2 * s;
-        u64 cur = slot[0];
-        bool mine = false;
+        unsigned long long* slot = (unsigned long long*)(a.table + s);
+        u64 cur = *slot;
         if (cur == 0) {
-            cur = atomicCAS((unsigned long long*)slot, 0ULL, (unsigned long long)key);
-            mine = cur == 0;
+            cur = atomicCAS(slot, 0ULL, (unsigned long long)mine);
+            if (cur == 0) {
+                a.slot_of[p] = s;
+                return;
+            }
         }
-        if (mine || cur == key) {
-            // a later candidate reads the mark first: an earlier arrival may
-            // already hold it
-            if (mine || slot[1] > p) atomicMin((unsigned long long*)(slot + 1), (unsigned long long)p);
+        if ((cur >> PRA_DENSE_PBITS) == key) {
+            // the key's slot: lower its arrival while ours is earlier (the
+            // key bits never change once set)
+            while ((cur & PRA_DENSE_PMASK) > p) {
+                const u64 prev = atomicCAS(slot, (unsigned long long)cur, (unsigned long long)mine);
+                if (prev == cur) break;
+                cur = prev;
+            }
             a.slot_of[p] = s;
             return;
         }
@@ -1475,16 +1484,16 @@ __device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n,
             const u64 s = a.slot_of[p];
             u32 kp = 0;
             if (s != ~0ULL) {
-                u64* slot = a.table + 2 * s;
-                if (slot[1] == p) {
+                u64* slot = a.table + s;
+                const u64 v = *slot;
+                // the earliest arrival of the key keeps it; the slot returns to
+                // empty, which no later arrival's position matches
+                if (v != 0 && (v & PRA_DENSE_PMASK) == p) {
                     kp = 1;
                     atomicAdd(&h[a.dest[p]], 1u);
-                    const u64 x = slot[0] - 1;
+                    const u64 x = (v >> PRA_DENSE_PBITS) - 1;
                     atomicOr(&a.bitmap[x >> 5], 1u << (x & 31));
-                    // only the winner writes the slot; the others compare the
-                    // mark with their own position, which ~0 never equals
-                    slot[0] = 0;
-                    slot[1] = ~0ULL;
+                    *slot = 0;
                 }
             }
             a.keep[p] = kp;
@@ -2621,7 +2630,7 @@ void pra_launch_dense_build(hipStream_t st, const u64* rows, u64 n, int w, int k
     }
 }
 
-void pra_launch_dense_table_clear(hipStream_t st, u64* table, u64 cap) { pra_launch_table_clear(st, table, cap, 2); }
+void pra_launch_dense_table_clear(hipStream_t st, u64* table, u64 cap) { pra_launch_fill_u64(st, table, cap, 0); }
 
 void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags)
 {

@@ -170,10 +170,14 @@ struct DecideJob {
 // (D = dictionary size) selects one bit of a bitmap over D^kw bits that holds
 // the keys of full u delta (and of newt, set as tuples are kept).  Within an
 // iteration, candidates that are not members claim a slot of a small table
-// keyed exactly by x + 1, slots [x + 1, earliest arrival] (empty: [0, ~0]);
-// the earliest arrival keeps the tuple, sets its bit and resets the slot, so
-// the table is empty again after every iteration.  A value missing from the
-// dictionary sets ctl[1] = 3 and the output is redone the general way.
+// keyed exactly by x + 1, one word per slot: (x + 1) << PRA_DENSE_PBITS |
+// earliest arrival (empty: 0); the earliest arrival keeps the tuple, sets its
+// bit and empties the slot, so the table is empty again after every
+// iteration.  Needs D^kw <= 2^35 (x + 1 in 36 bits) and fewer than 2^28
+// candidates.  A value missing from the dictionary sets ctl[1] = 3 and the
+// output is redone the general way.
+#define PRA_DENSE_PBITS 28
+#define PRA_DENSE_PMASK ((1ULL << PRA_DENSE_PBITS) - 1)
 struct DenseArgs {
     const u64* cand;
     u64* ctl;

@@ -51,6 +51,7 @@ class Recorder:
         self.scc_rels = []  # (scc, rel, status)
         self.rules = []     # text lines
         self.deps = []
+        self.logged = []    # SCCs printed after each batch (RAM::enable_logging)
 
     def rel_create(self, arity, jcc, tag):
         self.rels.append((arity, jcc, tag))
@@ -98,6 +99,8 @@ class Recorder:
         lines += self.rules
         for a, b in self.deps:
             lines.append(f"dep {a} {b}")
+        for a in self.logged:
+            lines.append(f"log {a}")
         lines.append(f"batch {batch}")
         spec = os.path.join(d, "spec")
         with open(spec, "w") as f:
@@ -277,9 +280,42 @@ def cases():
     return out
 
 
+LOG_LINE = re.compile(r"^(\S+ \d+ (FULL|DELTA) Rows \d+ columns \d+|F \[\d+\] [\d ]+|D [\d ]+)$")
+
+
+def relation_dump_lines(stdout):
+    """the lines relation::print writes (src/relation/balanced_hash_relation.cpp:433-522)"""
+    return [l.rstrip() for l in stdout.splitlines() if LOG_LINE.match(l.rstrip())]
+
+
+def write_logging_goldens():
+    """RAM::enable_logging on the fixed-point SCC of the TC program, at one
+    rank: the relation dumps the reference prints after every batch"""
+    for name in ("g5955_edge_2_1_2", "tiny8"):
+        edges = P.golden(name) if name != "tiny8" else tiny8_edges()
+        rec = Recorder()
+        P.tc_program(rec, edges)
+        rec.sccs = [(True, 1), (False, 2)]
+        rec.rels = [(2, 1, 257), (2, 1, 256)]
+        rec.logged = [0]
+        with tempfile.TemporaryDirectory() as d:
+            spec = rec.write(d, 1)
+            spec_text = open(spec).read().replace("rel R0 ", "rel T ").replace("rel R1 ", "rel G ")
+            open(spec, "w").write(spec_text)
+            os.makedirs(os.path.join(d, "out"))
+            r = subprocess.run([MPIRUN, "-n", "1", os.path.join(REF_BIN, "ref_driver"), spec, os.path.join(d, "out")],
+                               cwd=d, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+        path = os.path.join(HERE, f"ref_log_tc_{name.split('_')[0]}_k1.txt")
+        with open(path, "w") as f:
+            f.write("\n".join(relation_dump_lines(r.stdout)) + "\n")
+        print("wrote", path)
+
+
 def main():
     if not os.path.exists(os.path.join(REF_BIN, "ref_driver")):
         sys.exit("build the reference checkers first: make -C oracle ref")
+    write_logging_goldens()
     result = {}
     for name, build, edges, ks, batches in cases():
         rec = Recorder()

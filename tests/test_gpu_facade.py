@@ -70,7 +70,7 @@ def test_worstcase_program_on_gpu(tmp_path):
     r = run_facade_program(os.path.join(BUILD, "worstcase-110-terms-4-m"), "worstcase-110-terms-4-m",
                            str(tmp_path / "run"), 3, str(tmp_path / "dump"), timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr
-    print("\n".join(l for l in r.stdout.splitlines() if "batches to the fixed point" in l), flush=True)
+    print("\n".join(r.stdout.splitlines()[-8:]), flush=True)
     assert total_facts(r.stdout) == 6476108
     got = dump_digest(str(tmp_path / "dump"))
     bad = sorted(k for k in expect["relations"] if got.get(k) != expect["relations"][k])
@@ -92,3 +92,13 @@ def test_tc_checkpoint_restart_on_gpu(tmp_path):
     for d in e["checkpoints"]:
         assert got[d] == e["checkpoints"][d], d
     assert dump_digest(dump) == e["restart_dump"]
+
+
+@pytest.mark.parametrize("case,edges", [("g5955", "g5955_edge_2_1_2"), ("tiny8", None)])
+def test_gpu_enable_logging_prints_what_the_reference_prints(tmp_path, case, edges):
+    """RAM::enable_logging through the HIP library: the relation dumps after
+    every batch equal the reference's own (tests/golden/ref_log_tc_*_k1.txt)"""
+    from test_facade import logging_run
+    got, _ = logging_run(tmp_path, os.path.join(ROOT, "parallel-ra_amd"), "pra_amd", edges)
+    want = open(os.path.join(ROOT, "tests", "golden", f"ref_log_tc_{case}_k1.txt")).read().split("\n")[:-1]
+    assert got == want
