@@ -1173,6 +1173,18 @@ __global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host
 
 // ---------------------------------------------------- dedup / newt ---------
 
+// Count this lane in LDS counter h[d] when `on`, adding once per group of
+// lanes of the wave with the same d (d < 256): a tile's candidates fall into
+// few destination buckets, and one LDS atomic per lane would serialise on
+// them.  Every lane of the wave must call it.
+__device__ __forceinline__ void pra_wave_count(u32* h, u32 d, bool on)
+{
+    const u64 act = __ballot(on);
+    if (!act) return;
+    const u64 peers = pra_match<PRA_RADIX_BITS>(on ? d : 0u, act);
+    if (on && __lane_id() == (u32)(__ffsll((long long)peers) - 1)) atomicAdd(&h[d], (u32)__popcll(peers));
+}
+
 
 // Claim (or find) the slot of candidate p's key (p < the candidate count).
 // The first claimant stores the key; every candidate then lowers the slot's
@@ -1294,9 +1306,9 @@ __device__ __forceinline__ void pra_decide_tile(const u64* cand, u64 n, int cw, 
 #pragma unroll
     for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
         u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
+        u32 kp = 0;
         if (p < n) {
             u64 s = slot_of[p];
-            u32 kp = 0;
             if (s != ~0ULL) {
                 const u64* slot = table + s * (2 + kw);
                 const u64* key = cand + p * cw;
@@ -1315,13 +1327,11 @@ __device__ __forceinline__ void pra_decide_tile(const u64* cand, u64 n, int cw, 
                         if (slot[2 + c] != key[c]) { eq = false; break; }
                 }
                 if (!eq) atomicOr(flags, PRA_FLAG_COLLISION);
-                if (slot[1] == ((epoch << 40) | (p + 1))) {
-                    kp = 1;
-                    atomicAdd(&h[dest[p]], 1u);
-                }
+                if (slot[1] == ((epoch << 40) | (p + 1))) kp = 1;
             }
             keep[p] = kp;
         }
+        pra_wave_count(h, kp ? dest[p] : 0u, kp != 0);
     }
     __syncthreads();
     if (offs1) {
@@ -1480,9 +1490,9 @@ __device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n,
 #pragma unroll
     for (int k = 0; k < PRA_TILE / PRA_DECIDE_BLOCK; k++) {
         const u64 p = base + (u64)k * PRA_DECIDE_BLOCK + threadIdx.x;
+        u32 kp = 0, d = 0;
         if (p < n) {
             const u64 s = a.slot_of[p];
-            u32 kp = 0;
             if (s != ~0ULL) {
                 u64* slot = a.table + s;
                 const u64 v = *slot;
@@ -1490,7 +1500,7 @@ __device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n,
                 // empty, which no later arrival's position matches
                 if (v != 0 && (v & PRA_DENSE_PMASK) == p) {
                     kp = 1;
-                    atomicAdd(&h[a.dest[p]], 1u);
+                    d = a.dest[p];
                     const u64 x = (v >> PRA_DENSE_PBITS) - 1;
                     atomicOr(&a.bitmap[x >> 5], 1u << (x & 31));
                     *slot = 0;
@@ -1498,6 +1508,7 @@ __device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n,
             }
             a.keep[p] = kp;
         }
+        pra_wave_count(h, d, kp != 0);
     }
     __syncthreads();
     if (a.offs1) {
