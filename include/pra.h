@@ -49,6 +49,7 @@ typedef struct pra_scc pra_scc;
 #define PRA_E_COLLISION -5  /* 64-bit key hash collision detected */
 #define PRA_E_STATE -6      /* call out of order */
 #define PRA_E_BUDGET -7     /* the context's device memory budget would be exceeded */
+#define PRA_E_NOMEM -8      /* the device ran out of memory (after idle cached blocks were freed) */
 
 /* Context = one LIE (src/lie/lie.h:13-153) on one device with `nbuckets`
  * virtual buckets, i.e. the layout of the reference run on `nbuckets` MPI
@@ -82,8 +83,16 @@ int pra_ctx_step(pra_ctx* ctx, uint64_t* active);
  * null) */
 int pra_ctx_group_status(pra_ctx* ctx, int group, int* done, uint64_t* iterations, uint64_t* bytes);
 /* free instance `group` (its relations and SCCs; their handles become
- * invalid) and reset its counters, so that the id can build a new program */
+ * invalid) and reset its counters, so that the id can build a new program.
+ * An instance with an SCC in progress is abandoned where it stands (between
+ * steps; the other instances go on unaffected). */
 int pra_ctx_release_group(pra_ctx* ctx, int group);
+/* Between steps: give back the context's scratch (device buffers that hold
+ * nothing from one iteration to the next, kept at the size of the largest
+ * iteration so far); they grow again on demand.  A long-lived context that
+ * admits instances under a memory budget calls it so that one large merged
+ * iteration does not hold memory against later admissions. */
+int pra_ctx_trim(pra_ctx* ctx);
 /* device bytes the context holds now and at most so far (no counterpart in
  * the reference; what budget admission measures) */
 int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak);
@@ -205,17 +214,27 @@ int pra_execute_many(pra_ctx** ctxs, int n, int concurrency, int* rc);
  * one context and advance together, one lock-step iteration at a time (each
  * phase one batched launch over all of them); between iterations finished
  * requests are retired and queued ones admitted, in arrival order, when the
- * device bytes the context holds, plus the growth still expected of the
- * requests in flight, plus the new request's estimate fit into `hbm_budget`
- * (0 = no budget; an idle server always admits).  est_bytes 0 = the server's
- * own estimate (random graphs with 1.3 edges per vertex).  keep_rows != 0
- * keeps each result's rows on the host for pra_server_rows.  Times are
- * seconds since server creation. */
+ * requests in flight (each counted at the larger of its estimate and the bytes
+ * its relations hold), the new request's estimate and the largest context
+ * scratch seen after a step fit into `hbm_budget` (0 = no budget; an idle
+ * server always admits).  est_bytes 0 = the server's
+ * own estimate (random graphs with 1.3 edges per vertex).  Under memory
+ * pressure requests are restarted rather than failed: when the bytes held
+ * pass the budget after an iteration, the context's scratch is given back
+ * (pra_ctx_trim), then the most recently admitted requests in flight are
+ * released and queued again at the head of the queue; when an
+ * iteration runs the device out of memory (PRA_E_NOMEM / PRA_E_BUDGET) with
+ * several requests in flight, all of them are queued again, the context is
+ * rebuilt and the number in flight is halved, then raised by one per request
+ * completed.  A request fails only when it fails alone.  keep_rows != 0 keeps
+ * each result's rows on the host for pra_server_rows.  Times are seconds since
+ * server creation; t_start is the last admission. */
 typedef struct pra_server pra_server;
 typedef struct {
     int status;
     uint64_t tuples, joined, iterations;
     double t_submit, t_start, t_done;
+    uint32_t restarts;   /* times the request was released and queued again */
 } pra_request_result;
 int pra_server_create(int device, int nbuckets, int slots, uint64_t hbm_budget, int keep_rows, pra_server** out);
 int pra_server_submit_tc(pra_server* srv, const uint64_t* edge_rows, uint64_t m, uint64_t est_bytes, uint64_t* ticket);

@@ -876,6 +876,9 @@ int ora_scc_run(ora_ctx *c, int scc, int batch_size)
     return ora_scc_end(c, scc);
 }
 
+/* the SCC LIE::execute would run next, or -1 when none is left */
+int ora_next_runnable(ora_ctx *c) { return one_runnable(c); }
+
 int ora_execute(ora_ctx *c, int batch_size)
 {
     int t;

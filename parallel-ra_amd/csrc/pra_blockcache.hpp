@@ -5,7 +5,9 @@
 // Returning tens of GB to the driver, and first use of freshly mapped device
 // memory, cost far more than the work large blocks hold, so released blocks of
 // at least `min_block` bytes are kept and handed to later requests of the same
-// power-of-two size class (or the next one up).
+// size class or of the two classes below it.  Classes are quarter octaves
+// (min_block x 2^k x 1, 1.25, 1.5, 1.75), so a block is at most 25 % larger
+// than what was asked for when fresh, and at most about 50 % when reused.
 //
 // Ownership rule (no events, no cross-stream ordering):
 //   * put(stream, block): the block was released while work enqueued on
@@ -43,24 +45,31 @@ class BlockCache {
 
     size_t min_block() const { return min_; }
 
-    // power-of-two size class of a request
+    // quarter-octave size class of a request: the smallest
+    // min_block x 2^k x q / 4 (q = 4..7) that holds it
     size_t size_class(size_t bytes) const
     {
-        size_t c = min_;
-        while (c < bytes) c <<= 1;
-        return c;
+        size_t oct = min_;
+        while (oct * 2 <= bytes) oct <<= 1;
+        for (size_t q = 4; q <= 8; q++)
+            if (oct / 4 * q >= bytes) return oct / 4 * q;
+        return oct * 2;
     }
 
+    // the class above class `cls`
+    size_t next_class(size_t cls) const { return size_class(cls + 1); }
+
     // A cached block for `bytes` on `stream`: idle or owned by `stream`, of
-    // the request's class or the next one, smallest first.  nullptr on a miss.
+    // the request's class or one of the next two, smallest first.  nullptr on
+    // a miss.
     void* take(int device, const void* stream, size_t bytes, size_t* got)
     {
-        const size_t cls = size_class(bytes);
+        const size_t cls = size_class(bytes), top = next_class(next_class(cls));
         std::lock_guard<std::mutex> lk(mu_);
         long best = -1;
         for (size_t i = 0; i < free_.size(); i++) {
             const Entry& b = free_[i];
-            if (b.device == device && (!b.owner || b.owner == stream) && b.size >= cls && b.size <= 2 * cls &&
+            if (b.device == device && (!b.owner || b.owner == stream) && b.size >= cls && b.size <= top &&
                 (best < 0 || b.size < free_[(size_t)best].size))
                 best = (long)i;
         }

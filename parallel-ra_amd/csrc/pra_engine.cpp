@@ -118,7 +118,7 @@ void* big_get(size_t bytes, hipStream_t st, size_t* got)
         p = g_big.get(dev, st, bytes, got, kHipBlocks);
     }
     if (!p)
-        throw PraError(PRA_E_HIP, "hipMalloc: out of memory (" + std::to_string(g_big.size_class(bytes)) + " bytes)");
+        throw PraError(PRA_E_NOMEM, "hipMalloc: out of memory (" + std::to_string(g_big.size_class(bytes)) + " bytes)");
     return p;
 }
 
@@ -177,8 +177,11 @@ struct DBuf {
     void reserve(size_t bytes, hipStream_t s, size_t used = 0)
     {
         if (bytes <= cap) return;
+        // grow by doubling (fewer growth copies than growing by half, which
+        // measured slower); the quarter-octave block classes keep the
+        // rounding on top of that within 25 %
         size_t nc = std::max(bytes, 2 * cap);
-        nc = std::max<size_t>(nc, 256);
+        nc = std::max<size_t>((nc + 255) & ~(size_t)255, 256);
         void* np = nullptr;
         bool nbig = nc >= kBigBlock;
         if (nbig) nc = g_big.size_class(nc);
@@ -199,7 +202,12 @@ struct DBuf {
                     g_big.idle(s);
                     g_big.flush(dev, kHipBlocks);
                     trim_pool(dev);
-                    HIPCHK(hipMallocAsync(&np, nc, s));
+                    const hipError_t e = hipMallocAsync(&np, nc, s);
+                    if (e == hipErrorOutOfMemory) {
+                        (void)hipGetLastError();
+                        throw PraError(PRA_E_NOMEM, "hipMallocAsync: out of memory (" + std::to_string(nc) + " bytes)");
+                    }
+                    HIPCHK(e);
                 }
             }
         } catch (...) {
@@ -2852,8 +2860,6 @@ int pra_ctx_release_group(pra_ctx* ctx, int group)
 {
     if (!ctx || group < 0 || group >= (int)ctx->groups.size()) return PRA_E_ARG;
     API_BEGIN(ctx)
-    Group& G = *ctx->groups[(size_t)group];
-    if (G.running) throw PraError(PRA_E_STATE, "instance has an SCC in progress");
     auto& deps = ctx->deps;
     deps.erase(std::remove_if(deps.begin(), deps.end(),
                               [&](const std::pair<pra_scc*, pra_scc*>& d) { return d.first->group == group; }),
@@ -2873,6 +2879,22 @@ int pra_ctx_release_group(pra_ctx* ctx, int group)
             keep_r.push_back(r);
     ctx->rels.swap(keep_r);
     ctx->groups[(size_t)group].reset(new Group());
+    API_END
+}
+
+int pra_ctx_trim(pra_ctx* ctx)
+{
+    if (!ctx) return PRA_E_ARG;
+    API_BEGIN(ctx)
+    HIPCHK(hipSetDevice(ctx->device));
+    // written before they are read inside every iteration that uses them;
+    // released on the context's stream, so reuse stays ordered behind the
+    // work already enqueued
+    for (DBuf* b : {&ctx->sort_tmp, &ctx->stage, &ctx->stage2, &ctx->tmp64a, &ctx->tmp64b, &ctx->tmp64c, &ctx->cand,
+                    &ctx->dest, &ctx->slot_of, &ctx->keep, &ctx->hist, &ctx->offs, &ctx->ktmp, &ctx->idxa, &ctx->idxb,
+                    &ctx->posadj_dev, &ctx->ttab, &ctx->kp, &ctx->lb_status})
+        b->release();
+    ctx->lb_zeroed = 0;   // look-back words are zeroed again on next use
     API_END
 }
 

@@ -8,18 +8,28 @@
 // iteration, all of them in the same batched launches, and between iterations
 // the thread retires the requests that reached their fixed point (results
 // read, instance released) and admits queued ones into the freed slots when
-// their memory fits.  Admission counts the device bytes the context holds now
-// (pra_ctx_device_bytes), the growth still expected of the requests in flight
-// (estimate minus the bytes their relations hold already) and the new
-// request's estimate, against the HBM budget; the first request of an idle
-// server is always admitted.  Relations stay resident on the device from
-// admission to completion.  The program solved is the reference's
+// their memory fits.  Admission counts every request in flight at the larger
+// of its estimate and the bytes its relations hold (pra_ctx_group_status),
+// the new request's estimate, and the largest amount of context scratch seen
+// after any step (the device bytes of the context beyond its requests'
+// relations: candidate, sort and staging buffers shared by all instances),
+// against the HBM budget; the first request of an idle server is always
+// admitted.  Relations stay resident on the device from
+// admission to completion.  Estimates can be low, so the loop restarts
+// requests rather than fail them when memory runs short: past the budget after
+// an iteration, the context's scratch is given back (pra_ctx_trim), then the
+// newest admissions in flight are released and queued again at the head of
+// the queue (released bytes count at once); an iteration that
+// runs the device out of memory with several requests in flight requeues all
+// of them on a fresh context and halves the number in flight, which then
+// grows back by one per completed request.  The program solved is the reference's
 // tests/transitive_closure.cpp:56-77 (copy seed, then the T(DELTA) x G(FULL)
 // join to the fixed point).
 #include "../../include/pra.h"
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -33,7 +43,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 struct Request {
-    uint64_t id = 0;
+    uint64_t id = 0, admitted = 0;   // arrival number, admission number (last admission)
     std::vector<uint64_t> edges;
     uint64_t m = 0, est = 0;
     pra_request_result res{};
@@ -48,20 +58,25 @@ struct Slot {
 
 double seconds_since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-// device bytes a TC request on m edges is expected to need: random graphs with
-// 1.3 edges per vertex close to about 0.2 n^2 paths, held as 24-byte rows in
-// blocks up to twice that size, plus a 64 MiB allowance for scratch
+// device bytes a TC request on m edges is expected to need at its peak:
+// random graphs with 1.3 edges per vertex close to about 0.18 n^2 paths, and
+// a solve peaks at up to 90 bytes per final path (24-byte rows of the full,
+// delta and newt versions in blocks up to 1.5x their size, the growth copy,
+// the join candidates and the dense membership bitmap), plus a 64 MiB
+// allowance for scratch
 uint64_t default_estimate(uint64_t m)
 {
     const double n = (double)m / 1.3;
-    return (uint64_t)(0.2 * n * n * 48.0) + (64ULL << 20);
+    return (uint64_t)(0.18 * n * n * 90.0) + (64ULL << 20);
 }
 
 } // namespace
 
 struct pra_server {
     int device = 0, nb = 1, slots = 1, keep_rows = 0;
-    uint64_t budget = 0, next_id = 1;
+    int limit = 1;   // requests in flight at most for now: slots, halved after running out of memory
+    uint64_t budget = 0, next_id = 1, admissions = 0;
+    uint64_t scratch_hw = 0;   // largest context scratch seen after a step (bytes beyond the relations)
     bool stopping = false;
     Clock::time_point t0 = Clock::now();
     std::mutex mu;
@@ -80,20 +95,32 @@ struct pra_server {
         return k;
     }
 
+    // device bytes the relations of instance g hold
+    uint64_t held_by(int g)
+    {
+        uint64_t held = 0;
+        pra_ctx_group_status(ctx, g, nullptr, nullptr, &held);
+        return held;
+    }
+
+    // after a step: the context's bytes beyond the relations in flight
+    void observe_scratch()
+    {
+        uint64_t live = 0, held = 0;
+        if (pra_ctx_device_bytes(ctx, &live, nullptr) != PRA_OK) return;
+        for (int g = 0; g < slots; g++)
+            if (slot[(size_t)g].r) held += held_by(g);
+        if (live > held) scratch_hw = std::max(scratch_hw, live - held);
+    }
+
     // may the head of the queue join the requests in flight?
     bool fits(const Request* r)
     {
         if (!in_flight() || !budget) return true;
-        uint64_t live = 0;
-        if (pra_ctx_device_bytes(ctx, &live, nullptr) != PRA_OK) return false;
-        uint64_t reserved = 0;
-        for (size_t g = 0; g < slot.size(); g++) {
-            if (!slot[g].r) continue;
-            uint64_t held = 0;
-            pra_ctx_group_status(ctx, (int)g, nullptr, nullptr, &held);
-            if (slot[g].r->est > held) reserved += slot[g].r->est - held;
-        }
-        return live + reserved + r->est <= budget;
+        uint64_t need = scratch_hw + r->est;
+        for (int g = 0; g < slots; g++)
+            if (slot[(size_t)g].r) need += std::max(slot[(size_t)g].r->est, held_by(g));
+        return need <= budget;
     }
 
     // the program of tests/transitive_closure.cpp as instance g
@@ -145,21 +172,84 @@ struct pra_server {
             uint64_t got = 0;
             rc = pra_rel_download(s.T, PRA_FULL, r->rows.data(), r->res.tuples, &got);
         }
+        if (rc != PRA_OK)
+            std::fprintf(stderr, "pra_server: results of request %llu not read (%d): %s\n",
+                         (unsigned long long)r->id, rc, pra_last_error(ctx));
         const int rel = pra_ctx_release_group(ctx, g);
         s = Slot{};
         finish(r, rc != PRA_OK ? rc : rel);
     }
 
-    // a failed step leaves the context unusable: every request in flight fails
-    // and the next ones start on a fresh context
+    // a request released before its end needs at least what it held: its
+    // estimate becomes at least twice that, so that it is not admitted again
+    // into the same shortage
+    void raise_estimate(Request* r, int g)
+    {
+        if (ctx) r->est = std::max(r->est, 2 * held_by(g));
+    }
+
+    // requests back to the head of the queue, in arrival order
+    void requeue(std::vector<Request*> rs)
+    {
+        std::sort(rs.begin(), rs.end(), [](const Request* a, const Request* b) { return a->id > b->id; });
+        std::lock_guard<std::mutex> lk(mu);
+        for (Request* r : rs) {
+            r->res.restarts++;
+            queue.push_front(r);
+        }
+    }
+
+    // a failed step leaves the context unusable: the requests in flight are
+    // queued again after running out of memory with several in flight, else
+    // they fail; the next ones start on a fresh context
     void reset_context(int rc)
     {
-        for (Slot& s : slot)
-            if (s.r) finish(s.r, rc);
+        const int n = in_flight();
+        const bool retry = (rc == PRA_E_NOMEM || rc == PRA_E_BUDGET) && n > 1;
+        std::fprintf(stderr, "pra_server: step failed (%d): %s; %d request(s) in flight %s\n", rc,
+                     pra_last_error(ctx), n, retry ? "queued again" : "fail");
+        std::vector<Request*> back;
+        for (size_t g = 0; g < slot.size(); g++)
+            if (Request* r = slot[g].r) {
+                if (retry) {
+                    raise_estimate(r, (int)g);
+                    back.push_back(r);
+                } else {
+                    finish(r, rc);
+                }
+            }
+        if (retry) limit = std::max(1, n / 2);
         slot.assign((size_t)slots, Slot{});
         pra_ctx_destroy(ctx);
         ctx = nullptr;
         if (pra_ctx_create(device, nb, 0, &ctx) == PRA_OK) pra_ctx_set_batch_size(ctx, 1);
+        requeue(back);
+    }
+
+    // past the budget after an iteration: the context's scratch is given
+    // back, then the newest admissions are released until the bytes held fit
+    // again (one request always stays)
+    void relieve()
+    {
+        if (!budget) return;
+        bool trimmed = false;
+        while (in_flight() > 1) {
+            uint64_t live = 0;
+            if (pra_ctx_device_bytes(ctx, &live, nullptr) != PRA_OK || live <= budget) return;
+            if (!trimmed) {
+                trimmed = true;
+                if (pra_ctx_trim(ctx) == PRA_OK) continue;
+            }
+            int newest = -1;
+            for (int g = 0; g < slots; g++)
+                if (slot[(size_t)g].r && (newest < 0 || slot[(size_t)g].r->admitted > slot[(size_t)newest].r->admitted))
+                    newest = g;
+            Request* r = slot[(size_t)newest].r;
+            raise_estimate(r, newest);
+            pra_ctx_release_group(ctx, newest);
+            slot[(size_t)newest] = Slot{};
+            requeue({r});
+        }
     }
 
     void run()
@@ -173,7 +263,7 @@ struct pra_server {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stopping || !queue.empty() || in_flight(); });
                 if (stopping && queue.empty() && !in_flight()) break;
-                int free_slots = slots - in_flight();
+                int free_slots = std::min(slots, limit) - in_flight();
                 while (ctx && free_slots > 0 && !queue.empty() && fits(queue.front())) {
                     admit.push_back(queue.front());
                     queue.pop_front();
@@ -195,9 +285,12 @@ struct pra_server {
                 int g = 0;
                 while (slot[(size_t)g].r) g++;
                 slot[(size_t)g].r = r;
+                r->admitted = ++admissions;
                 r->res.t_start = seconds_since(t0);
                 const int rc = build(g, r);
                 if (rc != PRA_OK) {
+                    std::fprintf(stderr, "pra_server: request %llu (%llu edges) not set up (%d): %s\n",
+                                 (unsigned long long)r->id, (unsigned long long)r->m, rc, pra_last_error(ctx));
                     pra_ctx_release_group(ctx, g);
                     slot[(size_t)g] = Slot{};
                     finish(r, rc);
@@ -210,12 +303,17 @@ struct pra_server {
                 reset_context(rc);
                 continue;
             }
+            observe_scratch();
             for (int g = 0; g < slots; g++) {
                 if (!slot[(size_t)g].r) continue;
                 int done = 0;
                 pra_ctx_group_status(ctx, g, &done, nullptr, nullptr);
-                if (done) retire(g);
+                if (done) {
+                    retire(g);
+                    limit = std::min(slots, limit + 1);
+                }
             }
+            relieve();
         }
         if (ctx) pra_ctx_destroy(ctx);
         ctx = nullptr;
@@ -231,6 +329,7 @@ int pra_server_create(int device, int nbuckets, int slots, uint64_t hbm_budget, 
     s->device = device;
     s->nb = nbuckets;
     s->slots = slots;
+    s->limit = slots;
     s->budget = hbm_budget;
     s->keep_rows = keep_rows;
     s->loop = std::thread([s] { s->run(); });

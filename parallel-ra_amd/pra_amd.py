@@ -23,7 +23,8 @@ F_EQ, F_NEQ, F_LT, F_LE, F_GT, F_GE, F_TRUE, F_FALSE = range(8)
 EXPORTS = (
     "pra_ctx_create", "pra_ctx_destroy", "pra_last_error", "pra_ctx_set_batch_size",
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
-    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes", "pra_ctx_set_group", "pra_ctx_group_joined",
+    "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes", "pra_ctx_trim",
+    "pra_ctx_set_group", "pra_ctx_group_joined",
     "pra_ctx_step", "pra_ctx_group_status", "pra_ctx_release_group",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
     "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
@@ -48,7 +49,7 @@ class PraError(RuntimeError):
 class RequestResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("tuples", ctypes.c_uint64), ("joined", ctypes.c_uint64),
                 ("iterations", ctypes.c_uint64), ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
-                ("t_done", ctypes.c_double)]
+                ("t_done", ctypes.c_double), ("restarts", ctypes.c_uint32)]
 
 
 def build():
@@ -61,65 +62,78 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise PraError("libpra_amd.so is not built: run __graft_entry__.build() (make -C parallel-ra_amd)")
-        L = ctypes.CDLL(LIB_PATH)
-        sig = {
-            "pra_ctx_create": [ctypes.c_int, ctypes.c_int, _u64, ctypes.POINTER(_vp)],
-            "pra_ctx_destroy": [_vp],
-            "pra_ctx_set_batch_size": [_vp, ctypes.c_uint32],
-            "pra_ctx_set_join_threshold": [_vp, ctypes.c_int64],
-            "pra_ctx_joined_tuples": [_vp, _u64p],
-            "pra_ctx_iterations": [_vp, _u64p],
-            "pra_ctx_set_profiling": [_vp, ctypes.c_int],
-            "pra_ctx_kernel_stat": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _u64p,
-                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
-            "pra_ctx_synchronize": [_vp],
-            "pra_ctx_device_bytes": [_vp, _u64p, _u64p],
-            "pra_ctx_set_group": [_vp, ctypes.c_int],
-            "pra_ctx_group_joined": [_vp, ctypes.c_int, _u64p],
-            "pra_ctx_step": [_vp, _u64p],
-            "pra_ctx_group_status": [_vp, ctypes.c_int, _intp, _u64p, _u64p],
-            "pra_ctx_release_group": [_vp, ctypes.c_int],
-            "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
-                               ctypes.POINTER(_vp)],
-            "pra_rel_destroy": [_vp],
-            "pra_rel_load_rows": [_vp, _u64p, _u64, ctypes.c_int],
-            "pra_rel_load_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
-            "pra_rel_count": [_vp, ctypes.c_int, _u64p],
-            "pra_rel_download": [_vp, ctypes.c_int, _u64p, _u64, _u64p],
-            "pra_rel_write_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
-            "pra_rel_device_rows": [_vp, ctypes.c_int, ctypes.POINTER(_u64p), _u64p],
-            "pra_rel_checksum": [_vp, ctypes.c_int, _u64p],
-            "pra_scc_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
-            "pra_scc_destroy": [_vp],
-            "pra_scc_add_relation": [_vp, _vp, ctypes.c_int],
-            "pra_scc_add_join": [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _intp, ctypes.c_int],
-            "pra_scc_add_copy": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int],
-            "pra_scc_add_acopy": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int],
-            "pra_scc_add_copy_filter": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int, ctypes.c_int],
-            "pra_scc_add_dependency": [_vp, _vp],
-            "pra_scc_run": [_vp, _u64p, _u64, _u64p],
-            "pra_scc_begin": [_vp, ctypes.c_int],
-            "pra_scc_step": [_vp, _u64p],
-            "pra_scc_end": [_vp],
-            "pra_scc_history": [_vp, _u64p, _u64, _u64p],
-            "pra_partition_rows": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
-            "pra_partition_rows_noalloc": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
-            "pra_execute": [_vp],
-            "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],
-            "pra_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.POINTER(_vp)],
-            "pra_server_submit_tc": [_vp, _u64p, _u64, _u64, _u64p],
-            "pra_server_wait": [_vp, _u64, ctypes.POINTER(RequestResult)],
-            "pra_server_rows": [_vp, _u64, _u64p, _u64, _u64p],
-            "pra_server_destroy": [_vp],
-        }
-        for name, args in sig.items():
-            f = getattr(L, name)
-            f.argtypes = args
-            f.restype = ctypes.c_int
-        L.pra_last_error.argtypes = [_vp]
-        L.pra_last_error.restype = ctypes.c_char_p
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def load_library(path):
+    """Another build of the C ABI (for example the serving loop linked with
+    the CPU oracle, oracle/libpra_server_oracle.so), with the same signatures
+    set on whichever of its entry points it exports."""
+    return _bind(ctypes.CDLL(path))
+
+
+def _bind(L):
+    sig = {
+        "pra_ctx_create": [ctypes.c_int, ctypes.c_int, _u64, ctypes.POINTER(_vp)],
+        "pra_ctx_destroy": [_vp],
+        "pra_ctx_set_batch_size": [_vp, ctypes.c_uint32],
+        "pra_ctx_set_join_threshold": [_vp, ctypes.c_int64],
+        "pra_ctx_joined_tuples": [_vp, _u64p],
+        "pra_ctx_iterations": [_vp, _u64p],
+        "pra_ctx_set_profiling": [_vp, ctypes.c_int],
+        "pra_ctx_kernel_stat": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _u64p,
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+        "pra_ctx_synchronize": [_vp],
+        "pra_ctx_device_bytes": [_vp, _u64p, _u64p],
+        "pra_ctx_set_group": [_vp, ctypes.c_int],
+        "pra_ctx_group_joined": [_vp, ctypes.c_int, _u64p],
+        "pra_ctx_step": [_vp, _u64p],
+        "pra_ctx_group_status": [_vp, ctypes.c_int, _intp, _u64p, _u64p],
+        "pra_ctx_release_group": [_vp, ctypes.c_int],
+    "pra_ctx_trim": [_vp],
+        "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                           ctypes.POINTER(_vp)],
+        "pra_rel_destroy": [_vp],
+        "pra_rel_load_rows": [_vp, _u64p, _u64, ctypes.c_int],
+        "pra_rel_load_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
+        "pra_rel_count": [_vp, ctypes.c_int, _u64p],
+        "pra_rel_download": [_vp, ctypes.c_int, _u64p, _u64, _u64p],
+        "pra_rel_write_bin": [_vp, ctypes.c_char_p, ctypes.c_int],
+        "pra_rel_device_rows": [_vp, ctypes.c_int, ctypes.POINTER(_u64p), _u64p],
+        "pra_rel_checksum": [_vp, ctypes.c_int, _u64p],
+        "pra_scc_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+        "pra_scc_destroy": [_vp],
+        "pra_scc_add_relation": [_vp, _vp, ctypes.c_int],
+        "pra_scc_add_join": [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _intp, ctypes.c_int],
+        "pra_scc_add_copy": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int],
+        "pra_scc_add_acopy": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int],
+        "pra_scc_add_copy_filter": [_vp, _vp, _vp, ctypes.c_int, _intp, ctypes.c_int, ctypes.c_int],
+        "pra_scc_add_dependency": [_vp, _vp],
+        "pra_scc_run": [_vp, _u64p, _u64, _u64p],
+        "pra_scc_begin": [_vp, ctypes.c_int],
+        "pra_scc_step": [_vp, _u64p],
+        "pra_scc_end": [_vp],
+        "pra_scc_history": [_vp, _u64p, _u64, _u64p],
+        "pra_partition_rows": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
+        "pra_partition_rows_noalloc": [_vp, _vp, _u64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
+        "pra_execute": [_vp],
+        "pra_execute_many": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _intp],
+        "pra_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64, ctypes.c_int, ctypes.POINTER(_vp)],
+        "pra_server_submit_tc": [_vp, _u64p, _u64, _u64, _u64p],
+        "pra_server_wait": [_vp, _u64, ctypes.POINTER(RequestResult)],
+        "pra_server_rows": [_vp, _u64, _u64p, _u64, _u64p],
+        "pra_server_destroy": [_vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.pra_last_error.argtypes = [_vp]
+    L.pra_last_error.restype = ctypes.c_char_p
+    return L
 
 
 def _ints(xs):
@@ -219,6 +233,10 @@ class Context:
         self._chk(self.L.pra_ctx_group_status(self.c, int(g), ctypes.byref(d), ctypes.byref(it), ctypes.byref(by)),
                   "group_status")
         return bool(d.value), int(it.value), int(by.value)
+
+    def trim(self):
+        """between steps: give back the context's scratch buffers (they grow again on demand)"""
+        self._chk(self.L.pra_ctx_trim(self.c), "trim")
 
     def release_group(self, g):
         """free instance g's relations and SCCs (their handles become invalid) and reset it"""
@@ -356,8 +374,8 @@ class Server:
     """Transitive-closure request server (pra_server_*): `slots` workers,
     admission in arrival order under an HBM budget, overlapping requests."""
 
-    def __init__(self, nbuckets=1, slots=4, hbm_budget=0, keep_rows=False, device=0):
-        self.L = lib()
+    def __init__(self, nbuckets=1, slots=4, hbm_budget=0, keep_rows=False, device=0, library=None):
+        self.L = library if library is not None else lib()
         h = _vp()
         rc = self.L.pra_server_create(int(device), int(nbuckets), int(slots), int(hbm_budget), int(bool(keep_rows)),
                                       ctypes.byref(h))

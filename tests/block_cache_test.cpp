@@ -67,15 +67,18 @@ int main()
     const void* sB = &sb_tag;
     size_t got = 0;
 
-    // size classes are powers of two from the minimum block up
+    // size classes are quarter octaves from the minimum block up
     expect(cache.size_class(1) == 64 * MB, "class of 1 byte is the minimum block");
-    expect(cache.size_class(64 * MB + 1) == 128 * MB, "class rounds up to the next power of two");
+    expect(cache.size_class(64 * MB + 1) == 80 * MB, "class rounds up to the next quarter octave");
+    expect(cache.size_class(113 * MB) == 128 * MB && cache.size_class(129 * MB) == 160 * MB &&
+               cache.size_class(400 * MB) == 448 * MB,
+           "classes 64, 80, 96, 112, 128, 160, ... MiB");
 
     // a block released while stream A may still use it is owned by A
     void* b = cache.get(0, sA, 100 * MB, &got, a);
-    expect(b != nullptr && got == 128 * MB, "fresh block of the request's class");
+    expect(b != nullptr && got == 112 * MB, "fresh block of the request's class");
     cache.put(0, sA, b, got);
-    expect(cache.owned_bytes(0, sA) == 128 * MB, "released block is owned by its stream");
+    expect(cache.owned_bytes(0, sA) == 112 * MB, "released block is owned by its stream");
 
     // another stream never gets it while A owns it
     void* c = cache.get(0, sB, 100 * MB, &got, a);
@@ -95,33 +98,34 @@ int main()
     int sc_tag = 0;
     const void* sC = &sc_tag;   // a stream that owns nothing
     cache.idle(sA);
-    expect(cache.owned_bytes(0, sA) == 0 && cache.idle_bytes(0) == 128 * MB, "idle moves A's blocks to the pool");
-    expect(cache.owned_bytes(0, sB) == 128 * MB, "idle of A leaves B's blocks owned");
+    expect(cache.owned_bytes(0, sA) == 0 && cache.idle_bytes(0) == 112 * MB, "idle moves A's blocks to the pool");
+    expect(cache.owned_bytes(0, sB) == 112 * MB, "idle of A leaves B's blocks owned");
     void* b3 = cache.get(0, sC, 70 * MB, &got, a);
     expect(b3 == b, "an idle block goes to another stream");
     cache.put(0, sC, b3, got);
 
-    // classes: a request takes its class or the next one up, never larger
+    // classes: a request takes its class or one of the next two, never larger
     cache.idle(sB);
-    cache.idle(sC);   // pool: b, c (128 MiB each), all idle
-    void* small = cache.get(0, sA, 10 * MB, &got, a);
-    expect((small == b || small == c) && got == 128 * MB, "a 64 MiB request may take a block of the next class");
+    cache.idle(sC);   // pool: b, c (112 MiB each), all idle
+    void* small = cache.get(0, sA, 90 * MB, &got, a);
+    expect((small == b || small == c) && got == 112 * MB, "a 96 MiB request may take a block two classes up");
     cache.put(0, sA, small, got);
     cache.idle(sA);
     void* big = cache.get(0, sA, 300 * MB, &got, a);
-    expect(big != b && big != c && got == 512 * MB, "a 512 MiB request gets a fresh block");
+    expect(big != b && big != c && got == 320 * MB, "a 320 MiB request gets a fresh block");
     cache.put(0, sA, big, got);
     void* tiny = cache.get(0, sB, 1 * MB, &got, a);
-    expect(tiny != big && got <= 128 * MB, "a 64 MiB request never takes a 512 MiB block");
+    expect(tiny != big && tiny != b && tiny != c && got == 64 * MB,
+           "a 64 MiB request takes no block three classes up (112 MiB) or more");
     cache.put(0, sB, tiny, got);
     cache.idle(sB);
 
-    // out of room: 768 MiB are held (b and c idle, `big` owned by A) and C
-    // needs 512 MiB more; the idle blocks are freed to make space, the owned
-    // one is not
-    expect(dev.used == 768 * MB, "bookkeeping before the pressure case");
+    // out of room: 608 MiB are held (b, c and tiny idle, `big` owned by A)
+    // and C needs 448 MiB more; the idle blocks are freed to make space, the
+    // owned one is not
+    expect(dev.used == 608 * MB, "bookkeeping before the pressure case");
     void* d0 = cache.get(0, sC, 400 * MB, &got, a);
-    expect(d0 != nullptr && got == 512 * MB, "allocation succeeds after freeing idle blocks");
+    expect(d0 != nullptr && got == 448 * MB, "allocation succeeds after freeing idle blocks");
     expect(!dev.live.count(b) && !dev.live.count(c), "idle blocks were freed");
     expect(dev.live.count(big) == 1, "a block owned by A survives C's out-of-memory flush");
     if (d0) cache.put(0, sC, d0, got);

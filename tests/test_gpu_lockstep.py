@@ -119,3 +119,41 @@ def test_lockstep_many_instances_equal_oracle():
             assert g.group_joined(i) == o.joined()
     finally:
         g.close()
+
+
+def test_step_with_trim_and_abandoned_instance():
+    """pra_ctx_step with the scratch given back after every step
+    (pra_ctx_trim), an instance abandoned in the middle of its fixed point
+    (pra_ctx_release_group of a running instance) and a new program built in
+    its place: the other instances and the new one equal the oracle"""
+    import pra_amd
+    k = 3
+    graphs = [random_graph(n, int(1.3 * n), 4100 + n) for n in (400, 1500, 90, 2500)]
+    late = random_graph(700, 910, 4242)
+    g = pra_amd.Context(nbuckets=k, batch_size=1)
+    try:
+        handles = {}
+        for i, e in enumerate(graphs):
+            g.set_group(i)
+            handles[i] = (e, tc_program(g, e))
+        steps = 0
+        while True:
+            active = g.step()
+            g.trim()
+            steps += 1
+            if steps == 5:
+                assert not g.group_status(1)[0]
+                g.release_group(1)          # abandoned mid-run
+                del handles[1]
+                g.set_group(1)
+                handles[1] = (late, tc_program(g, late))
+            if not active and steps > 5:
+                break
+        for i, (e, (T, G, sfix, scopy)) in sorted(handles.items()):
+            print("instance", i, flush=True)
+            o, (To, Go, sfo, sco) = oracle_run(k, lambda b: tc_program(b, e))
+            np.testing.assert_array_equal(g.rows(T), o.rows(To))
+            assert g.history(sfix) == o.history(sfo)
+            assert g.group_joined(i) == o.joined()
+    finally:
+        g.close()
