@@ -113,11 +113,12 @@ struct pra_server {
         if (live > held) scratch_hw = std::max(scratch_hw, live - held);
     }
 
-    // may the head of the queue join the requests in flight?
-    bool fits(const Request* r)
+    // may the head of the queue join the requests in flight and the `pending`
+    // bytes of estimates admitted before it in the same pass?
+    bool fits(const Request* r, bool first, uint64_t pending)
     {
-        if (!in_flight() || !budget) return true;
-        uint64_t need = scratch_hw + r->est;
+        if ((first && !in_flight()) || !budget) return true;
+        uint64_t need = scratch_hw + pending + r->est;
         for (int g = 0; g < slots; g++)
             if (slot[(size_t)g].r) need += std::max(slot[(size_t)g].r->est, held_by(g));
         return need <= budget;
@@ -264,7 +265,9 @@ struct pra_server {
                 cv_work.wait(lk, [&] { return stopping || !queue.empty() || in_flight(); });
                 if (stopping && queue.empty() && !in_flight()) break;
                 int free_slots = std::min(slots, limit) - in_flight();
-                while (ctx && free_slots > 0 && !queue.empty() && fits(queue.front())) {
+                uint64_t pending = 0;
+                while (ctx && free_slots > 0 && !queue.empty() && fits(queue.front(), admit.empty(), pending)) {
+                    pending += queue.front()->est;
                     admit.push_back(queue.front());
                     queue.pop_front();
                     free_slots--;

@@ -123,3 +123,27 @@ def test_request_that_fails_alone_fails(library, stream, alone):
             assert r["status"] == PRA_E_NOMEM
         else:
             check_equal([r], [rw], [a])
+
+
+def max_in_flight(res):
+    """most requests between admission and completion at one time (an end
+    at the same instant as a start comes first)"""
+    ev = sorted([(r["t_start"], 1) for r in res] + [(r["t_done"], -1) for r in res], key=lambda e: (e[0], e[1]))
+    cur = top = 0
+    for _, d in ev:
+        cur += d
+        top = max(top, cur)
+    return top
+
+
+def test_admission_counts_requests_admitted_together(library, stream, alone):
+    """requests admitted in the same pass count against the budget together:
+    with every estimate a third of the budget, at most three are in flight at
+    once although eight slots are free, and none is restarted (the real bytes
+    stay far below the estimates)"""
+    _, graphs = stream
+    budget = 3 * 10**9
+    res, rows = serve(library, graphs, est=budget // 3, slots=8, hbm_budget=budget)
+    check_equal(res, rows, alone)
+    assert sum(r["restarts"] for r in res) == 0
+    assert max_in_flight(res) == 3
