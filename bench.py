@@ -26,6 +26,68 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+REF_TC = os.path.join(ROOT, "oracle", "_ref", "TC")
+MPIRUN = "/opt/conda/bin/mpirun"
+REF_RANKS = 8
+REF_SAMPLE_GRID = 100
+# the reference's own 8-rank run of the full 150x150 grid, measured in the
+# build container: 77.5 s over 298 iterations
+REF_FULL_GRID150_S = 77.5
+
+
+def grid_join_pairs(n):
+    """join pairs of semi-naive TC on the n x n grid (edges right and down):
+    every path x -> y enters delta once and is joined with y's out-edges, and
+    (i, j) is reached from the (i+1)(j+1)-1 cells up and to its left"""
+    return sum(((i + 1) * (j + 1) - 1) * ((i < n - 1) + (j < n - 1)) for i in range(n) for j in range(n))
+
+
+def cpu_baseline_reference(nb):
+    """The reference itself (tests/transitive_closure.cpp built from its own
+    sources into oracle/_ref/TC) under mpirun on REF_RANKS host processes, on a
+    bounded sample of the same workload: the REF_SAMPLE_GRID-square grid with
+    randomly permuted u64 ids.  Time = the sum of the reference's own
+    per-iteration Total_time column over its fixed-point SCC.  None when the
+    reference build or mpirun is absent."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    from programs import grid_graph, write_bin
+    if not (os.path.exists(REF_TC) and os.path.exists(MPIRUN)) or nb != REF_RANKS:
+        return None
+    n = REF_SAMPLE_GRID
+    with tempfile.TemporaryDirectory() as d:
+        # the program reads its (empty) T seed from ../data/g5955/path_2_1_2
+        os.makedirs(os.path.join(d, "tests"))
+        os.makedirs(os.path.join(d, "data", "g5955"))
+        write_bin(os.path.join(d, "data", "g5955", "path_2_1_2"), np.zeros((0, 3), dtype=np.uint64))
+        write_bin(os.path.join(d, "edges"), grid_graph(n, seed=150))
+        r = subprocess.run([MPIRUN, "-n", str(REF_RANKS), REF_TC, os.path.join(d, "edges")], cwd=os.path.join(d, "tests"),
+                           capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"reference baseline failed ({r.returncode}): {r.stderr[-500:]}")
+        return None
+    total, iters, in_fix = 0.0, 0, False
+    for line in r.stdout.splitlines():
+        if line.startswith("-------------------Executing SCC"):
+            in_fix = line.split()[-1].startswith("1-")   # the fixed-point SCC is RAM(true, 1)
+        f = line.split("\t")
+        if in_fix and len(f) == 9 and f[0].isdigit():
+            total += float(f[8])
+            iters += 1
+    facts = (n * (n + 1) // 2) ** 2 - n * n
+    if f"({facts} total facts)" not in r.stdout or not iters:
+        log("reference baseline: unexpected output")
+        return None
+    j = grid_join_pairs(n)
+    return {"value": j / total, "unit": "joined tuples/s", "cores": REF_RANKS, "kind": "reference",
+            "sample": f"the reference's tests/transitive_closure.cpp built from its sources (oracle/_ref/TC), "
+                      f"mpirun -n {REF_RANKS}, on the {n}x{n} grid with permuted u64 ids: {j} joined tuples, "
+                      f"{facts} tuples, {iters} iterations in {total:.2f} s (sum of its per-iteration Total_time); "
+                      f"its full 150x150 run at {REF_RANKS} ranks took "
+                      f"{REF_FULL_GRID150_S} s ({grid_join_pairs(150) / REF_FULL_GRID150_S / 1e6:.2f} M joined tuples/s)"}
+
+
 def cpu_baseline(edges, nb, iters):
     """The oracle (a scalar C restatement of the reference loop) on the first
     `iters` fixed-point iterations of the same workload."""
@@ -65,7 +127,8 @@ def main():
     ap.add_argument("--count", type=int, default=256, help="batch/serve: number of instances / requests")
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=50000)
-    ap.add_argument("--slots", type=int, default=64, help="serve: requests in flight at most")
+    ap.add_argument("--slots", type=int, default=None,
+                    help="batch/serve: instances / requests in flight at most (default 256 / 64)")
     ap.add_argument("--budget-gb", type=float, default=160.0, help="batch/serve: HBM admission budget")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--grid", type=int, default=150)
@@ -177,7 +240,11 @@ def main():
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches_per_step": d["launches"] // args.steps,
                     "algorithmic_bytes_per_launch": round(bytes_per)}
 
-    cpu = None if (args.no_cpu or args.workload != "tc") else cpu_baseline(edges, nb, args.cpu_iters)
+    cpu = None
+    if not args.no_cpu and args.workload == "tc":
+        cpu = cpu_baseline_reference(nb) if n == 150 else None
+        if cpu is None:
+            cpu = cpu_baseline(edges, nb, args.cpu_iters)
 
     out = {
         "metric": METRIC,
@@ -212,12 +279,14 @@ def est_bytes(n):
 
 def bench_batch(args):
     """Config 4: independent TC instances, plain (one context per instance,
-    one after another) against batched (waves of instances admitted by
-    estimated bytes under the HBM budget, each wave one context whose
-    instances run in lock step: every phase of an iteration one batched
-    launch over all of them).  Every instance is compared bit for bit
-    (device checksum of its rows in order, count, join pairs) and its |TC|
-    against the SCC-condensation count computed on the CPU."""
+    one after another) against batched (one context whose instances run in
+    lock step: every phase of an iteration is one batched launch over all of
+    them; instances are queued largest first and admitted between iterations
+    while their estimated bytes fit the HBM budget, up to --slots at once, and
+    a finished instance is read and released at once so that the next one
+    takes its place).  Every instance is compared bit for bit (device checksum
+    of its rows in order, count, join pairs) and its |TC| against the
+    SCC-condensation count computed on the CPU."""
     import pra_amd
     from programs import config4_instances, random_graph, tc_program, tc_size_condensation
     inst = config4_instances(count=args.count, nmin=args.nmin, nmax=args.nmax)
@@ -242,39 +311,57 @@ def bench_batch(args):
         if (k + 1) % 32 == 0:
             log(f"  plain: {k + 1}/{len(graphs)} instances, {t_plain:.2f} s")
 
-    # batched: waves admitted by estimated bytes, largest first
-    order = sorted(range(len(graphs)), key=lambda i: -inst[i][0])
+    # batched: continuous admission into one context.  As in plain, the timed
+    # figure starts with the inputs resident in HBM and ends before the result
+    # checks: building an admitted instance's program (its edges uploaded) and
+    # the checks of a finished one (its rows sorted into the reference's order
+    # and checksummed) run between the steps on a synchronized stream, and
+    # their time is taken out
+    from collections import deque
     budget = int(args.budget_gb * (1 << 30))
+    slots = args.slots or 256
+    queue = deque(sorted(range(len(graphs)), key=lambda i: -inst[i][0]))
     batched = [None] * len(graphs)
-    t_batch = 0.0
-    pos = 0
-    waves = 0
-    while pos < len(order):
-        wave, used = [], 0
-        while pos < len(order) and (not wave or used + est_bytes(inst[order[pos]][0]) <= budget):
-            wave.append(order[pos])
-            used += est_bytes(inst[order[pos]][0])
-            pos += 1
-        c = pra_amd.Context(nbuckets=nb, batch_size=1)
-        hs = []
-        for g, i in enumerate(wave):
-            c.set_group(g)
-            hs.append(tc_program(c, graphs[i]))
+    c = pra_amd.Context(nbuckets=nb, batch_size=1)
+    free = list(range(slots - 1, -1, -1))
+    flight = {}                        # instance id -> (graph index, handles)
+    held = 0                           # estimated bytes of the instances in flight
+    steps = top = 0
+    t_checks = t_admit = 0.0
+    t0 = time.perf_counter()
+    while queue or flight:
         c.synchronize()
-        t0 = time.perf_counter()
-        try:
-            c.execute()
-            c.synchronize()
-        except pra_amd.PraError as ex:
-            raise RuntimeError(f"wave of instances {wave} failed: {ex}")
-        t_batch += time.perf_counter() - t0
-        waves += 1
-        for g, (i, (T, G, sfix, _)) in enumerate(zip(wave, hs)):
-            batched[i] = (c.count(T), c.group_joined(g), c.checksum(T))
-        peak = c.device_bytes()[1]
-        c.close()
-        log(f"  batched: wave of {len(wave)} (estimated {used / 2**30:.1f} GiB, peak {peak / 2**30:.1f} GiB), "
-            f"{pos}/{len(order)} instances, {t_batch:.2f} s")
+        a0 = time.perf_counter()
+        while queue and free and (not flight or held + est_bytes(inst[queue[0]][0]) <= budget):
+            i = queue.popleft()
+            g = free.pop()
+            c.set_group(g)
+            flight[g] = (i, tc_program(c, graphs[i]))
+            held += est_bytes(inst[i][0])
+        c.synchronize()
+        t_admit += time.perf_counter() - a0
+        top = max(top, len(flight))
+        c.step()
+        steps += 1
+        for g in list(flight):
+            if c.group_status(g)[0]:
+                i, (T, G, sfix, _) = flight.pop(g)
+                c.synchronize()
+                k0 = time.perf_counter()
+                batched[i] = (c.count(T), c.group_joined(g), c.checksum(T))
+                t_checks += time.perf_counter() - k0
+                c.release_group(g)
+                free.append(g)
+                held -= est_bytes(inst[i][0])
+                done = sum(b is not None for b in batched)
+                if done % 32 == 0:
+                    log(f"  batched: {done}/{len(graphs)} instances, {time.perf_counter() - t0:.2f} s")
+    c.synchronize()
+    t_batch = time.perf_counter() - t0 - t_checks - t_admit
+    peak = c.device_bytes()[1]
+    c.close()
+    log(f"  batched: {steps} lock-step iterations, at most {top} instances in flight, peak {peak / 2**30:.1f} GiB, "
+        f"{t_admit:.2f} s of program builds and {t_checks:.2f} s of result checks taken out")
     exact = all(a == b for a, b in zip(plain, batched))
     closed = None
     if not args.no_closed_form:
@@ -284,16 +371,18 @@ def bench_batch(args):
             if (i + 1) % 32 == 0:
                 log(f"  closed form: {i + 1}/{len(graphs)} instances checked, all equal so far: {closed}")
     joined = sum(j for _, j, _ in plain)
-    log(f"plain {t_plain:.3f} s, batched {t_batch:.3f} s ({waves} waves), bit-exact per instance: {exact}, "
+    log(f"plain {t_plain:.3f} s, batched {t_batch:.3f} s, bit-exact per instance: {exact}, "
         f"|TC| = closed form: {closed}")
     out = {
         "metric": METRIC, "value": joined / t_batch, "unit": "joined tuples/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"tc_batch{len(graphs)}_n{args.nmin}-{args.nmax}", "buckets": nb,
-                   "budget_gb": args.budget_gb, "waves": waves, "tuples": sum(t for t, _, _ in plain),
+                   "budget_gb": args.budget_gb, "slots": slots, "lockstep_iterations": steps,
+                   "max_in_flight": top, "tuples": sum(t for t, _, _ in plain),
                    "joined_tuples": joined},
         "plain_value": joined / t_plain, "plain_s": t_plain, "batched_s": t_batch,
+        "batched_builds_s": t_admit, "batched_checks_s": t_checks,
         "batched_over_plain": t_plain / t_batch, "bit_exact_per_instance": exact, "tc_equals_closed_form": closed,
         "roofline": None, "cpu_baseline": None,
     }
@@ -332,7 +421,7 @@ def bench_serve(args):
     for rate in [float(r) for r in args.rates.split(",")]:
         rng = np.random.default_rng(int(rate))
         gaps = rng.exponential(1.0 / rate, size=len(graphs))
-        srv = pra_amd.Server(nbuckets=args.buckets, slots=args.slots, hbm_budget=budget)
+        srv = pra_amd.Server(nbuckets=args.buckets, slots=args.slots or 64, hbm_budget=budget)
         t0 = time.perf_counter()
         tickets = []
         due = 0.0
@@ -368,7 +457,7 @@ def bench_serve(args):
         "n_gpus": 1, "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"tc_serve{len(graphs)}_n{args.nmin}-{args.nmax}", "buckets": args.buckets,
-                   "slots": args.slots},
+                   "slots": args.slots or 64},
         "serve": rows, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -20,12 +20,16 @@
 //     may free them.
 //   * flush(device) frees idle blocks only; an owned block is never freed or
 //     handed to another stream before its owner is declared idle.
+// Out of memory, the idle blocks released longest ago are freed first, and
+// only until the failed request fits: freeing tens of GB and touching fresh
+// memory again costs far more than the request.
 // The caller must call idle() only after a synchronization that happened after
 // the put() calls it covers (the cache records no time, so it relies on the
 // host order of put, synchronize, idle).
 #ifndef PRA_BLOCKCACHE_HPP
 #define PRA_BLOCKCACHE_HPP
 
+#include <algorithm>
 #include <cstddef>
 #include <mutex>
 #include <vector>
@@ -81,17 +85,17 @@ class BlockCache {
     }
 
     // take(), else a fresh block of the request's class from the allocator;
-    // when the allocator fails, the idle blocks of the device are freed and
-    // the allocation retried once.  nullptr when that fails too.
+    // when the allocator fails, idle blocks of the device are freed, oldest
+    // first, until the request's size is freed, and the allocation retried;
+    // then once more after freeing every idle block.  nullptr when that fails
+    // too.
     void* get(int device, const void* stream, size_t bytes, size_t* got, const BlockAllocator& a)
     {
         if (void* p = take(device, stream, bytes, got)) return p;
         const size_t cls = size_class(bytes);
         void* p = a.alloc(cls, a.user);
-        if (!p) {
-            flush(device, a);
-            p = a.alloc(cls, a.user);
-        }
+        if (!p && flush(device, a, cls)) p = a.alloc(cls, a.user);
+        if (!p && flush(device, a)) p = a.alloc(cls, a.user);
         if (p) *got = cls;
         return p;
     }
@@ -99,7 +103,7 @@ class BlockCache {
     void put(int device, const void* stream, void* p, size_t size)
     {
         std::lock_guard<std::mutex> lk(mu_);
-        free_.push_back(Entry{p, size, device, stream});
+        free_.push_back(Entry{p, size, device, stream, ++clock_});
     }
 
     // every piece of work enqueued on `stream` so far has completed
@@ -110,19 +114,26 @@ class BlockCache {
             if (b.owner == stream) b.owner = nullptr;
     }
 
-    // free the idle blocks of a device; returns the bytes freed
-    size_t flush(int device, const BlockAllocator& a)
+    // free idle blocks of a device, released longest ago first, until at
+    // least `want` bytes are freed (all of them by default); returns the
+    // bytes freed
+    size_t flush(int device, const BlockAllocator& a, size_t want = ~(size_t)0)
     {
         std::vector<Entry> drop;
         {
             std::lock_guard<std::mutex> lk(mu_);
-            for (size_t i = 0; i < free_.size();)
-                if (free_[i].device == device && !free_[i].owner) {
-                    drop.push_back(free_[i]);
-                    free_.erase(free_.begin() + i);
-                } else {
-                    i++;
-                }
+            std::vector<size_t> idle;
+            for (size_t i = 0; i < free_.size(); i++)
+                if (free_[i].device == device && !free_[i].owner) idle.push_back(i);
+            std::sort(idle.begin(), idle.end(), [&](size_t x, size_t y) { return free_[x].stamp < free_[y].stamp; });
+            size_t freed = 0, k = 0;
+            for (; k < idle.size() && freed < want; k++) freed += free_[idle[k]].size;
+            idle.resize(k);
+            std::sort(idle.begin(), idle.end());
+            for (size_t j = idle.size(); j-- > 0;) {
+                drop.push_back(free_[idle[j]]);
+                free_.erase(free_.begin() + (long)idle[j]);
+            }
         }
         size_t bytes = 0;
         for (auto& b : drop) {
@@ -157,8 +168,10 @@ class BlockCache {
         size_t size;
         int device;
         const void* owner;   // stream that may still use the block, or null when idle
+        unsigned long long stamp;   // order of release
     };
     size_t min_;
+    unsigned long long clock_ = 0;
     mutable std::mutex mu_;
     std::vector<Entry> free_;
 };

@@ -54,9 +54,10 @@ static void check_launch(const char* what)
 // Large blocks (>= 64 MiB) go through the process-wide block cache of
 // pra_blockcache.hpp (its ownership rule: a block released while its stream
 // runs is reused only on that stream until a host synchronization of the
-// stream declares it idle).  Out of memory: idle blocks are freed, then the
-// requesting stream is synchronized, its own blocks freed and the
-// stream-ordered pool trimmed before one more attempt.
+// stream declares it idle).  Out of memory: idle blocks are freed, oldest
+// first, until the request fits; then the requesting stream is synchronized,
+// its own blocks declared idle and the stream-ordered pool trimmed before one
+// more attempt.
 namespace {
 constexpr size_t kBigBlock = 64ULL << 20;
 pra::BlockCache g_big(kBigBlock);
@@ -113,7 +114,6 @@ void* big_get(size_t bytes, hipStream_t st, size_t* got)
     if (!p) {
         HIPCHK(hipStreamSynchronize(st));
         g_big.idle(st);
-        g_big.flush(dev, kHipBlocks);
         trim_pool(dev);
         p = g_big.get(dev, st, bytes, got, kHipBlocks);
     }
@@ -195,7 +195,7 @@ struct DBuf {
                 int dev = 0;
                 (void)hipGetLastError();
                 HIPCHK(hipGetDevice(&dev));
-                g_big.flush(dev, kHipBlocks);
+                g_big.flush(dev, kHipBlocks, nc);
                 if (hipMallocAsync(&np, nc, s) != hipSuccess) {
                     (void)hipGetLastError();
                     HIPCHK(hipStreamSynchronize(s));

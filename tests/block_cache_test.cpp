@@ -143,6 +143,32 @@ int main()
     expect(dev.live.empty(), "flush of an all-idle cache frees every block");
     expect(dev.allocs == dev.frees, "allocations and releases balance");
 
+    // out of room with several idle blocks: only the ones released longest
+    // ago are freed, and only as many as the request needs
+    {
+        FakeDevice small;
+        small.capacity = 256 * MB;
+        pra::BlockAllocator sa{fake_alloc, fake_release, &small};
+        pra::BlockCache lru(64 * MB);
+        void* x[4];
+        for (void*& p : x) p = lru.get(0, sA, 64 * MB, &got, sa);
+        expect(small.used == 256 * MB, "four 64 MiB blocks fill the small device");
+        lru.put(0, sA, x[2], 64 * MB);   // released first
+        lru.put(0, sA, x[0], 64 * MB);
+        lru.put(0, sA, x[1], 64 * MB);   // released last
+        lru.idle(sA);
+        void* y = lru.get(0, sB, 100 * MB, &got, sa);   // 112 MiB: two blocks must go
+        expect(y != nullptr && got == 112 * MB, "the request fits after a partial flush");
+        expect(!small.live.count(x[2]) && !small.live.count(x[0]) && small.live.count(x[1]),
+               "the two blocks released longest ago were freed, the newest kept");
+        lru.put(0, sB, y, got);
+        lru.put(0, sA, x[3], 64 * MB);
+        lru.idle(sA);
+        lru.idle(sB);
+        lru.flush(0, sa);
+        expect(small.live.empty(), "a full flush still frees every idle block");
+    }
+
     // devices are separate
     void* e = cache.get(1, sA, 64 * MB, &got, a);
     cache.put(1, sA, e, got);
