@@ -425,13 +425,19 @@ def bench_serve(args):
         t0 = time.perf_counter()
         tickets = []
         due = 0.0
-        for e, (n, _, _), gap in zip(graphs, inst, gaps):
+        for k, (e, (n, _, _), gap) in enumerate(zip(graphs, inst, gaps)):
             due += gap
             now = time.perf_counter() - t0
             if due > now:
                 time.sleep(due - now)
             tickets.append(srv.submit(e, est_bytes=est_bytes(n)))
-        res = [srv.wait(t) for t in tickets]
+            if (k + 1) % 1000 == 0:
+                log(f"  rate {rate:g}: {k + 1} submitted at {time.perf_counter() - t0:.1f} s")
+        res = []
+        for t in tickets:
+            res.append(srv.wait(t))
+            if len(res) % 1000 == 0:
+                log(f"  rate {rate:g}: {len(res)} served at {time.perf_counter() - t0:.1f} s")
         srv.close()
         bad = [(i, inst[i][0], r["status"], round(r["t_start"], 3), round(r["t_done"], 3))
                for i, r in enumerate(res) if r["status"] != 0]
