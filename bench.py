@@ -329,22 +329,26 @@ def bench_batch(args):
     steps = top = 0
     t_checks = t_admit = 0.0
     t0 = time.perf_counter()
+    def admissible():
+        return queue and free and (not flight or held + est_bytes(inst[queue[0]][0]) <= budget)
+
     while queue or flight:
-        c.synchronize()
-        a0 = time.perf_counter()
-        while queue and free and (not flight or held + est_bytes(inst[queue[0]][0]) <= budget):
-            i = queue.popleft()
-            g = free.pop()
-            c.set_group(g)
-            flight[g] = (i, tc_program(c, graphs[i]))
-            held += est_bytes(inst[i][0])
-        c.synchronize()
-        t_admit += time.perf_counter() - a0
+        if admissible():
+            c.synchronize()
+            a0 = time.perf_counter()
+            while admissible():
+                i = queue.popleft()
+                g = free.pop()
+                c.set_group(g)
+                flight[g] = (i, tc_program(c, graphs[i]))
+                held += est_bytes(inst[i][0])
+            c.synchronize()
+            t_admit += time.perf_counter() - a0
         top = max(top, len(flight))
         c.step()
         steps += 1
         for g in list(flight):
-            if c.group_status(g)[0]:
+            if c.group_done(g):
                 i, (T, G, sfix, _) = flight.pop(g)
                 c.synchronize()
                 k0 = time.perf_counter()

@@ -144,7 +144,9 @@ int pra_scc_create(pra_ctx* ctx, int fixpoint, int id, pra_scc** out);
 int pra_scc_destroy(pra_scc* scc);
 /* RAM::add_relation(relation*&, bool move_full_to_delta) (src/RAM/RA_tasks.cpp:34-39) */
 int pra_scc_add_relation(pra_scc* scc, pra_rel* rel, int move_full_to_delta);
-/* parallel_join(out, in0, ver0, in1, ver1, projection) (src/RA/parallel_join.h:55-58) */
+/* parallel_join(out, in0, ver0, in1, ver1, projection) (src/RA/parallel_join.h:55-58).  The join
+ * matches on in0's join column count (src/RAM/RA_tasks.cpp:448); when in1's count differs, a pair is
+ * found only where the reference's ranks meet it: in1's row in the same bucket under both counts. */
 int pra_scc_add_join(pra_scc* scc, pra_rel* out, pra_rel* in0, int ver0, pra_rel* in1, int ver1, const int* proj,
                      int nproj);
 /* parallel_copy(dest, src, ver, reorder) (src/RA/parallel_copy.h:28-32) */

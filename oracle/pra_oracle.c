@@ -780,7 +780,10 @@ int ora_scc_add_join(ora_ctx *c, int scc, int out, int in0, int v0, int in1, int
 {
     relation_t *o = &c->rel[out];
     if (nproj != o->arity || nproj > MAXW) return -1;
-    if (c->rel[in0].jcc != c->rel[in1].jcc) return -1;
+    /* input 0's join column count is the one matched (src/RAM/RA_tasks.cpp:448);
+     * with a different count on input 1, local_join below still walks each
+     * bucket's own rows, which is what the reference's ranks see */
+    if (c->rel[in0].jcc > c->rel[in1].arity) return -1;
     int span = c->rel[in0].arity + 1 + c->rel[in1].arity + 1 - c->rel[in0].jcc;
     for (int i = 0; i < nproj; i++) if (proj[i] < 0 || proj[i] >= span) return -1;
     ra_t *r = scc_new_ra(&c->scc[scc]);

@@ -293,11 +293,19 @@ struct pra_rel {
     bool scc_changing = false, merge_full = false;
     // set while an SCC runs: a rule of it reads the relation's full version
     bool full_read = false;
+    // changes of the delta and full versions so far
+    u64 gen[2] = {0, 0};
+    // a view: the rows of relation view_of that a join keyed on this
+    // relation's jcc columns can match (built from view_of's versions of
+    // generation view_gen)
+    pra_rel* view_of = nullptr;
+    u64 view_gen[2] = {~0ULL, ~0ULL};
     void touched(int v)
     {
         if (v == PRA_DELTA || v == PRA_FULL) {
             idx_valid[v] = false;
             bnd_valid[v] = false;
+            gen[v]++;
         }
     }
 };
@@ -1009,7 +1017,7 @@ static void build_dictionary(pra_ctx* c, int gi)
     Group& G = grp(c, gi);
     u64 total = 0;
     for (pra_rel* R : c->rels)
-        if (R->group == gi)
+        if (R->group == gi && !R->view_of)
             for (int v : {PRA_FULL, PRA_DELTA}) total += R->ver[v].n * (u64)R->arity;
     if (total == 0) return;
     Rows V;
@@ -1017,7 +1025,7 @@ static void build_dictionary(pra_ctx* c, int gi)
     u64 off = 0;
     for (pra_rel* R : c->rels)
         for (int v : {PRA_FULL, PRA_DELTA}) {
-            if (R->group != gi) continue;
+            if (R->group != gi || R->view_of) continue;
             Rows& r = R->ver[v];
             pra_launch_collect_cols(c->st, r.d(), r.n, R->w, R->arity, V.d() + off);
             off += r.n * (u64)R->arity;
@@ -1484,8 +1492,50 @@ static bool is_intern_type(int type) { return type != RA_ACOPY; }
 // of the current SCCs of several instances in lock step: their rules, outputs
 // and per-phase launches are taken together, while dedup, intern ids and
 // dictionaries stay per instance.
+// A join whose inputs have different join column counts matches on input 0's
+// count (src/RAM/RA_tasks.cpp:448 passes input0->get_join_column_count() to
+// local_join) and reads input 1 through a view keyed on that count.
+// local_join probes input 1 in the bucket of the jcc0-column prefix
+// (src/RA/parallel_join.cpp:180-186), and a rank holds only the rows of its own
+// buckets, so a row of input 1 takes part only when its bucket under its first
+// jcc0 columns equals its own bucket; for a RIGHT join it is the outer row
+// whose bucket is checked the same way.  The view holds exactly those rows.
+// Input 1's version is sorted by (bucket, columns), so the kept rows are in
+// the view's order already.
+static void refresh_view(pra_ctx* c, pra_rel* V, int v)
+{
+    pra_rel* src = V->view_of;
+    ensure_sorted(c, src, v);
+    if (V->view_gen[v] == src->gen[v]) return;
+    Rows& S = src->ver[v];
+    Rows& D = V->ver[v];
+    const u64 n = S.n;
+    if (n >= (1ULL << 32)) throw PraError(PRA_E_UNSUPPORTED, "join view of 2^32 rows or more");
+    u64 kept = 0;
+    if (n) {
+        c->keep.reserve(n * 4, c->st);
+        c->kp.reserve(n * 4, c->st);
+        c->tmp64a.reserve(8, c->st);
+        pra_launch_fill_u64(c->st, c->tmp64a.as<u64>(), 1, 0);
+        pra_launch_view_keep(c->st, S.d(), n, src->w, V->jcc, src->jcc, (u64)c->nb, c->keep.as<u32>());
+        check_launch("view_keep");
+        scan_u32(c, c->keep.as<u32>(), c->kp.as<u32>(), n, c->tmp64a.as<u32>());
+        kept = readback(c, c->tmp64a.p, 1)[0] & 0xFFFFFFFFULL;
+        D.buf.reserve(std::max<u64>(1, kept) * src->w * 8, c->st);
+        pra_launch_view_compact(c->st, S.d(), n, src->w, c->keep.as<u32>(), c->kp.as<u32>(), D.d());
+        check_launch("view_compact");
+    }
+    D.n = kept;
+    D.sorted = true;
+    V->touched(v);
+    V->view_gen[v] = src->gen[v];
+}
+
 static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
 {
+    for (pra_scc* m : members)
+        for (auto& ra : m->ras)
+            if (ra.type == RA_JOIN && ra.in1->view_of) refresh_view(c, ra.in1, ra.v1);
     Merged merged(c, members);
     Merged* S = &merged;
     const int nb = c->nb;
@@ -2513,9 +2563,21 @@ static void scc_begin(pra_scc* S, bool move_status)
         if (ra.v0 == PRA_FULL) ra.in0->full_read = true;
         if (ra.type == RA_JOIN && ra.v1 == PRA_FULL) ra.in1->full_read = true;
     }
+    // a view changes with its relation, and reading its full reads the relation's
+    for (auto& ra : S->ras)
+        if (ra.type == RA_JOIN && ra.in1->view_of) {
+            ra.in1->scc_changing = ra.in1->view_of->scc_changing;
+            if (ra.v1 == PRA_FULL) ra.in1->view_of->full_read = true;
+        }
     for (auto& ra : S->ras)
         for (pra_rel* R : {ra.in0, ra.in1, ra.out})
             if (R) R->merge_full = R->full_read && R->scc_changing;
+    for (auto& ra : S->ras)
+        if (ra.type == RA_JOIN && ra.in1->view_of) {
+            pra_rel* src = ra.in1->view_of;
+            src->merge_full = src->full_read && src->scc_changing;
+            ra.in1->merge_full = false;
+        }
     if (move_status)
         for (size_t i = 0; i < S->rels.size(); i++)
             if (S->status[i]) full_to_delta(c, S->rels[i]);
@@ -2940,7 +3002,14 @@ int pra_rel_destroy(pra_rel* rel)
     if (!rel) return PRA_E_ARG;
     pra_ctx* c = rel->ctx;
     (void)hipStreamSynchronize(c->st);
-    c->rels.erase(std::remove(c->rels.begin(), c->rels.end(), rel), c->rels.end());
+    std::vector<pra_rel*> keep;
+    for (pra_rel* r : c->rels)
+        if (r == rel || r->view_of == rel) {
+            if (r != rel) delete r;
+        } else {
+            keep.push_back(r);
+        }
+    c->rels.swap(keep);
     delete rel;
     return PRA_OK;
 }
@@ -3111,7 +3180,7 @@ int pra_scc_add_join(pra_scc* scc, pra_rel* out, pra_rel* in0, int ver0, pra_rel
     if ((ver0 != PRA_DELTA && ver0 != PRA_FULL) || (ver1 != PRA_DELTA && ver1 != PRA_FULL))
         return fail(c, PRA_E_ARG, "join versions must be DELTA or FULL");
     if (nproj != out->arity) return fail(c, PRA_E_ARG, "projection width must equal the output arity");
-    if (in0->jcc != in1->jcc) return fail(c, PRA_E_ARG, "join inputs must share the join column count");
+    if (in0->jcc > in1->arity) return fail(c, PRA_E_ARG, "input 0's join column count exceeds input 1's arity");
     int span = in0->w + in1->w - in0->jcc;
     for (int i = 0; i < nproj; i++)
         if (proj[i] < 0 || proj[i] >= span) return fail(c, PRA_E_ARG, "projection index out of range");
@@ -3122,6 +3191,25 @@ int pra_scc_add_join(pra_scc* scc, pra_rel* out, pra_rel* in0, int ver0, pra_rel
     ra.v0 = ver0;
     ra.in1 = in1;
     ra.v1 = ver1;
+    if (in0->jcc != in1->jcc) {
+        // input 1 seen through its view keyed on input 0's join column count
+        pra_rel* view = nullptr;
+        for (pra_rel* r : c->rels)
+            if (r->view_of == in1 && r->jcc == in0->jcc) view = r;
+        if (!view) {
+            view = new pra_rel();
+            view->ctx = c;
+            view->group = in1->group;
+            view->arity = in1->arity;
+            view->jcc = in0->jcc;
+            view->tag = in1->tag;
+            view->canonical = in1->canonical;
+            view->w = in1->w;
+            view->view_of = in1;
+            c->rels.push_back(view);
+        }
+        ra.in1 = view;
+    }
     ra.map.assign(proj, proj + nproj);
     scc->ras.push_back(std::move(ra));
     return PRA_OK;

@@ -447,6 +447,26 @@ __global__ void k_collect_cols(const u64* rows, u64 n, int w, int ncols, u64* ou
     for (int c = 0; c < ncols; c++) out[i * ncols + c] = rows[i * w + c];
 }
 
+// keep[i] = 1 when row i falls in the same bucket under the hash of its first
+// ja columns as under the hash of its first jb columns
+__global__ void k_view_keep(const u64* rows, u64 n, int w, int ja, int jb, u64 nb, u32* keep)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64* r = rows + i * w;
+    keep[i] = (pra_fnv(r, ja) % nb) == (pra_fnv(r, jb) % nb) ? 1u : 0u;
+}
+
+// the kept rows, in order, at their exclusive-prefix positions
+__global__ void k_view_compact(const u64* rows, u64 n, int w, const u32* keep, const u32* pos, u64* out)
+{
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    const u64* r = rows + i * w;
+    u64* o = out + (u64)pos[i] * w;
+    for (int k = 0; k < w; k++) o[k] = r[k];
+}
+
 // values sorted and distinct: rank = position
 __global__ void k_dict_insert(const u64* vals, u64 n, u64* tab, u64 cap, u32* flags)
 {
@@ -2527,6 +2547,18 @@ void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int 
 {
     if (!n) return;
     hipLaunchKernelGGL(k_collect_cols, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, ncols, out);
+}
+
+void pra_launch_view_keep(hipStream_t st, const u64* rows, u64 n, int w, int ja, int jb, u64 nb, u32* keep)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_view_keep, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, ja, jb, nb, keep);
+}
+
+void pra_launch_view_compact(hipStream_t st, const u64* rows, u64 n, int w, const u32* keep, const u32* pos, u64* out)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_view_compact, dim3(grid_for(n)), dim3(PRA_BLOCK), 0, st, rows, n, w, keep, pos, out);
 }
 
 void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u64 cap, u32* flags)

@@ -329,6 +329,11 @@ void pra_launch_sort_scan(hipStream_t st, const u32* ghist, int np, u32* gofs, u
 void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int w, DigitSpec ds, const u32* gofs,
                              u64* status, u32* tile_ctr, u32 epoch, u32* flags, const u32* iin, u32* iout);
 void pra_launch_collect_cols(hipStream_t st, const u64* rows, u64 n, int w, int ncols, u64* out);
+// rows of a join input seen under another join column count: keep[i] = 1 when
+// row i's bucket under its first ja columns equals its bucket under its first
+// jb columns; then the kept rows compacted in order (pos = exclusive scan of keep)
+void pra_launch_view_keep(hipStream_t st, const u64* rows, u64 n, int w, int ja, int jb, u64 nb, u32* keep);
+void pra_launch_view_compact(hipStream_t st, const u64* rows, u64 n, int w, const u32* keep, const u32* pos, u64* out);
 void pra_launch_dict_insert(hipStream_t st, const u64* vals, u64 n, u64* tab, u64 cap, u32* flags);
 void pra_launch_gather_rows(hipStream_t st, const u64* in, const u32* idx, u64 n, int w, u64* out);
 void pra_launch_key_bounds(hipStream_t st, const u64* keys, u64 n, u64 bmul, int ls, u64 nb, u64* bnd);

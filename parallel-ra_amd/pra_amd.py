@@ -227,6 +227,12 @@ class Context:
         self._chk(self.L.pra_ctx_step(self.c, ctypes.byref(v)), "step")
         return int(v.value)
 
+    def group_done(self, g):
+        """instance g has no SCC in progress or runnable"""
+        d = ctypes.c_int()
+        self._chk(self.L.pra_ctx_group_status(self.c, int(g), ctypes.byref(d), None, None), "group_status")
+        return bool(d.value)
+
     def group_status(self, g):
         """(finished, iterations, device bytes of its relations) of instance g"""
         d, it, by = ctypes.c_int(), _u64(), _u64()

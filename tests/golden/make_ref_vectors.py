@@ -253,6 +253,49 @@ def threshold_split(b):
     return O
 
 
+def mixed_jcc(b):
+    """Joins whose inputs have different join column counts.  The reference
+    matches on input 0's count (src/RAM/RA_tasks.cpp:448), and a rank probes
+    input 1 only in its own bucket (src/RA/parallel_join.cpp:180-186), so a
+    match exists only where input 1's row falls in the same bucket under both
+    counts:
+         Q(z, x, c) <- R(y, x) DELTA JOIN W(y, z, c) FULL   (1 column against 2)
+         R(b, a)    <- Q(a, b, c) DELTA JOIN E(a, b) FULL   (2 columns against 1)
+         Q(b, a, c) <- W(a, b, c) FULL JOIN R(a, b) DELTA   (RIGHT: 2 against 1)
+         R(b, x)    <- R(y, x) DELTA JOIN Q(y, b, c) FULL   (1 against 2, input 1 changing)
+         R(z, x)    <- R(y, x) DELTA JOIN E(y, z) FULL      (equal counts: keeps R growing)
+    seeded by R(x, y) <- E(y, x); 30 vertices, 55 edges, 3 colours."""
+    rng = np.random.default_rng(97)
+    pairs = set()
+    while len(pairs) < 55:
+        x, y = (int(v) for v in rng.integers(0, 30, 2))
+        if x != y:
+            pairs.add((x, y))
+    pairs = sorted(pairs)
+    trip = sorted({(x, y, int(rng.integers(0, 3))) for x, y in pairs[::2]} |
+                  {(int(rng.integers(0, 30)), int(rng.integers(0, 30)), int(rng.integers(0, 3))) for _ in range(25)})
+    E = b.rel_create(2, 1, 41)
+    W = b.rel_create(3, 2, 42)
+    R = b.rel_create(2, 1, 43)
+    Q = b.rel_create(3, 2, 44)
+    b.load_rows(E, np.array([[x, y, 5000 + i] for i, (x, y) in enumerate(pairs)], dtype=np.uint64), P.FULL)
+    b.load_rows(W, np.array([[x, y, c, 6000 + i] for i, (x, y, c) in enumerate(trip)], dtype=np.uint64), P.FULL)
+    fix = b.scc_create(True, 1)
+    for rel, st in ((E, False), (W, False), (R, True), (Q, True)):
+        b.scc_add_relation(fix, rel, st)
+    b.add_join(fix, Q, R, P.DELTA, W, P.FULL, [3, 1, 4])
+    b.add_join(fix, R, Q, P.DELTA, E, P.FULL, [1, 0])
+    b.add_join(fix, Q, W, P.FULL, R, P.DELTA, [1, 0, 2])
+    b.add_join(fix, R, R, P.DELTA, Q, P.FULL, [3, 1])
+    b.add_join(fix, R, R, P.DELTA, E, P.FULL, [3, 1])
+    seed = b.scc_create(False, 2)
+    b.scc_add_relation(seed, E, False)
+    b.scc_add_relation(seed, R, True)
+    b.add_copy(seed, R, E, P.FULL, [1, 0])
+    b.add_dependency(seed, fix)
+    return R
+
+
 def cases():
     """(name, builder(rec) -> None, tc edges or None, rank counts, batch sizes)"""
     out = []
@@ -277,6 +320,7 @@ def cases():
         out.append((f"random_program_s{seed}", lambda b, s=seed: P.random_program(b, s), None, [1, 3], [1, 3]))
     out.append(("id_valued", lambda b: P.id_valued_program(b), None, [1, 3], [1]))
     out.append(("threshold_split", threshold_split, None, [1], [1]))
+    out.append(("mixed_jcc", mixed_jcc, None, [1, 2, 3, 5], [1, 2]))
     return out
 
 
@@ -313,11 +357,21 @@ def write_logging_goldens():
 
 
 def main():
+    """all cases, or with --only NAME[,NAME..] those cases, updated in place"""
     if not os.path.exists(os.path.join(REF_BIN, "ref_driver")):
         sys.exit("build the reference checkers first: make -C oracle ref")
-    write_logging_goldens()
+    only = None
+    if len(sys.argv) > 2 and sys.argv[1] == "--only":
+        only = set(sys.argv[2].split(","))
     result = {}
+    if only is None:
+        write_logging_goldens()
+    else:
+        with open(OUT) as f:
+            result = json.load(f)
     for name, build, edges, ks, batches in cases():
+        if only is not None and name not in only:
+            continue
         rec = Recorder()
         build(rec)
         for k in ks:
