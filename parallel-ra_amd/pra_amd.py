@@ -24,7 +24,7 @@ EXPORTS = (
     "pra_ctx_create", "pra_ctx_destroy", "pra_last_error", "pra_ctx_set_batch_size",
     "pra_ctx_set_join_threshold", "pra_ctx_joined_tuples", "pra_ctx_iterations",
     "pra_ctx_set_profiling", "pra_ctx_kernel_stat", "pra_ctx_synchronize", "pra_ctx_device_bytes", "pra_ctx_trim",
-    "pra_ctx_set_group", "pra_ctx_group_joined",
+    "pra_ctx_set_group", "pra_ctx_group_joined", "pra_autotune_report",
     "pra_ctx_step", "pra_ctx_group_status", "pra_ctx_release_group",
     "pra_rel_create", "pra_rel_destroy", "pra_rel_load_rows", "pra_rel_load_bin", "pra_rel_count",
     "pra_rel_download", "pra_rel_write_bin", "pra_rel_checksum", "pra_rel_device_rows", "pra_scc_create", "pra_scc_destroy",
@@ -66,6 +66,21 @@ def lib():
     return _lib
 
 
+def autotune_report():
+    """the autotuner's state: [(device, knob, band, choice, [mean cost per variant...])]"""
+    L = lib()
+    n = _u64()
+    if L.pra_autotune_report(None, 0, ctypes.byref(n)) != 0:
+        raise PraError("autotune_report failed")
+    buf = ctypes.create_string_buffer(int(n.value) + 1)
+    L.pra_autotune_report(buf, len(buf), ctypes.byref(n))
+    out = []
+    for line in buf.value.decode().splitlines():
+        f = line.split()
+        out.append((int(f[0]), f[1], int(f[2]), int(f[3]), [float(x) for x in f[4:]]))
+    return out
+
+
 def load_library(path):
     """Another build of the C ABI (for example the serving loop linked with
     the CPU oracle, oracle/libpra_server_oracle.so), with the same signatures
@@ -92,6 +107,7 @@ def _bind(L):
         "pra_ctx_group_status": [_vp, ctypes.c_int, _intp, _u64p, _u64p],
         "pra_ctx_release_group": [_vp, ctypes.c_int],
     "pra_ctx_trim": [_vp],
+    "pra_autotune_report": [ctypes.c_char_p, _u64, _u64p],
         "pra_rel_create": [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                            ctypes.POINTER(_vp)],
         "pra_rel_destroy": [_vp],
