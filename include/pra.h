@@ -98,7 +98,10 @@ int pra_ctx_trim(pra_ctx* ctx);
  * candidate of capacity) and the newt scatter's block width (256 or 1,024
  * threads) -- are timed on live single-output iterations, per device and band
  * of log2(work), three times each, and the cheapest is kept for the process.
- * PRA_AUTOTUNE=0 in the environment keeps the fixed defaults.  The report has
+ * PRA_AUTOTUNE=0 in the environment keeps the fixed defaults.  With
+ * PRA_AUTOTUNE_FILE=<path>, the choices settled in earlier processes are read
+ * from that file at the first context's creation (those cells run without
+ * trials) and the file is rewritten whenever a cell settles.  The report has
  * one line per (device, knob, band): "device knob band choice mean-cost..."
  * (choice -1 while still trying; costs in ms per candidate).  *len receives
  * the report's length; up to cap bytes of it, NUL-terminated, go to buf. */

@@ -212,6 +212,41 @@ const int kTuneVariants[TUNE_KNOBS] = {3, 2};
 constexpr u64 kTuneMinCandidates = 1ULL << 16;
 pra::Tuner g_tune(3);
 
+// Tuning cache: with PRA_AUTOTUNE_FILE set, the settled choices are read from
+// that file when the first context is created (those cells run without
+// trials) and the file is rewritten whenever a cell settles.
+std::once_flag g_tune_once;
+std::mutex g_tune_file_mu;
+std::string g_tune_file;
+
+void tune_init()
+{
+    std::call_once(g_tune_once, [] {
+        const char* f = getenv("PRA_AUTOTUNE_FILE");
+        if (!f || !*f) return;
+        g_tune_file = f;
+        FILE* in = fopen(f, "r");
+        if (!in) return;
+        std::string text;
+        char buf[4096];
+        size_t got;
+        while ((got = fread(buf, 1, sizeof(buf), in)) > 0) text.append(buf, got);
+        fclose(in);
+        g_tune.load(text, kTuneNames, TUNE_KNOBS);
+    });
+}
+
+void tune_save()
+{
+    if (g_tune_file.empty()) return;
+    const std::string r = g_tune.report(kTuneNames, TUNE_KNOBS);
+    std::lock_guard<std::mutex> lk(g_tune_file_mu);
+    FILE* out = fopen(g_tune_file.c_str(), "w");
+    if (!out) return;
+    fwrite(r.data(), 1, r.size(), out);
+    fclose(out);
+}
+
 int band_of(u64 n)
 {
     int b = 0;
@@ -2540,8 +2575,9 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
     for (auto& t : c->tune_pending) {
         const u64 n = st2[t.oi * SW + 4];
         float ms = 0;
-        if (n > 0 && n <= t.cap && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
-            g_tune.record(c->device, t.knob, t.band, t.var, (double)ms / (double)n, kTuneVariants[t.knob]);
+        if (n > 0 && n <= t.cap && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess &&
+            g_tune.record(c->device, t.knob, t.band, t.var, (double)ms / (double)n, kTuneVariants[t.knob]))
+            tune_save();
         c->evpool.push_back(t.a);
         c->evpool.push_back(t.b);
     }
@@ -2906,6 +2942,7 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->dense_on = !(dense_env && dense_env[0] == '0');
         const char* tune_env = getenv("PRA_AUTOTUNE");            // "0": the fixed defaults of every knob
         c->tune_on = !(tune_env && tune_env[0] == '0');
+        tune_init();
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);

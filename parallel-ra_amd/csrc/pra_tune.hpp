@@ -38,21 +38,54 @@ class Tuner {
         return best;
     }
 
-    // one measurement of variant var: `cost` per unit of work
-    void record(int device, int knob, int band, int var, double cost, int nvar)
+    // one measurement of variant var: `cost` per unit of work; true when it
+    // settled the cell
+    bool record(int device, int knob, int band, int var, double cost, int nvar)
     {
-        if (var < 0 || var >= nvar || nvar > kMaxVariants || !(cost >= 0)) return;
+        if (var < 0 || var >= nvar || nvar > kMaxVariants || !(cost >= 0)) return false;
         std::lock_guard<std::mutex> lk(mu_);
         Cell& c = cells_[{device, knob, band}];
-        if (c.choice >= 0) return;
+        if (c.choice >= 0) return false;
         c.sum[var] += cost;
         c.n[var]++;
         for (int v = 0; v < nvar; v++)
-            if (c.n[v] < samples_) return;
+            if (c.n[v] < samples_) return false;
         int best = 0;
         for (int v = 1; v < nvar; v++)
             if (c.sum[v] / c.n[v] < c.sum[best] / c.n[best]) best = v;
         c.choice = best;
+        return true;
+    }
+
+    // a choice settled earlier (a tuning cache read back): no more trials
+    void set(int device, int knob, int band, int choice)
+    {
+        if (choice < 0 || choice >= kMaxVariants) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        cells_[{device, knob, band}].choice = choice;
+    }
+
+    // the settled choices of a report ("device knob band choice ..." lines;
+    // unknown knobs and unsettled cells are skipped); returns the count taken
+    int load(const std::string& text, const char* const* knob_names, int nknobs)
+    {
+        int taken = 0;
+        size_t at = 0;
+        while (at < text.size()) {
+            size_t end = text.find('\n', at);
+            if (end == std::string::npos) end = text.size();
+            const std::string line = text.substr(at, end - at);
+            at = end + 1;
+            int device = 0, band = 0, choice = -1;
+            char name[96] = {0};
+            if (std::sscanf(line.c_str(), "%d %95s %d %d", &device, name, &band, &choice) != 4 || choice < 0) continue;
+            for (int k = 0; k < nknobs; k++)
+                if (std::string(name) == knob_names[k]) {
+                    set(device, k, band, choice);
+                    taken++;
+                }
+        }
+        return taken;
     }
 
     // the settled variant, or -1 while the variants are still being tried

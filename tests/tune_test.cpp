@@ -57,6 +57,15 @@ int main()
     expect(rep.find("0 alpha 20 1") != std::string::npos && rep.find("0 beta 5 1") != std::string::npos,
            "report names the settled choices");
 
+    // a tuning cache: the settled choices of one tuner taken by another, which
+    // then runs them without trials; unsettled cells and unknown knobs skipped
+    pra::Tuner u(3);
+    const std::string cache = t.report(names, 2) + "0 alpha 30 -1 1.0\n0 gamma 20 1 1.0\n";
+    expect(u.load(cache, names, 2) == 2, "two settled choices read back");
+    expect(u.settled(0, 0, 20) == 1 && u.settled(0, 1, 5) == 1, "read-back choices are settled");
+    expect(u.settled(0, 0, 30) == -1, "an unsettled cell is not taken");
+    expect(u.pick(0, 0, 20, 3) == 1, "a read-back choice runs without trials");
+
     if (failures) return 1;
     std::printf("ok\n");
     return 0;
