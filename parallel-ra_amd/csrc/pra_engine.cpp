@@ -538,6 +538,9 @@ struct pra_ctx {
     int early_parity = 0;          // early appends' slot of the last iteration
     bool dense_on = true;        // dense membership for outputs that allow it
     bool tune_on = true;         // autotuned knobs (PRA_AUTOTUNE=0: the fixed defaults)
+    bool seg_sort_on = true;     // segmented sort of newts too large for the bins (PRA_SEG_SORT=0: byte passes)
+    u64 bin_cap_test = 0;        // PRA_SORT_TEST_BIN_CAP: a lower bin capacity, so tests reach the large-newt paths
+    DBuf segb;                   // segment bounds of the segmented sort, then its largest segment
     // timed launches of an iteration, resolved per candidate after its readback
     struct TuneSample {
         int knob, band, var;
@@ -576,7 +579,7 @@ struct pra_ctx {
         groups.clear();
         for (DBuf* b : {&hist, &offs, &sort_tmp, &flags_buf, &stage, &stage2, &orand, &tmp64a, &tmp64b,
                         &tmp64c, &cand, &dest, &slot_of, &keep, &ghist, &gofs, &lb_status, &lb_ctr, &ktmp,
-                        &idxa, &idxb, &posadj_dev, &rep_words, &rctl, &ttab, &kp, &part_scratch})
+                        &idxa, &idxb, &posadj_dev, &rep_words, &rctl, &ttab, &kp, &part_scratch, &segb})
             b->release();
         for (DBuf& b : jobtab) b.release();
         if (st) small_flush(st);   // the small blocks kept for this stream, freed on it
@@ -1063,7 +1066,8 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
     const u32* top_ofs = R->kgofs.as<u32>();
     u64 n = r.n;
     u64 vary = kor ^ kand;
-    if (n > 1 && maxbin <= std::min<u64>(R->bin_cap, PRA_BIN_CAP)) {
+    const u64 bin_limit = std::min<u64>(std::min<u64>(R->bin_cap, PRA_BIN_CAP), c->bin_cap_test ? c->bin_cap_test : ~0ULL);
+    if (n > 1 && maxbin <= bin_limit) {
         u32 bytes = 0;
         for (int b = 0; b < 7; b++)
             if ((vary >> (8 * b)) & (PRA_RADIX - 1)) bytes |= 1u << b;
@@ -1085,19 +1089,20 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         r.sorted = true;
         return false;
     }
-    std::vector<DigitSpec> passes;
-    for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
-        if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{0, sh, 0, 0, 0});
-    if (n > 1 && !passes.empty()) {
+    // LSD passes over the keys (and the permutation in idxa, from `permuted`
+    // on): the byte-pass sort, and the first stage of the segmented one
+    bool permuted = false;
+    auto run_passes = [&](const std::vector<DigitSpec>& ps) {
+        if (ps.empty()) return;
         prepare_lookback(c, n);
         c->ktmp.reserve(n * 8, c->st);
         c->idxa.reserve(n * 4, c->st);
         c->idxb.reserve(n * 4, c->st);
-        int np = (int)passes.size();
+        const int np = (int)ps.size();
         PassList pl;
         memset(&pl, 0, sizeof(pl));
         pl.np = np;
-        for (int i = 0; i < np; i++) pl.p[i] = passes[i];
+        for (int i = 0; i < np; i++) pl.p[i] = ps[i];
         pra_launch_fill_u64(c->st, c->ghist.as<u64>(), np * PRA_RADIX / 2, 0);
         {
             Prof p(c, K_RADIX_HIST, 8.0 * n);
@@ -1109,14 +1114,63 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         for (int i = 0; i < np; i++) {
             u32 ep = next_sort_epoch(c);
             Prof p(c, K_RADIX_SCATTER, 24.0 * n);
-            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, passes[i],
+            pra_launch_sort_scatter(c->st, keys.as<u64>(), c->ktmp.as<u64>(), n, 1, ps[i],
                                     c->gofs.as<u32>() + (size_t)i * PRA_RADIX, c->lb_status.as<u64>(),
-                                    c->lb_ctr.as<u32>() + i, ep, c->dflags(), i == 0 ? nullptr : c->idxa.as<u32>(),
+                                    c->lb_ctr.as<u32>() + i, ep, c->dflags(), permuted ? c->idxa.as<u32>() : nullptr,
                                     c->idxb.as<u32>());
             check_launch("key_sort_scatter");
             keys.swap(c->ktmp);
             c->idxa.swap(c->idxb);
+            permuted = true;
         }
+    };
+    // Segmented sort: two counting passes over the keys' top two bytes
+    // order the pairs by their top 16 bits; then the segments of equal top
+    // `sb` bits (sized to average 2,048 pairs) are each sorted in LDS by
+    // their lower varying bytes, rows gathered in the same kernel.  A segment
+    // over PRA_BIN_CAP pairs sends the sort on to the byte passes, which take
+    // the permutation as it stands.
+    const u64 seg_min = c->bin_cap_test ? 2 : (1ULL << 16);
+    std::vector<DigitSpec> top2;
+    for (int sh : {48, 56})
+        if ((vary >> sh) & (PRA_RADIX - 1)) top2.push_back(DigitSpec{0, sh, 0, 0, 0});
+    if (c->seg_sort_on && n >= seg_min && n < (1ULL << 32) && !top2.empty()) {
+        run_passes(top2);
+        // segments of about 2,048 pairs on average (the keys of one iteration
+        // are skewed: with smaller segments the largest still overflows the
+        // narrow blocks, and the wide ones then sort many tiny segments)
+        const int sb = std::min(16, std::max(9, ceil_log2(std::max<u64>(1, n / 2048))));
+        const int top = 64 - sb;
+        const u32 nseg = 1u << sb;
+        c->segb.reserve(((size_t)nseg + 2) * 8, c->st);
+        u64* seg = c->segb.as<u64>();
+        pra_launch_fill_u64(c->st, seg + nseg + 1, 1, 0);
+        pra_launch_seg_bounds(c->st, keys.as<u64>(), n, top, nseg, seg, seg + nseg + 1);
+        check_launch("seg_bounds");
+        const u64 maxseg = readback(c, seg + nseg + 1, 1)[0];
+        if (maxseg <= PRA_BIN_CAP) {
+            const u64 low = vary & ((1ULL << top) - 1);
+            u32 bytes = 0;
+            for (int b = 0; b < 8; b++)
+                if ((low >> (8 * b)) & (PRA_RADIX - 1)) bytes |= 1u << b;
+            c->sort_tmp.reserve(n * w * 8, c->st);
+            {
+                Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
+                pra_launch_seg_sort(c->st, keys.as<u64>(), c->idxa.as<u32>(), seg, top, nseg, n, bytes, r.d(), w,
+                                    c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags(),
+                                    maxseg <= PRA_SEG_NARROW_CAP);
+                check_launch("seg_sort");
+            }
+            r.buf.swap(c->sort_tmp);
+            r.sorted = true;
+            return false;
+        }
+    }
+    std::vector<DigitSpec> passes;
+    for (int sh = 0; sh < 64; sh += PRA_RADIX_BITS)
+        if ((vary >> sh) & (PRA_RADIX - 1)) passes.push_back(DigitSpec{0, sh, 0, 0, 0});
+    if (n > 1 && !passes.empty()) {
+        run_passes(passes);
         c->sort_tmp.reserve(n * w * 8, c->st);
         {
             Prof p(c, K_GATHER, 16.0 * n * w + 4.0 * n);
@@ -2943,6 +2997,10 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         const char* tune_env = getenv("PRA_AUTOTUNE");            // "0": the fixed defaults of every knob
         c->tune_on = !(tune_env && tune_env[0] == '0');
         tune_init();
+        const char* seg_env = getenv("PRA_SEG_SORT");             // "0": byte passes for newts over the bins
+        c->seg_sort_on = !(seg_env && seg_env[0] == '0');
+        const char* cap_env = getenv("PRA_SORT_TEST_BIN_CAP");
+        c->bin_cap_test = cap_env ? strtoull(cap_env, nullptr, 10) : 0;
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);

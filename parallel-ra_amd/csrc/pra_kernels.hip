@@ -507,15 +507,22 @@ __global__ void k_gather_rows(const u64* in, const u32* idx, u64 n, int w, u64* 
 // and is left unsorted).  Bucket s starts at the first key >= (s * bmul) <<
 // ls (the key is bucket * bmul + column ranks, shifted left by ls); the block
 // whose bin holds that value counts its keys below it.
-__device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n,
-                                                   const u32* bofs, u32 bytes, const u64* rows, int w, u64* out,
-                                                   u64 bmul, int ls, u64 nb, u64* bnd, u32* flags, const u32 b)
+// The sort of one bin (or segment) b by a block of BLOCK threads (at most
+// BLOCK * PRA_BIN_ITEMS pairs): its pairs are keys[0..end-start) /
+// idx[0..end-start), all with the same key bits from bit `top` up (the bin's
+// value b), and its rows go to [start, end) of the sorted order.
+template <int BLOCK>
+__device__ __forceinline__ void pra_bin_sort_range(const u64* keys, const u32* idx, u64 start, u64 end, u32 cap, u64 n,
+                                                   u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls,
+                                                   u64 nb, u64* bnd, u32* flags, const u32 b, const int top)
 {
     constexpr int IT = PRA_BIN_ITEMS;
-    constexpr int WAVES = PRA_BIN_BLOCK / 64;
-    constexpr u32 PER_WAVE = PRA_BIN_CAP / WAVES;
-    __shared__ u64 skey[PRA_BIN_CAP];
-    __shared__ u32 sidx[PRA_BIN_CAP];
+    constexpr u32 CAP = BLOCK * PRA_BIN_ITEMS;
+    constexpr int WAVES = BLOCK / 64;
+    constexpr u32 PER_WAVE = CAP / WAVES;
+    static_assert(BLOCK >= PRA_RADIX, "the digit scan takes a thread per digit");
+    __shared__ u64 skey[CAP];
+    __shared__ u32 sidx[CAP];
     __shared__ u32 wh[WAVES][PRA_RADIX];
     __shared__ u32 wsum[PRA_RADIX / 64];
     __shared__ u32 below;
@@ -523,19 +530,16 @@ __device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u3
         bnd[0] = 0;
         bnd[nb] = n;
     }
-    const u64 start = bofs[b], end = b + 1 < PRA_RADIX ? bofs[b + 1] : n;
     if (end <= start) {
-        for (u64 s = 1 + threadIdx.x; s < nb; s += PRA_BIN_BLOCK)
-            if ((((s * bmul) << ls) >> 56) == b) bnd[s] = start;
+        for (u64 s = 1 + threadIdx.x; s < nb; s += BLOCK)
+            if ((((s * bmul) << ls) >> top) == b) bnd[s] = start;
         return;
     }
-    if (end - start > PRA_BIN_CAP || end - start > bin_cap) {
+    if (end - start > CAP || end - start > cap) {
         if (threadIdx.x == 0) atomicOr(flags, PRA_FLAG_BIN_OVERFLOW);
         return;
     }
     const u32 m = (u32)(end - start);
-    const u64* keys = bin_keys + (u64)b * bin_cap;
-    const u32* idx = bin_idx + (u64)b * bin_cap;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const u64 lt = pra_lanemask_lt();
     u64 key[IT];
@@ -551,9 +555,9 @@ __device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u3
         }
     }
 #pragma unroll 1
-    for (int p = 0; p < 7; p++) {
+    for (int p = 0; p < 8; p++) {
         if (!((bytes >> p) & 1u)) continue;
-        for (int i = threadIdx.x; i < WAVES * PRA_RADIX; i += PRA_BIN_BLOCK) (&wh[0][0])[i] = 0;
+        for (int i = threadIdx.x; i < WAVES * PRA_RADIX; i += BLOCK) (&wh[0][0])[i] = 0;
         __syncthreads();
         // stable rank of every pair among the pairs of its digit: wave-local
         // counts in element order, then a scan over waves and digits
@@ -620,7 +624,7 @@ __device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u3
     }
     for (u64 s = 1; s < nb; s++) {
         const u64 T = (s * bmul) << ls;
-        if ((T >> 56) != b) continue;
+        if ((T >> top) != b) continue;
         if (threadIdx.x == 0) below = 0;
         __syncthreads();
         u32 cnt = 0;
@@ -645,6 +649,17 @@ __device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u3
     }
 }
 
+// bin b of the newt scatter's top-byte bins (bofs = the bins' first sorted positions)
+__device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n,
+                                                   const u32* bofs, u32 bytes, const u64* rows, int w, u64* out,
+                                                   u64 bmul, int ls, u64 nb, u64* bnd, u32* flags, const u32 b)
+{
+    const u64 start = bofs[b], end = b + 1 < PRA_RADIX ? bofs[b + 1] : n;
+    pra_bin_sort_range<PRA_BIN_BLOCK>(bin_keys + (u64)b * bin_cap, bin_idx + (u64)b * bin_cap, start, end, bin_cap, n,
+                                      bytes & 0x7Fu,
+                       rows, w, out, bmul, ls, nb, bnd, flags, b, 56);
+}
+
 __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys, const u32* bin_idx, u32 bin_cap,
                                                             u64 n, const u32* bofs, u32 bytes, const u64* rows, int w,
                                                             u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
@@ -660,6 +675,46 @@ __global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort_many(const BinSortJo
     const BinSortJob& j = jobs[blockIdx.x / PRA_RADIX];
     pra_bin_sort_block(j.bin_keys, j.bin_idx, j.bin_cap, j.n, j.bofs, j.bytes, j.rows, j.w, j.out, j.bmul, j.ls, j.nb,
                        j.bnd, flags, blockIdx.x % PRA_RADIX);
+}
+
+// Segmented form for newts too large for the top-byte bins: the pairs are
+// already ordered by the keys' top 16 bits (two counting passes), segment b
+// is the pairs whose key bits from `top` up equal b ([seg[b], seg[b + 1])),
+// and block b sorts it by the varying bytes below (bit p of `bytes`).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_sort(const u64* keys, const u32* idx, const u64* seg, int top, u64 n,
+                                                    u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls,
+                                                    u64 nb, u64* bnd, u32* flags)
+{
+    const u32 b = blockIdx.x;
+    const u64 start = seg[b], end = seg[b + 1];
+    pra_bin_sort_range<BLOCK>(keys + start, idx + start, start, end, BLOCK * PRA_BIN_ITEMS, n, bytes, rows, w, out,
+                              bmul, ls, nb, bnd, flags, b, top);
+}
+
+// seg[v] = first position of a sorted key array whose bits from `top` up are
+// >= v, for v in [0, nseg] (seg[nseg] = n); *maxseg = the largest segment
+__global__ void k_seg_bounds(const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg)
+{
+    const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nseg) return;
+    u64 b2[2];
+    for (int k = 0; k < 2; k++) {
+        const u64 t = v + (u64)k;
+        if (t >= nseg) {
+            b2[k] = n;
+            continue;
+        }
+        u64 lo = 0, hi = n;
+        while (lo < hi) {
+            const u64 mid = (lo + hi) >> 1;
+            if ((keys[mid] >> top) < t) lo = mid + 1; else hi = mid;
+        }
+        b2[k] = lo;
+    }
+    seg[v] = b2[0];
+    if (v + 1 == nseg) seg[nseg] = n;
+    atomicMax((unsigned long long*)maxseg, (unsigned long long)(b2[1] - b2[0]));
 }
 
 // bnd[s] = first position whose key is >= (s * bmul) << ls (the key is
@@ -2516,6 +2571,23 @@ void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int
         hipLaunchKernelGGL((k_sort_scatter<false, PRA_ITEMS>), dim3(nt ? nt : 1), dim3(PRA_BLOCK), 0, st, in, out, n, w, ds,
                            gofs, status, tile_ctr, epoch, flags, iin, iout);
     }
+}
+
+void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg)
+{
+    hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(nseg)), dim3(PRA_BLOCK), 0, st, keys, n, top, nseg, seg, maxseg);
+}
+
+void pra_launch_seg_sort(hipStream_t st, const u64* keys, const u32* idx, const u64* seg, int top, u32 nseg, u64 n,
+                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
+                         bool narrow)
+{
+    if (narrow)
+        hipLaunchKernelGGL(k_seg_sort<PRA_SEG_NARROW_BLOCK>, dim3(nseg), dim3(PRA_SEG_NARROW_BLOCK), 0, st, keys, idx,
+                           seg, top, n, bytes, rows, w, out, bmul, ls, nb, bnd, flags);
+    else
+        hipLaunchKernelGGL(k_seg_sort<PRA_BIN_BLOCK>, dim3(nseg), dim3(PRA_BIN_BLOCK), 0, st, keys, idx, seg, top, n,
+                           bytes, rows, w, out, bmul, ls, nb, bnd, flags);
 }
 
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,

@@ -344,6 +344,22 @@ void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jc
 void pra_launch_small_sort_many(hipStream_t st, const SmallSortBatch& b, int njobs);
 void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const PassList& pl, u64* out, u64* bnd,
                            int jcc, u64 nb);
+// Segmented packed-key sort of a newt too large for the top-byte bins: keys
+// (and their newt positions idx) already ordered by their top 16 bits;
+// segment v = the keys whose bits from `top` up equal v (nseg segments).
+// pra_launch_seg_bounds writes seg[0..nseg] and the largest segment into
+// *maxseg (zeroed by the caller); pra_launch_seg_sort sorts every segment in
+// LDS by its varying bytes (bit p of `bytes`), gathers the rows into out and
+// writes the bucket bounds as pra_launch_bin_sort does.  narrow: blocks of
+// PRA_SEG_NARROW_BLOCK threads, segments of at most PRA_SEG_NARROW_CAP pairs
+// (more blocks resident per compute unit); else PRA_BIN_BLOCK threads and
+// PRA_BIN_CAP pairs.  A larger segment raises PRA_FLAG_BIN_OVERFLOW.
+#define PRA_SEG_NARROW_BLOCK 256
+#define PRA_SEG_NARROW_CAP (PRA_SEG_NARROW_BLOCK * PRA_BIN_ITEMS)
+void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg);
+void pra_launch_seg_sort(hipStream_t st, const u64* keys, const u32* idx, const u64* seg, int top, u32 nseg, u64 n,
+                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
+                         bool narrow);
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
                          u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
