@@ -122,8 +122,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["tc", "sg", "batch", "serve"], default="tc",
-                    help="tc: config 2 grid transitive closure (default); sg: config 3 same generation; "
-                         "batch: config 4 batched vs plain; serve: config 5 request stream")
+                    help="tc: transitive closure of the 150x150 grid (default); sg: same generation on a "
+                         "12-layer 10K-vertex DAG; batch: 256 random graphs, batched vs plain; serve: a Poisson "
+                         "request stream of random-graph reachability solves")
     ap.add_argument("--count", type=int, default=256, help="batch/serve: number of instances / requests")
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=50000)
@@ -290,8 +291,8 @@ def bench_batch(args):
     of its rows in order, count, join pairs) and its |TC| against the
     SCC-condensation count computed on the CPU."""
     import pra_amd
-    from programs import config4_instances, random_graph, tc_program, tc_size_condensation
-    inst = config4_instances(count=args.count, nmin=args.nmin, nmax=args.nmax)
+    from programs import log_uniform_instances, random_graph, tc_program, tc_size_condensation
+    inst = log_uniform_instances(count=args.count, nmin=args.nmin, nmax=args.nmax)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     nb = args.buckets
     log(f"batch: {len(graphs)} instances, n in [{min(n for n, _, _ in inst)}, {max(n for n, _, _ in inst)}], "
@@ -414,9 +415,9 @@ def bench_serve(args):
     |TC| computed on the CPU."""
     import numpy as np
     import pra_amd
-    from programs import config4_instances, random_graph, tc_size_condensation
+    from programs import log_uniform_instances, random_graph, tc_size_condensation
     import make_serve_sizes as S
-    inst = config4_instances(count=args.count, seed=S.SEED, nmin=args.nmin, nmax=args.nmax)
+    inst = log_uniform_instances(count=args.count, seed=S.SEED, nmin=args.nmin, nmax=args.nmax)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     want = None
     if not args.no_closed_form:

@@ -307,14 +307,14 @@ def cases():
     tc("tc_g13236", P.golden("g13236_edge_2_1_2"), [1, 2, 3, 5])
     tc("tc_tiny8", tiny8_edges(), [1, 2, 3])
     for seed in range(3):
-        tc(f"tc_config1_s{seed}", P.config1_graph(seed), [2, 3, 5])
+        tc(f"tc_small_s{seed}", P.small_random_graph(seed), [2, 3, 5])
     tc("tc_grid30_u64", P.grid_graph(30, seed=30), [8])
     out.append(("two_rule", lambda b: P.two_rule_program(b), None, [1, 2, 3], [1]))
     out.append(("two_rule_wide", two_rule_wide, None, [1, 2, 3, 5], [1]))
     out.append(("sg_dag8", lambda b: P.sg_program(b, small_dag()), None, [1, 3], [1, 2]))
     out.append(("sg_dag600", lambda b: P.sg_program(b, P.layered_dag(11, nvert=600, nlayers=6, nedges=840)), None,
                 [1, 3], [1, 3]))
-    out.append(("nonlinear_tc_config1_s0", lambda b: P.nonlinear_tc_program(b, P.config1_graph(0)), None, [1, 3],
+    out.append(("nonlinear_tc_small_s0", lambda b: P.nonlinear_tc_program(b, P.small_random_graph(0)), None, [1, 3],
                 [1]))
     for seed in range(6):
         out.append((f"random_program_s{seed}", lambda b, s=seed: P.random_program(b, s), None, [1, 3], [1, 3]))

@@ -1,5 +1,5 @@
 """Writes serve_tc_sizes_<count>.json: |TC| of every graph in the served
-request stream of random graphs (programs.config4_instances with the serving
+request stream of random graphs (programs.log_uniform_instances with the serving
 seed 5, vertex counts log-uniform in [nmin, nmax], 1.3 edges per vertex),
 computed on the CPU by SCC condensation (programs.tc_size_condensation).
 
@@ -19,7 +19,7 @@ from multiprocessing import Pool
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
-from programs import config4_instances, random_graph, tc_size_condensation  # noqa: E402
+from programs import log_uniform_instances, random_graph, tc_size_condensation  # noqa: E402
 
 SEED = 5
 
@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--nmax", type=int, default=50000)
     ap.add_argument("--procs", type=int, default=8)
     a = ap.parse_args()
-    inst = config4_instances(count=a.count, seed=SEED, nmin=a.nmin, nmax=a.nmax)
+    inst = log_uniform_instances(count=a.count, seed=SEED, nmin=a.nmin, nmax=a.nmax)
     with Pool(a.procs) as pool:
         sizes = pool.map(tc_size, inst, chunksize=8)
     out = {"count": a.count, "seed": SEED, "nmin": a.nmin, "nmax": a.nmax,

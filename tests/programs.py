@@ -104,7 +104,7 @@ def sg_program(b, edges):
 
 # ------------------------------------------------------------ generators ----
 
-def config1_graph(seed, n=40, m=60):
+def small_random_graph(seed, n=40, m=60):
     """40 vertices, 60 distinct edges uniform without self-loops, random.Random(seed)."""
     rng = random.Random(seed)
     edges = []
@@ -186,7 +186,7 @@ def random_graph(n, m, seed):
     return with_rowid(ids[np.stack([u[sel], v[sel]], axis=1)])
 
 
-def config4_instances(count=256, seed=4, nmin=1000, nmax=50000, density=1.3):
+def log_uniform_instances(count=256, seed=4, nmin=1000, nmax=50000, density=1.3):
     """Batched TC instances: vertex count log-uniform in [nmin, nmax], edge
     count density x vertex count, one seed per instance."""
     rng = np.random.default_rng(seed)

@@ -1,4 +1,4 @@
-"""Batched execution on the config-4 generator (uniform random graphs, vertex
+"""Batched execution on random graphs (uniform random edges, vertex
 counts log-uniform, 1.3 edges per vertex, uint64 ids): all instances as
 program instances of one context run in lock step, every phase one batched
 launch over all of them.  Per instance the rows (intern ids included), the
@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from programs import config4_instances, random_graph, tc_program, tc_size_condensation
+from programs import log_uniform_instances, random_graph, tc_program, tc_size_condensation
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,7 @@ def plain_runs(graphs, nb):
 @pytest.mark.parametrize("nb", [1, 8])
 def test_lockstep_batch_equals_plain_and_closed_form(nb):
     import pra_amd
-    inst = config4_instances(count=24, seed=41, nmin=1000, nmax=8000)
+    inst = log_uniform_instances(count=24, seed=41, nmin=1000, nmax=8000)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     print(f"[batch] nb={nb} {len(graphs)} instances, sizes {[n for n, _, _ in inst]}", flush=True)
     plain = plain_runs(graphs, nb)
@@ -57,7 +57,7 @@ def test_lockstep_batch_equals_plain_and_closed_form(nb):
 
 def test_host_threads_equal_plain():
     import pra_amd
-    inst = config4_instances(count=12, seed=42, nmin=200, nmax=3000)
+    inst = log_uniform_instances(count=12, seed=42, nmin=200, nmax=3000)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     plain = plain_runs(graphs, 8)
     ctxs, hs = [], []

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from programs import (config1_graph, random_graph, random_program, sg_program, layered_dag, tc_program,
+from programs import (small_random_graph, random_graph, random_program, sg_program, layered_dag, tc_program,
                       two_rule_program)
 
 pytestmark = pytest.mark.gpu
@@ -23,7 +23,7 @@ def oracle_run(k, build, bs=1):
 def test_lockstep_tc_instances_equal_oracle(k):
     import pra_amd
     graphs = [random_graph(n, int(1.3 * n), 900 + n) for n in (50, 300, 120, 800, 60, 2000, 10)]
-    graphs.append(config1_graph(4))
+    graphs.append(small_random_graph(4))
     g = pra_amd.Context(nbuckets=k)
     try:
         handles = []
@@ -48,7 +48,7 @@ def test_lockstep_mixed_programs_equal_oracle(k):
     two-rule program and random multi-SCC programs"""
     import pra_amd
     builders = [
-        lambda b: tc_program(b, config1_graph(7)),
+        lambda b: tc_program(b, small_random_graph(7)),
         lambda b: sg_program(b, layered_dag(5, nvert=300, nlayers=5, nedges=420)),
         lambda b: two_rule_program(b),
         lambda b: random_program(b, 11),

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
+from programs import (DELTA, FULL, bfs_distance_histogram, small_random_graph, golden, grid_deltas, grid_graph,
                       grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_program,
                       two_rule_expected, two_rule_program, TWO_RULE_ORDERS, with_rowid)
 
@@ -59,8 +59,8 @@ def test_tc_golden_graphs(nb):
 
 
 @pytest.mark.parametrize("seed", range(6))
-def test_tc_config1(seed):
-    edges = config1_graph(seed)
+def test_tc_small_random(seed):
+    edges = small_random_graph(seed)
     pairs = [tuple(r) for r in edges[:, :2].tolist()]
     total, deltas = bfs_distance_histogram(pairs)
     for nb in [1, 3, 5]:
@@ -70,7 +70,7 @@ def test_tc_config1(seed):
 
 
 def test_tc_batch_size():
-    check_tc(3, config1_graph(11), batch_size=3)
+    check_tc(3, small_random_graph(11), batch_size=3)
 
 
 def test_tc_empty_and_tiny():
@@ -127,10 +127,10 @@ def test_acopy_and_filter_parity():
             assert_same_rel(o, ho[i], g, hg[i], f"rel{i}")
 
 
-def test_config2_grid150_properties():
+def test_grid150_tc_properties():
     # the full-size config: 150x150 grid, uint64 ids, 8 virtual buckets
     n, nb = 150, 8
-    print("[config2] grid150 nb=8", flush=True)
+    print("[grid150] grid150 nb=8", flush=True)
     g = gpu_ctx(nb)
     T, G, sfix, scopy = tc_program(g, grid_graph(n, seed=150))
     g.execute()
@@ -150,11 +150,11 @@ def test_config2_grid150_properties():
         assert np.array_equal(s, np.arange(s.size, dtype=np.uint64))
 
 
-def test_config3_same_generation_full_size():
+def test_same_generation_dag10k_full_size():
     # 10,000-vertex 12-layer DAG, 14,000 uint64 edges, 8 virtual buckets:
     # 1,392,703 sg tuples; every relation's rows (ids included) and deltas vs the oracle
     edges = layered_dag(3)
-    print("[config3] same generation nb=8", flush=True)
+    print("[sg_dag10k] same generation nb=8", flush=True)
     o, ho, g, hg = both(8, lambda b: sg_program(b, edges))
     for k in ["E", "I", "SG", "A", "B"]:
         assert_same_rel(o, ho[k], g, hg[k], k)

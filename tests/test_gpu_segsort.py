@@ -3,14 +3,14 @@ segmented sort (two counting passes over the keys' top bytes, then one LDS
 sort per segment) and the byte-pass sort it falls back to.  A lowered bin
 capacity (PRA_SORT_TEST_BIN_CAP) sends small programs down both paths, which
 must equal the oracle, ids and per-batch deltas included; at full size the
-largest config-4 instance gives the same rows down both paths."""
+largest instance of the 256-graph batch (49,691 vertices) gives the same rows down both paths."""
 import os
 
 import numpy as np
 import pytest
 
 import oracle as O
-from programs import (config4_instances, layered_dag, random_graph, random_program, sg_program, tc_program,
+from programs import (log_uniform_instances, layered_dag, random_graph, random_program, sg_program, tc_program,
                       tc_size_condensation)
 
 pytestmark = pytest.mark.gpu
@@ -80,9 +80,9 @@ def test_large_newt_paths_equal_oracle(name, build, k, path):
         g.close()
 
 
-def test_largest_config4_instance_both_paths():
+def test_largest_batch_instance_both_paths():
     import time
-    n, m, sd = max(config4_instances(count=256), key=lambda i: i[0])
+    n, m, sd = max(log_uniform_instances(count=256), key=lambda i: i[0])
     e = random_graph(n, m, sd)
     out = {}
     for path in ("segmented", "byte_passes"):

@@ -9,14 +9,14 @@ import numpy as np
 import pytest
 
 import oracle as O
-from programs import config4_instances, random_graph, tc_program, tc_size_condensation
+from programs import log_uniform_instances, random_graph, tc_program, tc_size_condensation
 
 pytestmark = pytest.mark.gpu
 
 
 def test_served_results_alone_and_in_mix():
     import pra_amd
-    inst = config4_instances(count=16, seed=51, nmin=150, nmax=2500)
+    inst = log_uniform_instances(count=16, seed=51, nmin=150, nmax=2500)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     print(f"[serve] {len(graphs)} requests", flush=True)
     alone = pra_amd.Server(nbuckets=4, slots=1, keep_rows=True)
@@ -49,11 +49,11 @@ def test_served_results_alone_and_in_mix():
 
 def test_poisson_stream_continuous_batching():
     """a seeded Poisson stream of 300 requests at 1,000 req/s (sizes from the
-    config-4 distribution, reduced range) into 16 slots under an HBM budget:
+    log-uniform size distribution, reduced range) into 16 slots under an HBM budget:
     every result equals its SCC-condensation |TC| and its alone run's tuple
     count and join pairs; requests joined while others were in flight"""
     import pra_amd
-    inst = config4_instances(count=300, seed=55, nmin=200, nmax=4000)
+    inst = log_uniform_instances(count=300, seed=55, nmin=200, nmax=4000)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     want = [tc_size_condensation(e[:, :2])[0] for e in graphs]
     rng = np.random.default_rng(1000)

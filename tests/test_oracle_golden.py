@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from programs import (DELTA, FULL, bfs_distance_histogram, config1_graph, golden, grid_deltas, grid_graph,
+from programs import (DELTA, FULL, bfs_distance_histogram, small_random_graph, golden, grid_deltas, grid_graph,
                       grid_tc_size, layered_dag, nonlinear_tc_program, random_graph, sg_bruteforce, sg_program, tc_join_pairs, tc_pairs_bruteforce,
                       tc_program, two_rule_expected, two_rule_program, TWO_RULE_ORDERS, with_rowid, random_program)
 
@@ -171,8 +171,8 @@ def test_intern_seq_follows_iteration_order():
 
 
 @pytest.mark.parametrize("seed", range(8))
-def test_config1_bfs_histogram(seed):
-    edges = config1_graph(seed)
+def test_small_random_bfs_histogram(seed):
+    edges = small_random_graph(seed)
     pairs = [tuple(r) for r in edges[:, :2].tolist()]
     total, deltas = bfs_distance_histogram(pairs)
     for nb in [1, 3, 5]:
@@ -198,7 +198,7 @@ def test_grid30_reference_prefix():
 
 
 def test_batch_size_and_threshold_do_not_change_sets():
-    edges = config1_graph(3)
+    edges = small_random_graph(3)
     o1, T1, *_ = run_tc(3, edges, batch_size=1)
     o2, T2, *_ = run_tc(3, edges, batch_size=4)
     o3, T3, *_ = run_tc(3, edges, batch_size=1, threshold=5)
@@ -315,10 +315,10 @@ def test_random_programs_bucket_and_batch_invariance():
 
 
 def test_tc_size_condensation_matches_bruteforce():
-    """the SCC-condensation |TC| used for config-4-sized checks equals brute
+    """the SCC-condensation |TC| used for checks of graphs of up to 50 K vertices equals brute
     force on small graphs with cycles, self-loops and isolated parts"""
     from programs import tc_size_condensation
-    graphs = [config1_graph(s)[:, :2] for s in range(4)]
+    graphs = [small_random_graph(s)[:, :2] for s in range(4)]
     graphs += [random_graph(n, int(1.3 * n), 31 + n)[:, :2] for n in (50, 400, 1500)]
     graphs.append(np.array([[1, 1], [1, 2], [2, 3], [3, 2], [7, 8]], dtype=np.uint64))
     for p in graphs:

@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from programs import config4_instances, random_graph, tc_size_condensation
+from programs import log_uniform_instances, random_graph, tc_size_condensation
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SERVER_LIB = os.path.join(ROOT, "oracle", "libpra_server_oracle.so")
@@ -20,7 +20,7 @@ PRA_E_NOMEM = -8
 
 @pytest.fixture(scope="module")
 def stream():
-    inst = config4_instances(count=40, seed=77, nmin=30, nmax=400)
+    inst = log_uniform_instances(count=40, seed=77, nmin=30, nmax=400)
     graphs = [random_graph(n, m, sd) for n, m, sd in inst]
     return inst, graphs
 
