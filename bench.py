@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--slots", type=int, default=None,
                     help="batch/serve: instances / requests in flight at most (default 256 / 64)")
     ap.add_argument("--budget-gb", type=float, default=160.0, help="batch/serve: HBM admission budget")
+    ap.add_argument("--alone-n", type=int, default=10000,
+                    help="batch: instances of at least this many vertices run one at a time (0: no such rule)")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--grid", type=int, default=150)
     ap.add_argument("--buckets", type=int, default=8)
@@ -287,7 +289,12 @@ def bench_batch(args):
     them; instances are queued largest first and admitted between iterations
     while their estimated bytes fit the HBM budget, up to --slots at once, and
     a finished instance is read and released at once so that the next one
-    takes its place).  Every instance is compared bit for bit (device checksum
+    takes its place).  An instance of --alone-n vertices or more fills the
+    device by itself: two such run together take longer than one after the
+    other (their membership bitmaps, 12-300 MB each, evict each other from
+    the 256 MB last-level cache), so they have a queue of their own and run
+    one at a time, while the smaller instances ride along in the same
+    lock-step launches.  Every instance is compared bit for bit (device checksum
     of its rows in order, count, join pairs) and its |TC| against the
     SCC-condensation count computed on the CPU."""
     import pra_amd
@@ -326,7 +333,12 @@ def bench_batch(args):
     from collections import deque
     budget = int(args.budget_gb * (1 << 30))
     slots = args.slots or 256
-    queue = deque(sorted(range(len(graphs)), key=lambda i: -inst[i][0]))
+    order = sorted(range(len(graphs)), key=lambda i: -inst[i][0])
+    def large(i):
+        return args.alone_n > 0 and inst[i][0] >= args.alone_n
+    big_queue = deque(i for i in order if large(i))
+    queue = deque(i for i in order if not large(i))
+    big_in_flight = 0
     batched = [None] * len(graphs)
     c = pra_amd.Context(nbuckets=nb, batch_size=1)
     free = list(range(slots - 1, -1, -1))
@@ -337,18 +349,25 @@ def bench_batch(args):
     finished = []                      # (instance id, graph index, handles) left for after the run
     t0 = time.perf_counter()
     def admissible():
-        return queue and free and (not flight or held + est_bytes(inst[queue[0]][0]) <= budget)
+        """the queue the next admission comes from, or None"""
+        if not free:
+            return None
+        for q in (big_queue, queue):
+            if q and (q is queue or not big_in_flight) and (not flight or held + est_bytes(inst[q[0]][0]) <= budget):
+                return q
+        return None
 
-    while queue or flight:
+    while queue or big_queue or flight:
         if admissible():
             c.synchronize()
             a0 = time.perf_counter()
-            while admissible():
-                i = queue.popleft()
+            while (q := admissible()) is not None:
+                i = q.popleft()
                 g = free.pop()
                 c.set_group(g)
                 flight[g] = (i, tc_program(c, graphs[i]))
                 held += est_bytes(inst[i][0])
+                big_in_flight += large(i)
             c.synchronize()
             t_admit += time.perf_counter() - a0
         top = max(top, len(flight))
@@ -357,7 +376,8 @@ def bench_batch(args):
         for g in list(flight):
             if c.group_done(g):
                 i, hs = flight.pop(g)
-                if not queue:
+                big_in_flight -= large(i)
+                if not queue and not big_queue:
                     # nothing waits for its memory: checked and released after the run
                     finished.append((g, i, hs))
                     continue
@@ -395,7 +415,8 @@ def bench_batch(args):
         "warmup": 0, "ms_per_step": 1e3 * t_batch, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"tc_batch{len(graphs)}_n{args.nmin}-{args.nmax}", "buckets": nb,
-                   "budget_gb": args.budget_gb, "slots": slots, "lockstep_iterations": steps,
+                   "budget_gb": args.budget_gb, "slots": slots, "alone_n": args.alone_n,
+                   "lockstep_iterations": steps,
                    "max_in_flight": top, "tuples": sum(t for t, _, _ in plain),
                    "joined_tuples": joined},
         "plain_value": joined / t_plain, "plain_s": t_plain, "batched_s": t_batch,

@@ -233,7 +233,12 @@ int pra_execute_many(pra_ctx** ctxs, int n, int concurrency, int* rc);
  * its relations hold), the new request's estimate and the largest context
  * scratch seen after a step fit into `hbm_budget` (0 = no budget; an idle
  * server always admits).  est_bytes 0 = the server's
- * own estimate (random graphs with 1.3 edges per vertex).  Under memory
+ * own estimate (random graphs with 1.3 edges per vertex).  Large requests
+ * (estimate at least that of a 10,000-vertex graph; PRA_SERVE_LARGE_BYTES in
+ * the environment at creation sets the bound, 0 turns the rule off) are in
+ * flight one at a time, since each fills the device alone: one that waits
+ * only for the large request in flight lets the requests behind it pass, one
+ * that waits for memory holds the queue.  Under memory
  * pressure requests are restarted rather than failed: when the bytes held
  * pass the budget after an iteration, the context's scratch is given back
  * (pra_ctx_trim), then the most recently admitted requests in flight are

@@ -22,7 +22,13 @@
 // the queue (released bytes count at once); an iteration that
 // runs the device out of memory with several requests in flight requeues all
 // of them on a fresh context and halves the number in flight, which then
-// grows back by one per completed request.  The program solved is the reference's
+// grows back by one per completed request.  A large request (estimate at or
+// above `large`, by default that of a 10,000-vertex graph) fills the device
+// by itself: two of them in flight together finish later than one after the
+// other, their random-access membership bitmaps evicting each other from the
+// last-level cache.  So one large request is in flight at a time; a large
+// request that waits only for that one lets the requests behind it pass, and
+// one that waits for memory holds the queue as any other.  The program solved is the reference's
 // tests/transitive_closure.cpp:56-77 (copy seed, then the T(DELTA) x G(FULL)
 // join to the fixed point).
 #include "../../include/pra.h"
@@ -30,6 +36,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -74,6 +81,7 @@ uint64_t default_estimate(uint64_t m)
 
 struct pra_server {
     int device = 0, nb = 1, slots = 1, keep_rows = 0;
+    uint64_t large = 0;   // estimates from here up run one at a time (0: no such rule)
     int limit = 1;   // requests in flight at most for now: slots, halved after running out of memory
     uint64_t budget = 0, next_id = 1, admissions = 0;
     uint64_t scratch_hw = 0;   // largest context scratch seen after a step (bytes beyond the relations)
@@ -111,6 +119,15 @@ struct pra_server {
         for (int g = 0; g < slots; g++)
             if (slot[(size_t)g].r) held += held_by(g);
         if (live > held) scratch_hw = std::max(scratch_hw, live - held);
+    }
+
+    bool is_large(const Request* r) const { return large && r->est >= large; }
+
+    bool large_in_flight() const
+    {
+        for (const Slot& s : slot)
+            if (s.r && is_large(s.r)) return true;
+        return false;
     }
 
     // may the head of the queue join the requests in flight and the `pending`
@@ -266,11 +283,20 @@ struct pra_server {
                 if (stopping && queue.empty() && !in_flight()) break;
                 int free_slots = std::min(slots, limit) - in_flight();
                 uint64_t pending = 0;
-                while (ctx && free_slots > 0 && !queue.empty() && fits(queue.front(), admit.empty(), pending)) {
-                    pending += queue.front()->est;
-                    admit.push_back(queue.front());
-                    queue.pop_front();
+                bool large_busy = large_in_flight();
+                for (auto it = queue.begin(); ctx && free_slots > 0 && it != queue.end();) {
+                    Request* r = *it;
+                    const bool lg = is_large(r);
+                    if (lg && large_busy) {   // waits for the large request in flight
+                        ++it;
+                        continue;
+                    }
+                    if (!fits(r, admit.empty(), pending)) break;
+                    pending += r->est;
+                    admit.push_back(r);
+                    it = queue.erase(it);
                     free_slots--;
+                    large_busy = large_busy || lg;
                 }
             }
             if (!ctx) {   // no context could be created: fail what is queued
@@ -335,6 +361,8 @@ int pra_server_create(int device, int nbuckets, int slots, uint64_t hbm_budget, 
     s->limit = slots;
     s->budget = hbm_budget;
     s->keep_rows = keep_rows;
+    s->large = default_estimate((uint64_t)(1.3 * 10000));
+    if (const char* v = std::getenv("PRA_SERVE_LARGE_BYTES")) s->large = std::strtoull(v, nullptr, 10);
     s->loop = std::thread([s] { s->run(); });
     *out = s;
     return PRA_OK;

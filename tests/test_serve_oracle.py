@@ -147,3 +147,27 @@ def test_admission_counts_requests_admitted_together(library, stream, alone):
     check_equal(res, rows, alone)
     assert sum(r["restarts"] for r in res) == 0
     assert max_in_flight(res) == 3
+
+
+def test_large_requests_run_one_at_a_time(library, stream, alone):
+    """requests whose estimate reaches PRA_SERVE_LARGE_BYTES (here: the
+    server's own estimate for 200 vertices) are in flight one at a time;
+    smaller ones arriving after a waiting large request are admitted before
+    it, and every result equals the request served alone"""
+    inst, graphs = stream
+    # the server's own estimate from the edge count (1.3 edges per vertex)
+    est = [int(0.18 * (len(e) / 1.3) ** 2 * 90) + (64 << 20) for e in graphs]
+    bound = int(0.18 * 200 ** 2 * 90) + (64 << 20)
+    large = [x >= bound for x in est]
+    assert 2 <= sum(large) < len(graphs) - 2
+    os.environ["PRA_SERVE_LARGE_BYTES"] = str(bound)
+    try:
+        res, rows = serve(library, graphs, slots=8)
+    finally:
+        os.environ.pop("PRA_SERVE_LARGE_BYTES", None)
+    check_equal(res, rows, alone)
+    spans = sorted((r["t_start"], r["t_done"]) for r, lg in zip(res, large) if lg)
+    assert all(spans[i][1] <= spans[i + 1][0] for i in range(len(spans) - 1))
+    overtaken = [(i, j) for i in range(len(graphs)) for j in range(i + 1, len(graphs))
+                 if large[i] and not large[j] and res[j]["t_start"] < res[i]["t_start"]]
+    assert overtaken
