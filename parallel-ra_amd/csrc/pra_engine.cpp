@@ -2242,6 +2242,14 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                 a.keys = R->nkeys.as<u64>();
                 a.rank_radix = G.dict_n;
                 a.key_lshift = pk_ls[oi];
+                if (dn) {
+                    // the dense decision leaves each kept candidate's rank
+                    // index in its slot_of word
+                    u64 scale = 1;
+                    for (int i = 0; i < cw; i++) scale *= G.dict_n;
+                    a.xk = slot0;
+                    a.xk_scale = scale;
+                }
             }
             a.fold_here = fold_here ? 1 : 0;
             a.fold = f;

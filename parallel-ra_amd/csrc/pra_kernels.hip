@@ -1499,6 +1499,8 @@ __device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u
     u64 s = pra_mix64(key) & m;
     for (u64 q = 0; q < a.cap; q++) {
         unsigned long long* slot = (unsigned long long*)(a.table + s);
+        // a plain load first: a slot held by another key then costs no
+        // atomic (swapping straight away measured slower)
         u64 cur = *slot;
         if (cur == 0) {
             cur = atomicCAS(slot, 0ULL, (unsigned long long)mine);
@@ -1579,6 +1581,8 @@ __device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n,
                     const u64 x = (v >> PRA_DENSE_PBITS) - 1;
                     atomicOr(&a.bitmap[x >> 5], 1u << (x & 31));
                     *slot = 0;
+                    // the rank index, for the newt scatter's packed key
+                    a.slot_of[p] = x;
                 }
             }
             a.keep[p] = kp;
@@ -1755,38 +1759,47 @@ __device__ __forceinline__ void pra_scatter_block(NewtScatterArgs& a, const u32 
     u64 o[NC], an[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) { o[c] = 0; an[c] = ~0ULL; }
-    // packed keys: the column ranks of all items are looked up column by
-    // column with every item's first probe in flight at once (a dictionary
-    // slot resolves at its first probe unless its chain is longer)
+    // packed keys: from the rank index the dense decision left, or else the
+    // column ranks of all items are looked up column by column with every
+    // item's first probe in flight at once (a dictionary slot resolves at its
+    // first probe unless its chain is longer)
     u64 K[PACKED ? IT : 1];
     u32 lr[PACKED ? IT : 1];
     if (packed) {
-#pragma unroll
-        for (int k = 0; k < IT; k++) K[k] = dg[k];
-        for (int c = 0; c < a.cw; c++) {
-            u64 v[IT], e[IT], r1[IT];
+        if (a.xk) {
 #pragma unroll
             for (int k = 0; k < IT; k++) {
-                v[k] = e[k] = r1[k] = 0;
-                if ((kept >> k) & 1u) v[k] = a.cand[(base + (u64)k * 64 + lane) * a.cw + c];
+                K[k] = 0;
+                if ((kept >> k) & 1u) K[k] = (u64)dg[k] * a.xk_scale + a.xk[base + (u64)k * 64 + lane];
             }
+        } else {
 #pragma unroll
-            for (int k = 0; k < IT; k++)
-                if ((kept >> k) & 1u) {
-                    const u64* sl = a.dict + 2 * pra_dict_slot(v[k], a.dict_cap);
-                    e[k] = sl[0];
-                    r1[k] = sl[1];
+            for (int k = 0; k < IT; k++) K[k] = dg[k];
+            for (int c = 0; c < a.cw; c++) {
+                u64 v[IT], e[IT], r1[IT];
+#pragma unroll
+                for (int k = 0; k < IT; k++) {
+                    v[k] = e[k] = r1[k] = 0;
+                    if ((kept >> k) & 1u) v[k] = a.cand[(base + (u64)k * 64 + lane) * a.cw + c];
                 }
 #pragma unroll
-            for (int k = 0; k < IT; k++)
-                if ((kept >> k) & 1u) {
-                    u64 r = 0;
-                    if (r1[k] != 0 && e[k] == v[k])
-                        r = r1[k] - 1;
-                    else if (!pra_dict_rank(a.dict, a.dict_cap, v[k], &r))
-                        kmiss = 1;
-                    K[k] = K[k] * a.rank_radix + r;
-                }
+                for (int k = 0; k < IT; k++)
+                    if ((kept >> k) & 1u) {
+                        const u64* sl = a.dict + 2 * pra_dict_slot(v[k], a.dict_cap);
+                        e[k] = sl[0];
+                        r1[k] = sl[1];
+                    }
+#pragma unroll
+                for (int k = 0; k < IT; k++)
+                    if ((kept >> k) & 1u) {
+                        u64 r = 0;
+                        if (r1[k] != 0 && e[k] == v[k])
+                            r = r1[k] - 1;
+                        else if (!pra_dict_rank(a.dict, a.dict_cap, v[k], &r))
+                            kmiss = 1;
+                        K[k] = K[k] * a.rank_radix + r;
+                    }
+            }
         }
         // every row's (key, newt position) goes to the bin of the key's top
         // byte: a rank inside the block's share of the bin, then one counter

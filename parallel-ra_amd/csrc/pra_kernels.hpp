@@ -181,7 +181,7 @@ struct DecideJob {
 struct DenseArgs {
     const u64* cand;
     u64* ctl;
-    u64* slot_of;
+    u64* slot_of;        // claim: the slot (~0: dropped); decision: the kept candidates' x
     u32* keep;
     const u32* dest;
     u32* hist;
@@ -272,6 +272,12 @@ struct NewtScatterArgs {
     u64* bin_keys;     // 256 bins of bin_cap packed keys
     u32* bin_idx;      // 256 bins of bin_cap newt positions
     u32* bin_cur;      // 256 bin fill counters (all rows, also those past bin_cap)
+    // packed keys of a dense-membership iteration: each kept candidate's rank
+    // index (sum of rank_c * D^(cw-1-c), left by the dense decision), so the
+    // packed key is bucket * xk_scale + index with xk_scale = D^cw; null:
+    // the ranks are looked up in the dictionary
+    const u64* xk;
+    u64 xk_scale;
     const u64* cand;
     const u32* keep;
     const u32* dest;
