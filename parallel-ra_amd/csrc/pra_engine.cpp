@@ -397,6 +397,8 @@ struct pra_rel {
     // per-iteration claim table (empty between iterations)
     DBuf bm, dtab;
     u64 bm_gen = ~0ULL, dtab_cap = 0, dtab_use = 0;   // slots allocated, slots in use this iteration
+    int dtab_eb = -1;      // epoch bits of the claim table's slots (0: winners empty their slots)
+    u32 dtab_epoch = 0;    // this iteration's epoch (slots of any other count as empty)
     int bm_kw = 0;
     bool bm_valid = false;
     // join index (key -> first row, count) per version
@@ -540,6 +542,7 @@ struct pra_ctx {
     bool tune_on = true;         // autotuned knobs (PRA_AUTOTUNE=0: the fixed defaults)
     bool seg_sort_on = true;     // segmented sort of newts too large for the bins (PRA_SEG_SORT=0: byte passes)
     u64 bin_cap_test = 0;        // PRA_SORT_TEST_BIN_CAP: a lower bin capacity, so tests reach the large-newt paths
+    int dense_epoch_bits = 16;   // PRA_DENSE_EPOCH_BITS: at most this many epoch bits in claim slots (0: none)
     DBuf segb;                   // segment bounds of the segmented sort, then its largest segment
     // timed launches of an iteration, resolved per candidate after its readback
     struct TuneSample {
@@ -1401,12 +1404,30 @@ static bool ensure_dense(pra_ctx* c, pra_rel* R, int kw, u64 C, u64 slots_per_ca
     // table is allocated for the largest use so far, and every slot is empty
     // between iterations, so a smaller use is a prefix of it
     const u64 cap = pow2_at_least(std::max<u64>(1024, slots_per_candidate * C));
+    // the bits x + 1 <= D^kw needs leave the rest of its 36 bits to an epoch
+    // tag (at most 16, or the context's setting): each use of the table takes
+    // the next epoch, so the slots of earlier ones count as empty, and the
+    // table is filled with zeros only when the epochs wrap
+    int kbits = 0;
+    while (kbits < 36 && ((unsigned __int128)1 << kbits) <= bits) kbits++;
+    const int eb = std::min(c->dense_epoch_bits, 36 - kbits);
+    bool clear = false;
     if (cap > R->dtab_cap) {
         R->dtab.reserve(cap * 8, c->st);
-        pra_launch_dense_table_clear(c->st, R->dtab.as<u64>(), cap);
-        check_launch("dense_table_clear");
         R->dtab_cap = cap;
+        clear = true;
     }
+    if (eb != R->dtab_eb) {
+        R->dtab_eb = eb;
+        clear = true;
+    }
+    if (eb > 0 && !clear && R->dtab_epoch + 1 >= (1u << eb)) clear = true;
+    if (clear) {
+        pra_launch_dense_table_clear(c->st, R->dtab.as<u64>(), R->dtab_cap);
+        check_launch("dense_table_clear");
+        R->dtab_epoch = 0;
+    }
+    if (eb > 0) R->dtab_epoch++;
     R->dtab_use = cap;
     return true;
 }
@@ -2120,6 +2141,10 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                 da.bitmap = R->bm.as<u32>();
                 da.table = R->dtab.as<u64>();
                 da.cap = R->dtab_use;
+                if (R->dtab_eb > 0) {
+                    da.emask = ~0ULL << (64 - R->dtab_eb);
+                    da.stamp = (u64)R->dtab_epoch << (64 - R->dtab_eb);
+                }
                 da.nblocks = nblocks;
                 da.nb = (u32)nb;
                 da.cw = cw;
@@ -3009,6 +3034,8 @@ int pra_ctx_create(int device, int nbuckets, uint64_t hbm_budget, pra_ctx** out)
         c->seg_sort_on = !(seg_env && seg_env[0] == '0');
         const char* cap_env = getenv("PRA_SORT_TEST_BIN_CAP");
         c->bin_cap_test = cap_env ? strtoull(cap_env, nullptr, 10) : 0;
+        const char* eb_env = getenv("PRA_DENSE_EPOCH_BITS");
+        if (eb_env) c->dense_epoch_bits = std::max(0, std::min(16, atoi(eb_env)));
         c->flags_buf.reserve(64, c->st);
         pra_launch_fill_u64(c->st, c->flags_buf.as<u64>(), 1, 0);
         c->rep_words.reserve((size_t)PRA_NREP * PRA_PART_WORDS * 8, c->st);

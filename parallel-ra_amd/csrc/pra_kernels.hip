@@ -1477,12 +1477,15 @@ __device__ __forceinline__ bool pra_dense_index(const u64* key, const u64* dict,
     return true;
 }
 
+// a slot written in this iteration (the others count as empty)
+__device__ __forceinline__ bool pra_dense_live(u64 v, const DenseArgs& a) { return v != 0 && (v & a.emask) == a.stamp; }
+
 // A candidate whose key is a member (bit set) is dropped at once
 // (slot_of = ~0); any other claims or finds the slot holding x + 1 and lowers
 // the slot's arrival to its own position, so the earliest arrival wins.  A
-// slot is one word, (x + 1) << PRA_DENSE_PBITS | arrival (0 = empty): the
-// first candidate of a key claims it with one compare-and-swap, a later one
-// reads it and swaps only while its own arrival is earlier.
+// slot is one word, epoch | (x + 1) << PRA_DENSE_PBITS | arrival: the first
+// candidate of a key claims an empty slot with one compare-and-swap, a later
+// one reads it and swaps only while its own arrival is earlier.
 template <int KW>
 __device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u32* flags)
 {
@@ -1495,21 +1498,24 @@ __device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u
         a.slot_of[p] = ~0ULL;
         return;
     }
-    const u64 key = x + 1, m = a.cap - 1, mine = (key << PRA_DENSE_PBITS) | p;
+    const u64 key = x + 1, m = a.cap - 1, mine = a.stamp | (key << PRA_DENSE_PBITS) | p;
     u64 s = pra_mix64(key) & m;
     for (u64 q = 0; q < a.cap; q++) {
         unsigned long long* slot = (unsigned long long*)(a.table + s);
         // a plain load first: a slot held by another key then costs no
         // atomic (swapping straight away measured slower)
         u64 cur = *slot;
-        if (cur == 0) {
-            cur = atomicCAS(slot, 0ULL, (unsigned long long)mine);
-            if (cur == 0) {
+        if (!pra_dense_live(cur, a)) {
+            // empty: every other write in this launch leaves a live slot, so
+            // a failed swap returns one
+            const u64 prev = atomicCAS(slot, (unsigned long long)cur, (unsigned long long)mine);
+            if (prev == cur) {
                 a.slot_of[p] = s;
                 return;
             }
+            cur = prev;
         }
-        if ((cur >> PRA_DENSE_PBITS) == key) {
+        if (((cur & ~a.emask) >> PRA_DENSE_PBITS) == key) {
             // the key's slot: lower its arrival while ours is earlier (the
             // key bits never change once set)
             while ((cur & PRA_DENSE_PMASK) > p) {
@@ -1573,14 +1579,15 @@ __device__ __forceinline__ void pra_dense_decide_tile(const DenseArgs& a, u64 n,
             if (s != ~0ULL) {
                 u64* slot = a.table + s;
                 const u64 v = *slot;
-                // the earliest arrival of the key keeps it; the slot returns to
-                // empty, which no later arrival's position matches
-                if (v != 0 && (v & PRA_DENSE_PMASK) == p) {
+                // the earliest arrival of the key keeps it; without epochs
+                // the slot returns to empty, which no later arrival's
+                // position matches
+                if (pra_dense_live(v, a) && (v & PRA_DENSE_PMASK) == p) {
                     kp = 1;
                     d = a.dest[p];
-                    const u64 x = (v >> PRA_DENSE_PBITS) - 1;
+                    const u64 x = ((v & ~a.emask) >> PRA_DENSE_PBITS) - 1;
                     atomicOr(&a.bitmap[x >> 5], 1u << (x & 31));
-                    *slot = 0;
+                    if (!a.emask) *slot = 0;
                     // the rank index, for the newt scatter's packed key
                     a.slot_of[p] = x;
                 }

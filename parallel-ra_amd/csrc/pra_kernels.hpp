@@ -170,10 +170,13 @@ struct DecideJob {
 // (D = dictionary size) selects one bit of a bitmap over D^kw bits that holds
 // the keys of full u delta (and of newt, set as tuples are kept).  Within an
 // iteration, candidates that are not members claim a slot of a small table
-// keyed exactly by x + 1, one word per slot: (x + 1) << PRA_DENSE_PBITS |
-// earliest arrival (empty: 0); the earliest arrival keeps the tuple, sets its
-// bit and empties the slot, so the table is empty again after every
-// iteration.  Needs D^kw <= 2^35 (x + 1 in 36 bits) and fewer than 2^28
+// keyed exactly by x + 1, one word per slot: epoch | (x + 1) <<
+// PRA_DENSE_PBITS | earliest arrival; the earliest arrival keeps the tuple and
+// sets its bit.  A slot is live in this iteration when its top bits (emask)
+// hold this iteration's epoch (stamp) and it is not 0; any other slot counts
+// as empty, so a new epoch empties the table without touching it.  When x + 1
+// leaves no bits for an epoch (emask 0), the winner writes its slot back to
+// 0 instead.  Needs D^kw <= 2^35 (x + 1 in 36 bits) and fewer than 2^28
 // candidates.  A value missing from the dictionary sets ctl[1] = 3 and the
 // output is redone the general way.
 #define PRA_DENSE_PBITS 28
@@ -191,6 +194,7 @@ struct DenseArgs {
     u32* bitmap;
     u64* table;
     u64 cap;
+    u64 emask, stamp;    // the slots' epoch bits and this iteration's epoch in them
     u32 nblocks, nb;
     int cw, kw;
 };
