@@ -1085,7 +1085,7 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         {
             Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
             pra_launch_bin_sort(c->st, R->bin_keys.as<u64>(), R->bin_idx.as<u32>(), R->bin_cap, n, top_ofs, bytes,
-                                r.d(), w, c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags());
+                                r.d(), w, c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags(), maxbin);
             check_launch("bin_sort");
         }
         r.buf.swap(c->sort_tmp);
@@ -1160,8 +1160,7 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
             {
                 Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
                 pra_launch_seg_sort(c->st, keys.as<u64>(), c->idxa.as<u32>(), seg, top, nseg, n, bytes, r.d(), w,
-                                    c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags(),
-                                    maxseg <= PRA_SEG_NARROW_CAP);
+                                    c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags(), maxseg);
                 check_launch("seg_sort");
             }
             r.buf.swap(c->sort_tmp);
@@ -2708,6 +2707,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
     // launch, each into its emptied delta buffer; one sorts at once
     std::vector<BinSortJob> bin_jobs;
     std::vector<size_t> bin_out;
+    u64 bins_max = 0;   // the largest bin of the deferred bin sorts
     size_t n_packed_sorts = 0;
     for (size_t oi = 0; oi < no; oi++) n_packed_sorts += packed[oi] && !in_rounds[oi] && st2[oi * SW + 3] == 0;
     auto flush_small_sorts = [&]() {
@@ -2736,6 +2736,7 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
                                    bnd, n_packed_sorts > 1 ? &job : nullptr, &R->ver[PRA_DELTA].buf)) {
                 bin_jobs.push_back(job);
                 bin_out.push_back(oi);
+                bins_max = std::max<u64>(bins_max, st2[oi * SW + 6]);
             }
             bnd_ok[oi] = 1;
             continue;
@@ -2768,10 +2769,10 @@ static void run_iteration(pra_ctx* c, const std::vector<pra_scc*>& members)
         if (bin_jobs.size() == 1) {
             const BinSortJob& j = bin_jobs[0];
             pra_launch_bin_sort(c->st, j.bin_keys, j.bin_idx, j.bin_cap, j.n, j.bofs, j.bytes, j.rows, j.w, j.out,
-                                j.bmul, j.ls, j.nb, j.bnd, c->dflags());
+                                j.bmul, j.ls, j.nb, j.bnd, c->dflags(), bins_max);
         } else {
             const char* d = stage_table(c, JOB_SORT, bin_jobs.data(), bin_jobs.size() * sizeof(BinSortJob));
-            pra_launch_bin_sort_many(c->st, (const BinSortJob*)d, (int)bin_jobs.size(), c->dflags());
+            pra_launch_bin_sort_many(c->st, (const BinSortJob*)d, (int)bin_jobs.size(), c->dflags(), bins_max);
         }
         check_launch("bin_sort");
         for (size_t oi2 : bin_out) outs[oi2]->ver[PRA_NEWT].buf.swap(outs[oi2]->ver[PRA_DELTA].buf);

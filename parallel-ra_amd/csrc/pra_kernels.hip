@@ -649,32 +649,39 @@ __device__ __forceinline__ void pra_bin_sort_range(const u64* keys, const u32* i
     }
 }
 
-// bin b of the newt scatter's top-byte bins (bofs = the bins' first sorted positions)
+// bin b of the newt scatter's top-byte bins (bofs = the bins' first sorted
+// positions) by a block of BLOCK threads
+template <int BLOCK>
 __device__ __forceinline__ void pra_bin_sort_block(const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n,
                                                    const u32* bofs, u32 bytes, const u64* rows, int w, u64* out,
                                                    u64 bmul, int ls, u64 nb, u64* bnd, u32* flags, const u32 b)
 {
     const u64 start = bofs[b], end = b + 1 < PRA_RADIX ? bofs[b + 1] : n;
-    pra_bin_sort_range<PRA_BIN_BLOCK>(bin_keys + (u64)b * bin_cap, bin_idx + (u64)b * bin_cap, start, end, bin_cap, n,
-                                      bytes & 0x7Fu,
-                       rows, w, out, bmul, ls, nb, bnd, flags, b, 56);
+    pra_bin_sort_range<BLOCK>(bin_keys + (u64)b * bin_cap, bin_idx + (u64)b * bin_cap, start, end, bin_cap, n,
+                              bytes & 0x7Fu, rows, w, out, bmul, ls, nb, bnd, flags, b, 56);
 }
 
-__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort(const u64* bin_keys, const u32* bin_idx, u32 bin_cap,
-                                                            u64 n, const u32* bofs, u32 bytes, const u64* rows, int w,
-                                                            u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
+// BLOCK threads per bin: the narrowest of 256, 512 and PRA_BIN_BLOCK whose
+// BLOCK * PRA_BIN_ITEMS pairs hold the largest bin (a wave holds 512 of a
+// bin's pairs, so small bins leave most waves of a wide block without items,
+// and every wave still takes part in each pass's barriers and digit scan)
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bin_sort(const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n,
+                                                    const u32* bofs, u32 bytes, const u64* rows, int w, u64* out,
+                                                    u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
 {
-    pra_bin_sort_block(bin_keys, bin_idx, bin_cap, n, bofs, bytes, rows, w, out, bmul, ls, nb, bnd, flags,
-                       blockIdx.x);
+    pra_bin_sort_block<BLOCK>(bin_keys, bin_idx, bin_cap, n, bofs, bytes, rows, w, out, bmul, ls, nb, bnd, flags,
+                              blockIdx.x);
 }
 
 // The bin sorts of several outputs in one launch: block b sorts bin b % 256
 // of job b / 256 (job table in device memory).
-__global__ __launch_bounds__(PRA_BIN_BLOCK) void k_bin_sort_many(const BinSortJob* jobs, u32* flags)
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bin_sort_many(const BinSortJob* jobs, u32* flags)
 {
     const BinSortJob& j = jobs[blockIdx.x / PRA_RADIX];
-    pra_bin_sort_block(j.bin_keys, j.bin_idx, j.bin_cap, j.n, j.bofs, j.bytes, j.rows, j.w, j.out, j.bmul, j.ls, j.nb,
-                       j.bnd, flags, blockIdx.x % PRA_RADIX);
+    pra_bin_sort_block<BLOCK>(j.bin_keys, j.bin_idx, j.bin_cap, j.n, j.bofs, j.bytes, j.rows, j.w, j.out, j.bmul, j.ls,
+                              j.nb, j.bnd, flags, blockIdx.x % PRA_RADIX);
 }
 
 // Segmented form for newts too large for the top-byte bins: the pairs are
@@ -2600,21 +2607,34 @@ void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 
 
 void pra_launch_seg_sort(hipStream_t st, const u64* keys, const u32* idx, const u64* seg, int top, u32 nseg, u64 n,
                          u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
-                         bool narrow)
+                         u64 maxseg)
 {
-    if (narrow)
-        hipLaunchKernelGGL(k_seg_sort<PRA_SEG_NARROW_BLOCK>, dim3(nseg), dim3(PRA_SEG_NARROW_BLOCK), 0, st, keys, idx,
-                           seg, top, n, bytes, rows, w, out, bmul, ls, nb, bnd, flags);
+#define PRA_SEG_SORT(B)                                                                                              \
+    hipLaunchKernelGGL(k_seg_sort<B>, dim3(nseg), dim3(B), 0, st, keys, idx, seg, top, n, bytes, rows, w, out, bmul, ls, \
+                       nb, bnd, flags)
+    if (maxseg <= 256 * PRA_BIN_ITEMS)
+        PRA_SEG_SORT(256);
+    else if (maxseg <= 512 * PRA_BIN_ITEMS)
+        PRA_SEG_SORT(512);
     else
-        hipLaunchKernelGGL(k_seg_sort<PRA_BIN_BLOCK>, dim3(nseg), dim3(PRA_BIN_BLOCK), 0, st, keys, idx, seg, top, n,
-                           bytes, rows, w, out, bmul, ls, nb, bnd, flags);
+        PRA_SEG_SORT(PRA_BIN_BLOCK);
+#undef PRA_SEG_SORT
 }
 
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
-                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags)
+                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
+                         u64 maxbin)
 {
-    hipLaunchKernelGGL(k_bin_sort, dim3(PRA_RADIX), dim3(PRA_BIN_BLOCK), 0, st, bin_keys, bin_idx, bin_cap, n, bofs,
-                       bytes, rows, w, out, bmul, ls, nb, bnd, flags);
+#define PRA_BIN_SORT(B)                                                                                              \
+    hipLaunchKernelGGL(k_bin_sort<B>, dim3(PRA_RADIX), dim3(B), 0, st, bin_keys, bin_idx, bin_cap, n, bofs, bytes, rows, \
+                       w, out, bmul, ls, nb, bnd, flags)
+    if (maxbin <= 256 * PRA_BIN_ITEMS)
+        PRA_BIN_SORT(256);
+    else if (maxbin <= 512 * PRA_BIN_ITEMS)
+        PRA_BIN_SORT(512);
+    else
+        PRA_BIN_SORT(PRA_BIN_BLOCK);
+#undef PRA_BIN_SORT
 }
 
 void pra_launch_small_sort(hipStream_t st, const u64* rows, u64 n, int w, int jcc, int kc, u64 nb, int bucketed,
@@ -2815,10 +2835,17 @@ void pra_launch_dedup_decide_hist(hipStream_t st, const u64* cand, const u64* ct
 #undef PRA_DECIDE
 }
 
-void pra_launch_bin_sort_many(hipStream_t st, const BinSortJob* jobs, int njobs, u32* flags)
+void pra_launch_bin_sort_many(hipStream_t st, const BinSortJob* jobs, int njobs, u32* flags, u64 maxbin)
 {
-    if (njobs > 0)
-        hipLaunchKernelGGL(k_bin_sort_many, dim3(PRA_RADIX * njobs), dim3(PRA_BIN_BLOCK), 0, st, jobs, flags);
+    if (njobs <= 0) return;
+#define PRA_BIN_SORT(B) hipLaunchKernelGGL(k_bin_sort_many<B>, dim3(PRA_RADIX * njobs), dim3(B), 0, st, jobs, flags)
+    if (maxbin <= 256 * PRA_BIN_ITEMS)
+        PRA_BIN_SORT(256);
+    else if (maxbin <= 512 * PRA_BIN_ITEMS)
+        PRA_BIN_SORT(512);
+    else
+        PRA_BIN_SORT(PRA_BIN_BLOCK);
+#undef PRA_BIN_SORT
 }
 
 void pra_launch_scan_small_many(hipStream_t st, const ScanJob* jobs, int njobs)

@@ -360,18 +360,19 @@ void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const 
 // pra_launch_seg_bounds writes seg[0..nseg] and the largest segment into
 // *maxseg (zeroed by the caller); pra_launch_seg_sort sorts every segment in
 // LDS by its varying bytes (bit p of `bytes`), gathers the rows into out and
-// writes the bucket bounds as pra_launch_bin_sort does.  narrow: blocks of
-// PRA_SEG_NARROW_BLOCK threads, segments of at most PRA_SEG_NARROW_CAP pairs
-// (more blocks resident per compute unit); else PRA_BIN_BLOCK threads and
-// PRA_BIN_CAP pairs.  A larger segment raises PRA_FLAG_BIN_OVERFLOW.
-#define PRA_SEG_NARROW_BLOCK 256
-#define PRA_SEG_NARROW_CAP (PRA_SEG_NARROW_BLOCK * PRA_BIN_ITEMS)
+// writes the bucket bounds as pra_launch_bin_sort does.  maxseg (the largest
+// segment) picks the narrowest block of 256, 512 or PRA_BIN_BLOCK threads
+// that holds it (PRA_BIN_ITEMS pairs per thread); a segment over PRA_BIN_CAP
+// pairs raises PRA_FLAG_BIN_OVERFLOW.
 void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg);
 void pra_launch_seg_sort(hipStream_t st, const u64* keys, const u32* idx, const u64* seg, int top, u32 nseg, u64 n,
                          u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
-                         bool narrow);
+                         u64 maxseg);
+// maxbin: the largest bin (of every job, for the batched form), which picks
+// the narrowest block of 256, 512 or PRA_BIN_BLOCK threads that holds it
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
-                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags);
+                         u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
+                         u64 maxbin);
 void pra_launch_col_orand2(hipStream_t st, const u64* rows, u64 n, int w, int kc, u64* out);
 void pra_launch_dedup_claim2(hipStream_t st, const u64* cand, const u64* ctl, u64 capn, int cw, int kw, u64* table,
                              u64 cap, u64* slot_of, u64 epoch, u32* flags);
@@ -396,7 +397,7 @@ void pra_launch_dense_table_clear(hipStream_t st, u64* table, u64 cap);
 // batched forms of k_bin_sort (256 blocks per job), k_scan_small (u32, one
 // block per job of at most PRA_SMALL_SCAN elements), D2D copies (job table
 // followed by njobs + 1 first blocks; 2,048 words per block) and k_newt_fold
-void pra_launch_bin_sort_many(hipStream_t st, const BinSortJob* jobs, int njobs, u32* flags);
+void pra_launch_bin_sort_many(hipStream_t st, const BinSortJob* jobs, int njobs, u32* flags, u64 maxbin);
 void pra_launch_scan_small_many(hipStream_t st, const ScanJob* jobs, int njobs);
 void pra_launch_copy_many(hipStream_t st, const CopyJob* jobs, int njobs, u32 nblocks);
 void pra_launch_newt_fold_many(hipStream_t st, const NewtFoldArgs* jobs, int njobs);
