@@ -1145,12 +1145,17 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
         const int sb = std::min(16, std::max(9, ceil_log2(std::max<u64>(1, n / 2048))));
         const int top = 64 - sb;
         const u32 nseg = 1u << sb;
-        c->segb.reserve(((size_t)nseg + 2) * 8, c->st);
+        // segment bounds, then the largest segment and the number of those
+        // over PRA_SEG_SMALL pairs, then their list
+        c->segb.reserve(((size_t)nseg + 3) * 8 + (size_t)nseg * 4, c->st);
         u64* seg = c->segb.as<u64>();
-        pra_launch_fill_u64(c->st, seg + nseg + 1, 1, 0);
-        pra_launch_seg_bounds(c->st, keys.as<u64>(), n, top, nseg, seg, seg + nseg + 1);
+        u64* sst = seg + nseg + 1;
+        u32* large = (u32*)(seg + nseg + 3);
+        pra_launch_fill_u64(c->st, sst, 2, 0);
+        pra_launch_seg_bounds(c->st, keys.as<u64>(), n, top, nseg, seg, sst, large);
         check_launch("seg_bounds");
-        const u64 maxseg = readback(c, seg + nseg + 1, 1)[0];
+        const u64* sstat = readback(c, sst, 2);
+        const u64 maxseg = sstat[0], nlarge = sstat[1];
         if (maxseg <= PRA_BIN_CAP) {
             const u64 low = vary & ((1ULL << top) - 1);
             u32 bytes = 0;
@@ -1160,7 +1165,8 @@ static bool sort_by_packed_key(pra_ctx* c, pra_rel* R, u64 kor, u64 kand, u64 ma
             {
                 Prof p(c, K_BIN_SORT, 12.0 * n + 16.0 * n * w);
                 pra_launch_seg_sort(c->st, keys.as<u64>(), c->idxa.as<u32>(), seg, top, nseg, n, bytes, r.d(), w,
-                                    c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags(), maxseg);
+                                    c->sort_tmp.as<u64>(), bmul, ls, (u64)c->nb, bnd, c->dflags(), maxseg, large,
+                                    nlarge);
                 check_launch("seg_sort");
             }
             r.buf.swap(c->sort_tmp);

@@ -688,20 +688,25 @@ __global__ __launch_bounds__(BLOCK) void k_bin_sort_many(const BinSortJob* jobs,
 // already ordered by the keys' top 16 bits (two counting passes), segment b
 // is the pairs whose key bits from `top` up equal b ([seg[b], seg[b + 1])),
 // and block b sorts it by the varying bytes below (bit p of `bytes`).
+// Block i sorts segment list[i] (list null: segment i, skipping a segment
+// of more than skip_above pairs, which a launch over the list sorts).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_seg_sort(const u64* keys, const u32* idx, const u64* seg, int top, u64 n,
                                                     u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls,
-                                                    u64 nb, u64* bnd, u32* flags)
+                                                    u64 nb, u64* bnd, u32* flags, const u32* list, u32 skip_above)
 {
-    const u32 b = blockIdx.x;
+    const u32 b = list ? list[blockIdx.x] : blockIdx.x;
     const u64 start = seg[b], end = seg[b + 1];
+    if (end - start > skip_above) return;
     pra_bin_sort_range<BLOCK>(keys + start, idx + start, start, end, BLOCK * PRA_BIN_ITEMS, n, bytes, rows, w, out,
                               bmul, ls, nb, bnd, flags, b, top);
 }
 
 // seg[v] = first position of a sorted key array whose bits from `top` up are
-// >= v, for v in [0, nseg] (seg[nseg] = n); *maxseg = the largest segment
-__global__ void k_seg_bounds(const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg)
+// >= v, for v in [0, nseg] (seg[nseg] = n); stat[0] = the largest segment,
+// and the stat[1] segments of more than PRA_SEG_SMALL pairs are listed in
+// large (in no particular order)
+__global__ void k_seg_bounds(const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* stat, u32* large)
 {
     const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nseg) return;
@@ -721,7 +726,9 @@ __global__ void k_seg_bounds(const u64* keys, u64 n, int top, u32 nseg, u64* seg
     }
     seg[v] = b2[0];
     if (v + 1 == nseg) seg[nseg] = n;
-    atomicMax((unsigned long long*)maxseg, (unsigned long long)(b2[1] - b2[0]));
+    const u64 size = b2[1] - b2[0];
+    atomicMax((unsigned long long*)stat, (unsigned long long)size);
+    if (size > PRA_SEG_SMALL) large[atomicAdd((unsigned long long*)(stat + 1), 1ULL)] = (u32)v;
 }
 
 // bnd[s] = first position whose key is >= (s * bmul) << ls (the key is
@@ -2600,24 +2607,28 @@ void pra_launch_sort_scatter(hipStream_t st, const u64* in, u64* out, u64 n, int
     }
 }
 
-void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg)
+void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* stat, u32* large)
 {
-    hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(nseg)), dim3(PRA_BLOCK), 0, st, keys, n, top, nseg, seg, maxseg);
+    hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(nseg)), dim3(PRA_BLOCK), 0, st, keys, n, top, nseg, seg, stat,
+                       large);
 }
 
 void pra_launch_seg_sort(hipStream_t st, const u64* keys, const u32* idx, const u64* seg, int top, u32 nseg, u64 n,
                          u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
-                         u64 maxseg)
+                         u64 maxseg, const u32* large, u64 nlarge)
 {
-#define PRA_SEG_SORT(B)                                                                                              \
-    hipLaunchKernelGGL(k_seg_sort<B>, dim3(nseg), dim3(B), 0, st, keys, idx, seg, top, n, bytes, rows, w, out, bmul, ls, \
-                       nb, bnd, flags)
-    if (maxseg <= 256 * PRA_BIN_ITEMS)
-        PRA_SEG_SORT(256);
-    else if (maxseg <= 512 * PRA_BIN_ITEMS)
-        PRA_SEG_SORT(512);
-    else
-        PRA_SEG_SORT(PRA_BIN_BLOCK);
+#define PRA_SEG_SORT(B, G, L, S)                                                                                     \
+    hipLaunchKernelGGL(k_seg_sort<B>, dim3(G), dim3(B), 0, st, keys, idx, seg, top, n, bytes, rows, w, out, bmul, ls, \
+                       nb, bnd, flags, L, S)
+    static_assert(PRA_SEG_SMALL == 256 * PRA_BIN_ITEMS, "the small segments fit the 256-thread blocks");
+    // every segment that fits the 256-thread blocks, then the larger ones
+    PRA_SEG_SORT(256, nseg, (const u32*)nullptr, maxseg <= PRA_SEG_SMALL ? ~0u : (u32)PRA_SEG_SMALL);
+    if (maxseg > PRA_SEG_SMALL && nlarge > 0) {
+        if (maxseg <= 512 * PRA_BIN_ITEMS)
+            PRA_SEG_SORT(512, nlarge, large, ~0u);
+        else
+            PRA_SEG_SORT(PRA_BIN_BLOCK, nlarge, large, ~0u);
+    }
 #undef PRA_SEG_SORT
 }
 

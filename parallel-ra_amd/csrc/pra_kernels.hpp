@@ -360,14 +360,17 @@ void pra_launch_block_sort(hipStream_t st, const u64* rows, u64 n, int w, const 
 // pra_launch_seg_bounds writes seg[0..nseg] and the largest segment into
 // *maxseg (zeroed by the caller); pra_launch_seg_sort sorts every segment in
 // LDS by its varying bytes (bit p of `bytes`), gathers the rows into out and
-// writes the bucket bounds as pra_launch_bin_sort does.  maxseg (the largest
-// segment) picks the narrowest block of 256, 512 or PRA_BIN_BLOCK threads
-// that holds it (PRA_BIN_ITEMS pairs per thread); a segment over PRA_BIN_CAP
-// pairs raises PRA_FLAG_BIN_OVERFLOW.
-void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* maxseg);
+// writes the bucket bounds as pra_launch_bin_sort does.  Segments of at most
+// PRA_SEG_SMALL pairs are sorted by 256-thread blocks; the nlarge larger ones
+// (their ids in `large`, listed by pra_launch_seg_bounds) by a second launch
+// of the narrowest blocks, 512 or PRA_BIN_BLOCK threads, that hold maxseg
+// (the largest); a segment over PRA_BIN_CAP pairs raises
+// PRA_FLAG_BIN_OVERFLOW.
+#define PRA_SEG_SMALL 2048
+void pra_launch_seg_bounds(hipStream_t st, const u64* keys, u64 n, int top, u32 nseg, u64* seg, u64* stat, u32* large);
 void pra_launch_seg_sort(hipStream_t st, const u64* keys, const u32* idx, const u64* seg, int top, u32 nseg, u64 n,
                          u32 bytes, const u64* rows, int w, u64* out, u64 bmul, int ls, u64 nb, u64* bnd, u32* flags,
-                         u64 maxseg);
+                         u64 maxseg, const u32* large, u64 nlarge);
 // maxbin: the largest bin (of every job, for the batched form), which picks
 // the narrowest block of 256, 512 or PRA_BIN_BLOCK threads that holds it
 void pra_launch_bin_sort(hipStream_t st, const u64* bin_keys, const u32* bin_idx, u32 bin_cap, u64 n, const u32* bofs,
