@@ -1531,13 +1531,18 @@ __device__ __forceinline__ void pra_dense_claim_one(const DenseArgs& a, u64 p, u
         }
         if (((cur & ~a.emask) >> PRA_DENSE_PBITS) == key) {
             // the key's slot: lower its arrival while ours is earlier (the
-            // key bits never change once set)
+            // key bits never change once set); a candidate that never got in
+            // has lost to an earlier arrival for good, and the decision skips it
+            bool in = false;
             while ((cur & PRA_DENSE_PMASK) > p) {
                 const u64 prev = atomicCAS(slot, (unsigned long long)cur, (unsigned long long)mine);
-                if (prev == cur) break;
+                if (prev == cur) {
+                    in = true;
+                    break;
+                }
                 cur = prev;
             }
-            a.slot_of[p] = s;
+            a.slot_of[p] = in ? s : ~0ULL;
             return;
         }
         s = (s + 1) & m;
