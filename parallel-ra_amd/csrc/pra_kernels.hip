@@ -1757,6 +1757,24 @@ __device__ __forceinline__ void pra_scatter_block(NewtScatterArgs& a, const u32 
             if ((peers >> lane) == 1ULL) wh[wave][d] = before + (u32)__popcll(peers);
         }
     }
+    // the wide form loads the kept rows' words (up to PW) and rank indexes
+    // now, so that they arrive while the block forms its offsets and bin
+    // counts (loads are not moved across the barriers in between)
+    constexpr int PW = SB == PRA_SCATTER_BLOCK ? 2 : 0;
+    const bool pre_rows = PW > 0 && a.cw <= PW;
+    u64 prow[IT][PW > 0 ? PW : 1], pxk[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const u64 p = base + (u64)k * 64 + lane;
+        pxk[k] = 0;
+        if ((kept >> k) & 1u) {
+            if (PACKED && PW > 0 && a.xk) pxk[k] = a.xk[p];
+            if (pre_rows)
+#pragma unroll
+                for (int c = 0; c < (PW > 0 ? PW : 1); c++)
+                    if (c < a.cw) prow[k][c] = a.cand[p * a.cw + c];
+        }
+    }
     __syncthreads();
     for (u32 d = threadIdx.x; d < a.nb; d += SB) {
         u32 acc = a.offs[(u64)d * a.nblocks + bid];
@@ -1796,7 +1814,8 @@ __device__ __forceinline__ void pra_scatter_block(NewtScatterArgs& a, const u32 
 #pragma unroll
             for (int k = 0; k < IT; k++) {
                 K[k] = 0;
-                if ((kept >> k) & 1u) K[k] = (u64)dg[k] * a.xk_scale + a.xk[base + (u64)k * 64 + lane];
+                if ((kept >> k) & 1u)
+                    K[k] = (u64)dg[k] * a.xk_scale + (PW > 0 ? pxk[k] : a.xk[base + (u64)k * 64 + lane]);
             }
         } else {
 #pragma unroll
@@ -1848,7 +1867,13 @@ __device__ __forceinline__ void pra_scatter_block(NewtScatterArgs& a, const u32 
             u64 pos = (u64)wh[wave][d] + lc[k];
             const u64* src = a.cand + p * a.cw;
             u64* dst = a.newt + pos * a.wn;
-            pra_copy_row(dst, src, a.cw);
+            if (pre_rows) {
+#pragma unroll
+                for (int c = 0; c < (PW > 0 ? PW : 1); c++)
+                    if (c < a.cw) dst[c] = prow[k][c];
+            } else {
+                pra_copy_row(dst, src, a.cw);
+            }
             if (a.intern) {
                 u64 rank = pos - a.offs[(u64)d * a.nblocks];
                 u64 key = ((u64)a.tag << 46) | ((u64)d << 28);
