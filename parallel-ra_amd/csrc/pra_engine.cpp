@@ -559,6 +559,7 @@ struct pra_ctx {
     size_t host_stage_cap = 0;
     u64* mailbox = nullptr;     // pinned, written by the publish kernel
     size_t mailbox_cap = 0;
+    u64 pub_seq = 0;            // sequence number of the last publication
     hipEvent_t pub_ev = nullptr;
     std::string err;
     // profiling
@@ -832,19 +833,22 @@ static u64* readback(pra_ctx* c, const void* src, size_t words)
 static void sync_check(pra_ctx* c) { readback(c, nullptr, 0); }
 
 // The per-iteration readback: one kernel copies the summary words and the
-// error flags into pinned host memory (no copy-engine transfers), and the host
-// polls an event recorded behind it instead of sleeping in a blocking
-// synchronization.  Completion of the event (the end of the kernel, with its
-// system-scope release) is what makes the words valid to read; after a while
-// the host falls back to a blocking wait.
+// error flags into pinned host memory (no copy-engine transfers), then, once
+// they are visible to the host, a sequence number in the word before them.  The host spins
+// on that word instead of sleeping in a blocking synchronization or waiting
+// for the kernel's completion to be signalled; an event recorded behind the
+// kernel reports errors, and after a while the host falls back to a blocking
+// wait on it.
 // fold: a newt fold still to run, done by the publishing kernel itself
 static u64* publish_begin(pra_ctx* c, const u64* src, size_t words, const NewtFoldArgs* fold = nullptr)
 {
-    u64* h = c->mailbox_buf(words + 1);
+    // mailbox word 0: the sequence number; the summary and flags after it
+    u64* h = c->mailbox_buf(words + 2) + 1;
+    const u64 seq = ++c->pub_seq;
     if (fold)
-        pra_launch_newt_fold_publish(c->st, *fold, src, words, c->flags_buf.as<u32>(), h);
+        pra_launch_newt_fold_publish(c->st, *fold, src, words, c->flags_buf.as<u32>(), h, seq);
     else
-        pra_launch_publish(c->st, src, words, c->flags_buf.as<u32>(), h);
+        pra_launch_publish(c->st, src, words, c->flags_buf.as<u32>(), h, seq);
     check_launch("publish");
     if (!c->pub_ev) HIPCHK(hipEventCreateWithFlags(&c->pub_ev, hipEventDisableTiming));
     HIPCHK(hipEventRecord(c->pub_ev, c->st));
@@ -857,12 +861,14 @@ static u64* publish_wait(pra_ctx* c, u64* h, size_t words)
 {
     auto t0 = std::chrono::steady_clock::now();
     for (u32 k = 0;; k++) {
-        hipError_t q = hipEventQuery(c->pub_ev);
-        if (q == hipSuccess) break;
-        if (q != hipErrorNotReady) HIPCHK(q);
-        if ((k & 63) == 63 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
-            HIPCHK(hipEventSynchronize(c->pub_ev));
-            break;
+        if (__atomic_load_n(h - 1, __ATOMIC_ACQUIRE) == c->pub_seq) break;
+        if ((k & 63) == 63) {
+            const hipError_t q = hipEventQuery(c->pub_ev);
+            if (q != hipSuccess && q != hipErrorNotReady) HIPCHK(q);
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+                HIPCHK(hipEventSynchronize(c->pub_ev));
+                break;
+            }
         }
     }
     check_flags((u32)h[words]);

@@ -1254,10 +1254,24 @@ __global__ void k_find_split(const u64* off, u64 n, const u32* kp, long long thr
 
 // Copy `words` summary words and then the error flags into pinned host
 // memory; the host reads them once the kernel has completed.
-__global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host)
+// The summary words and the error flags into pinned host memory, then
+// host[-1] = seq once every word is visible to the host (each thread's
+// system-scope release before the barrier, then one system-scope store), so
+// the host can start on the summary before the kernel's completion is
+// signalled.  host[-1] is written by publications only, so a sequence number
+// never meets a stale summary word there.
+__device__ __forceinline__ void pra_publish_words(const u64* src, u64 words, const u32* flags, u64* host, u64 seq)
 {
     for (u64 i = threadIdx.x; i < words; i += blockDim.x) host[i] = src[i];
     if (threadIdx.x == 0) host[words] = *flags;
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(host - 1, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void k_publish(const u64* src, u64 words, const u32* flags, u64* host, u64 seq)
+{
+    pra_publish_words(src, words, flags, host, seq);
 }
 
 // ---------------------------------------------------- dedup / newt ---------
@@ -1991,7 +2005,7 @@ __global__ void k_newt_fold(NewtFoldArgs a)
 // (k_publish) in one block: the fold's parts run in turn (skipped, as in
 // k_newt_fold, when the output overflowed its candidate capacity), then
 // `words` summary words and the error flags go to pinned host memory.
-__global__ void k_newt_fold_publish(NewtFoldArgs a, const u64* src, u64 words, const u32* flags, u64* host)
+__global__ void k_newt_fold_publish(NewtFoldArgs a, const u64* src, u64 words, const u32* flags, u64* host, u64 seq)
 {
     __shared__ u32 lds[PRA_WAVES + 1];
     __shared__ u32 mx;
@@ -2000,8 +2014,7 @@ __global__ void k_newt_fold_publish(NewtFoldArgs a, const u64* src, u64 words, c
             pra_newt_fold_part(a, y, threadIdx.x, lds, &mx);
             __syncthreads();
         }
-    for (u64 i = threadIdx.x; i < words; i += blockDim.x) host[i] = src[i];
-    if (threadIdx.x == 0) host[words] = *flags;
+    pra_publish_words(src, words, flags, host, seq);
 }
 
 // identity values of the summary replicas (all counts 0, OR words 0, AND
@@ -2542,14 +2555,14 @@ void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp,
 }
 
 void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,
-                                  const u32* flags, u64* host)
+                                  const u32* flags, u64* host, u64 seq)
 {
-    hipLaunchKernelGGL(k_newt_fold_publish, dim3(1), dim3(PRA_BLOCK), 0, st, a, src, words, flags, host);
+    hipLaunchKernelGGL(k_newt_fold_publish, dim3(1), dim3(PRA_BLOCK), 0, st, a, src, words, flags, host, seq);
 }
 
-void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host)
+void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host, u64 seq)
 {
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(PRA_BLOCK), 0, st, src, words, flags, host);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(PRA_BLOCK), 0, st, src, words, flags, host, seq);
 }
 
 void pra_launch_copy_expand(hipStream_t st, const CopyExpandArgs& a)

@@ -320,7 +320,9 @@ u32 pra_join_expand_blocks(u64 cap);   // grid of a join expansion of up to cap 
 void pra_launch_expand_many(hipStream_t st, const void* jobs, int njobs, u32 nblocks, bool join);
 void pra_launch_plan(hipStream_t st, const PlanArgs& a);
 void pra_launch_plan_all(hipStream_t st, const PlanAllArgs& a, int nblocks);
-void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host);
+// host[0..words) = src, host[words] = *flags, then host[-1] = seq once they
+// are visible to the host
+void pra_launch_publish(hipStream_t st, const u64* src, u64 words, const u32* flags, u64* host, u64 seq);
 void pra_launch_find_split(hipStream_t st, const u64* off, u64 n, const u32* kp, long long thresh, u64* out);
 void pra_launch_group_maxlen(hipStream_t st, const u64* flags, const u64* gpos, const u64* gstart, u64 n, u64* maxlen);
 void pra_launch_count_scan_many(hipStream_t st, const CountScanBatch& b, u64* status, u32* tile_ctr, u32 epoch,
@@ -407,7 +409,7 @@ void pra_launch_newt_fold_many(hipStream_t st, const NewtFoldArgs* jobs, int njo
 void pra_launch_dedup_claim_many(hipStream_t st, const ClaimBatch& b, int cls, u32* flags);
 void pra_launch_dedup_decide_many(hipStream_t st, const DecideBatch& b, int cls, u32* flags);
 void pra_launch_newt_fold_publish(hipStream_t st, const NewtFoldArgs& a, const u64* src, u64 words,
-                                  const u32* flags, u64* host);
+                                  const u32* flags, u64* host, u64 seq);
 // `sets` consecutive replica sets (one per output of a batched partition)
 void pra_launch_newt_rep_init(hipStream_t st, u64* rep_words, int sets = 1);
 void pra_launch_rows_checksum(hipStream_t st, const u64* rows, u64 n, int w, u64* out);
