@@ -283,7 +283,7 @@ def est_bytes(n):
 
 
 def bench_batch(args):
-    """Config 4: independent TC instances, plain (one context per instance,
+    """Independent TC instances on random graphs, plain (one context per instance,
     one after another) against batched (one context whose instances run in
     lock step: every phase of an iteration is one batched launch over all of
     them; instances are queued largest first and admitted between iterations
@@ -428,7 +428,7 @@ def bench_batch(args):
 
 
 def bench_serve(args):
-    """Config 5: Poisson request streams of TC solves (sizes from the batched
+    """Served reachability: Poisson request streams of TC solves (sizes from the batched
     distribution) into the serving loop (continuous batching: up to --slots
     requests in flight advance together, admitted by measured device bytes
     under the HBM budget); p50/p99 latency and aggregate joined tuples/s per
